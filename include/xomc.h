@@ -1,0 +1,309 @@
+/*
+ * xomc.h — C ABI of libxomc.so, the MI355X (gfx950) photon-packet transport
+ * engine for layered (mcml) and voxelized (mcvox) turbid media.
+ *
+ * Every entry point takes plain pointers and sizes. No torch, no HIP types.
+ * The structs below are the packed forms of the PyXOpto plugin objects; each
+ * cites the reference structure it replaces (paths relative to the PyXOpto
+ * source tree). All floats are IEEE binary32, all accumulators are uint64
+ * fixed point with K = 0x7FFFFF (xopto/mcbase/kernel/mcbase.template.h:201-205).
+ *
+ * Status codes: 0 on success, negative on error; xo_last_error() returns the
+ * message of the last failing call on the calling thread.
+ */
+#ifndef XOMC_H
+#define XOMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XO_ABI_VERSION 1
+
+/* Fixed-point accumulator scale: weight_to_int(w) = (uint32)(w*K + 0.5).
+ * xopto/mcml/kernel/mcml.template.h:606; mcbase.template.h:201-205. */
+#define XO_ACCU_K 0x7FFFFF
+
+/* Number of rows of the safe-prime MWC seed table
+ * (xopto/data/primes/safeprimes_base32_500k.npz, 500000 rows; the first row
+ * seeds the others, so 499999 generator streams are available:
+ * xopto/cl/clrng.py:438-439). */
+#define XO_NUM_PRIMES 500000u
+#define XO_MAX_SEEDS (XO_NUM_PRIMES - 1u)
+/* First multiplier of the table, searched downwards from here
+ * (xopto/src/rng/find_primes.c:50). */
+#define XO_FIRST_PRIME_A 4294967118u
+
+/* A packet still in flight after this many transport steps is terminated
+ * (counted in xo_run_report.truncated). The reference has no such limit; it
+ * only matters for packets trapped by total internal reflection in a
+ * non-absorbing, non-scattering layer, which would otherwise never end. */
+#define XO_MAX_PACKET_STEPS (1u << 26)
+
+#define XO_MAX_LAYERS 16
+#define XO_MAX_MATERIALS 32
+
+/* Scattering phase function kinds (one kind per simulation, as
+ * xopto/mcml/mclayer/layer.py:862-872 requires). */
+enum xo_pf_kind {
+    XO_PF_HG = 0,   /* xopto/mcbase/mcpf/hg.py:70-97   */
+    XO_PF_MHG = 1,  /* xopto/mcbase/mcpf/mhg.py:76-110 */
+    XO_PF_LUT = 2   /* xopto/mcbase/mcpf/lut.py:111-150 */
+};
+
+/* Surface detector kinds. */
+enum xo_det_kind {
+    XO_DET_NONE = 0,
+    XO_DET_TOTAL = 1,  /* xopto/mcml/mcdetector/total.py:86-108   */
+    XO_DET_RADIAL = 2  /* xopto/mcml/mcdetector/radial.py:117-150 */
+};
+
+/* Photon packet source kinds. */
+enum xo_src_kind {
+    XO_SRC_LINE = 0,     /* xopto/mcml/mcsource/line.py:95-110, mcvox/mcsource/line.py:84-104 */
+    XO_SRC_GAUSSIAN = 1  /* xopto/mcvox/mcsource/gaussianbeam.py:112-189 */
+};
+
+/* Stepping method (xopto/mcbase/mcoptions.py:269-318). Only albedo weight. */
+enum xo_method { XO_METHOD_AW = 0 };
+
+/* Run flags. */
+enum xo_run_flags {
+    /* Lane g runs packets g, g+L, g+2L, ... (L lanes) instead of claiming them
+     * from the shared counter: results become a deterministic function of the
+     * seeds, the configuration and L. */
+    XO_FLAG_STATIC_SCHEDULE = 1u << 0,
+    /* Keep detector bins in global memory even if they fit in LDS. */
+    XO_FLAG_NO_LDS_BINS = 1u << 1
+};
+
+/* Phase-function parameters; the union of McPf for Hg, MHg and Lut. */
+typedef struct xo_pf {
+    float g;          /* Hg, MHg: anisotropy                       */
+    float beta;       /* MHg: fraction of the HG term              */
+    float a, b, c;    /* Lut: nonlinear index transform parameters */
+    uint32_t offset;  /* Lut: first entry in the fp lookup array   */
+    uint32_t size;    /* Lut: number of entries                    */
+    uint32_t reserved;
+} xo_pf; /* 32 B */
+
+/* One layer of the stack; McLayer (xopto/mcml/mclayer/layer.py:40-72). Layer 0
+ * and layer num_layers-1 are the surrounding media. */
+typedef struct xo_layer {
+    float thickness;
+    float top;                  /* z of the top surface (-inf for layer 0)    */
+    float bottom;               /* z of the bottom surface (+inf for last)    */
+    float n;
+    float cos_critical_top;
+    float cos_critical_bottom;
+    float mus;
+    float mua;
+    float inv_mut;              /* 1/(mua+mus), inf if both are 0             */
+    float mua_inv_mut;          /* mua/(mua+mus), 1 if mus is 0               */
+    float reserved[2];
+    xo_pf pf;
+} xo_layer; /* 80 B */
+
+/* Line source; McSource of xopto/mcml/mcsource/line.py:33-64 (the mcvox Line
+ * source uses the same packed fields, xopto/mcvox/mcsource/line.py). */
+typedef struct xo_source_line {
+    float position[3];
+    float direction_medium[3];
+    float direction_sample[3];
+    float direction_reflected[3];
+    float reflectance;
+    float reserved[3];
+} xo_source_line; /* 64 B */
+
+/* Gaussian beam; McSource of xopto/mcvox/mcsource/gaussianbeam.py:33-85. */
+typedef struct xo_source_gaussian {
+    float transformation[9];    /* row-major 3x3, beam frame -> sample frame */
+    float position[3];
+    float direction[3];
+    float sigma[2];
+    float clip;
+} xo_source_gaussian; /* 72 B */
+
+/* A surface detector: Radial (radial.py:33-78) or Total (total.py:33-70). */
+typedef struct xo_detector {
+    int32_t kind;               /* enum xo_det_kind                           */
+    float direction[3];
+    float position[2];          /* Radial: centre                             */
+    float r_min;                /* Radial: (log-)scaled start of the axis     */
+    float inv_dr;               /* Radial: 1/step of the (log-)scaled axis    */
+    float cos_min;
+    uint32_t n;                 /* number of bins (1 for Total)               */
+    uint32_t offset;            /* first bin in the accumulator buffer        */
+    int32_t log_scale;
+} xo_detector; /* 48 B */
+
+/* Cartesian fluence/deposition accumulator; McFluence of
+ * xopto/mcbase/mcfluence/fluence.py:35-65. Deposition mode only. */
+typedef struct xo_fluence {
+    int32_t enabled;
+    float inv_step[3];
+    float top_left[3];
+    uint32_t shape[3];
+    uint32_t offset;
+    int32_t k;
+} xo_fluence; /* 48 B */
+
+/* Everything one layered-media run needs besides the LUT array. Compile-time
+ * options of the reference (McMinimumPacketWeight, McPacketLotteryChance,
+ * McUseLottery: xopto/mcbase/mcoptions.py:600-670) are runtime fields here. */
+typedef struct xo_mcml_cfg {
+    uint32_t num_layers;        /* including the two surrounding media (>= 3) */
+    int32_t pf_kind;            /* enum xo_pf_kind                            */
+    int32_t method;             /* enum xo_method                             */
+    float rmax;                 /* simulation radius from the source (m)      */
+    float w_min;                /* roulette threshold (1e-4)                  */
+    float lottery_chance;       /* roulette survival probability (0.1)        */
+    int32_t use_lottery;
+    uint32_t reserved;
+    xo_source_line source;
+    xo_detector top, bottom, specular;
+    xo_layer layers[XO_MAX_LAYERS];
+} xo_mcml_cfg;
+
+/* A voxel material; McMaterial of xopto/mcbase/mcmaterial.py:37-63. */
+typedef struct xo_material {
+    float n, mus, mua, inv_mut, mua_inv_mut;
+    float reserved[3];
+    xo_pf pf;
+} xo_material; /* 64 B */
+
+/* Voxel grid; McVoxelConfig of xopto/mcvox/mcgeometry/voxel.py:96-120. The
+ * material index array is int32[z][y][x] (C order). */
+typedef struct xo_voxel_cfg {
+    float top_left[3];
+    float bottom_right[3];
+    float size[3];
+    int32_t shape[3];
+} xo_voxel_cfg; /* 48 B */
+
+typedef struct xo_mcvox_cfg {
+    uint32_t num_materials;     /* material 0 is the surrounding medium       */
+    int32_t pf_kind;
+    int32_t method;
+    float rmax;
+    float w_min;
+    float lottery_chance;
+    int32_t use_lottery;
+    int32_t source_kind;        /* enum xo_src_kind                           */
+    xo_voxel_cfg voxels;
+    xo_source_line line;
+    xo_source_gaussian gaussian;
+    xo_fluence fluence;
+    xo_detector top, bottom, specular;
+    xo_material materials[XO_MAX_MATERIALS];
+} xo_mcvox_cfg;
+
+/* Run report; mirrors Mc.run_report (xopto/mcml/mc.py:329-332, 1003-1016)
+ * plus the step counters the roofline needs. */
+typedef struct xo_run_report {
+    uint64_t packets;           /* packets launched                           */
+    uint64_t steps;             /* iterations of the transport loop           */
+    uint64_t boundary_steps;    /* of which ended on a boundary               */
+    uint64_t voxel_crossings;   /* mcvox: voxel faces crossed (incl. steps)   */
+    uint64_t deposits;          /* fluence deposits issued                    */
+    uint64_t truncated;         /* packets stopped at XO_MAX_PACKET_STEPS     */
+    uint32_t lanes;             /* lanes launched                             */
+    uint32_t threads_used;      /* lanes that ran >= 1 packet (num_kernels)   */
+    double kernel_ms;           /* device time of the transport kernel        */
+    double total_ms;            /* host wall time of the whole call           */
+} xo_run_report;
+
+/* ---------------- library ---------------- */
+
+/* ABI version of the loaded library. */
+int xo_abi_version(void);
+
+/* Message of the last error on this thread ("" if none). */
+const char *xo_last_error(void);
+
+/* Number of visible HIP devices (0 if none). Replaces the OpenCL device
+ * discovery of xopto/cl/clinfo.py:359-425. */
+int xo_device_count(void);
+
+/* sizeof() of each packed struct as the library was built; the analogue of
+ * the sizeof_datatypes kernel (xopto/mcml/kernel/mcml.template.c:827-844).
+ * Writes up to n values in the order: xo_pf, xo_layer, xo_source_line,
+ * xo_source_gaussian, xo_detector, xo_fluence, xo_mcml_cfg, xo_material,
+ * xo_voxel_cfg, xo_mcvox_cfg, xo_run_report. Returns the count written. */
+int xo_struct_sizes(uint32_t *out, int n);
+
+/* ---------------- seeds (host) ---------------- */
+
+/* The first n multipliers a of the safe-prime table: a counts down from
+ * XO_FIRST_PRIME_A and is kept when a*2^32-1 and a*2^31-1 are both prime.
+ * Regenerates xopto/data/primes/safeprimes_base32_500k.npz[:,0]
+ * (xopto/src/rng/find_primes.c:43-93). n <= XO_NUM_PRIMES. */
+int xo_safeprime_multipliers(uint32_t n, uint32_t *a_out);
+
+/* Install the first n multipliers of the table computed earlier (for
+ * example read back from a cache file) so that later calls do not search
+ * again. a[0] must be XO_FIRST_PRIME_A. */
+int xo_safeprime_preload(const uint32_t *a, uint32_t n);
+
+/* MWC seeds for n streams: replaces init_RNG (xopto/src/rng/rng.cpp:64-103)
+ * called by Random.seeds (xopto/cl/clrng.py:464-530) with the built-in
+ * table. Returns 1 (and leaves x, a untouched) if xinit is not a valid
+ * initializer, exactly as init_RNG does. n <= XO_MAX_SEEDS. */
+int xo_init_rng(uint32_t n, uint64_t xinit, uint64_t *x_out, uint32_t *a_out);
+
+/* Uniform floats from one MWC stream, as the RngKernel test harness
+ * (xopto/mcbase/kernel/mcbase.template.c:1640-1646) computes them, on the
+ * host. Advances *x. */
+int xo_rng_host(uint64_t *x, uint32_t a, uint32_t n, float *out);
+
+/* ---------------- simulation contexts (device) ---------------- */
+
+typedef struct xo_ctx xo_ctx;
+
+/* A context owns the device-resident state of one simulator object (Mc):
+ * max_lanes MWC streams seeded by xinit on device `device`, the accumulator,
+ * LUT and voxel buffers, and one HIP stream. Replaces ClWorker +
+ * ClWorkerRngMixin (xopto/mcbase/mcworker.py:46-898, 1191-1248).
+ * max_lanes = 0 selects XO_MAX_SEEDS. Returns NULL on error. */
+xo_ctx *xo_ctx_create(int device, uint32_t max_lanes, uint64_t xinit);
+void xo_ctx_destroy(xo_ctx *ctx);
+
+/* Lanes the device runs concurrently for the transport kernels (the
+ * persistent grid size) and the context's seed capacity. */
+int xo_ctx_info(xo_ctx *ctx, uint32_t *persistent_lanes, uint32_t *max_lanes,
+                uint32_t *compute_units);
+
+/* Copy the mutable RNG state x of the first n lanes to the host
+ * (Mc.run(copyseeds=True), xopto/mcml/mc.py:962-966) or overwrite it. */
+int xo_ctx_get_rng(xo_ctx *ctx, uint32_t n, uint64_t *x_out, uint32_t *a_out);
+int xo_ctx_set_rng(xo_ctx *ctx, uint32_t n, const uint64_t *x, const uint32_t *a);
+
+/* Run `npackets` packets through a layered medium: the McKernel enqueue of
+ * Mc.run (xopto/mcml/mc.py:922-953; kernel mcml.template.c:346-824).
+ * lut/nlut: the fp lookup-table array (Lut phase functions), may be NULL.
+ * lanes: 0 = persistent grid (capped by max_lanes), else the lane count
+ * (maxthreads of Mc.run). accum/naccum: host output, zeroed and filled with
+ * the integer accumulators (detector bins at their offsets). The RNG state
+ * of the used lanes persists in the context across runs. */
+int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg,
+                const float *lut, uint32_t nlut,
+                uint64_t npackets, uint32_t lanes, uint32_t flags,
+                uint64_t *accum, uint64_t naccum, xo_run_report *report);
+
+/* Voxelized medium: McKernel of xopto/mcvox/kernel/mcvox.template.c:548-1048
+ * called by xopto/mcvox/mc.py:772-1068. voxels is int32[shape z][y][x]; pass
+ * NULL to reuse the grid uploaded by the previous call (the
+ * Voxels.update_required() path, xopto/mcvox/mc.py:895-899). */
+int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg,
+                 const int32_t *voxels, uint64_t nvoxels,
+                 const float *lut, uint32_t nlut,
+                 uint64_t npackets, uint32_t lanes, uint32_t flags,
+                 uint64_t *accum, uint64_t naccum, xo_run_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XOMC_H */

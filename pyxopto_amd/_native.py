@@ -1,0 +1,356 @@
+"""ctypes binding of libxomc.so (see include/xomc.h).
+
+The structures below mirror the C structs field for field; their sizes are
+checked against the library's own ``xo_struct_sizes`` when the library is
+loaded (the analogue of the reference's ``sizeof_datatypes`` check,
+xopto/mcml/mc.py:674-728). A missing library raises immediately: there is no
+fallback implementation.
+"""
+import ctypes
+import os
+import threading
+from ctypes import (POINTER, c_char_p, c_double, c_float, c_int, c_int32,
+                    c_uint32, c_uint64, c_void_p)
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, 'libxomc.so')
+DATA_DIR = os.path.join(PKG_DIR, 'data')
+PRIME_CACHE = os.path.join(DATA_DIR, 'safeprimes_base32.u32')
+
+XO_ACCU_K = 0x7FFFFF
+XO_NUM_PRIMES = 500000
+XO_MAX_SEEDS = XO_NUM_PRIMES - 1
+XO_FIRST_PRIME_A = 4294967118
+XO_MAX_PACKET_STEPS = 1 << 26
+XO_MAX_LAYERS = 16
+XO_MAX_MATERIALS = 32
+
+XO_PF_HG, XO_PF_MHG, XO_PF_LUT = 0, 1, 2
+XO_DET_NONE, XO_DET_TOTAL, XO_DET_RADIAL = 0, 1, 2
+XO_SRC_LINE, XO_SRC_GAUSSIAN = 0, 1
+XO_METHOD_AW = 0
+XO_FLAG_STATIC_SCHEDULE = 1 << 0
+XO_FLAG_NO_LDS_BINS = 1 << 1
+
+f3 = c_float * 3
+
+
+class XoPf(ctypes.Structure):
+    _fields_ = [('g', c_float), ('beta', c_float), ('a', c_float),
+                ('b', c_float), ('c', c_float), ('offset', c_uint32),
+                ('size', c_uint32), ('reserved', c_uint32)]
+
+
+class XoLayer(ctypes.Structure):
+    _fields_ = [('thickness', c_float), ('top', c_float), ('bottom', c_float),
+                ('n', c_float), ('cos_critical_top', c_float),
+                ('cos_critical_bottom', c_float), ('mus', c_float),
+                ('mua', c_float), ('inv_mut', c_float),
+                ('mua_inv_mut', c_float), ('reserved', c_float * 2),
+                ('pf', XoPf)]
+
+
+class XoSourceLine(ctypes.Structure):
+    _fields_ = [('position', f3), ('direction_medium', f3),
+                ('direction_sample', f3), ('direction_reflected', f3),
+                ('reflectance', c_float), ('reserved', c_float * 3)]
+
+
+class XoSourceGaussian(ctypes.Structure):
+    _fields_ = [('transformation', c_float * 9), ('position', f3),
+                ('direction', f3), ('sigma', c_float * 2), ('clip', c_float)]
+
+
+class XoDetector(ctypes.Structure):
+    _fields_ = [('kind', c_int32), ('direction', f3),
+                ('position', c_float * 2), ('r_min', c_float),
+                ('inv_dr', c_float), ('cos_min', c_float), ('n', c_uint32),
+                ('offset', c_uint32), ('log_scale', c_int32)]
+
+
+class XoFluence(ctypes.Structure):
+    _fields_ = [('enabled', c_int32), ('inv_step', f3), ('top_left', f3),
+                ('shape', c_uint32 * 3), ('offset', c_uint32), ('k', c_int32)]
+
+
+class XoMcmlCfg(ctypes.Structure):
+    _fields_ = [('num_layers', c_uint32), ('pf_kind', c_int32),
+                ('method', c_int32), ('rmax', c_float), ('w_min', c_float),
+                ('lottery_chance', c_float), ('use_lottery', c_int32),
+                ('reserved', c_uint32), ('source', XoSourceLine),
+                ('top', XoDetector), ('bottom', XoDetector),
+                ('specular', XoDetector),
+                ('layers', XoLayer * XO_MAX_LAYERS)]
+
+
+class XoMaterial(ctypes.Structure):
+    _fields_ = [('n', c_float), ('mus', c_float), ('mua', c_float),
+                ('inv_mut', c_float), ('mua_inv_mut', c_float),
+                ('reserved', c_float * 3), ('pf', XoPf)]
+
+
+class XoVoxelCfg(ctypes.Structure):
+    _fields_ = [('top_left', f3), ('bottom_right', f3), ('size', f3),
+                ('shape', c_int32 * 3)]
+
+
+class XoMcvoxCfg(ctypes.Structure):
+    _fields_ = [('num_materials', c_uint32), ('pf_kind', c_int32),
+                ('method', c_int32), ('rmax', c_float), ('w_min', c_float),
+                ('lottery_chance', c_float), ('use_lottery', c_int32),
+                ('source_kind', c_int32), ('voxels', XoVoxelCfg),
+                ('line', XoSourceLine), ('gaussian', XoSourceGaussian),
+                ('fluence', XoFluence), ('top', XoDetector),
+                ('bottom', XoDetector), ('specular', XoDetector),
+                ('materials', XoMaterial * XO_MAX_MATERIALS)]
+
+
+class XoRunReport(ctypes.Structure):
+    _fields_ = [('packets', c_uint64), ('steps', c_uint64),
+                ('boundary_steps', c_uint64), ('voxel_crossings', c_uint64),
+                ('deposits', c_uint64), ('truncated', c_uint64),
+                ('lanes', c_uint32),
+                ('threads_used', c_uint32), ('kernel_ms', c_double),
+                ('total_ms', c_double)]
+
+    def todict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+STRUCTS_IN_ABI_ORDER = [XoPf, XoLayer, XoSourceLine, XoSourceGaussian,
+                        XoDetector, XoFluence, XoMcmlCfg, XoMaterial,
+                        XoVoxelCfg, XoMcvoxCfg, XoRunReport]
+
+# Every entry point of include/xomc.h: name -> (restype, argtypes)
+PROTOTYPES = {
+    'xo_abi_version': (c_int, []),
+    'xo_last_error': (c_char_p, []),
+    'xo_device_count': (c_int, []),
+    'xo_struct_sizes': (c_int, [POINTER(c_uint32), c_int]),
+    'xo_safeprime_multipliers': (c_int, [c_uint32, POINTER(c_uint32)]),
+    'xo_safeprime_preload': (c_int, [POINTER(c_uint32), c_uint32]),
+    'xo_init_rng': (c_int, [c_uint32, c_uint64, POINTER(c_uint64),
+                            POINTER(c_uint32)]),
+    'xo_rng_host': (c_int, [POINTER(c_uint64), c_uint32, c_uint32,
+                            POINTER(c_float)]),
+    'xo_ctx_create': (c_void_p, [c_int, c_uint32, c_uint64]),
+    'xo_ctx_destroy': (None, [c_void_p]),
+    'xo_ctx_info': (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32),
+                            POINTER(c_uint32)]),
+    'xo_ctx_get_rng': (c_int, [c_void_p, c_uint32, POINTER(c_uint64),
+                               POINTER(c_uint32)]),
+    'xo_ctx_set_rng': (c_int, [c_void_p, c_uint32, POINTER(c_uint64),
+                               POINTER(c_uint32)]),
+    'xo_mcml_run': (c_int, [c_void_p, POINTER(XoMcmlCfg), POINTER(c_float),
+                            c_uint32, c_uint64, c_uint32, c_uint32,
+                            POINTER(c_uint64), c_uint64,
+                            POINTER(XoRunReport)]),
+    'xo_mcvox_run': (c_int, [c_void_p, POINTER(XoMcvoxCfg), POINTER(c_int32),
+                             c_uint64, POINTER(c_float), c_uint32, c_uint64,
+                             c_uint32, c_uint32, POINTER(c_uint64), c_uint64,
+                             POINTER(XoRunReport)]),
+}
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.isfile(LIB_PATH):
+        raise ImportError(
+            'libxomc.so is not built ({}); run '
+            '"python -m pyxopto_amd.build_native" first'.format(LIB_PATH))
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (restype, argtypes) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    sizes = (c_uint32 * 16)()
+    count = lib.xo_struct_sizes(sizes, 16)
+    for i, struct in enumerate(STRUCTS_IN_ABI_ORDER[:count]):
+        if ctypes.sizeof(struct) != sizes[i]:
+            raise RuntimeError(
+                'Size of {} differs between the host ({} B) and libxomc.so '
+                '({} B)!'.format(struct.__name__, ctypes.sizeof(struct),
+                                 sizes[i]))
+    return lib
+
+
+def lib():
+    """The loaded library (loaded on first use)."""
+    global _lib
+    if _lib is None:
+        with _lib_lock:
+            if _lib is None:
+                _lib = _load()
+                _preload_primes(_lib)
+    return _lib
+
+
+def reload_library():
+    global _lib
+    with _lib_lock:
+        _lib = None
+
+
+def last_error() -> str:
+    msg = lib().xo_last_error()
+    return msg.decode() if msg else ''
+
+
+def check(rc, what='libxomc'):
+    if rc != 0:
+        raise NativeError('{}: {}'.format(what, last_error()))
+
+
+def _ptr(arr, ctype):
+    return arr.ctypes.data_as(POINTER(ctype))
+
+
+def _read_prime_cache():
+    if not os.path.isfile(PRIME_CACHE):
+        return None
+    data = np.fromfile(PRIME_CACHE, dtype='<u4')
+    if data.size == 0 or data[0] != XO_FIRST_PRIME_A:
+        return None
+    return data
+
+
+def _preload_primes(l):
+    data = _read_prime_cache()
+    if data is not None:
+        data = np.ascontiguousarray(data, dtype=np.uint32)
+        l.xo_safeprime_preload(_ptr(data, c_uint32), c_uint32(data.size))
+
+
+def safeprime_table(n: int = XO_NUM_PRIMES, ensure_cache: bool = False):
+    """The first n multipliers of the safe-prime seed table (uint32)."""
+    l = lib()
+    out = np.empty(n, dtype=np.uint32)
+    check(l.xo_safeprime_multipliers(c_uint32(n), _ptr(out, c_uint32)),
+          'xo_safeprime_multipliers')
+    if ensure_cache:
+        cached = _read_prime_cache()
+        if cached is None or cached.size < n:
+            os.makedirs(DATA_DIR, exist_ok=True)
+            tmp = PRIME_CACHE + '.tmp'
+            out.astype('<u4').tofile(tmp)
+            os.replace(tmp, PRIME_CACHE)
+    return out
+
+
+def init_rng(n: int, xinit: int):
+    """init_RNG semantics: (x, a) seeds of n streams, or ValueError."""
+    x = np.zeros(n, dtype=np.uint64)
+    a = np.zeros(n, dtype=np.uint32)
+    rc = lib().xo_init_rng(c_uint32(n), c_uint64(int(xinit)),
+                           _ptr(x, c_uint64), _ptr(a, c_uint32))
+    if rc == 1:
+        raise ValueError('The provided initializer {} is not valid. Try a '
+                         'different 64-bit unsigned integer!'.format(xinit))
+    check(rc, 'xo_init_rng')
+    return x, a
+
+
+def rng_host(x: int, a: int, n: int):
+    """n floats of one MWC stream; returns (values, new x)."""
+    xs = c_uint64(int(x))
+    out = np.empty(n, dtype=np.float32)
+    check(lib().xo_rng_host(ctypes.byref(xs), c_uint32(int(a)), c_uint32(n),
+                            _ptr(out, c_float)), 'xo_rng_host')
+    return out, int(xs.value)
+
+
+def device_count() -> int:
+    return int(lib().xo_device_count())
+
+
+class Context:
+    """Owner of one xo_ctx: RNG streams and buffers on one device."""
+
+    def __init__(self, device: int = 0, max_lanes: int = 0,
+                 xinit: int = 0x5EED000000000001):
+        l = lib()
+        h = l.xo_ctx_create(c_int(device), c_uint32(max_lanes),
+                            c_uint64(int(xinit)))
+        if not h:
+            msg = last_error()
+            if 'not valid' in msg:
+                raise ValueError(msg)
+            raise NativeError('xo_ctx_create: ' + msg)
+        self._h = h
+        self.device = device
+        p, m, c = c_uint32(), c_uint32(), c_uint32()
+        check(l.xo_ctx_info(h, ctypes.byref(p), ctypes.byref(m),
+                            ctypes.byref(c)), 'xo_ctx_info')
+        self.persistent_lanes = p.value
+        self.max_lanes = m.value
+        self.compute_units = c.value
+
+    def close(self):
+        if getattr(self, '_h', None):
+            lib().xo_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_rng(self, n: int):
+        x = np.empty(n, dtype=np.uint64)
+        a = np.empty(n, dtype=np.uint32)
+        check(lib().xo_ctx_get_rng(self._h, c_uint32(n), _ptr(x, c_uint64),
+                                   _ptr(a, c_uint32)), 'xo_ctx_get_rng')
+        return x, a
+
+    def set_rng(self, x, a=None):
+        x = np.ascontiguousarray(x, dtype=np.uint64)
+        ap = None
+        if a is not None:
+            a = np.ascontiguousarray(a, dtype=np.uint32)
+            ap = _ptr(a, c_uint32)
+        check(lib().xo_ctx_set_rng(self._h, c_uint32(x.size),
+                                   _ptr(x, c_uint64), ap), 'xo_ctx_set_rng')
+
+    def run_mcml(self, cfg: XoMcmlCfg, lut, npackets: int, naccum: int,
+                 lanes: int = 0, flags: int = 0):
+        accum = np.zeros(naccum, dtype=np.uint64)
+        lut_p, nlut = None, 0
+        if lut is not None and len(lut):
+            lut = np.ascontiguousarray(lut, dtype=np.float32)
+            lut_p, nlut = _ptr(lut, c_float), lut.size
+        rep = XoRunReport()
+        check(lib().xo_mcml_run(self._h, ctypes.byref(cfg), lut_p,
+                                c_uint32(nlut), c_uint64(int(npackets)),
+                                c_uint32(lanes), c_uint32(flags),
+                                _ptr(accum, c_uint64), c_uint64(naccum),
+                                ctypes.byref(rep)), 'xo_mcml_run')
+        return accum, rep
+
+    def run_mcvox(self, cfg: XoMcvoxCfg, voxels, lut, npackets: int,
+                  naccum: int, lanes: int = 0, flags: int = 0):
+        accum = np.zeros(naccum, dtype=np.uint64)
+        vox_p, nvox = None, 0
+        if voxels is not None:
+            voxels = np.ascontiguousarray(voxels, dtype=np.int32)
+            vox_p, nvox = _ptr(voxels, c_int32), voxels.size
+        lut_p, nlut = None, 0
+        if lut is not None and len(lut):
+            lut = np.ascontiguousarray(lut, dtype=np.float32)
+            lut_p, nlut = _ptr(lut, c_float), lut.size
+        rep = XoRunReport()
+        check(lib().xo_mcvox_run(self._h, ctypes.byref(cfg), vox_p,
+                                 c_uint64(nvox), lut_p, c_uint32(nlut),
+                                 c_uint64(int(npackets)), c_uint32(lanes),
+                                 c_uint32(flags), _ptr(accum, c_uint64),
+                                 c_uint64(naccum), ctypes.byref(rep)),
+              'xo_mcvox_run')
+        return accum, rep

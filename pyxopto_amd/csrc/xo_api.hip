@@ -1,0 +1,514 @@
+/*
+ * xo_api.hip — the C ABI of libxomc.so: contexts (device-resident RNG
+ * streams and buffers, one HIP stream each) and the run entry points that
+ * replace Mc.run's upload / McKernel enqueue / download sequence
+ * (xopto/mcml/mc.py:730-1018, xopto/mcvox/mc.py:772-1068).
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/xomc.h"
+#include "xo_internal.h"
+#include "xo_kernels.h"
+
+static_assert(sizeof(xo_pf) == 32, "xo_pf layout");
+static_assert(sizeof(xo_layer) == 80, "xo_layer layout");
+static_assert(sizeof(xo_source_line) == 64, "xo_source_line layout");
+static_assert(sizeof(xo_source_gaussian) == 72, "xo_source_gaussian layout");
+static_assert(sizeof(xo_detector) == 48, "xo_detector layout");
+static_assert(sizeof(xo_fluence) == 48, "xo_fluence layout");
+static_assert(sizeof(xo_material) == 64, "xo_material layout");
+static_assert(sizeof(xo_voxel_cfg) == 48, "xo_voxel_cfg layout");
+static_assert(sizeof(xo::McvoxParams) <= 4096, "kernel argument too large");
+static_assert(sizeof(xo::McmlParams) <= 4096, "kernel argument too large");
+
+namespace xo {
+
+static thread_local std::string g_error;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+void clear_error() { g_error.clear(); }
+
+} // namespace xo
+
+#define XO_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            xo::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),      \
+                          __FILE__, __LINE__);                                        \
+            return -1;                                                                \
+        }                                                                             \
+    } while (0)
+
+struct xo_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t max_lanes = 0;
+    uint32_t cus = 0;
+    uint64_t *rng_x = nullptr;
+    uint32_t *rng_a = nullptr;
+    unsigned long long *counters = nullptr; /* [0] packet counter, [1..] stats */
+    unsigned long long *accum = nullptr;
+    size_t accum_cap = 0;
+    float *lut = nullptr;
+    size_t lut_cap = 0;
+    uint8_t *voxels = nullptr;
+    size_t voxels_cap = 0;
+    int32_t vox_shape[3] = {0, 0, 0};
+    bool vox_valid = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+};
+
+namespace {
+
+int ensure_buffer(void **ptr, size_t *cap, size_t bytes) {
+    if (bytes <= *cap && *ptr) return 0;
+    if (*ptr) XO_HIP(hipFree(*ptr));
+    *ptr = nullptr;
+    *cap = 0;
+    size_t alloc = bytes < 256 ? 256 : bytes;
+    XO_HIP(hipMalloc(ptr, alloc));
+    *cap = alloc;
+    return 0;
+}
+
+int check_detector(const char *name, const xo_detector &d, uint64_t naccum,
+                   uint64_t *end) {
+    if (d.kind == XO_DET_NONE) return 0;
+    if (d.kind != XO_DET_TOTAL && d.kind != XO_DET_RADIAL) {
+        xo::set_error("%s detector: unknown kind %d", name, d.kind);
+        return -1;
+    }
+    uint64_t n = d.kind == XO_DET_TOTAL ? 1u : d.n;
+    if (n == 0 || (uint64_t)d.offset + n > naccum) {
+        xo::set_error("%s detector: bins [%u, %llu) exceed the accumulator (%llu)", name,
+                      d.offset, (unsigned long long)((uint64_t)d.offset + n),
+                      (unsigned long long)naccum);
+        return -1;
+    }
+    if ((uint64_t)d.offset + n > *end) *end = (uint64_t)d.offset + n;
+    return 0;
+}
+
+int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint32_t idx) {
+    if (pf_kind == XO_PF_LUT) {
+        if (pf.size < 2 || (uint64_t)pf.offset + pf.size > nlut) {
+            xo::set_error("%s %u: lookup table [%u, %u+%u) outside the LUT array (%u)", what,
+                          idx, pf.offset, pf.offset, pf.size, nlut);
+            return -1;
+        }
+    } else if (pf_kind != XO_PF_HG && pf_kind != XO_PF_MHG) {
+        xo::set_error("unknown phase function kind %d", pf_kind);
+        return -1;
+    }
+    return 0;
+}
+
+uint32_t choose_lanes(xo_ctx *ctx, uint32_t requested, int blocks_per_cu) {
+    uint32_t lanes = requested;
+    if (lanes == 0) {
+        uint64_t persistent =
+            (uint64_t)(blocks_per_cu > 0 ? blocks_per_cu : 1) * ctx->cus * XO_BLOCK;
+        lanes = (uint32_t)(persistent < ctx->max_lanes ? persistent : ctx->max_lanes);
+    }
+    if (lanes > ctx->max_lanes) lanes = ctx->max_lanes;
+    return lanes;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+        .count();
+}
+
+} // namespace
+
+extern "C" {
+
+int xo_abi_version(void) { return XO_ABI_VERSION; }
+
+const char *xo_last_error(void) { return xo::g_error.c_str(); }
+
+int xo_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int xo_struct_sizes(uint32_t *out, int n) {
+    const uint32_t sizes[] = {
+        (uint32_t)sizeof(xo_pf),          (uint32_t)sizeof(xo_layer),
+        (uint32_t)sizeof(xo_source_line), (uint32_t)sizeof(xo_source_gaussian),
+        (uint32_t)sizeof(xo_detector),    (uint32_t)sizeof(xo_fluence),
+        (uint32_t)sizeof(xo_mcml_cfg),    (uint32_t)sizeof(xo_material),
+        (uint32_t)sizeof(xo_voxel_cfg),   (uint32_t)sizeof(xo_mcvox_cfg),
+        (uint32_t)sizeof(xo_run_report)};
+    int count = (int)(sizeof(sizes) / sizeof(sizes[0]));
+    if (n < count) count = n;
+    for (int i = 0; i < count; ++i) out[i] = sizes[i];
+    return count;
+}
+
+xo_ctx *xo_ctx_create(int device, uint32_t max_lanes, uint64_t xinit) {
+    xo::clear_error();
+    if (max_lanes == 0 || max_lanes > XO_MAX_SEEDS) max_lanes = XO_MAX_SEEDS;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        xo::set_error("no HIP device is available");
+        return nullptr;
+    }
+    if (device < 0 || device >= ndev) {
+        xo::set_error("device %d out of range (%d devices)", device, ndev);
+        return nullptr;
+    }
+    std::vector<uint64_t> hx(max_lanes);
+    std::vector<uint32_t> ha(max_lanes);
+    int rc = xo_init_rng(max_lanes, xinit, hx.data(), ha.data());
+    if (rc == 1) {
+        xo::set_error("The provided initializer %llu is not valid. Try a different 64-bit "
+                      "unsigned integer!",
+                      (unsigned long long)xinit);
+        return nullptr;
+    }
+    if (rc != 0) return nullptr;
+
+    xo_ctx *ctx = new xo_ctx();
+    ctx->device = device;
+    ctx->max_lanes = max_lanes;
+    auto fail = [&](const char *what, hipError_t e) -> xo_ctx * {
+        xo::set_error("%s failed: %s", what, hipGetErrorString(e));
+        xo_ctx_destroy(ctx);
+        return nullptr;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return fail("hipSetDevice", e);
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess)
+        return fail("hipGetDeviceProperties", e);
+    ctx->cus = (uint32_t)prop.multiProcessorCount;
+    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail("hipStreamCreate", e);
+    if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess) return fail("hipEventCreate", e);
+    if ((e = hipEventCreate(&ctx->ev1)) != hipSuccess) return fail("hipEventCreate", e);
+    if ((e = hipMalloc(&ctx->rng_x, sizeof(uint64_t) * max_lanes)) != hipSuccess)
+        return fail("hipMalloc(rng_x)", e);
+    if ((e = hipMalloc(&ctx->rng_a, sizeof(uint32_t) * max_lanes)) != hipSuccess)
+        return fail("hipMalloc(rng_a)", e);
+    if ((e = hipMalloc(&ctx->counters, sizeof(unsigned long long) * (1 + XO_STAT_COUNT))) !=
+        hipSuccess)
+        return fail("hipMalloc(counters)", e);
+    if ((e = hipMemcpy(ctx->rng_x, hx.data(), sizeof(uint64_t) * max_lanes,
+                       hipMemcpyHostToDevice)) != hipSuccess)
+        return fail("hipMemcpy(rng_x)", e);
+    if ((e = hipMemcpy(ctx->rng_a, ha.data(), sizeof(uint32_t) * max_lanes,
+                       hipMemcpyHostToDevice)) != hipSuccess)
+        return fail("hipMemcpy(rng_a)", e);
+    return ctx;
+}
+
+void xo_ctx_destroy(xo_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->rng_x) (void)hipFree(ctx->rng_x);
+    if (ctx->rng_a) (void)hipFree(ctx->rng_a);
+    if (ctx->counters) (void)hipFree(ctx->counters);
+    if (ctx->accum) (void)hipFree(ctx->accum);
+    if (ctx->lut) (void)hipFree(ctx->lut);
+    if (ctx->voxels) (void)hipFree(ctx->voxels);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int xo_ctx_info(xo_ctx *ctx, uint32_t *persistent_lanes, uint32_t *max_lanes,
+                uint32_t *compute_units) {
+    if (!ctx) {
+        xo::set_error("null context");
+        return -1;
+    }
+    int bpc = 1;
+    XO_HIP(hipSetDevice(ctx->device));
+    XO_HIP(xo::occupancy_mcml(XO_PF_HG, XO_MAX_LAYERS * 64 + 8 * 102, &bpc));
+    if (persistent_lanes) *persistent_lanes = choose_lanes(ctx, 0, bpc);
+    if (max_lanes) *max_lanes = ctx->max_lanes;
+    if (compute_units) *compute_units = ctx->cus;
+    return 0;
+}
+
+int xo_ctx_get_rng(xo_ctx *ctx, uint32_t n, uint64_t *x_out, uint32_t *a_out) {
+    if (!ctx || n > ctx->max_lanes) {
+        xo::set_error("xo_ctx_get_rng: bad context or n");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    if (x_out)
+        XO_HIP(hipMemcpyAsync(x_out, ctx->rng_x, sizeof(uint64_t) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    if (a_out)
+        XO_HIP(hipMemcpyAsync(a_out, ctx->rng_a, sizeof(uint32_t) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    XO_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int xo_ctx_set_rng(xo_ctx *ctx, uint32_t n, const uint64_t *x, const uint32_t *a) {
+    if (!ctx || n > ctx->max_lanes) {
+        xo::set_error("xo_ctx_set_rng: bad context or n");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    if (x)
+        XO_HIP(hipMemcpyAsync(ctx->rng_x, x, sizeof(uint64_t) * n, hipMemcpyHostToDevice,
+                              ctx->stream));
+    if (a)
+        XO_HIP(hipMemcpyAsync(ctx->rng_a, a, sizeof(uint32_t) * n, hipMemcpyHostToDevice,
+                              ctx->stream));
+    XO_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+static void fill_report(xo_run_report *r, const unsigned long long *stats, uint32_t lanes,
+                        float kernel_ms, double total_ms) {
+    if (!r) return;
+    r->packets = stats[XO_STAT_PACKETS];
+    r->steps = stats[XO_STAT_STEPS];
+    r->boundary_steps = stats[XO_STAT_BOUNDARY];
+    r->voxel_crossings = stats[XO_STAT_CROSSINGS];
+    r->deposits = stats[XO_STAT_DEPOSITS];
+    r->truncated = stats[XO_STAT_TRUNCATED];
+    r->lanes = lanes;
+    r->threads_used = (uint32_t)stats[XO_STAT_THREADS];
+    r->kernel_ms = kernel_ms;
+    r->total_ms = total_ms;
+}
+
+int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t nlut,
+                uint64_t npackets, uint32_t lanes, uint32_t flags, uint64_t *accum,
+                uint64_t naccum, xo_run_report *report) {
+    auto t0 = std::chrono::steady_clock::now();
+    xo::clear_error();
+    if (!ctx || !cfg || !accum) {
+        xo::set_error("xo_mcml_run: null argument");
+        return -1;
+    }
+    if (cfg->num_layers < 3 || cfg->num_layers > XO_MAX_LAYERS) {
+        xo::set_error("xo_mcml_run: %u layers (3..%d supported)", cfg->num_layers,
+                      XO_MAX_LAYERS);
+        return -1;
+    }
+    if (cfg->method != XO_METHOD_AW) {
+        xo::set_error("xo_mcml_run: only the albedo-weight method is implemented");
+        return -1;
+    }
+    if (naccum == 0) {
+        xo::set_error("xo_mcml_run: empty accumulator");
+        return -1;
+    }
+    for (uint32_t i = 0; i < cfg->num_layers; ++i)
+        if (check_pf(cfg->pf_kind, cfg->layers[i].pf, nlut, "layer", i)) return -1;
+    if (cfg->pf_kind == XO_PF_LUT && !lut) {
+        xo::set_error("xo_mcml_run: a lookup-table phase function needs the LUT array");
+        return -1;
+    }
+    uint64_t det_end = 0;
+    if (check_detector("top", cfg->top, naccum, &det_end) ||
+        check_detector("bottom", cfg->bottom, naccum, &det_end) ||
+        check_detector("specular", cfg->specular, naccum, &det_end))
+        return -1;
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    if (ensure_buffer((void **)&ctx->accum, &ctx->accum_cap, naccum * 8)) return -1;
+    if (nlut && ensure_buffer((void **)&ctx->lut, &ctx->lut_cap, (size_t)nlut * 4)) return -1;
+
+    xo::McmlParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.cfg = *cfg;
+    P.lut = nlut ? ctx->lut : nullptr;
+    P.rng_x = ctx->rng_x;
+    P.rng_a = ctx->rng_a;
+    P.counter = ctx->counters;
+    P.stats = ctx->counters + 1;
+    P.accum = ctx->accum;
+    P.npackets = npackets;
+    P.flags = flags;
+    P.lds_bins = (!(flags & XO_FLAG_NO_LDS_BINS) && naccum <= XO_LDS_BINS_MAX)
+                     ? (uint32_t)naccum
+                     : 0u;
+    P.lds_lut = (nlut && nlut <= XO_LDS_LUT_MAX) ? nlut : 0u;
+    int bpc = 1;
+    XO_HIP(xo::occupancy_mcml(cfg->pf_kind, xo::mcml_lds_bytes(P), &bpc));
+    P.lanes = choose_lanes(ctx, lanes, bpc);
+    uint32_t blocks = (P.lanes + XO_BLOCK - 1) / XO_BLOCK;
+
+    if (nlut)
+        XO_HIP(hipMemcpyAsync(ctx->lut, lut, (size_t)nlut * 4, hipMemcpyHostToDevice,
+                              ctx->stream));
+    XO_HIP(hipMemsetAsync(ctx->accum, 0, naccum * 8, ctx->stream));
+    XO_HIP(hipMemsetAsync(ctx->counters, 0, sizeof(unsigned long long) * (1 + XO_STAT_COUNT),
+                          ctx->stream));
+    XO_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (npackets > 0) XO_HIP(xo::launch_mcml(P, blocks, ctx->stream));
+    XO_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    unsigned long long stats[1 + XO_STAT_COUNT];
+    XO_HIP(hipMemcpyAsync(accum, ctx->accum, naccum * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XO_HIP(hipMemcpyAsync(stats, ctx->counters, sizeof(stats), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    XO_HIP(hipStreamSynchronize(ctx->stream));
+    float kms = 0.f;
+    XO_HIP(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+    fill_report(report, stats + 1, P.lanes, kms, ms_since(t0));
+    return 0;
+}
+
+int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
+                 uint64_t nvoxels, const float *lut, uint32_t nlut, uint64_t npackets,
+                 uint32_t lanes, uint32_t flags, uint64_t *accum, uint64_t naccum,
+                 xo_run_report *report) {
+    auto t0 = std::chrono::steady_clock::now();
+    xo::clear_error();
+    if (!ctx || !cfg || !accum) {
+        xo::set_error("xo_mcvox_run: null argument");
+        return -1;
+    }
+    if (cfg->num_materials < 1 || cfg->num_materials > XO_MAX_MATERIALS) {
+        xo::set_error("xo_mcvox_run: %u materials (1..%d supported)", cfg->num_materials,
+                      XO_MAX_MATERIALS);
+        return -1;
+    }
+    if (cfg->method != XO_METHOD_AW) {
+        xo::set_error("xo_mcvox_run: only the albedo-weight method is implemented");
+        return -1;
+    }
+    if (cfg->source_kind != XO_SRC_LINE && cfg->source_kind != XO_SRC_GAUSSIAN) {
+        xo::set_error("xo_mcvox_run: unknown source kind %d", cfg->source_kind);
+        return -1;
+    }
+    const int32_t *shape = cfg->voxels.shape;
+    if (shape[0] <= 0 || shape[1] <= 0 || shape[2] <= 0) {
+        xo::set_error("xo_mcvox_run: invalid voxel grid shape");
+        return -1;
+    }
+    const uint64_t nvox = (uint64_t)shape[0] * shape[1] * shape[2];
+    if (naccum == 0) {
+        xo::set_error("xo_mcvox_run: empty accumulator");
+        return -1;
+    }
+    for (uint32_t i = 0; i < cfg->num_materials; ++i)
+        if (check_pf(cfg->pf_kind, cfg->materials[i].pf, nlut, "material", i)) return -1;
+    if (cfg->pf_kind == XO_PF_LUT && !lut) {
+        xo::set_error("xo_mcvox_run: a lookup-table phase function needs the LUT array");
+        return -1;
+    }
+    uint64_t det_end = 0;
+    if (check_detector("top", cfg->top, naccum, &det_end) ||
+        check_detector("bottom", cfg->bottom, naccum, &det_end) ||
+        check_detector("specular", cfg->specular, naccum, &det_end))
+        return -1;
+    if (cfg->fluence.enabled) {
+        const uint32_t *fs = cfg->fluence.shape;
+        uint64_t nf = (uint64_t)fs[0] * fs[1] * fs[2];
+        if (nf == 0 || cfg->fluence.offset < det_end ||
+            (uint64_t)cfg->fluence.offset + nf > naccum) {
+            xo::set_error("xo_mcvox_run: fluence bins [%u, +%llu) must follow the detector "
+                          "bins (%llu) inside the accumulator (%llu)",
+                          cfg->fluence.offset, (unsigned long long)nf,
+                          (unsigned long long)det_end, (unsigned long long)naccum);
+            return -1;
+        }
+    }
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    if (voxels) {
+        if (nvoxels != nvox) {
+            xo::set_error("xo_mcvox_run: %llu voxels given, grid has %llu",
+                          (unsigned long long)nvoxels, (unsigned long long)nvox);
+            return -1;
+        }
+        std::vector<uint8_t> packed(nvox);
+        for (uint64_t i = 0; i < nvox; ++i) {
+            int32_t m = voxels[i];
+            if (m < 0 || (uint32_t)m >= cfg->num_materials) {
+                xo::set_error("xo_mcvox_run: voxel %llu has material index %d (valid 0..%u)",
+                              (unsigned long long)i, m, cfg->num_materials - 1);
+                return -1;
+            }
+            packed[i] = (uint8_t)m;
+        }
+        ctx->vox_valid = false;
+        if (ensure_buffer((void **)&ctx->voxels, &ctx->voxels_cap, nvox)) return -1;
+        XO_HIP(hipMemcpyAsync(ctx->voxels, packed.data(), nvox, hipMemcpyHostToDevice,
+                              ctx->stream));
+        XO_HIP(hipStreamSynchronize(ctx->stream));
+        std::memcpy(ctx->vox_shape, shape, sizeof(ctx->vox_shape));
+        ctx->vox_valid = true;
+    } else if (!ctx->vox_valid || std::memcmp(ctx->vox_shape, shape, sizeof(ctx->vox_shape))) {
+        xo::set_error("xo_mcvox_run: no voxel grid of this shape has been uploaded");
+        return -1;
+    }
+    if (ensure_buffer((void **)&ctx->accum, &ctx->accum_cap, naccum * 8)) return -1;
+    if (nlut && ensure_buffer((void **)&ctx->lut, &ctx->lut_cap, (size_t)nlut * 4)) return -1;
+
+    xo::McvoxParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.cfg = *cfg;
+    P.voxels = ctx->voxels;
+    P.lut = nlut ? ctx->lut : nullptr;
+    P.rng_x = ctx->rng_x;
+    P.rng_a = ctx->rng_a;
+    P.counter = ctx->counters;
+    P.stats = ctx->counters + 1;
+    P.accum = ctx->accum;
+    P.npackets = npackets;
+    P.flags = flags;
+    P.lds_bins = (!(flags & XO_FLAG_NO_LDS_BINS) && det_end <= XO_LDS_BINS_MAX)
+                     ? (uint32_t)det_end
+                     : 0u;
+    P.lds_lut = (nlut && nlut <= XO_LDS_LUT_MAX) ? nlut : 0u;
+    int bpc = 1;
+    XO_HIP(xo::occupancy_mcvox(cfg->pf_kind, xo::mcvox_lds_bytes(P), &bpc));
+    P.lanes = choose_lanes(ctx, lanes, bpc);
+    uint32_t blocks = (P.lanes + XO_BLOCK - 1) / XO_BLOCK;
+
+    if (nlut)
+        XO_HIP(hipMemcpyAsync(ctx->lut, lut, (size_t)nlut * 4, hipMemcpyHostToDevice,
+                              ctx->stream));
+    XO_HIP(hipMemsetAsync(ctx->accum, 0, naccum * 8, ctx->stream));
+    XO_HIP(hipMemsetAsync(ctx->counters, 0, sizeof(unsigned long long) * (1 + XO_STAT_COUNT),
+                          ctx->stream));
+    XO_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (npackets > 0) XO_HIP(xo::launch_mcvox(P, blocks, ctx->stream));
+    XO_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    unsigned long long stats[1 + XO_STAT_COUNT];
+    XO_HIP(hipMemcpyAsync(accum, ctx->accum, naccum * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XO_HIP(hipMemcpyAsync(stats, ctx->counters, sizeof(stats), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    XO_HIP(hipStreamSynchronize(ctx->stream));
+    float kms = 0.f;
+    XO_HIP(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+    fill_report(report, stats + 1, P.lanes, kms, ms_since(t0));
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,231 @@
+/*
+ * xo_device.h — device building blocks shared by the transport kernels:
+ * the MWC generator, detector and fluence deposits, phase-function sampling,
+ * direction update and Fresnel physics. Each function states the reference
+ * lines whose arithmetic it performs; the operation order (and every fused
+ * multiply-add) is part of the contract, because the kernels are built with
+ * -ffp-contract=off and must round like the host oracle.
+ */
+#ifndef XO_DEVICE_H
+#define XO_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/xomc.h"
+#include "xo_math.h"
+
+namespace xo {
+
+#define XO_DEV __device__ __forceinline__
+
+constexpr float kCos0 = 1.0f;   /* FP_COS_0,  mcbase.template.h:512 */
+constexpr float kCos90 = 0.0f;  /* FP_COS_90, mcbase.template.h:510 */
+constexpr float kRmin = 1e-12f; /* FP_RMIN,   mcbase.template.h:476 */
+constexpr float kInv2p32 = 2.3283064365386963e-10f; /* 2^-32 */
+
+/* Layer as staged in LDS: the per-step fields in the first 16 bytes. */
+struct alignas(16) LayerL {
+    float top, bottom, inv_mut, mua_inv_mut;
+    float n, cc_top, cc_bottom, g;
+    float beta, la, lb, lc;
+    uint32_t loff, lsize_m1, pad0, pad1;
+};
+
+/* Material as staged in LDS. */
+struct alignas(16) MaterialL {
+    float inv_mut, mua_inv_mut, n, g;
+    float beta, la, lb, lc;
+    uint32_t loff, lsize_m1, pad0, pad1;
+};
+
+/* fp_random_single (mcbase.template.c:1576-1594): one MWC step. The state
+ * update is one v_mad_u64_u32; (float)(uint32)x / (float)0xFFFFFFFF divides
+ * by 2^32 exactly, which is the multiply below. */
+XO_DEV float rng_next(uint64_t &x, uint32_t a) {
+    x = (uint64_t)(uint32_t)x * (uint64_t)a + (x >> 32);
+    return (float)(uint32_t)x * kInv2p32;
+}
+
+/* convert_int with saturation (mc_int). */
+XO_DEV int32_t f2i_sat(float v) {
+    if (!(v == v)) return 0;
+    if (v >= 2147483520.0f) return INT32_MAX;
+    if (v <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)v;
+}
+
+XO_DEV uint32_t f2u_sat(float v) {
+    if (!(v > 0.0f)) return 0u;
+    if (v >= 4294967040.0f) return UINT32_MAX;
+    return (uint32_t)v;
+}
+
+/* Where detector bins live for this launch: LDS copies (flushed at the end
+ * of the block) when every detector bin fits, else global memory. */
+struct Bins {
+    unsigned long long *lds;  /* nullptr if not privatized */
+    unsigned long long *glob;
+};
+
+XO_DEV void bins_add(const Bins &b, uint32_t index, uint32_t w) {
+    if (b.lds)
+        atomicAdd(&b.lds[index], (unsigned long long)w);
+    else
+        atomicAdd(&b.glob[index], (unsigned long long)w);
+}
+
+/* Radial (radial.py:117-150) / Total (total.py:86-108) detector deposit.
+ * accu_64_deposit_32 (mcbase.template.c:58-61) becomes one u64 atomic add:
+ * the modular sum is identical. */
+XO_DEV void detector_deposit(const xo_detector &d, const Bins &bins, float px,
+                             float py, float dx, float dy, float dz, float w) {
+    if (d.kind == XO_DET_NONE) return;
+    uint32_t index = 0;
+    if (d.kind == XO_DET_RADIAL) {
+        float ddx = px - d.position[0];
+        float ddy = py - d.position[1];
+        float r = __builtin_sqrtf(ddx * ddx + ddy * ddy);
+        if (d.log_scale) r = xo_logf(fmaxf(r, kRmin));
+        int32_t ri = f2i_sat((r - d.r_min) * d.inv_dr);
+        ri = ri < 0 ? 0 : (ri > (int32_t)d.n - 1 ? (int32_t)d.n - 1 : ri);
+        index = (uint32_t)ri;
+    }
+    float cosv = fabsf(dx * d.direction[0] + dy * d.direction[1] +
+                       dz * d.direction[2]);
+    float wi = (w * (float)XO_ACCU_K + 0.5f) * (float)(d.cos_min <= cosv);
+    uint32_t ui = f2u_sat(wi);
+    if (ui > 0u) bins_add(bins, d.offset + index, ui);
+}
+
+/* Phase-function parameters of one layer/material, read from LDS. */
+struct PfL {
+    float g, beta, la, lb, lc;
+    uint32_t loff, lsize_m1;
+};
+
+/* mcsim_pf_sample_angles: returns cos(theta), the azimuth as a fraction of
+ * a turn in xi_fi. HG hg.py:80-96; MHg mhg.py:87-109; Lut lut.py:125-150
+ * (index clamped to the table). */
+template <int PF>
+XO_DEV float pf_sample(const PfL &pf, const float *lut, uint64_t &x, uint32_t a,
+                       float &xi_fi) {
+    xi_fi = rng_next(x, a);
+    if constexpr (PF == XO_PF_HG) {
+        float g = pf.g;
+        float k = (1.0f - g * g) / (1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
+        float ct = (1.0f + g * g - k * k) / (2.0f * g);
+        if (g == 0.0f) ct = 1.0f - 2.0f * rng_next(x, a);
+        return fmaxf(fminf(ct, 1.0f), -1.0f);
+    } else if constexpr (PF == XO_PF_MHG) {
+        float g = pf.g;
+        float ct;
+        if (rng_next(x, a) <= pf.beta) {
+            float k = (1.0f - g * g) /
+                      (1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
+            ct = (1.0f + g * g - k * k) / (2.0f * g);
+            if (g == 0.0f) ct = 1.0f - 2.0f * rng_next(x, a);
+        } else {
+            ct = xo_cbrtf(2.0f * rng_next(x, a) - 1.0f);
+        }
+        return fminf(fmaxf(ct, -1.0f), 1.0f);
+    } else {
+        float fsize = (float)pf.lsize_m1;
+        float fi = (pf.la / (rng_next(x, a) - pf.lc) - pf.lb + 1.0f) * fsize *
+                   0.5f;
+        fi = fminf(fmaxf(fi, 0.0f), fsize);
+        float fl = floorf(fi);
+        float d = fi - fl;
+        uint32_t i0 = (uint32_t)f2i_sat(fl);
+        uint32_t i1 = i0 + 1u < pf.lsize_m1 ? i0 + 1u : pf.lsize_m1;
+        return lut[pf.loff + i0] * (1.0f - d) + lut[pf.loff + i1] * d;
+    }
+}
+
+/* scatter_direction (mcbase.template.c:1522-1556) with the single-precision
+ * renormalization (mc_normalize_fpv3, :975-981). */
+XO_DEV void scatter_direction(float &dx, float &dy, float &dz, float ct,
+                              float xi_fi) {
+    float sin_fi, cos_fi;
+    float st = __builtin_sqrtf(1.0f - ct * ct);
+    xo_sincos_turns(xi_fi, &sin_fi, &cos_fi);
+    float stcf = st * cos_fi;
+    float stsf = st * sin_fi;
+    float px = dx, py = dy, pz = dz;
+    if (fabsf(pz) >= kCos0) {
+        dx = stcf;
+        dy = stsf;
+        dz = copysignf(ct, pz * ct);
+    } else {
+        float k = __builtin_sqrtf(1.0f - pz * pz);
+        dx = (stcf * px * pz - stsf * py) / k + px * ct;
+        dy = (stcf * py * pz + stsf * px) / k + py * ct;
+        dz = (-stcf) * k + pz * ct;
+    }
+    float inv = 1.0f / __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    dx = dx * inv;
+    dy = dy * inv;
+    dz = dz * inv;
+}
+
+/* cos_critical (mcbase.template.c:1229-1231). */
+XO_DEV float cos_critical(float n1, float n2) {
+    return (n1 > n2) ? __builtin_sqrtf(1.0f - (n2 * n2) / (n1 * n1)) : 0.0f;
+}
+
+/* reflectance (mcbase.template.c:1244-1290). */
+XO_DEV float reflectance(float n1, float n2, float cos1, float cc) {
+    float R = 1.0f;
+    cos1 = fabsf(cos1);
+    if (n1 == n2) return 0.0f;
+    if (cos1 > cc) {
+        float n1_d_n2 = n1 / n2;
+        float sin1 = __builtin_sqrtf(1.0f - cos1 * cos1);
+        if (cos1 >= kCos0) sin1 = 0.0f;
+        float sin2 = fminf(1.0f, n1_d_n2 * sin1);
+        float cos2 = __builtin_sqrtf(1.0f - sin2 * sin2);
+        float n_cos1 = n1_d_n2 * cos1;
+        float n_cos2 = n1_d_n2 * cos2;
+        float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
+        Rs *= Rs;
+        float Rp = (n_cos2 - cos1) / (n_cos2 + cos1);
+        Rp *= Rp;
+        R = 0.5f * (Rp + Rs);
+        if (cos1 <= kCos90 || sin2 == 1.0f) return 1.0f;
+    }
+    return R;
+}
+
+XO_DEV float fsign(float v) { return (v >= 0.0f) ? 1.0f : -1.0f; }
+
+/* Roulette (mcml.template.c:738-753); returns true if the packet ends. */
+XO_DEV bool roulette(float &w, uint64_t &x, uint32_t a, float w_min,
+                     float chance, int use_lottery) {
+    if (w < w_min) {
+        if (use_lottery) {
+            if (rng_next(x, a) > chance) return true;
+            w = w / chance;
+        } else {
+            return true;
+        }
+    }
+    return false;
+}
+
+/* Wave-level helpers (64-lane wavefronts). */
+XO_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+XO_DEV uint32_t prefix_count(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+XO_DEV uint64_t readlane_u64(uint64_t v, uint32_t lane) {
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)lane);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+} // namespace xo
+
+#endif /* XO_DEVICE_H */

@@ -1,0 +1,709 @@
+/*
+ * xo_kernels.hip — persistent photon-packet transport kernels for gfx950.
+ *
+ * One lane = one packet in flight, its state (position, direction, weight,
+ * layer/voxel, MWC state) in VGPRs. Lanes take a new packet when theirs ends:
+ * with the dynamic schedule, finished lanes of a wave claim consecutive
+ * packet indices with one atomic per wave (ballot + mbcnt prefix) instead of
+ * one atomic per packet (the reference's pkt_cnt_atomic_inc,
+ * mcml.template.c:460, 790-812); with the static schedule lane g runs packets
+ * g, g+L, ... . The layer/material table is staged once per block in LDS;
+ * detector bins are privatized in LDS when they fit and flushed with one u64
+ * atomic per bin and block; fluence deposits are u64 atomics to HBM.
+ * Everything is binary32 VALU work; nothing here is a contraction, so no MFMA.
+ */
+#include <hip/hip_runtime.h>
+
+#include "xo_device.h"
+#include "xo_kernels.h"
+
+namespace xo {
+
+/* ------------------------------------------------------------------ */
+/* mcml                                                               */
+/* ------------------------------------------------------------------ */
+
+/* mcsim_boundary, mcml.template.c:80-203 (no surface layouts). */
+XO_DEV void mcml_boundary(const LayerL *sl, int32_t &layer, int32_t next,
+                          float &dx, float &dy, float &dz, uint64_t &x,
+                          uint32_t a) {
+    const LayerL &cur = sl[layer];
+    const LayerL &nxt = sl[layer + (dz >= 0.0f ? 1 : -1)];
+    float cos_crit = (dz < 0.0f) ? cur.cc_top : cur.cc_bottom;
+    float n1 = cur.n;
+    float n2 = nxt.n;
+    if (n1 == n2) {
+        layer = next;
+        return;
+    }
+    dz = -dz;
+    float cos1 = fabsf(dz);
+    if (cos1 > cos_crit) {
+        if (cos1 > kCos0) {
+            float R = (n1 - n2) / (n1 + n2);
+            if (R * R < rng_next(x, a)) {
+                layer = next;
+                dz = -dz;
+            }
+            return;
+        }
+        float n1_d_n2 = n1 / n2;
+        float sin1 = __builtin_sqrtf(1.0f - cos1 * cos1);
+        if (cos1 >= kCos0) sin1 = 0.0f;
+        float sin2 = fminf(1.0f, n1_d_n2 * sin1);
+        float cos2 = __builtin_sqrtf(1.0f - sin2 * sin2);
+        float n_cos1 = n1_d_n2 * cos1;
+        float n_cos2 = n1_d_n2 * cos2;
+        float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
+        Rs *= Rs;
+        float Rp = (n_cos2 - cos1) / (n_cos2 + cos1);
+        Rp *= Rp;
+        float R = 0.5f * (Rs + Rp);
+        if (cos1 <= kCos90 || sin2 == 1.0f) R = 1.0f;
+        if (R < rng_next(x, a)) {
+            layer = next;
+            dx *= n1_d_n2;
+            dy *= n1_d_n2;
+            dz = -copysignf(cos2, dz);
+        }
+    }
+}
+
+/* Block-level prologue/epilogue shared by the kernels. */
+struct BlockStats {
+    unsigned long long v[XO_STAT_COUNT];
+};
+
+XO_DEV void stats_flush(BlockStats *ss, const uint32_t *mine,
+                        unsigned long long *gstats) {
+    for (int i = 0; i < XO_STAT_COUNT; ++i)
+        if (mine[i]) atomicAdd(&ss->v[i], (unsigned long long)mine[i]);
+    __syncthreads();
+    if (threadIdx.x < XO_STAT_COUNT && ss->v[threadIdx.x])
+        atomicAdd(&gstats[threadIdx.x], ss->v[threadIdx.x]);
+}
+
+template <int PF>
+__global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ BlockStats sstats;
+    LayerL *sl = reinterpret_cast<LayerL *>(smem);
+    unsigned long long *sbins =
+        reinterpret_cast<unsigned long long *>(smem + XO_MAX_LAYERS * sizeof(LayerL));
+    float *slut = reinterpret_cast<float *>(sbins + P.lds_bins);
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nl = P.cfg.num_layers;
+    if (tid < nl) {
+        const xo_layer &L = P.cfg.layers[tid];
+        LayerL o;
+        o.top = L.top;
+        o.bottom = L.bottom;
+        o.inv_mut = L.inv_mut;
+        o.mua_inv_mut = L.mua_inv_mut;
+        o.n = L.n;
+        o.cc_top = L.cos_critical_top;
+        o.cc_bottom = L.cos_critical_bottom;
+        o.g = L.pf.g;
+        o.beta = L.pf.beta;
+        o.la = L.pf.a;
+        o.lb = L.pf.b;
+        o.lc = L.pf.c;
+        o.loff = L.pf.offset;
+        o.lsize_m1 = L.pf.size > 0u ? L.pf.size - 1u : 0u;
+        o.pad0 = o.pad1 = 0u;
+        sl[tid] = o;
+    }
+    for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK) sbins[i] = 0ull;
+    for (uint32_t i = tid; i < P.lds_lut; i += XO_BLOCK) slut[i] = P.lut[i];
+    if (tid < XO_STAT_COUNT) sstats.v[tid] = 0ull;
+    __syncthreads();
+
+    const Bins bins{P.lds_bins ? sbins : nullptr, P.accum};
+    const float *lut = P.lds_lut ? slut : P.lut;
+
+    const uint32_t gid = blockIdx.x * XO_BLOCK + tid;
+    const bool lane_ok = gid < P.lanes;
+    const bool static_sched = (P.flags & XO_FLAG_STATIC_SCHEDULE) != 0u;
+    uint64_t x = 0;
+    uint32_t a = 0;
+    uint64_t my_n = 0, my_k = 0;
+    if (lane_ok) {
+        x = P.rng_x[gid];
+        a = P.rng_a[gid];
+        if (static_sched)
+            my_n = P.npackets / P.lanes + ((uint64_t)gid < P.npackets % P.lanes ? 1u : 0u);
+    }
+
+    const float rmax2 = P.cfg.rmax * P.cfg.rmax;
+    const float spx = P.cfg.source.position[0];
+    const float spy = P.cfg.source.position[1];
+    const float spz = P.cfg.source.position[2];
+    const float w_min = P.cfg.w_min, chance = P.cfg.lottery_chance;
+    const int use_lottery = P.cfg.use_lottery;
+
+    bool active = lane_ok;
+    bool alive = false;
+    float px = 0.f, py = 0.f, pz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, w = 0.f;
+    int32_t layer = 1;
+    uint32_t st[XO_STAT_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t pkt_steps = 0;
+
+    for (;;) {
+        /* ---- refill lanes whose packet ended ---- */
+        bool want = active && !alive;
+        bool launch = false;
+        if (static_sched) {
+            if (want) {
+                if (my_k < my_n) {
+                    ++my_k;
+                    launch = true;
+                } else {
+                    active = false;
+                }
+            }
+        } else {
+            uint64_t m = __ballot(want);
+            if (m) {
+                uint32_t leader = (uint32_t)__builtin_ctzll(m);
+                uint64_t base = 0;
+                if (lane_id() == leader)
+                    base = atomicAdd(P.counter, (unsigned long long)__builtin_popcountll(m));
+                base = readlane_u64(base, leader);
+                if (want) {
+                    uint64_t idx = base + prefix_count(m);
+                    if (idx < P.npackets)
+                        launch = true;
+                    else
+                        active = false;
+                }
+            }
+        }
+        if (launch) {
+            /* Line source, xopto/mcml/mcsource/line.py:95-110 */
+            const xo_source_line &src = P.cfg.source;
+            w = 1.0f - src.reflectance;
+            px = src.position[0];
+            py = src.position[1];
+            pz = src.position[2];
+            dx = src.direction_sample[0];
+            dy = src.direction_sample[1];
+            dz = src.direction_sample[2];
+            detector_deposit(P.cfg.specular, bins, px, py, src.direction_reflected[0],
+                             src.direction_reflected[1], src.direction_reflected[2],
+                             src.reflectance);
+            layer = 1;
+            alive = true;
+            pkt_steps = 0;
+            st[XO_STAT_PACKETS]++;
+        }
+        if (!__any(alive)) break;
+        if (!alive) continue;
+        /* ---- one iteration of while(!done), mcml.template.c:514-788 ---- */
+        const LayerL &L = sl[layer];
+        st[XO_STAT_STEPS]++;
+        float step = -xo_logf(rng_next(x, a)) * L.inv_mut;
+        step = fminf(step, XO_FLT_MAX);
+        int32_t next = layer;
+        if (__builtin_fmaf(step, dz, pz) < L.top) {
+            --next;
+            if (fabsf(dz) != 0.0f) step = (L.top - pz) / dz;
+        }
+        if (__builtin_fmaf(step, dz, pz) >= L.bottom) {
+            ++next;
+            if (fabsf(dz) != 0.0f) step = (L.bottom - pz) / dz;
+        }
+        px = __builtin_fmaf(dx, step, px);
+        py = __builtin_fmaf(dy, step, py);
+        pz = __builtin_fmaf(dz, step, pz);
+        if (layer < next) pz = L.bottom;
+        if (layer > next) pz = L.top;
+
+        bool done = false;
+        if (next != layer) {
+            st[XO_STAT_BOUNDARY]++;
+            mcml_boundary(sl, layer, next, dx, dy, dz, x, a);
+            if (!(layer > 0 && layer < (int32_t)nl - 1)) {
+                if (layer <= 0)
+                    detector_deposit(P.cfg.top, bins, px, py, dx, dy, dz, w);
+                else
+                    detector_deposit(P.cfg.bottom, bins, px, py, dx, dy, dz, w);
+                done = true;
+            }
+        } else {
+            float deposit = w * L.mua_inv_mut;
+            w -= deposit;
+            PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
+            float xi_fi;
+            float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
+            scatter_direction(dx, dy, dz, ct, xi_fi);
+            if (roulette(w, x, a, w_min, chance, use_lottery)) done = true;
+        }
+        float ex = px - spx, ey = py - spy, ez = pz - spz;
+        if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
+        if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
+            done = true;
+            st[XO_STAT_TRUNCATED]++;
+        }
+        if (done) alive = false;
+    }
+
+    if (lane_ok) {
+        P.rng_x[gid] = x;
+        if (st[XO_STAT_PACKETS]) st[XO_STAT_THREADS] = 1;
+    }
+    stats_flush(&sstats, st, P.stats);
+    if (P.lds_bins) {
+        for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK)
+            if (sbins[i]) atomicAdd(&P.accum[i], sbins[i]);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* mcvox                                                              */
+/* ------------------------------------------------------------------ */
+
+struct VoxGeom {
+    float tl[3], br[3], size[3];
+    int32_t shape[3];
+};
+
+XO_DEV bool vox_valid(const VoxGeom &g, int32_t ix, int32_t iy, int32_t iz) {
+    return ix >= 0 && ix < g.shape[0] && iy >= 0 && iy < g.shape[1] && iz >= 0 &&
+           iz < g.shape[2];
+}
+
+XO_DEV uint32_t vox_material(const uint8_t *vox, const VoxGeom &g, int32_t ix,
+                             int32_t iy, int32_t iz) {
+    size_t flat = ((size_t)iz * (size_t)g.shape[1] + (size_t)iy) * (size_t)g.shape[0] +
+                  (size_t)ix;
+    return vox[flat];
+}
+
+/* mcsim_sample_box_intersect, mcvox.template.c:94-162 */
+XO_DEV bool box_intersect(const VoxGeom &g, const float pos[3], const float dir[3],
+                          float inter[3], float nrm[3]) {
+    float inv[3];
+    for (int k = 0; k < 3; ++k) inv[k] = (dir[k] != 0.0f) ? 1.0f / dir[k] : XO_INF_F;
+    float t1 = (g.tl[0] - pos[0]) * inv[0];
+    float t2 = (g.br[0] - pos[0]) * inv[0];
+    float t3 = (g.tl[1] - pos[1]) * inv[1];
+    float t4 = (g.br[1] - pos[1]) * inv[1];
+    float t5 = (g.tl[2] - pos[2]) * inv[2];
+    float t6 = (g.br[2] - pos[2]) * inv[2];
+    float txmin = fminf(t1, t2), txmax = fmaxf(t1, t2);
+    float tymin = fminf(t3, t4), tymax = fmaxf(t3, t4);
+    float tzmin = fminf(t5, t6), tzmax = fmaxf(t5, t6);
+    float tmin = fmaxf(fmaxf(txmin, tymin), tzmin);
+    float tmax = fminf(fminf(txmax, tymax), tzmax);
+    if (tmin > tmax) return false;
+    nrm[0] = (float)(txmin >= tymin && txmin >= tzmin);
+    nrm[1] = (float)((nrm[0] == 0.0f) && tymin >= tzmin);
+    nrm[2] = (float)(nrm[0] == 0.0f && nrm[1] == 0.0f);
+    nrm[0] *= fsign(dir[0]);
+    nrm[1] *= fsign(dir[1]);
+    nrm[2] *= fsign(dir[2]);
+    float t = (tmin > 0.0f) ? tmin : tmax;
+    for (int k = 0; k < 3; ++k) inter[k] = pos[k] + t * dir[k];
+    for (int k = 0; k < 3; ++k)
+        inter[k] = fminf(fmaxf(inter[k], g.tl[k]), g.br[k] - 1.1920929e-07f);
+    return tmax >= tmin && tmax >= 0.0f;
+}
+
+template <int PF>
+__global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ BlockStats sstats;
+    MaterialL *sm = reinterpret_cast<MaterialL *>(smem);
+    unsigned long long *sbins = reinterpret_cast<unsigned long long *>(
+        smem + XO_MAX_MATERIALS * sizeof(MaterialL));
+    float *slut = reinterpret_cast<float *>(sbins + P.lds_bins);
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nm = P.cfg.num_materials;
+    if (tid < nm) {
+        const xo_material &M = P.cfg.materials[tid];
+        MaterialL o;
+        o.inv_mut = M.inv_mut;
+        o.mua_inv_mut = M.mua_inv_mut;
+        o.n = M.n;
+        o.g = M.pf.g;
+        o.beta = M.pf.beta;
+        o.la = M.pf.a;
+        o.lb = M.pf.b;
+        o.lc = M.pf.c;
+        o.loff = M.pf.offset;
+        o.lsize_m1 = M.pf.size > 0u ? M.pf.size - 1u : 0u;
+        o.pad0 = o.pad1 = 0u;
+        sm[tid] = o;
+    }
+    for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK) sbins[i] = 0ull;
+    for (uint32_t i = tid; i < P.lds_lut; i += XO_BLOCK) slut[i] = P.lut[i];
+    if (tid < XO_STAT_COUNT) sstats.v[tid] = 0ull;
+    __syncthreads();
+
+    const Bins bins{P.lds_bins ? sbins : nullptr, P.accum};
+    const float *lut = P.lds_lut ? slut : P.lut;
+    const uint8_t *vox = P.voxels;
+
+    VoxGeom g;
+    for (int k = 0; k < 3; ++k) {
+        g.tl[k] = P.cfg.voxels.top_left[k];
+        g.br[k] = P.cfg.voxels.bottom_right[k];
+        g.size[k] = P.cfg.voxels.size[k];
+        g.shape[k] = P.cfg.voxels.shape[k];
+    }
+    const xo_fluence &F = P.cfg.fluence;
+    const bool use_fluence = F.enabled != 0;
+
+    const uint32_t gid = blockIdx.x * XO_BLOCK + tid;
+    const bool lane_ok = gid < P.lanes;
+    const bool static_sched = (P.flags & XO_FLAG_STATIC_SCHEDULE) != 0u;
+    uint64_t x = 0;
+    uint32_t a = 0;
+    uint64_t my_n = 0, my_k = 0;
+    if (lane_ok) {
+        x = P.rng_x[gid];
+        a = P.rng_a[gid];
+        if (static_sched)
+            my_n = P.npackets / P.lanes + ((uint64_t)gid < P.npackets % P.lanes ? 1u : 0u);
+    }
+
+    const bool line_src = P.cfg.source_kind == XO_SRC_LINE;
+    const float *sp = line_src ? P.cfg.line.position : P.cfg.gaussian.position;
+    const float spx = sp[0], spy = sp[1], spz = sp[2];
+    const float rmax2 = P.cfg.rmax * P.cfg.rmax;
+    const float w_min = P.cfg.w_min, chance = P.cfg.lottery_chance;
+    const int use_lottery = P.cfg.use_lottery;
+
+    bool active = lane_ok;
+    bool alive = false;
+    float p[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f};
+    float w = 0.f;
+    int32_t vi[3] = {0, 0, 0};
+    uint32_t mi = 0;
+    uint32_t st[XO_STAT_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t pkt_steps = 0;
+
+    for (;;) {
+        bool want = active && !alive;
+        bool launch = false;
+        if (static_sched) {
+            if (want) {
+                if (my_k < my_n) {
+                    ++my_k;
+                    launch = true;
+                } else {
+                    active = false;
+                }
+            }
+        } else {
+            uint64_t m = __ballot(want);
+            if (m) {
+                uint32_t leader = (uint32_t)__builtin_ctzll(m);
+                uint64_t base = 0;
+                if (lane_id() == leader)
+                    base = atomicAdd(P.counter, (unsigned long long)__builtin_popcountll(m));
+                base = readlane_u64(base, leader);
+                if (want) {
+                    uint64_t idx = base + prefix_count(m);
+                    if (idx < P.npackets)
+                        launch = true;
+                    else
+                        active = false;
+                }
+            }
+        }
+        if (launch) {
+            if (line_src) {
+                /* xopto/mcvox/mcsource/line.py:84-104 */
+                const xo_source_line &src = P.cfg.line;
+                w = 1.0f - src.reflectance;
+                for (int k = 0; k < 3; ++k) {
+                    p[k] = src.position[k];
+                    d[k] = src.direction_sample[k];
+                }
+                detector_deposit(P.cfg.specular, bins, p[0], p[1],
+                                 src.direction_reflected[0], src.direction_reflected[1],
+                                 src.direction_reflected[2], src.reflectance);
+            } else {
+                /* xopto/mcvox/mcsource/gaussianbeam.py:112-189 */
+                const xo_source_gaussian &src = P.cfg.gaussian;
+                float specular_r = 0.0f;
+                float r = __builtin_sqrtf(-2.0f * xo_logf(1.0f - rng_next(x, a)));
+                r = fminf(r, src.clip);
+                float sin_fi, cos_fi;
+                xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+                float ps0 = r * cos_fi * src.sigma[0];
+                float ps1 = r * sin_fi * src.sigma[1];
+                float ps2 = 0.0f;
+                const float *T = src.transformation;
+                float pm[3];
+                for (int k = 0; k < 3; ++k)
+                    pm[k] = T[3 * k] * ps0 + T[3 * k + 1] * ps1 + T[3 * k + 2] * ps2;
+                for (int k = 0; k < 3; ++k) pm[k] += src.position[k];
+                float dir[3] = {src.direction[0], src.direction[1], src.direction[2]};
+                bool inside = pm[0] >= g.tl[0] && pm[0] < g.br[0] && pm[1] >= g.tl[1] &&
+                              pm[1] < g.br[1] && pm[2] >= g.tl[2] && pm[2] < g.br[2];
+                if (inside) {
+                    dir[0] = -dir[0];
+                    dir[1] = -dir[1];
+                    dir[2] = -dir[2];
+                }
+                float inter[3], nrm[3];
+                if (box_intersect(g, pm, dir, inter, nrm)) {
+                    for (int k = 0; k < 3; ++k) {
+                        p[k] = inter[k];
+                        dir[k] = -dir[k];
+                        nrm[k] = -nrm[k];
+                    }
+                    float n_out = sm[0].n;
+                    int32_t ci[3];
+                    for (int k = 0; k < 3; ++k) {
+                        int32_t t = f2i_sat((inter[k] - g.tl[k]) / g.size[k]);
+                        ci[k] = t < 0 ? 0 : (t > g.shape[k] - 1 ? g.shape[k] - 1 : t);
+                    }
+                    uint32_t cm = vox_material(vox, g, ci[0], ci[1], ci[2]);
+                    float n_in = sm[cm].n;
+                    float refr[3] = {dir[0], dir[1], dir[2]};
+                    {
+                        /* refract_safe, mcbase.template.c:1487-1508 */
+                        float cos1 = dir[0] * nrm[0] + dir[1] * nrm[1] + dir[2] * nrm[2];
+                        float n1_d_n2 = n_out / n_in;
+                        float s2 = n1_d_n2 * n1_d_n2 * (1.0f - cos1 * cos1);
+                        if (!(s2 > 1.0f)) {
+                            float k = fsign(cos1) *
+                                      (n1_d_n2 * fabsf(cos1) - __builtin_sqrtf(1.0f - s2));
+                            for (int q = 0; q < 3; ++q) refr[q] = n1_d_n2 * dir[q] - k * nrm[q];
+                        }
+                    }
+                    for (int k = 0; k < 3; ++k) d[k] = refr[k];
+                    specular_r = reflectance(n_out, n_in,
+                                             nrm[0] * dir[0] + nrm[1] * dir[1] + nrm[2] * dir[2],
+                                             cos_critical(n_out, n_in));
+                    detector_deposit(P.cfg.specular, bins, p[0], p[1], dir[0], dir[1], dir[2],
+                                     specular_r);
+                } else {
+                    for (int k = 0; k < 3; ++k) p[k] = g.tl[k];
+                    d[0] = 0.0f;
+                    d[1] = 0.0f;
+                    d[2] = 1.0f;
+                    specular_r = 1.0f;
+                }
+                w = 1.0f - specular_r;
+            }
+            /* mcsim_set_voxel_index_from_position, mcvox.template.c:245-251 */
+            for (int k = 0; k < 3; ++k) vi[k] = f2i_sat((p[k] - g.tl[k]) / g.size[k]);
+            mi = vox_valid(g, vi[0], vi[1], vi[2]) ? vox_material(vox, g, vi[0], vi[1], vi[2])
+                                                    : 0u;
+            alive = true;
+            pkt_steps = 0;
+            st[XO_STAT_PACKETS]++;
+        }
+        if (!__any(alive)) break;
+        if (!alive) continue;
+
+        /* ---- one iteration of while(!done), mcvox.template.c:748-1012 ---- */
+        const MaterialL &M = sm[mi];
+        st[XO_STAT_STEPS]++;
+        float step = -xo_logf(rng_next(x, a)) * M.inv_mut;
+        step = fminf(step, XO_FLT_MAX);
+        /* mcsim_intersect, mcvox.template.c:173-196 */
+        float dist[3];
+        for (int k = 0; k < 3; ++k) {
+            float side = (float)((int32_t)(d[k] >= 0.0f) + vi[k]);
+            dist[k] = g.tl[k] + side * g.size[k] - p[k];
+            dist[k] = (d[k] != 0.0f) ? dist[k] / d[k] : XO_INF_F;
+        }
+        float dd = fminf(dist[0], fminf(dist[1], dist[2]));
+        float d_ok = fminf(dd, step);
+        p[0] = __builtin_fmaf(d_ok, d[0], p[0]);
+        p[1] = __builtin_fmaf(d_ok, d[1], p[1]);
+        p[2] = __builtin_fmaf(d_ok, d[2], p[2]);
+
+        bool done = false;
+        if (dd < step) {
+            st[XO_STAT_BOUNDARY]++;
+            /* mcsim_boundary, mcvox.template.c:275-388 */
+            int32_t nrm[3];
+            nrm[0] = (dist[0] <= dist[1]) && (dist[0] <= dist[2]);
+            nrm[1] = nrm[0] ? 0 : ((dist[1] <= dist[0]) && (dist[1] <= dist[2]));
+            nrm[2] = !(nrm[0] + nrm[1]);
+            for (int k = 0; k < 3; ++k)
+                if (d[k] < 0.0f) nrm[k] = -nrm[k];
+            int32_t nv0 = vi[0] + nrm[0], nv1 = vi[1] + nrm[1], nv2 = vi[2] + nrm[2];
+            bool escaping = !vox_valid(g, nv0, nv1, nv2);
+            uint32_t next_mi = !escaping ? vox_material(vox, g, nv0, nv1, nv2) : 0u;
+            bool crossed = false;
+            if (mi == next_mi && !escaping) {
+                crossed = true;
+            } else {
+                float n1 = sm[mi].n, n2 = sm[next_mi].n;
+                if (n1 == n2) {
+                    crossed = true;
+                } else {
+                    float cc = cos_critical(n1, n2);
+                    float cos1 = (float)nrm[0] * d[0] + (float)nrm[1] * d[1] +
+                                 (float)nrm[2] * d[2];
+                    float fn0 = (float)nrm[0], fn1 = (float)nrm[1], fn2 = (float)nrm[2];
+                    bool refracted = false;
+                    if (cos1 > cc) {
+                        float R = reflectance(n1, n2, cos1, cc);
+                        if (R < rng_next(x, a)) {
+                            /* refract, mcbase.template.c:1393-1410 */
+                            float c1 = d[0] * fn0 + d[1] * fn1 + d[2] * fn2;
+                            float n1_d_n2 = n1 / n2;
+                            float s2 = n1_d_n2 * n1_d_n2 * (1.0f - c1 * c1);
+                            float k = fsign(c1) *
+                                      (n1_d_n2 * fabsf(c1) - __builtin_sqrtf(1.0f - s2));
+                            d[0] = n1_d_n2 * d[0] - k * fn0;
+                            d[1] = n1_d_n2 * d[1] - k * fn1;
+                            d[2] = n1_d_n2 * d[2] - k * fn2;
+                            refracted = true;
+                            crossed = true;
+                        }
+                    }
+                    if (!refracted) {
+                        /* reflect, mcbase.template.c:1342-1350 */
+                        float pn2 = 2.0f * (d[0] * fn0 + d[1] * fn1 + d[2] * fn2);
+                        d[0] = d[0] - fn0 * pn2;
+                        d[1] = d[1] - fn1 * pn2;
+                        d[2] = d[2] - fn2 * pn2;
+                    }
+                }
+            }
+            if (crossed) {
+                vi[0] = nv0;
+                vi[1] = nv1;
+                vi[2] = nv2;
+                mi = next_mi;
+                st[XO_STAT_CROSSINGS]++;
+                if (escaping) {
+                    if (vi[2] < 0)
+                        detector_deposit(P.cfg.top, bins, p[0], p[1], d[0], d[1], d[2], w);
+                    else if (vi[2] >= g.shape[2])
+                        detector_deposit(P.cfg.bottom, bins, p[0], p[1], d[0], d[1], d[2], w);
+                    done = true;
+                }
+            }
+        } else {
+            float deposit = w * M.mua_inv_mut;
+            w -= deposit;
+            if (use_fluence) {
+                /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
+                float fx = (p[0] - F.top_left[0]) * F.inv_step[0];
+                float fy = (p[1] - F.top_left[1]) * F.inv_step[1];
+                float fz = (p[2] - F.top_left[2]) * F.inv_step[2];
+                if (fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)F.shape[0] &&
+                    fy < (float)F.shape[1] && fz < (float)F.shape[2]) {
+                    uint64_t ix = f2u_sat(fx), iy = f2u_sat(fy), iz = f2u_sat(fz);
+                    uint64_t index = (iz * F.shape[1] + iy) * F.shape[0] + ix;
+                    uint32_t ui = f2u_sat(deposit * (float)F.k + 0.5f);
+                    if (ui > 0u) {
+                        atomicAdd(&P.accum[F.offset + index], (unsigned long long)ui);
+                        st[XO_STAT_DEPOSITS]++;
+                    }
+                }
+            }
+            PfL pf{M.g, M.beta, M.la, M.lb, M.lc, M.loff, M.lsize_m1};
+            float xi_fi;
+            float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
+            scatter_direction(d[0], d[1], d[2], ct, xi_fi);
+            if (roulette(w, x, a, w_min, chance, use_lottery)) done = true;
+        }
+        float ex = p[0] - spx, ey = p[1] - spy, ez = p[2] - spz;
+        if (ex * ex + ey * ey + ez * ez > rmax2 || w <= 0.0f) done = true;
+        if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
+            done = true;
+            st[XO_STAT_TRUNCATED]++;
+        }
+        if (done) alive = false;
+    }
+
+    if (lane_ok) {
+        P.rng_x[gid] = x;
+        if (st[XO_STAT_PACKETS]) st[XO_STAT_THREADS] = 1;
+    }
+    stats_flush(&sstats, st, P.stats);
+    if (P.lds_bins) {
+        for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK)
+            if (sbins[i]) atomicAdd(&P.accum[i], sbins[i]);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* launchers                                                          */
+/* ------------------------------------------------------------------ */
+size_t mcml_lds_bytes(const McmlParams &P) {
+    return XO_MAX_LAYERS * sizeof(LayerL) + 8u * P.lds_bins + 4u * P.lds_lut;
+}
+
+size_t mcvox_lds_bytes(const McvoxParams &P) {
+    return XO_MAX_MATERIALS * sizeof(MaterialL) + 8u * P.lds_bins + 4u * P.lds_lut;
+}
+
+hipError_t launch_mcml(const McmlParams &P, uint32_t blocks, hipStream_t stream) {
+    size_t lds = mcml_lds_bytes(P);
+    switch (P.cfg.pf_kind) {
+    case XO_PF_HG:
+        hipLaunchKernelGGL(mcml_kernel<XO_PF_HG>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+        break;
+    case XO_PF_MHG:
+        hipLaunchKernelGGL(mcml_kernel<XO_PF_MHG>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+        break;
+    case XO_PF_LUT:
+        hipLaunchKernelGGL(mcml_kernel<XO_PF_LUT>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mcvox(const McvoxParams &P, uint32_t blocks, hipStream_t stream) {
+    size_t lds = mcvox_lds_bytes(P);
+    switch (P.cfg.pf_kind) {
+    case XO_PF_HG:
+        hipLaunchKernelGGL(mcvox_kernel<XO_PF_HG>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+        break;
+    case XO_PF_MHG:
+        hipLaunchKernelGGL(mcvox_kernel<XO_PF_MHG>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+        break;
+    case XO_PF_LUT:
+        hipLaunchKernelGGL(mcvox_kernel<XO_PF_LUT>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t occupancy_mcml(int pf_kind, size_t lds, int *blocks_per_cu) {
+    switch (pf_kind) {
+    case XO_PF_MHG:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_kernel<XO_PF_MHG>,
+                                                            XO_BLOCK, lds);
+    case XO_PF_LUT:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_kernel<XO_PF_LUT>,
+                                                            XO_BLOCK, lds);
+    default:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_kernel<XO_PF_HG>,
+                                                            XO_BLOCK, lds);
+    }
+}
+
+hipError_t occupancy_mcvox(int pf_kind, size_t lds, int *blocks_per_cu) {
+    switch (pf_kind) {
+    case XO_PF_MHG:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcvox_kernel<XO_PF_MHG>,
+                                                            XO_BLOCK, lds);
+    case XO_PF_LUT:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcvox_kernel<XO_PF_LUT>,
+                                                            XO_BLOCK, lds);
+    default:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcvox_kernel<XO_PF_HG>,
+                                                            XO_BLOCK, lds);
+    }
+}
+
+} // namespace xo

@@ -1,0 +1,144 @@
+"""Layered-media Monte Carlo simulator: the drop-in for xopto.mcml.mc.Mc
+(xopto/mcml/mc.py:77-1240).
+
+    from pyxopto_amd.mcml import mc
+    layers = mc.mclayer.Layers([...])
+    mc_obj = mc.Mc(layers, mc.mcsource.Line(), mc.mcdetector.Detectors(...))
+    mc_obj.rmax = 25e-3
+    trace, fluence, detectors = mc_obj.run(1e6)
+
+Each run packs the plugin objects into one xo_mcml_cfg and calls
+xo_mcml_run of libxomc.so (one persistent gfx950 kernel launch).
+"""
+import numpy as np
+
+from .. import _native as N
+from ..mcbase import mcdetector, mcoptions, mcpf
+from ..mcbase.mcutil import axis as _axis
+from ..mcbase.mcworker import McWorker
+from . import mclayer, mcsource
+
+# Module layout of xopto.mcml.mc: the plugin modules are reachable from here.
+mcdetector.Axis = _axis.Axis
+mcdetector.RadialAxis = _axis.RadialAxis
+
+__all__ = ['Mc', 'mclayer', 'mcsource', 'mcdetector', 'mcpf', 'mcoptions']
+
+
+class Mc(McWorker):
+    def __init__(self, layers, source, detectors: mcdetector.Detectors = None,
+                 trace=None, fluence=None, surface=None, types=None,
+                 options=None, rnginit=None, cl_devices=None,
+                 cl_build_options=None, cl_profiling: bool = False):
+        if trace is not None or surface is not None:
+            raise NotImplementedError(
+                'Trace and surface layouts are not part of this engine.')
+        if fluence is not None:
+            raise NotImplementedError(
+                'Fluence accumulation is implemented for the voxelized '
+                'simulator (pyxopto_amd.mcvox); the layered kernel has none.')
+        super().__init__(options=options, rnginit=rnginit,
+                         cl_devices=cl_devices,
+                         cl_build_options=cl_build_options,
+                         cl_profiling=cl_profiling, types=types)
+        self._layers = mclayer.Layers(layers)
+        self._pf_type = type(self._layers[1].pf)
+        self._source = source
+        self._source_type = type(source)
+        if not isinstance(source, mcsource.Line):
+            raise NotImplementedError('Only the Line source is implemented '
+                                      'for layered media.')
+        self._detectors = detectors
+        self._detector_types = detectors.types() if detectors else None
+        self._rmax = float('inf')
+
+    # ---------------- properties (xopto/mcml/mc.py:1190-1290) ----------
+    def _get_rmax(self):
+        return self._rmax
+
+    def _set_rmax(self, rmax):
+        rmax = float(rmax)
+        if rmax == 0.0:
+            raise ValueError('Simulation radius must be > 0.0 m!')
+        self._rmax = rmax
+    rmax = property(_get_rmax, _set_rmax, None,
+                    'Maximum simulation radius measured from the photon '
+                    'packet source position (m).')
+
+    layers = property(lambda self: self._layers, None, None, 'Layer stack.')
+    source = property(lambda self: self._source, None, None, 'Source.')
+    detectors = property(lambda self: self._detectors, None, None,
+                         'Detectors.')
+    options = property(lambda self: list(self._options), None, None,
+                       'Simulator options.')
+
+    def layer(self, index: int) -> mclayer.Layer:
+        return self._layers.layer(index)
+
+    def layer_index(self, z: float) -> int:
+        return self._layers.layer_index(z)
+
+    # ---------------- packing (xopto/mcml/mc.py:466-529) ----------------
+    def pack(self) -> tuple:
+        """Pack the configuration: (cfg, lut, naccum)."""
+        self._reset_allocations()
+        if type(self._layers[1].pf) != self._pf_type:
+            raise ValueError('The scattering phase function kind/type must '
+                             'not change between simulation calls!')
+        if type(self._source) != self._source_type:
+            raise ValueError('The photon packet source type/kind must not '
+                             'change between simulation calls!')
+        cfg = N.XoMcmlCfg()
+        self._apply_options(cfg)
+        cfg.pf_kind = self._layers.pf_kind
+        cfg.rmax = self._rmax
+        self._layers.pack(self, cfg)
+        self._source.pack(self, cfg.source)
+        if self._detectors is not None:
+            if self._detectors.types() != self._detector_types:
+                raise ValueError('Detector types must not change between '
+                                 'simulation calls!')
+            self._detectors.pack(self, cfg)
+        naccum = max(self._alloc_offset, 1)
+        return cfg, self._lut_array(), naccum
+
+    # ---------------- run (xopto/mcml/mc.py:730-1018) -------------------
+    def run(self, nphotons: int, out=None, wgsize: int = None,
+            maxthreads: int = None, copyseeds: bool = False,
+            exportsrc=None, verbose: bool = False,
+            static_schedule: bool = False):
+        """Simulate nphotons packets; returns (trace, fluence, detectors).
+
+        maxthreads caps the lanes (default: the persistent grid of the GPU).
+        static_schedule=True makes lane g run packets g, g+L, ... so that the
+        result is a deterministic function of the seeds and the lane count.
+        """
+        if out is None:
+            out = (None, None, None)
+        nphotons = self._check_nphotons(nphotons)
+        cfg, lut, naccum = self.pack()
+        ctx = self._context()
+        lanes = 0 if maxthreads is None else int(maxthreads)
+        accum, rep = ctx.run_mcml(cfg, lut, nphotons, naccum, lanes,
+                                  self._run_flags(static_schedule))
+        self._record_report(rep, nphotons)
+        self._last_accum = accum
+        detectors_res = out[2]
+        if self._detectors is not None:
+            if detectors_res is None:
+                detectors_res = mcdetector.Detectors(self._detectors)
+            for name, det in self._detectors.items():
+                if det is None:
+                    continue
+                off, n = self.allocation(det)
+                getattr(detectors_res, name).update_data(
+                    self, [accum[off:off + n]], nphotons)
+        if verbose:
+            print('McKernel processed {:,d} packets in {:,d} threads: '
+                  '{:.3f} ms'.format(nphotons, rep.threads_used,
+                                     rep.kernel_ms))
+        return None, out[1], detectors_res
+
+    def __repr__(self):
+        return 'Mc({}, {}, {})'.format(self._layers, self._source,
+                                       self._detectors)

@@ -1,0 +1,70 @@
+"""Photon packet sources of the layered simulator: Line
+(xopto/mcml/mcsource/line.py:30-252)."""
+from typing import Tuple
+
+import numpy as np
+
+from .. import _native as N
+from ..mcbase.mcutil import boundary
+
+
+class Line:
+    kind = N.XO_SRC_LINE
+
+    def __init__(self, position: Tuple[float, float, float] = (0.0, 0.0, 0.0),
+                 direction: Tuple[float, float, float] = (0.0, 0.0, 1.0)):
+        self._position = np.zeros(3)
+        self._direction = np.zeros(3)
+        self.position = position
+        self.direction = direction
+
+    def _set_position(self, v):
+        self._position[:] = v
+
+    def _set_direction(self, v):
+        self._direction[:] = v
+        norm = np.linalg.norm(self._direction)
+        if norm == 0.0:
+            raise ValueError('The norm/length of the propagation direction '
+                             'vector must not be 0!')
+        self._direction /= norm
+
+    position = property(lambda self: self._position, _set_position, None,
+                        'Source position.')
+    direction = property(lambda self: self._direction, _set_direction, None,
+                         'Source direction.')
+
+    def pack(self, mc, target: N.XoSourceLine) -> N.XoSourceLine:
+        """Project to the sample surface, refract, specular reflectance
+        (line.py:181-237)."""
+        d = self._direction
+        t = -self._position[2] / d[2]
+        position = self._position + d * t
+        position[2] = 0.0
+        n1 = mc.layer(0).n
+        n2 = mc.layer(1).n
+        reflectance = boundary.reflectance(n1, n2, d[2])
+        if reflectance >= 1.0:
+            raise ValueError('The line source is fully reflected from the '
+                             'top surface of the sample!')
+        sin1 = (1.0 - d[2] ** 2) ** 0.5
+        sin2 = max(min(n1 * sin1 / n2, 1.0), -1.0)
+        ds = (d[0] * n1 / n2, d[1] * n1 / n2,
+              np.sign(d[2]) * (1.0 - sin2 ** 2) ** 0.5)
+        for i in range(3):
+            target.position[i] = position[i]
+            target.direction_medium[i] = d[i]
+            target.direction_sample[i] = ds[i]
+        target.direction_reflected[0] = d[0]
+        target.direction_reflected[1] = d[1]
+        target.direction_reflected[2] = -d[2]
+        target.reflectance = reflectance
+        return target
+
+    def todict(self):
+        return {'position': self._position.tolist(),
+                'direction': self._direction.tolist(), 'type': 'Line'}
+
+    def __repr__(self):
+        return 'Line(position=({}, {}, {}), direction=({}, {}, {}))'.format(
+            *self._position, *self._direction)
