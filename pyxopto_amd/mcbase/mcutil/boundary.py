@@ -1,4 +1,4 @@
-"""Host-side boundary physics (xopto/mcbase/mcutil/boundary.py:25-130)."""
+"""Host-side boundary physics (xopto/mcbase/mcutil/boundary.py:25-150)."""
 import numpy as np
 
 
@@ -28,21 +28,27 @@ def reflectance(n1: float, n2: float, costheta: float = 1.0) -> float:
 
 
 def refract(direction, normal, n1: float, n2: float) -> np.ndarray:
-    """Refracted direction for a unit normal pointing along the incidence."""
+    """Refracted direction for a normal pointing either way
+    (xopto/mcbase/mcutil/boundary.py:80-117)."""
     p = np.asarray(direction, dtype=np.float64)
+    p = p / np.linalg.norm(p)
     n = np.asarray(normal, dtype=np.float64)
+    n = n / np.linalg.norm(n)
     cos1 = float(np.dot(p, n))
+    if abs(cos1) < cos_critical(n1, n2):
+        raise ValueError('Cannot refract, the incidence angle exceeds the '
+                         'critical angle!')
     k_n = n1 / n2
     sin2_sq = k_n * k_n * (1.0 - cos1 * cos1)
-    if sin2_sq > 1.0:
-        raise ValueError('Total internal reflection: no refracted beam!')
-    k = np.sign(cos1 if cos1 != 0.0 else 1.0) * \
-        (k_n * abs(cos1) - (1.0 - sin2_sq) ** 0.5)
-    r = k_n * p - k * n
-    return r / np.linalg.norm(r)
+    k = k_n * abs(cos1) - np.sqrt(1.0 - sin2_sq)
+    return k_n * p - np.sign(cos1) * k * n
 
 
 def reflect(direction, normal) -> np.ndarray:
+    """Mirror direction about the boundary (boundary.py:119-150)."""
     p = np.asarray(direction, dtype=np.float64)
+    if np.linalg.norm(p) > 0.0:
+        p = p / np.linalg.norm(p)
     n = np.asarray(normal, dtype=np.float64)
+    n = n / np.linalg.norm(n)
     return p - 2.0 * np.dot(p, n) * n

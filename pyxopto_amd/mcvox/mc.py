@@ -1,0 +1,139 @@
+"""Voxelized-media Monte Carlo simulator: the drop-in for xopto.mcvox.mc.Mc
+(xopto/mcvox/mc.py:89-1240). Each run packs the plugin objects into one
+xo_mcvox_cfg and calls xo_mcvox_run of libxomc.so; the voxel grid is
+uploaded (as uint8 material indices) only when it changed."""
+import numpy as np
+
+from .. import _native as N
+from ..mcbase import mcdetector, mcfluence, mcmaterial, mcoptions, mcpf
+from ..mcbase.mcutil import axis as _axis
+from ..mcbase.mcworker import McWorker
+from . import mcgeometry, mcsource
+
+mcdetector.Axis = _axis.Axis
+mcdetector.RadialAxis = _axis.RadialAxis
+mcfluence.Axis = _axis.Axis
+mcgeometry.Axis = _axis.Axis
+
+__all__ = ['Mc', 'mcgeometry', 'mcsource', 'mcdetector', 'mcpf', 'mcoptions',
+           'mcfluence', 'mcmaterial']
+
+
+class Mc(McWorker):
+    def __init__(self, voxels: mcgeometry.Voxels, materials, source,
+                 detectors: mcdetector.Detectors = None, trace=None,
+                 fluence: mcfluence.Fluence = None, surface=None, types=None,
+                 options=None, rnginit=None, cl_devices=None,
+                 cl_build_options=None, cl_profiling: bool = False):
+        if trace is not None or surface is not None:
+            raise NotImplementedError(
+                'Trace and surface layouts are not part of this engine.')
+        super().__init__(options=options, rnginit=rnginit,
+                         cl_devices=cl_devices,
+                         cl_build_options=cl_build_options,
+                         cl_profiling=cl_profiling, types=types)
+        self._voxels = voxels
+        self._materials = mcmaterial.Materials(materials)
+        self._pf_type = type(self._materials[0].pf)
+        self._source = source
+        self._source_type = type(source)
+        self._detectors = detectors
+        self._detector_types = detectors.types() if detectors else None
+        self._fluence = fluence
+        self._rmax = float('inf')
+        self._uploaded_shape = None
+
+    def _get_rmax(self):
+        return self._rmax
+
+    def _set_rmax(self, rmax):
+        rmax = float(rmax)
+        if rmax == 0.0:
+            raise ValueError('Simulation radius must be > 0.0 m!')
+        self._rmax = rmax
+    rmax = property(_get_rmax, _set_rmax, None,
+                    'Maximum simulation radius from the source (m).')
+
+    voxels = property(lambda self: self._voxels, None, None, 'Voxels.')
+    materials = property(lambda self: self._materials, None, None,
+                         'Materials.')
+    source = property(lambda self: self._source, None, None, 'Source.')
+    detectors = property(lambda self: self._detectors, None, None,
+                         'Detectors.')
+    fluence = property(lambda self: self._fluence, None, None, 'Fluence.')
+
+    def material(self, index: int) -> mcmaterial.Material:
+        return self._materials[index]
+
+    def material_at(self, position) -> mcmaterial.Material:
+        ix, iy, iz = self._voxels.index(position)
+        nz, ny, nx = self._voxels.shape
+        ix, iy, iz = (min(max(ix, 0), nx - 1), min(max(iy, 0), ny - 1),
+                      min(max(iz, 0), nz - 1))
+        return self._materials[int(self._voxels.material[iz, iy, ix])]
+
+    def pack(self):
+        """(cfg, lut, naccum): detector bins first, then the fluence bins."""
+        self._reset_allocations()
+        if type(self._materials[0].pf) != self._pf_type:
+            raise ValueError('The scattering phase function kind/type must '
+                             'not change between simulation calls!')
+        if type(self._source) != self._source_type:
+            raise ValueError('The photon packet source type/kind must not '
+                             'change between simulation calls!')
+        cfg = N.XoMcvoxCfg()
+        self._apply_options(cfg)
+        cfg.pf_kind = self._materials.pf_kind
+        cfg.rmax = self._rmax
+        cfg.num_materials = len(self._materials)
+        for i, m in enumerate(self._materials):
+            m.pack(self, cfg.materials[i])
+        self._voxels.pack(self, cfg.voxels)
+        self._source.pack(self, cfg)
+        if self._detectors is not None:
+            if self._detectors.types() != self._detector_types:
+                raise ValueError('Detector types must not change between '
+                                 'simulation calls!')
+            self._detectors.pack(self, cfg)
+        if self._fluence is not None:
+            self._fluence.pack(self, cfg.fluence)
+        naccum = max(self._alloc_offset, 1)
+        return cfg, self._lut_array(), naccum
+
+    def run(self, nphotons: int, out=None, wgsize: int = None,
+            maxthreads: int = None, copyseeds: bool = False, exportsrc=None,
+            verbose: bool = False, static_schedule: bool = False):
+        """Simulate nphotons packets; returns (trace, fluence, detectors)."""
+        if out is None:
+            out = (None, None, None)
+        nphotons = self._check_nphotons(nphotons)
+        cfg, lut, naccum = self.pack()
+        upload = self._voxels.update_required() or \
+            self._uploaded_shape != self._voxels.shape
+        voxels = self._voxels.material if upload else None
+        ctx = self._context()
+        lanes = 0 if maxthreads is None else int(maxthreads)
+        accum, rep = ctx.run_mcvox(cfg, voxels, lut, nphotons, naccum, lanes,
+                                   self._run_flags(static_schedule))
+        self._uploaded_shape = self._voxels.shape
+        self._record_report(rep, nphotons)
+        fluence_res = out[1]
+        if self._fluence is not None:
+            if fluence_res is None:
+                fluence_res = mcfluence.Fluence(self._fluence)
+            off, n = self.allocation(self._fluence)
+            fluence_res.update_data(self, [accum[off:off + n]], nphotons)
+        detectors_res = out[2]
+        if self._detectors is not None:
+            if detectors_res is None:
+                detectors_res = mcdetector.Detectors(self._detectors)
+            for name, det in self._detectors.items():
+                if det is None:
+                    continue
+                off, n = self.allocation(det)
+                getattr(detectors_res, name).update_data(
+                    self, [accum[off:off + n]], nphotons)
+        if verbose:
+            print('McKernel processed {:,d} packets: {:.3f} ms'.format(
+                nphotons, rep.kernel_ms))
+        return None, fluence_res, detectors_res

@@ -1,10 +1,12 @@
 """Shared builders and statistics for the tests.
 
-Statistical parity follows the plan of the reference-free oracle: the
-reference keeps no per-bin variance, so the standard error of each bin comes
-from K independent batches (batch means); a bin passes when
-|z| = |m_gpu - m_ref| / sqrt(se_gpu^2 + se_ref^2) < 4, and totals within 3
-combined standard errors.
+Statistical parity: the reference keeps no per-bin variance, so the standard
+error of each bin comes from K independent batches (batch means); a bin
+passes when |z| = |m_gpu - m_ref| / sqrt(se_gpu^2 + se_ref^2) < 4, and totals
+within 3 combined standard errors. The standard errors are themselves
+estimates (t-distributed with about 2K-2 degrees of freedom), so the tests
+use K >= 40 batches per side: with K = 10 a 100-bin comparison would fail
+by chance in several percent of runs.
 """
 import numpy as np
 

@@ -99,7 +99,7 @@ def _normalized(sim, accum, npk):
 def test_dynamic_schedule_statistical_parity(name):
     sim = sims()[name]
     print('case', name, flush=True)
-    k, npk = 10, 400000
+    k, npk = 40, 100000
     cfg, lut, naccum = sim.pack()
     ctx = N.Context(0, 0, XINIT)
     g = []
@@ -108,7 +108,7 @@ def test_dynamic_schedule_statistical_parity(name):
         assert rep.packets == npk
         g.append(_normalized(sim, acc, npk))
     g = np.asarray(g)
-    lanes = 4096
+    lanes = 1024
     o = oracle_batches(
         lambda x, a: _normalized(sim, O.mcml_run(cfg, lut, x, a, npk, naccum,
                                                  O.MATH_LIBM)[0], npk),
