@@ -45,6 +45,29 @@ def algorithmic_slots(rep, voxel: bool) -> float:
 
 def make_workload(name: str, packets):
     from pyxopto_amd import workloads
+    if name == 'layered_batch_4096':
+        from pyxopto_amd.mcml.batch import Batch
+        scale = 1.0 if not packets else packets / 8.77e8
+        insts = workloads.layered_batch(budget_scale=scale)
+        batch = Batch(insts, rnginit=0x5EED000000000001)
+        packed = batch.pack()
+        from pyxopto_amd.mcml.autotune import tuned_opts
+        opts = tuned_opts(batch, packed)
+        arr, lut, naccum = packed
+        npk = sum(int(i.nphotons) for i in insts)
+        ctx = batch.context
+
+        def step(first):
+            return ctx.run_batch(arr, lut, opts, naccum)[1]
+
+        config = {'workload': 'layered_batch_4096', 'instances': len(insts),
+                  'layers': '1-5 turbid + 2 ambient', 'packets': npk,
+                  'phase_functions': 'HG or 2000-point LUT of MHG',
+                  'detectors': '100-bin radial + total R/T per instance',
+                  'launch': {'chunk': opts.chunk, 'block': opts.block,
+                             'blocks_per_cu': opts.blocks_per_cu}}
+        batch.bench_plain = lambda: plain_rate(batch, insts)
+        return batch, npk, step, False, config
     if name == 'skin_vessel_200':
         sim = workloads.skin_vessel()
         npk = packets or workloads.SKIN_VESSEL_PACKETS
@@ -76,6 +99,26 @@ def make_workload(name: str, packets):
     raise SystemExit('unknown workload ' + name)
 
 
+def plain_rate(batch, insts, every: int = 16):
+    """The same instances one launch each (Mc.run: xo_mcml_run with
+    lane-owned generators), on every 16th instance: packets/s."""
+    from pyxopto_amd.mcml import mc
+    sub = insts[::every]
+    sims = [mc.Mc(i.layers, i.source, i.detectors, rnginit=0x5EED000000000001)
+            for i in sub]
+    for s in sims:             # one context, created outside the timing
+        s._ctx = batch.context
+    packed = [s.pack() for s in sims]
+    ctx = batch.context
+    npk = sum(int(i.nphotons) for i in sub)
+    t0 = time.perf_counter()
+    for s, (cfg, lut, naccum), i in zip(sims, packed, sub):
+        ctx.run_mcml(cfg, lut, i.nphotons, naccum)
+    dt = time.perf_counter() - t0
+    return {'packets_per_s': npk / dt, 'instances': len(sub),
+            'packets': npk, 'seconds': dt}
+
+
 def cpu_baseline(sim, voxel: bool, budget_s: float = 12.0):
     """The oracle (C, OpenMP, C-library math) on a bounded sample of the
     same workload, timed on this host's cores."""
@@ -83,6 +126,8 @@ def cpu_baseline(sim, voxel: bool, budget_s: float = 12.0):
     from pyxopto_amd import _native as N
     from oracle import pyoracle as O
     cores = len(os.sched_getaffinity(0))
+    if hasattr(sim, 'instances'):
+        return cpu_baseline_batch(sim, cores, budget_s)
     cfg, lut, naccum = sim.pack()
     lanes = 4 * cores * 64
     x, a = N.init_rng(lanes, 0x0123456789ABCDEF)
@@ -103,13 +148,38 @@ def cpu_baseline(sim, voxel: bool, budget_s: float = 12.0):
                 done, dt)}
 
 
+def cpu_baseline_batch(batch, cores: int, budget_s: float):
+    """The oracle on a seeded 256-instance subset of the batch (budgets cut
+    to 2000 packets each), repeated until the time budget is spent."""
+    from pyxopto_amd.mcml.batch import Batch
+    from oracle import pyoracle as O
+    import ctypes
+    sub = Batch(batch.instances[::16], rnginit=0x5EED000000000001)
+    arr, lut, naccum = sub.pack()
+    for i in range(len(arr)):
+        arr[i].npackets = min(int(arr[i].npackets), 2000)
+    per = sum(int(arr[i].npackets) for i in range(len(arr)))
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        O.batch_run(arr, lut, naccum, math_mode=O.MATH_LIBM, nthreads=cores)
+        done += per
+        for i in range(len(arr)):
+            arr[i].seed += 1
+    dt = time.perf_counter() - t0
+    return {'value': done / dt, 'unit': 'photon packets/s', 'cores': cores,
+            'kind': 'port',
+            'sample': '{} packets of a 256-instance subset (2000 packets '
+                      'each) in {:.1f} s'.format(done, dt)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--workload', default='skin_vessel_200',
-                    choices=['skin_vessel_200', 'slab_1mm'])
+    ap.add_argument('--workload', default='layered_batch_4096',
+                    choices=['layered_batch_4096', 'skin_vessel_200',
+                             'slab_1mm'])
     ap.add_argument('--packets', type=int, default=None)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     args = ap.parse_args()
@@ -173,6 +243,10 @@ def main():
         out['atomic_GBps'] = dep * 8 / kernel_s / 1e9
         out['voxel_crossings_per_s'] = sum(r.voxel_crossings
                                            for r in reps) / kernel_s
+    if hasattr(sim, 'bench_plain'):
+        plain = sim.bench_plain()
+        out['plain_one_launch_per_instance'] = plain
+        out['batched_vs_plain'] = value / plain['packets_per_s']
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(sim, voxel)
     print(json.dumps(out), flush=True)

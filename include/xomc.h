@@ -44,6 +44,7 @@ extern "C" {
 
 #define XO_MAX_LAYERS 16
 #define XO_MAX_MATERIALS 32
+#define XO_BATCH_MAX_LAYERS 8
 
 /* Scattering phase function kinds (one kind per simulation, as
  * xopto/mcml/mclayer/layer.py:862-872 requires). */
@@ -215,6 +216,38 @@ typedef struct xo_run_report {
     double total_ms;            /* host wall time of the whole call           */
 } xo_run_report;
 
+/* One layered-media simulation of a batch. There is no counterpart struct in
+ * the reference, whose batch workflow calls Mc.run once per instance
+ * (xopto/mcml/mcrun/helper.py:640-686); each field has the meaning of the
+ * xo_mcml_cfg field of the same name. The generator of packet p is derived
+ * from (seed, p) (xo_packet_seed), so an instance's result does not depend
+ * on the batch it runs in or on the schedule. */
+typedef struct xo_batch_instance {
+    uint32_t num_layers;
+    int32_t pf_kind;
+    float rmax;
+    uint32_t accum_base;        /* this instance's bins: [base, base+count)  */
+    uint32_t accum_count;
+    uint32_t reserved;
+    uint64_t npackets;
+    uint64_t seed;
+    xo_source_line source;
+    xo_detector top, bottom, specular; /* offsets are absolute              */
+    xo_layer layers[XO_BATCH_MAX_LAYERS];
+} xo_batch_instance; /* 888 B */
+
+/* Launch shape of a batch run (chosen by the autotuner). */
+typedef struct xo_batch_opts {
+    uint32_t chunk;             /* packets per work-queue item (>= 1)        */
+    uint32_t block;             /* threads per block: 64, 128 or 256          */
+    uint32_t blocks_per_cu;     /* 0: as many as fit                          */
+    uint32_t flags;             /* XO_FLAG_NO_LDS_BINS is ignored             */
+    float w_min;                /* roulette threshold for every instance     */
+    float lottery_chance;
+    int32_t use_lottery;
+    uint32_t reserved;
+} xo_batch_opts;
+
 /* ---------------- library ---------------- */
 
 /* ABI version of the loaded library. */
@@ -231,7 +264,8 @@ int xo_device_count(void);
  * the sizeof_datatypes kernel (xopto/mcml/kernel/mcml.template.c:827-844).
  * Writes up to n values in the order: xo_pf, xo_layer, xo_source_line,
  * xo_source_gaussian, xo_detector, xo_fluence, xo_mcml_cfg, xo_material,
- * xo_voxel_cfg, xo_mcvox_cfg, xo_run_report. Returns the count written. */
+ * xo_voxel_cfg, xo_mcvox_cfg, xo_run_report, xo_batch_instance,
+ * xo_batch_opts. Returns the count written. */
 int xo_struct_sizes(uint32_t *out, int n);
 
 /* ---------------- seeds (host) ---------------- */
@@ -301,6 +335,24 @@ int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg,
                  const float *lut, uint32_t nlut,
                  uint64_t npackets, uint32_t lanes, uint32_t flags,
                  uint64_t *accum, uint64_t naccum, xo_run_report *report);
+
+/* Run a batch of layered-media instances in one persistent launch: a device
+ * work queue of (instance, chunk of packets) items; lanes take packets of
+ * whichever item their wave holds and the wave takes the next item as soon
+ * as the current one has no unclaimed packet left. Replaces the sequential
+ * one-launch-per-instance loop of McRunHelper.run_batch
+ * (xopto/mcml/mcrun/helper.py:640-686). accum/naccum: host output holding
+ * every instance's bins; report->packets counts all instances. */
+int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *instances,
+                 uint32_t ninstances, const float *lut, uint32_t nlut,
+                 const xo_batch_opts *opts, uint64_t *accum, uint64_t naccum,
+                 xo_run_report *report);
+
+/* The MWC state (x, a) of packet p of a batch instance with key seed:
+ * a = table multiplier (rows 1..XO_MAX_SEEDS) chosen by a hash of
+ * (seed, p), x a valid state (0 <= c < a-1, x < 2^32-1) from a second hash.
+ * Host copy of the device function, for checks. */
+int xo_packet_seed(uint64_t seed, uint64_t p, uint64_t *x, uint32_t *a);
 
 #ifdef __cplusplus
 }

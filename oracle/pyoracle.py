@@ -48,6 +48,11 @@ def lib():
         l.oracle_rng.argtypes = [POINTER(c_uint64), c_uint32, c_uint32,
                                  POINTER(c_float)]
         l.oracle_max_threads.restype = c_int
+        l.oracle_batch_run.restype = c_int
+        l.oracle_batch_run.argtypes = [
+            POINTER(N.XoBatchInstance), c_uint32, POINTER(c_float),
+            POINTER(c_uint32), c_float, c_float, c_int, c_int, c_int,
+            POINTER(c_uint64), c_uint64, POINTER(c_uint64)]
         l.oracle_pf_samples.restype = c_int
         l.oracle_pf_samples.argtypes = [
             c_int, POINTER(N.XoPf), POINTER(c_float), c_uint64, c_uint32,
@@ -172,6 +177,29 @@ def mcvox_run(cfg, voxels, lut, rng_x, rng_a, npackets: int, naccum: int,
     if rc != 0:
         raise RuntimeError('oracle_mcvox_run failed ({})'.format(rc))
     return accum, counts, x
+
+
+def batch_run(instances, lut, naccum: int, w_min=1e-4, lottery_chance=0.1,
+              use_lottery=True, math_mode: int = MATH_KERNEL,
+              nthreads: int = 0):
+    """A batch of layered instances with per-packet generators
+    (schedule independent). instances: ctypes array of XoBatchInstance."""
+    table = np.ascontiguousarray(N.safeprime_table(N.XO_NUM_PRIMES)[1:])
+    accum = np.zeros(naccum, dtype=np.uint64)
+    counts = np.zeros(4, dtype=np.uint64)
+    lut_p = None
+    if lut is not None and len(lut):
+        lut = np.ascontiguousarray(lut, dtype=np.float32)
+        lut_p = _p(lut, c_float)
+    rc = lib().oracle_batch_run(instances, c_uint32(len(instances)), lut_p,
+                                _p(table, c_uint32), c_float(w_min),
+                                c_float(lottery_chance), c_int(int(use_lottery)),
+                                c_int(math_mode), c_int(nthreads),
+                                _p(accum, c_uint64), c_uint64(naccum),
+                                _p(counts, c_uint64))
+    if rc != 0:
+        raise RuntimeError('oracle_batch_run failed ({})'.format(rc))
+    return accum, counts
 
 
 def ref_rng_lib():

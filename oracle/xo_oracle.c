@@ -40,6 +40,7 @@
 
 #include "../include/xomc.h"
 #include "../pyxopto_amd/csrc/xo_math.h"
+#include "../pyxopto_amd/csrc/xo_seedhash.h"
 
 #define XO_OMATH_LIBM 0
 #define XO_OMATH_KERNEL 1
@@ -778,6 +779,63 @@ int oracle_mcvox_run(const xo_mcvox_cfg *cfg, const int32_t *voxels,
         counts_out[1] = sb;
         counts_out[2] = sc;
         counts_out[3] = sd;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* batches of layered instances                                       */
+/* ------------------------------------------------------------------ */
+/* Every packet p of instance i runs the layered packet loop above with the
+ * generator xo_packet_state(seed_i, p) (pyxopto_amd/csrc/xo_seedhash.h), so
+ * the result is independent of scheduling; table holds the multipliers of
+ * seed-table rows 1..499999. */
+int oracle_batch_run(const xo_batch_instance *inst, uint32_t ninst,
+                     const float *lut, const uint32_t *table, float w_min,
+                     float lottery_chance, int use_lottery, int math_mode,
+                     int nthreads, uint64_t *accum, uint64_t naccum,
+                     uint64_t *counts_out) {
+    memset(accum, 0, naccum * sizeof(uint64_t));
+    xo_mcml_cfg *cfgs = (xo_mcml_cfg *)calloc(ninst, sizeof(xo_mcml_cfg));
+    if (!cfgs) return -1;
+    for (uint32_t i = 0; i < ninst; ++i) {
+        xo_mcml_cfg *c = &cfgs[i];
+        if (inst[i].num_layers < 3 || inst[i].num_layers > XO_BATCH_MAX_LAYERS) {
+            free(cfgs);
+            return -1;
+        }
+        c->num_layers = inst[i].num_layers;
+        c->pf_kind = inst[i].pf_kind;
+        c->rmax = inst[i].rmax;
+        c->w_min = w_min;
+        c->lottery_chance = lottery_chance;
+        c->use_lottery = use_lottery;
+        c->source = inst[i].source;
+        c->top = inst[i].top;
+        c->bottom = inst[i].bottom;
+        c->specular = inst[i].specular;
+        memcpy(c->layers, inst[i].layers, sizeof(xo_layer) * inst[i].num_layers);
+    }
+    uint64_t st = 0, sb = 0;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const int64_t np = (int64_t)inst[i].npackets;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : st, sb)
+        for (int64_t p = 0; p < np; ++p) {
+            o_pkt s;
+            o_counts c = {0, 0, 0, 0, 0};
+            xo_packet_state(inst[i].seed, (uint64_t)p, table, &s.x, &s.a);
+            o_mcml_packet(&cfgs[i], lut, &s, accum, &c, math_mode);
+            st += c.steps;
+            sb += c.boundary;
+        }
+    }
+    free(cfgs);
+    if (counts_out) {
+        counts_out[0] = st;
+        counts_out[1] = sb;
+        counts_out[2] = 0;
+        counts_out[3] = 0;
     }
     return 0;
 }

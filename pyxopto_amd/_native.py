@@ -26,6 +26,7 @@ XO_FIRST_PRIME_A = 4294967118
 XO_MAX_PACKET_STEPS = 1 << 26
 XO_MAX_LAYERS = 16
 XO_MAX_MATERIALS = 32
+XO_BATCH_MAX_LAYERS = 8
 
 XO_PF_HG, XO_PF_MHG, XO_PF_LUT = 0, 1, 2
 XO_DET_NONE, XO_DET_TOTAL, XO_DET_RADIAL = 0, 1, 2
@@ -107,6 +108,23 @@ class XoMcvoxCfg(ctypes.Structure):
                 ('materials', XoMaterial * XO_MAX_MATERIALS)]
 
 
+class XoBatchInstance(ctypes.Structure):
+    _fields_ = [('num_layers', c_uint32), ('pf_kind', c_int32),
+                ('rmax', c_float), ('accum_base', c_uint32),
+                ('accum_count', c_uint32), ('reserved', c_uint32),
+                ('npackets', c_uint64), ('seed', c_uint64),
+                ('source', XoSourceLine), ('top', XoDetector),
+                ('bottom', XoDetector), ('specular', XoDetector),
+                ('layers', XoLayer * XO_BATCH_MAX_LAYERS)]
+
+
+class XoBatchOpts(ctypes.Structure):
+    _fields_ = [('chunk', c_uint32), ('block', c_uint32),
+                ('blocks_per_cu', c_uint32), ('flags', c_uint32),
+                ('w_min', c_float), ('lottery_chance', c_float),
+                ('use_lottery', c_int32), ('reserved', c_uint32)]
+
+
 class XoRunReport(ctypes.Structure):
     _fields_ = [('packets', c_uint64), ('steps', c_uint64),
                 ('boundary_steps', c_uint64), ('voxel_crossings', c_uint64),
@@ -121,7 +139,8 @@ class XoRunReport(ctypes.Structure):
 
 STRUCTS_IN_ABI_ORDER = [XoPf, XoLayer, XoSourceLine, XoSourceGaussian,
                         XoDetector, XoFluence, XoMcmlCfg, XoMaterial,
-                        XoVoxelCfg, XoMcvoxCfg, XoRunReport]
+                        XoVoxelCfg, XoMcvoxCfg, XoRunReport, XoBatchInstance,
+                        XoBatchOpts]
 
 # Every entry point of include/xomc.h: name -> (restype, argtypes)
 PROTOTYPES = {
@@ -147,6 +166,12 @@ PROTOTYPES = {
                             c_uint32, c_uint64, c_uint32, c_uint32,
                             POINTER(c_uint64), c_uint64,
                             POINTER(XoRunReport)]),
+    'xo_batch_run': (c_int, [c_void_p, POINTER(XoBatchInstance), c_uint32,
+                             POINTER(c_float), c_uint32, POINTER(XoBatchOpts),
+                             POINTER(c_uint64), c_uint64,
+                             POINTER(XoRunReport)]),
+    'xo_packet_seed': (c_int, [c_uint64, c_uint64, POINTER(c_uint64),
+                               POINTER(c_uint32)]),
     'xo_mcvox_run': (c_int, [c_void_p, POINTER(XoMcvoxCfg), POINTER(c_int32),
                              c_uint64, POINTER(c_float), c_uint32, c_uint64,
                              c_uint32, c_uint32, POINTER(c_uint64), c_uint64,
@@ -267,6 +292,15 @@ def rng_host(x: int, a: int, n: int):
     return out, int(xs.value)
 
 
+def packet_seed(seed: int, p: int):
+    """(x, a) of packet p of a batch instance keyed by seed."""
+    x, a = c_uint64(), c_uint32()
+    check(lib().xo_packet_seed(c_uint64(int(seed)), c_uint64(int(p)),
+                               ctypes.byref(x), ctypes.byref(a)),
+          'xo_packet_seed')
+    return int(x.value), int(a.value)
+
+
 def device_count() -> int:
     return int(lib().xo_device_count())
 
@@ -333,6 +367,20 @@ class Context:
                                 c_uint32(lanes), c_uint32(flags),
                                 _ptr(accum, c_uint64), c_uint64(naccum),
                                 ctypes.byref(rep)), 'xo_mcml_run')
+        return accum, rep
+
+    def run_batch(self, instances, lut, opts: XoBatchOpts, naccum: int):
+        """instances: ctypes array of XoBatchInstance."""
+        accum = np.zeros(naccum, dtype=np.uint64)
+        lut_p, nlut = None, 0
+        if lut is not None and len(lut):
+            lut = np.ascontiguousarray(lut, dtype=np.float32)
+            lut_p, nlut = _ptr(lut, c_float), lut.size
+        rep = XoRunReport()
+        check(lib().xo_batch_run(self._h, instances, c_uint32(len(instances)),
+                                 lut_p, c_uint32(nlut), ctypes.byref(opts),
+                                 _ptr(accum, c_uint64), c_uint64(naccum),
+                                 ctypes.byref(rep)), 'xo_batch_run')
         return accum, rep
 
     def run_mcvox(self, cfg: XoMcvoxCfg, voxels, lut, npackets: int,

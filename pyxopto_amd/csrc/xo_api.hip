@@ -15,8 +15,10 @@
 #include <vector>
 
 #include "../../include/xomc.h"
+#include "xo_batch.h"
 #include "xo_internal.h"
 #include "xo_kernels.h"
+#include "xo_seedhash.h"
 
 static_assert(sizeof(xo_pf) == 32, "xo_pf layout");
 static_assert(sizeof(xo_layer) == 80, "xo_layer layout");
@@ -26,6 +28,7 @@ static_assert(sizeof(xo_detector) == 48, "xo_detector layout");
 static_assert(sizeof(xo_fluence) == 48, "xo_fluence layout");
 static_assert(sizeof(xo_material) == 64, "xo_material layout");
 static_assert(sizeof(xo_voxel_cfg) == 48, "xo_voxel_cfg layout");
+static_assert(sizeof(xo_batch_instance) == 888, "xo_batch_instance layout");
 static_assert(sizeof(xo::McvoxParams) <= 4096, "kernel argument too large");
 static_assert(sizeof(xo::McmlParams) <= 4096, "kernel argument too large");
 
@@ -73,6 +76,14 @@ struct xo_ctx {
     int32_t vox_shape[3] = {0, 0, 0};
     bool vox_valid = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    /* batch buffers */
+    uint32_t *table = nullptr;          /* seed multipliers, rows 1..      */
+    void *batch_layers = nullptr;
+    size_t batch_layers_cap = 0;
+    void *batch_hdr = nullptr;
+    size_t batch_hdr_cap = 0;
+    void *batch_items = nullptr;
+    size_t batch_items_cap = 0;
     std::mutex mu;
 };
 
@@ -158,7 +169,8 @@ int xo_struct_sizes(uint32_t *out, int n) {
         (uint32_t)sizeof(xo_detector),    (uint32_t)sizeof(xo_fluence),
         (uint32_t)sizeof(xo_mcml_cfg),    (uint32_t)sizeof(xo_material),
         (uint32_t)sizeof(xo_voxel_cfg),   (uint32_t)sizeof(xo_mcvox_cfg),
-        (uint32_t)sizeof(xo_run_report)};
+        (uint32_t)sizeof(xo_run_report),  (uint32_t)sizeof(xo_batch_instance),
+        (uint32_t)sizeof(xo_batch_opts)};
     int count = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (n < count) count = n;
     for (int i = 0; i < count; ++i) out[i] = sizes[i];
@@ -232,6 +244,10 @@ void xo_ctx_destroy(xo_ctx *ctx) {
     if (ctx->accum) (void)hipFree(ctx->accum);
     if (ctx->lut) (void)hipFree(ctx->lut);
     if (ctx->voxels) (void)hipFree(ctx->voxels);
+    if (ctx->table) (void)hipFree(ctx->table);
+    if (ctx->batch_layers) (void)hipFree(ctx->batch_layers);
+    if (ctx->batch_hdr) (void)hipFree(ctx->batch_hdr);
+    if (ctx->batch_items) (void)hipFree(ctx->batch_items);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -508,6 +524,171 @@ int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
     float kms = 0.f;
     XO_HIP(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
     fill_report(report, stats + 1, P.lanes, kms, ms_since(t0));
+    return 0;
+}
+
+int xo_packet_seed(uint64_t seed, uint64_t p, uint64_t *x, uint32_t *a) {
+    static std::vector<uint32_t> table;
+    static std::mutex m;
+    std::lock_guard<std::mutex> lock(m);
+    if (table.empty()) {
+        std::vector<uint32_t> fora(XO_NUM_PRIMES);
+        if (xo::table_get(XO_NUM_PRIMES, fora.data()) != 0) {
+            xo::set_error("xo_packet_seed: safe-prime table unavailable");
+            return -1;
+        }
+        table.assign(fora.begin() + 1, fora.end());
+    }
+    xo_packet_state(seed, p, table.data(), x, a);
+    return 0;
+}
+
+int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
+                 const float *lut, uint32_t nlut, const xo_batch_opts *opts,
+                 uint64_t *accum, uint64_t naccum, xo_run_report *report) {
+    auto t0 = std::chrono::steady_clock::now();
+    xo::clear_error();
+    if (!ctx || !inst || !opts || !accum || ninst == 0 || naccum == 0) {
+        xo::set_error("xo_batch_run: null or empty argument");
+        return -1;
+    }
+    const uint32_t block = opts->block ? opts->block : 256u;
+    if (block != 64 && block != 128 && block != 256) {
+        xo::set_error("xo_batch_run: block must be 64, 128 or 256 (got %u)", block);
+        return -1;
+    }
+    const uint64_t chunk = opts->chunk ? opts->chunk : 1024u;
+    std::vector<xo::LayerL> layers((size_t)ninst * XO_BATCH_MAX_LAYERS);
+    std::vector<xo::BatchHdr> hdr(ninst);
+    std::vector<xo::BatchItem> items;
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const xo_batch_instance &I = inst[i];
+        if (I.num_layers < 3 || I.num_layers > XO_BATCH_MAX_LAYERS) {
+            xo::set_error("instance %u: %u layers (3..%d supported)", i, I.num_layers,
+                          XO_BATCH_MAX_LAYERS);
+            return -1;
+        }
+        for (uint32_t l = 0; l < I.num_layers; ++l)
+            if (check_pf(I.pf_kind, I.layers[l].pf, nlut, "layer", l)) return -1;
+        if (I.pf_kind == XO_PF_LUT && !lut) {
+            xo::set_error("instance %u: a lookup-table phase function needs the LUT", i);
+            return -1;
+        }
+        if ((uint64_t)I.accum_base + I.accum_count > naccum) {
+            xo::set_error("instance %u: bins [%u, +%u) outside the accumulator (%llu)", i,
+                          I.accum_base, I.accum_count, (unsigned long long)naccum);
+            return -1;
+        }
+        uint64_t end = 0;
+        const xo_detector *dets[3] = {&I.top, &I.bottom, &I.specular};
+        for (int k = 0; k < 3; ++k) {
+            if (check_detector("instance", *dets[k], naccum, &end)) return -1;
+            if (dets[k]->kind != XO_DET_NONE &&
+                (dets[k]->offset < I.accum_base ||
+                 (uint64_t)dets[k]->offset +
+                         (dets[k]->kind == XO_DET_TOTAL ? 1u : dets[k]->n) >
+                     (uint64_t)I.accum_base + I.accum_count)) {
+                xo::set_error("instance %u: detector bins outside its range", i);
+                return -1;
+            }
+        }
+        for (uint32_t l = 0; l < XO_BATCH_MAX_LAYERS; ++l)
+            layers[(size_t)i * XO_BATCH_MAX_LAYERS + l] =
+                xo::layer_to_device(I.layers[l < I.num_layers ? l : I.num_layers - 1]);
+        xo::BatchHdr &H = hdr[i];
+        std::memset(&H, 0, sizeof(H));
+        H.num_layers = (int32_t)I.num_layers;
+        H.pf_kind = I.pf_kind;
+        H.rmax2 = I.rmax * I.rmax;
+        H.reflectance = I.source.reflectance;
+        for (int k = 0; k < 3; ++k) {
+            H.src_pos[k] = I.source.position[k];
+            H.dir_sample[k] = I.source.direction_sample[k];
+            H.dir_reflected[k] = I.source.direction_reflected[k];
+        }
+        H.seed = I.seed;
+        H.top = I.top;
+        H.bottom = I.bottom;
+        H.specular = I.specular;
+        for (uint64_t first = 0; first < I.npackets; first += chunk) {
+            uint64_t cnt = I.npackets - first < chunk ? I.npackets - first : chunk;
+            items.push_back(xo::BatchItem{i, (uint32_t)cnt, first});
+        }
+        total += I.npackets;
+        if (items.size() >= 0xFFFFFFF0ull) {
+            xo::set_error("xo_batch_run: too many work items; raise the chunk size");
+            return -1;
+        }
+    }
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    if (!ctx->table) {
+        std::vector<uint32_t> fora(XO_NUM_PRIMES);
+        if (xo::table_get(XO_NUM_PRIMES, fora.data()) != 0) {
+            xo::set_error("xo_batch_run: safe-prime table unavailable");
+            return -1;
+        }
+        XO_HIP(hipMalloc(&ctx->table, sizeof(uint32_t) * XO_SEED_TABLE_ROWS));
+        XO_HIP(hipMemcpy(ctx->table, fora.data() + 1, sizeof(uint32_t) * XO_SEED_TABLE_ROWS,
+                         hipMemcpyHostToDevice));
+    }
+    const size_t lbytes = layers.size() * sizeof(xo::LayerL);
+    const size_t hbytes = hdr.size() * sizeof(xo::BatchHdr);
+    const size_t ibytes = (items.empty() ? 1 : items.size()) * sizeof(xo::BatchItem);
+    if (ensure_buffer(&ctx->batch_layers, &ctx->batch_layers_cap, lbytes) ||
+        ensure_buffer(&ctx->batch_hdr, &ctx->batch_hdr_cap, hbytes) ||
+        ensure_buffer(&ctx->batch_items, &ctx->batch_items_cap, ibytes) ||
+        ensure_buffer((void **)&ctx->accum, &ctx->accum_cap, naccum * 8))
+        return -1;
+    if (nlut && ensure_buffer((void **)&ctx->lut, &ctx->lut_cap, (size_t)nlut * 4)) return -1;
+    XO_HIP(hipMemcpyAsync(ctx->batch_layers, layers.data(), lbytes, hipMemcpyHostToDevice,
+                          ctx->stream));
+    XO_HIP(hipMemcpyAsync(ctx->batch_hdr, hdr.data(), hbytes, hipMemcpyHostToDevice,
+                          ctx->stream));
+    if (!items.empty())
+        XO_HIP(hipMemcpyAsync(ctx->batch_items, items.data(),
+                              items.size() * sizeof(xo::BatchItem), hipMemcpyHostToDevice,
+                              ctx->stream));
+    if (nlut)
+        XO_HIP(hipMemcpyAsync(ctx->lut, lut, (size_t)nlut * 4, hipMemcpyHostToDevice,
+                              ctx->stream));
+
+    xo::BatchParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.layers = (const xo::LayerL *)ctx->batch_layers;
+    P.hdr = (const xo::BatchHdr *)ctx->batch_hdr;
+    P.items = (const xo::BatchItem *)ctx->batch_items;
+    P.lut = nlut ? ctx->lut : nullptr;
+    P.table = ctx->table;
+    P.item_counter = (unsigned int *)ctx->counters;
+    P.accum = ctx->accum;
+    P.stats = ctx->counters + 1;
+    P.nitems = (uint32_t)items.size();
+    P.w_min = opts->w_min;
+    P.chance = opts->lottery_chance;
+    P.use_lottery = opts->use_lottery;
+    int bpc = 1;
+    XO_HIP(xo::occupancy_batch(block, &bpc));
+    if (opts->blocks_per_cu && (int)opts->blocks_per_cu < bpc) bpc = (int)opts->blocks_per_cu;
+    if (bpc < 1) bpc = 1;
+    uint32_t blocks = (uint32_t)bpc * ctx->cus;
+
+    XO_HIP(hipMemsetAsync(ctx->accum, 0, naccum * 8, ctx->stream));
+    XO_HIP(hipMemsetAsync(ctx->counters, 0, sizeof(unsigned long long) * (1 + XO_STAT_COUNT),
+                          ctx->stream));
+    XO_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (total > 0) XO_HIP(xo::launch_batch(P, blocks, block, ctx->stream));
+    XO_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    unsigned long long stats[1 + XO_STAT_COUNT];
+    XO_HIP(hipMemcpyAsync(accum, ctx->accum, naccum * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XO_HIP(hipMemcpyAsync(stats, ctx->counters, sizeof(stats), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    XO_HIP(hipStreamSynchronize(ctx->stream));
+    float kms = 0.f;
+    XO_HIP(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+    fill_report(report, stats + 1, blocks * block, kms, ms_since(t0));
     return 0;
 }
 

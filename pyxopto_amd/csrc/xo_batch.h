@@ -1,0 +1,80 @@
+/*
+ * xo_batch.h — device-side form of a batch (built on the host from the
+ * xo_batch_instance array) and the launch parameters of the batch kernel.
+ */
+#ifndef XO_BATCH_H
+#define XO_BATCH_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/xomc.h"
+#include "xo_device.h"
+#include "xo_kernels.h"
+
+namespace xo {
+
+/* Per-instance constants the lanes read at packet launch and escape. */
+struct BatchHdr {
+    int32_t num_layers;
+    int32_t pf_kind;
+    float rmax2;
+    float reflectance;
+    float src_pos[3];
+    float dir_sample[3];
+    float dir_reflected[3];
+    uint32_t pad0;
+    uint64_t seed;
+    xo_detector top, bottom, specular;
+};
+
+/* One work-queue item: packets [first, first + count) of one instance. */
+struct BatchItem {
+    uint32_t inst;
+    uint32_t count;
+    uint64_t first;
+};
+
+struct BatchParams {
+    const LayerL *layers;        /* [instance][XO_BATCH_MAX_LAYERS] */
+    const BatchHdr *hdr;
+    const BatchItem *items;
+    const float *lut;
+    const uint32_t *table;       /* seed-table multipliers, rows 1.. */
+    unsigned int *item_counter;
+    unsigned long long *accum;
+    unsigned long long *stats;
+    uint32_t nitems;
+    float w_min;
+    float chance;
+    int32_t use_lottery;
+};
+
+hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,
+                        hipStream_t stream);
+hipError_t occupancy_batch(uint32_t block, int *blocks_per_cu);
+
+/* Host conversion of one layer into the device layout. */
+inline LayerL layer_to_device(const xo_layer &L) {
+    LayerL o;
+    o.top = L.top;
+    o.bottom = L.bottom;
+    o.inv_mut = L.inv_mut;
+    o.mua_inv_mut = L.mua_inv_mut;
+    o.n = L.n;
+    o.cc_top = L.cos_critical_top;
+    o.cc_bottom = L.cos_critical_bottom;
+    o.g = L.pf.g;
+    o.beta = L.pf.beta;
+    o.la = L.pf.a;
+    o.lb = L.pf.b;
+    o.lc = L.pf.c;
+    o.loff = L.pf.offset;
+    o.lsize_m1 = L.pf.size > 0u ? L.pf.size - 1u : 0u;
+    o.pad0 = o.pad1 = 0u;
+    return o;
+}
+
+} // namespace xo
+
+#endif /* XO_BATCH_H */

@@ -1,0 +1,193 @@
+/*
+ * xo_batch.hip — one persistent launch for many layered-media instances.
+ *
+ * Work queue: items (instance, first packet, packet count), claimed by whole
+ * wavefronts with one atomic each. Lanes are independent: a lane whose packet
+ * ends takes the next packet of its wave's current item; when the item has
+ * no unclaimed packet left the wave claims the next item at once, so lanes
+ * of one wave may be on different instances and no lane waits for a slow
+ * packet of another. Each lane carries its instance index; the instance's
+ * layers (64 B each, a few instances per wave at a time) are read through
+ * the L1/L2 caches, detector deposits are u64 atomics to HBM.
+ *
+ * Each packet's MWC generator is keyed on (instance seed, packet index)
+ * (xo_seedhash.h), so the integer accumulators of an instance are a pure
+ * function of the instance: identical alone, in any mix, at any launch shape.
+ * The transport step is the layered kernel's (mcml.template.c:514-788).
+ */
+#include <hip/hip_runtime.h>
+
+#include "xo_batch.h"
+#include "xo_device.h"
+#include "xo_seedhash.h"
+
+namespace xo {
+
+struct BatchBlockStats {
+    unsigned long long v[XO_STAT_COUNT];
+};
+
+XO_DEV float pf_sample_rt(int kind, const PfL &pf, const float *lut, uint64_t &x,
+                          uint32_t a, float &xi_fi) {
+    if (kind == XO_PF_HG) return pf_sample<XO_PF_HG>(pf, lut, x, a, xi_fi);
+    if (kind == XO_PF_MHG) return pf_sample<XO_PF_MHG>(pf, lut, x, a, xi_fi);
+    return pf_sample<XO_PF_LUT>(pf, lut, x, a, xi_fi);
+}
+
+__global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
+    __shared__ BatchBlockStats sstats;
+    if (threadIdx.x < XO_STAT_COUNT) sstats.v[threadIdx.x] = 0ull;
+    __syncthreads();
+
+    const Bins bins{nullptr, P.accum};
+    const uint32_t lane = lane_id();
+
+    /* wave-uniform queue state */
+    uint32_t cur_inst = 0;
+    uint64_t cur_next = 0, cur_end = 0;
+    bool queue_empty = false;
+
+    /* lane state */
+    bool alive = false;
+    uint32_t inst = 0;
+    const LayerL *layers = P.layers;
+    int32_t nl = 3, pf_kind = 0;
+    float rmax2 = 0.f, spx = 0.f, spy = 0.f, spz = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, w = 0.f;
+    int32_t layer = 1;
+    uint64_t x = 0;
+    uint32_t a = 0;
+    uint32_t pkt_steps = 0;
+    uint32_t st[XO_STAT_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+
+    for (;;) {
+        /* ---- hand out packets to the lanes that need one ---- */
+        const bool want = !alive && !queue_empty;
+        const uint64_t m = __ballot(want);
+        const uint32_t nwant = (uint32_t)__builtin_popcountll(m);
+        const uint32_t rank = prefix_count(m);
+        uint32_t assigned = 0;
+        bool launch = false;
+        uint64_t my_p = 0;
+        while (assigned < nwant) {
+            if (cur_next >= cur_end) {
+                uint32_t it = 0;
+                if (lane == 0) it = atomicAdd(P.item_counter, 1u);
+                it = (uint32_t)__builtin_amdgcn_readlane((int)it, 0);
+                if (it >= P.nitems) {
+                    queue_empty = true;
+                    break;
+                }
+                const BatchItem item = P.items[it];
+                cur_inst = item.inst;
+                cur_next = item.first;
+                cur_end = item.first + item.count;
+            }
+            const uint64_t avail = cur_end - cur_next;
+            const uint32_t left = nwant - assigned;
+            const uint32_t take = avail < (uint64_t)left ? (uint32_t)avail : left;
+            if (want && rank >= assigned && rank < assigned + take) {
+                launch = true;
+                inst = cur_inst;
+                my_p = cur_next + (rank - assigned);
+            }
+            cur_next += take;
+            assigned += take;
+        }
+        if (launch) {
+            const BatchHdr &H = P.hdr[inst];
+            layers = P.layers + (size_t)inst * XO_BATCH_MAX_LAYERS;
+            nl = H.num_layers;
+            pf_kind = H.pf_kind;
+            rmax2 = H.rmax2;
+            spx = H.src_pos[0];
+            spy = H.src_pos[1];
+            spz = H.src_pos[2];
+            xo_packet_state(H.seed, my_p, P.table, &x, &a);
+            /* Line source, xopto/mcml/mcsource/line.py:95-110 */
+            w = 1.0f - H.reflectance;
+            px = spx;
+            py = spy;
+            pz = spz;
+            dx = H.dir_sample[0];
+            dy = H.dir_sample[1];
+            dz = H.dir_sample[2];
+            detector_deposit(H.specular, bins, px, py, H.dir_reflected[0],
+                             H.dir_reflected[1], H.dir_reflected[2], H.reflectance);
+            layer = 1;
+            pkt_steps = 0;
+            alive = true;
+            st[XO_STAT_PACKETS]++;
+        }
+        if (!__any(alive)) break;
+        if (!alive) continue;
+
+        /* ---- one transport step, mcml.template.c:514-788 ---- */
+        const LayerL L = layers[layer];
+        st[XO_STAT_STEPS]++;
+        float step = -xo_logf(rng_next(x, a)) * L.inv_mut;
+        step = fminf(step, XO_FLT_MAX);
+        int32_t next = layer;
+        if (__builtin_fmaf(step, dz, pz) < L.top) {
+            --next;
+            if (fabsf(dz) != 0.0f) step = (L.top - pz) / dz;
+        }
+        if (__builtin_fmaf(step, dz, pz) >= L.bottom) {
+            ++next;
+            if (fabsf(dz) != 0.0f) step = (L.bottom - pz) / dz;
+        }
+        px = __builtin_fmaf(dx, step, px);
+        py = __builtin_fmaf(dy, step, py);
+        pz = __builtin_fmaf(dz, step, pz);
+        if (layer < next) pz = L.bottom;
+        if (layer > next) pz = L.top;
+
+        bool done = false;
+        if (next != layer) {
+            st[XO_STAT_BOUNDARY]++;
+            mcml_boundary(layers, layer, next, dx, dy, dz, x, a);
+            if (!(layer > 0 && layer < nl - 1)) {
+                const BatchHdr &H = P.hdr[inst];
+                if (layer <= 0)
+                    detector_deposit(H.top, bins, px, py, dx, dy, dz, w);
+                else
+                    detector_deposit(H.bottom, bins, px, py, dx, dy, dz, w);
+                done = true;
+            }
+        } else {
+            float deposit = w * L.mua_inv_mut;
+            w -= deposit;
+            PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
+            float xi_fi;
+            float ct = pf_sample_rt(pf_kind, pf, P.lut, x, a, xi_fi);
+            scatter_direction(dx, dy, dz, ct, xi_fi);
+            if (roulette(w, x, a, P.w_min, P.chance, P.use_lottery)) done = true;
+        }
+        float ex = px - spx, ey = py - spy, ez = pz - spz;
+        if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
+        if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
+            done = true;
+            st[XO_STAT_TRUNCATED]++;
+        }
+        if (done) alive = false;
+    }
+
+    for (int i = 0; i < XO_STAT_COUNT; ++i)
+        if (st[i]) atomicAdd(&sstats.v[i], (unsigned long long)st[i]);
+    __syncthreads();
+    if (threadIdx.x < XO_STAT_COUNT && sstats.v[threadIdx.x])
+        atomicAdd(&P.stats[threadIdx.x], sstats.v[threadIdx.x]);
+}
+
+hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL(batch_kernel, dim3(blocks), dim3(block), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t occupancy_batch(uint32_t block, int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_kernel,
+                                                        (int)block, 0);
+}
+
+} // namespace xo

@@ -23,52 +23,6 @@ namespace xo {
 /* mcml                                                               */
 /* ------------------------------------------------------------------ */
 
-/* mcsim_boundary, mcml.template.c:80-203 (no surface layouts). */
-XO_DEV void mcml_boundary(const LayerL *sl, int32_t &layer, int32_t next,
-                          float &dx, float &dy, float &dz, uint64_t &x,
-                          uint32_t a) {
-    const LayerL &cur = sl[layer];
-    const LayerL &nxt = sl[layer + (dz >= 0.0f ? 1 : -1)];
-    float cos_crit = (dz < 0.0f) ? cur.cc_top : cur.cc_bottom;
-    float n1 = cur.n;
-    float n2 = nxt.n;
-    if (n1 == n2) {
-        layer = next;
-        return;
-    }
-    dz = -dz;
-    float cos1 = fabsf(dz);
-    if (cos1 > cos_crit) {
-        if (cos1 > kCos0) {
-            float R = (n1 - n2) / (n1 + n2);
-            if (R * R < rng_next(x, a)) {
-                layer = next;
-                dz = -dz;
-            }
-            return;
-        }
-        float n1_d_n2 = n1 / n2;
-        float sin1 = __builtin_sqrtf(1.0f - cos1 * cos1);
-        if (cos1 >= kCos0) sin1 = 0.0f;
-        float sin2 = fminf(1.0f, n1_d_n2 * sin1);
-        float cos2 = __builtin_sqrtf(1.0f - sin2 * sin2);
-        float n_cos1 = n1_d_n2 * cos1;
-        float n_cos2 = n1_d_n2 * cos2;
-        float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
-        Rs *= Rs;
-        float Rp = (n_cos2 - cos1) / (n_cos2 + cos1);
-        Rp *= Rp;
-        float R = 0.5f * (Rs + Rp);
-        if (cos1 <= kCos90 || sin2 == 1.0f) R = 1.0f;
-        if (R < rng_next(x, a)) {
-            layer = next;
-            dx *= n1_d_n2;
-            dy *= n1_d_n2;
-            dz = -copysignf(cos2, dz);
-        }
-    }
-}
-
 /* Block-level prologue/epilogue shared by the kernels. */
 struct BlockStats {
     unsigned long long v[XO_STAT_COUNT];

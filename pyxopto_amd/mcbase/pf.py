@@ -5,7 +5,8 @@ builder (xopto/pf/pfbase.py:102-567, xopto/pf/hg.py, xopto/pf/mhg.py).
 the phase function is integrated with Simpson's rule on a fine grid, the
 model F(x) = a/(x + b) + c with a = 2c(1 - c), b = 2c - 1 (so that F(-1) = 0
 and F(1) = 1) is fitted to it by least squares on both admissible branches
-(c > 1 and c < 0), and the table holds the inverse CDF evaluated at the model
+(c > 1 and c < 0; bounded Brent searches instead of the reference's
+L-BFGS-B, same objective), and the table holds the inverse CDF evaluated at the model
 F of N uniformly spaced cosines. The sampler then draws
 index = (a/(xi - c) - b + 1)(N - 1)/2 and interpolates linearly.
 """
@@ -13,7 +14,7 @@ from typing import Tuple
 
 import numpy as np
 from scipy.interpolate import interp1d
-from scipy.optimize import minimize
+from scipy.optimize import minimize_scalar
 
 
 def lut_function(costheta, params):
@@ -57,17 +58,26 @@ class PfBase:
         costhetai = np.linspace(-1.0, 1.0, ncd)
         cumpfi = self.cdf(costhetai)
 
-        def cost(v):
-            return np.linalg.norm(lut_function(costhetai, v) - cumpfi)
+        def cost(c):
+            diff = lut_function(costhetai, c) - cumpfi
+            return float(np.sqrt(np.sum(diff * diff)))
 
-        eps = np.finfo(np.float64).eps
-        res1 = minimize(cost, [2.0], bounds=((1.0 + eps, np.inf),))
-        res2 = minimize(cost, [-1.0], bounds=((-np.inf, -eps),))
-        res = res1 if res1.fun < res2.fun else res2
-        randlut = np.clip(lut_function(costheta, res.x), 0.0, 1.0)
+        # The model has one free parameter on two branches, c > 1 and c < 0;
+        # each is searched with bounded Brent on a logarithmic scale of
+        # |c - 1| and |c| (the least-squares optimum of the reference's
+        # L-BFGS-B fit, found with fewer evaluations).
+        r1 = minimize_scalar(lambda t: cost(1.0 + np.exp(t)),
+                             bounds=(-30.0, 15.0), method='bounded',
+                             options={'xatol': 1e-10})
+        r2 = minimize_scalar(lambda t: cost(-np.exp(t)),
+                             bounds=(-30.0, 15.0), method='bounded',
+                             options={'xatol': 1e-10})
+        c = 1.0 + np.exp(r1.x) if r1.fun < r2.fun else -np.exp(r2.x)
+        res_x = np.array([c])
+        randlut = np.clip(lut_function(costheta, res_x), 0.0, 1.0)
         lut = interp1d(cumpfi, costhetai, bounds_error=False, fill_value=0.0,
                        kind='linear')(randlut)
-        c = float(np.asarray(res.x).ravel()[0])
+        c = float(res_x[0])
         return [2.0 * c * (1.0 - c), 2.0 * c - 1.0, c], lut
 
 

@@ -1,0 +1,137 @@
+"""Batched layered-media simulations: many Mc-like instances in one
+persistent launch (xo_batch_run). The reference's batch workflow calls
+Mc.run once per instance (xopto/mcml/mcrun/helper.py:640-686); here every
+instance keeps its own layers, phase functions, detectors and packet budget,
+and the GPU works through a device queue of (instance, packet chunk) items.
+
+Packet p of an instance draws from a generator keyed on (instance seed, p),
+so an instance's integer accumulators do not depend on what else runs in
+the batch or on the launch shape.
+"""
+from typing import List, Optional
+
+import numpy as np
+
+from .. import _native as N
+from ..mcbase import mcdetector
+from ..mcbase.mcworker import McWorker
+from . import mclayer, mcsource
+
+
+class Instance:
+    """One layered simulation: layers, Line source, detectors, budget."""
+
+    def __init__(self, layers, source=None, detectors=None,
+                 nphotons: int = 100000, seed: Optional[int] = None,
+                 rmax: float = float('inf')):
+        self.layers = mclayer.Layers(layers)
+        if len(self.layers) > N.XO_BATCH_MAX_LAYERS:
+            raise ValueError('A batch instance supports at most {} layers '
+                             'including the surrounding media.'.format(
+                                 N.XO_BATCH_MAX_LAYERS))
+        self.source = source if source is not None else mcsource.Line()
+        self.detectors = detectors
+        self.nphotons = int(nphotons)
+        self.seed = seed
+        self.rmax = float(rmax)
+
+    def layer(self, index: int):
+        return self.layers.layer(index)
+
+
+class _InstancePacker:
+    """The mc object an instance's plugins pack against: layer access from
+    the instance, LUT array and accumulator allocation from the batch."""
+
+    def __init__(self, batch, inst):
+        self._batch, self._inst = batch, inst
+        self.allocations = {}
+
+    def layer(self, index):
+        return self._inst.layer(index)
+
+    def append_r_lut(self, data):
+        return self._batch.append_r_lut(data)
+
+    def allocate_accumulator(self, owner, n):
+        offset = self._batch.allocate_accumulator(owner, n)
+        self.allocations[id(owner)] = (offset, int(n))
+        return offset
+
+
+class Batch(McWorker):
+    def __init__(self, instances: List[Instance], options=None, rnginit=None,
+                 cl_devices=None):
+        super().__init__(options=options, rnginit=rnginit,
+                         cl_devices=cl_devices)
+        self.instances = list(instances)
+        if not self.instances:
+            raise ValueError('A batch needs at least one instance.')
+
+    def instance_seed(self, index: int) -> int:
+        inst = self.instances[index]
+        if inst.seed is not None:
+            return int(inst.seed) & 0xFFFFFFFFFFFFFFFF
+        return (self._rnginit * 0x9E3779B97F4A7C15 + index) & \
+            0xFFFFFFFFFFFFFFFF
+
+    def pack(self):
+        """(XoBatchInstance array, lut, naccum)."""
+        self._reset_allocations()
+        arr = (N.XoBatchInstance * len(self.instances))()
+        self._packers = []
+        for i, inst in enumerate(self.instances):
+            t = arr[i]
+            packer = _InstancePacker(self, inst)
+            self._packers.append(packer)
+            cfg = N.XoMcmlCfg()
+            inst.layers.pack(packer, cfg)
+            t.num_layers = cfg.num_layers
+            for l in range(cfg.num_layers):
+                t.layers[l] = cfg.layers[l]
+            t.pf_kind = inst.layers.pf_kind
+            t.rmax = inst.rmax
+            inst.source.pack(packer, t.source)
+            base = self._alloc_offset
+            if inst.detectors is not None:
+                inst.detectors.pack(packer, t)
+            t.accum_base = base
+            t.accum_count = self._alloc_offset - base
+            t.npackets = inst.nphotons
+            t.seed = self.instance_seed(i)
+        return arr, self._lut_array(), max(self._alloc_offset, 1)
+
+    def opts(self, chunk: int = 2048, block: int = 256,
+             blocks_per_cu: int = 0) -> N.XoBatchOpts:
+        o = N.XoBatchOpts()
+        o.chunk, o.block, o.blocks_per_cu = chunk, block, blocks_per_cu
+        o.w_min = float(self._opts['w_min'])
+        o.lottery_chance = float(self._opts['lottery_chance'])
+        o.use_lottery = int(bool(self._opts['use_lottery']))
+        return o
+
+    def run(self, opts: N.XoBatchOpts = None, packed=None):
+        """Run every instance; returns (list of Detectors, run report)."""
+        arr, lut, naccum = packed if packed is not None else self.pack()
+        if opts is None:
+            from .autotune import tuned_opts
+            opts = tuned_opts(self, (arr, lut, naccum))
+        accum, rep = self._context().run_batch(arr, lut, opts, naccum)
+        self._record_report(rep, int(sum(i.nphotons for i in self.instances)))
+        return self.results(accum), rep
+
+    def results(self, accum) -> List[mcdetector.Detectors]:
+        out = []
+        for inst, packer in zip(self.instances, self._packers):
+            if inst.detectors is None:
+                out.append(None)
+                continue
+            res = mcdetector.Detectors(inst.detectors)
+            for name, det in inst.detectors.items():
+                if det is None:
+                    continue
+                off, n = packer.allocations[id(det)]
+                getattr(res, name).update_data(self, [accum[off:off + n]],
+                                               inst.nphotons)
+            out.append(res)
+        return out

@@ -11,7 +11,11 @@
   n 1.40 as the dermis); Gaussian beam sigma 50 um at normal incidence;
   energy deposition in every voxel; 1e8 packets. The surrounding medium is
   n 1.0 with no absorption or scattering.
+- ``layered_batch``: 4096 layered instances with 1-5 turbid layers and
+  uneven packet budgets for the batched engine (see its docstring).
 """
+import os
+
 import numpy as np
 
 from .mcml import mc as mcml
@@ -62,3 +66,75 @@ def skin_vessel(n: int = 200, voxel: float = 10e-6,
     mat[x ** 2 + (z - vessel_depth) ** 2 <= vessel_radius ** 2] = 3
     sim.voxels.update()
     return sim
+
+
+BATCH_INSTANCES = 4096
+
+
+def _lut_params(args):
+    from .mcbase import pf as hostpf
+    g, b = args
+    return hostpf.MHg(g, b).mclut(2000)
+
+
+def layered_batch(n: int = BATCH_INSTANCES, seed: int = 20261202,
+                  budget_scale: float = 1.0, processes: int = None):
+    """``n`` layered instances drawn from a seeded generator: 1-5 turbid
+    layers each (thickness log-uniform 0.05-5 mm, mua log-uniform
+    0.01-50 /cm, mus log-uniform 10-500 /cm, n uniform 1.33-1.50) between
+    air above and below; phase function per instance either HG (g uniform
+    0.6-0.95 per layer) or a 2000-point lookup table of modified HG (g
+    uniform 0.6-0.95, beta uniform 0-1 per layer); packet budget
+    log-uniform 1e4-1e6 (times budget_scale); 100-bin radial reflectance
+    0-5 mm plus total transmittance and specular reflectance each.
+    Returns a list of batch.Instance."""
+    from .mcml import batch as mbatch
+    rng = np.random.default_rng(seed)
+    specs = []
+    lut_jobs = []
+    for i in range(n):
+        nl = int(rng.integers(1, 6))
+        d = np.exp(rng.uniform(np.log(0.05e-3), np.log(5e-3), nl))
+        mua = np.exp(rng.uniform(np.log(1.0), np.log(5000.0), nl))
+        mus = np.exp(rng.uniform(np.log(1e3), np.log(5e4), nl))
+        nn = rng.uniform(1.33, 1.50, nl)
+        use_lut = bool(rng.integers(0, 2))
+        g = rng.uniform(0.6, 0.95, nl)
+        beta = rng.uniform(0.0, 1.0, nl)
+        budget = int(np.exp(rng.uniform(np.log(1e4), np.log(1e6))) *
+                     budget_scale)
+        inst_seed = int(rng.integers(1, 2 ** 63))
+        specs.append((nl, d, mua, mus, nn, use_lut, g, beta, max(budget, 1),
+                      inst_seed))
+        if use_lut:
+            lut_jobs.extend((float(g[k]), float(beta[k])) for k in range(nl))
+    luts = {}
+    if lut_jobs:
+        if processes == 1:
+            results = [_lut_params(j) for j in lut_jobs]
+        else:
+            import multiprocessing as mp
+            procs = processes or min(16, len(os.sched_getaffinity(0)))
+            with mp.get_context('fork').Pool(procs) as pool:
+                results = pool.map(_lut_params, lut_jobs, chunksize=16)
+        luts = dict(zip(lut_jobs, results))
+    L = mcml.mclayer
+    D = mcml.mcdetector
+    out = []
+    for nl, d, mua, mus, nn, use_lut, g, beta, budget, inst_seed in specs:
+        if use_lut:
+            pfs = [mcml.mcpf.Lut(*luts[(float(g[k]), float(beta[k]))])
+                   for k in range(nl)]
+            amb = pfs[0]
+        else:
+            pfs = [mcml.mcpf.Hg(float(g[k])) for k in range(nl)]
+            amb = mcml.mcpf.Hg(0.0)
+        layers = [L.Layer(d=np.inf, n=1.0, mua=0.0, mus=0.0, pf=amb)]
+        layers += [L.Layer(d=float(d[k]), n=float(nn[k]), mua=float(mua[k]),
+                           mus=float(mus[k]), pf=pfs[k]) for k in range(nl)]
+        layers += [L.Layer(d=np.inf, n=1.0, mua=0.0, mus=0.0, pf=amb)]
+        det = D.Detectors(top=D.Radial(D.RadialAxis(0.0, 5e-3, 100)),
+                          bottom=D.Total(), specular=D.Total())
+        out.append(mbatch.Instance(layers, mcml.mcsource.Line(), det,
+                                   nphotons=budget, seed=inst_seed))
+    return out

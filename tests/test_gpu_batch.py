@@ -1,0 +1,60 @@
+"""Batched engine on the GPU: bit-exact with the oracle whatever the
+schedule, the same result alone and in the mix, and at every launch
+shape."""
+import numpy as np
+import pytest
+
+from pyxopto_amd import _native as N
+from pyxopto_amd.mcml.batch import Batch
+from oracle import pyoracle as O
+
+from batch_helpers import small_batch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def mixed():
+    insts = small_batch(40)
+    b = Batch(insts, rnginit=0x5EED000000000001)
+    return b, b.pack()
+
+
+def test_batch_bit_exact_with_oracle(mixed):
+    b, (arr, lut, naccum) = mixed
+    acc, rep = b.context.run_batch(arr, lut, b.opts(chunk=1024), naccum)
+    ref, counts = O.batch_run(arr, lut, naccum, math_mode=O.MATH_KERNEL)
+    assert rep.packets == sum(i.nphotons for i in b.instances)
+    assert int(rep.steps) == int(counts[0])
+    assert int(rep.boundary_steps) == int(counts[1])
+    np.testing.assert_array_equal(acc, ref)
+
+
+@pytest.mark.parametrize('chunk,block,bpc', [(64, 64, 1), (333, 128, 0),
+                                             (8192, 256, 2)])
+def test_launch_shape_does_not_change_results(mixed, chunk, block, bpc):
+    b, (arr, lut, naccum) = mixed
+    a1, _ = b.context.run_batch(arr, lut, b.opts(1024, 256, 0), naccum)
+    a2, _ = b.context.run_batch(arr, lut, b.opts(chunk, block, bpc), naccum)
+    np.testing.assert_array_equal(a1, a2)
+
+
+def test_instance_alone_equals_instance_in_mix(mixed):
+    b, (arr, lut, naccum) = mixed
+    mix, _ = b.context.run_batch(arr, lut, b.opts(), naccum)
+    for i in (0, 7, 21, 39):
+        alone = Batch([b.instances[i]], rnginit=0x5EED000000000001)
+        res, _ = alone.run(alone.opts(chunk=256))
+        base = arr[i].accum_base
+        got = np.concatenate([res[0].top.raw, res[0].bottom.raw,
+                              res[0].specular.raw]) * N.XO_ACCU_K
+        np.testing.assert_array_equal(np.rint(got).astype(np.uint64),
+                                      mix[base:base + 102])
+
+
+def test_batch_api_autotuned_run(mixed):
+    b, _ = mixed
+    res, rep = b.run()
+    assert len(res) == len(b.instances)
+    assert res[0].top.nphotons == b.instances[0].nphotons
+    assert rep.truncated == 0
