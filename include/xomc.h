@@ -77,7 +77,12 @@ enum xo_run_flags {
      * seeds, the configuration and L. */
     XO_FLAG_STATIC_SCHEDULE = 1u << 0,
     /* Keep detector bins in global memory even if they fit in LDS. */
-    XO_FLAG_NO_LDS_BINS = 1u << 1
+    XO_FLAG_NO_LDS_BINS = 1u << 1,
+    /* Packet p (counted from cfg->first_packet) draws from the generator
+     * keyed on (cfg->packet_seed, p) (xo_packet_seed) instead of the lane's
+     * stream; the context's lane streams are left untouched. Makes a run a
+     * function of the configuration alone, like a batch instance. */
+    XO_FLAG_PACKET_KEYED = 1u << 2
 };
 
 /* Phase-function parameters; the union of McPf for Hg, MHg and Lut. */
@@ -166,6 +171,8 @@ typedef struct xo_mcml_cfg {
     xo_source_line source;
     xo_detector top, bottom, specular;
     xo_layer layers[XO_MAX_LAYERS];
+    uint64_t packet_seed;       /* XO_FLAG_PACKET_KEYED only                  */
+    uint64_t first_packet;      /* XO_FLAG_PACKET_KEYED only                  */
 } xo_mcml_cfg;
 
 /* A voxel material; McMaterial of xopto/mcbase/mcmaterial.py:37-63. */
@@ -199,6 +206,8 @@ typedef struct xo_mcvox_cfg {
     xo_fluence fluence;
     xo_detector top, bottom, specular;
     xo_material materials[XO_MAX_MATERIALS];
+    uint64_t packet_seed;       /* XO_FLAG_PACKET_KEYED only                  */
+    uint64_t first_packet;      /* XO_FLAG_PACKET_KEYED only                  */
 } xo_mcvox_cfg;
 
 /* Run report; mirrors Mc.run_report (xopto/mcml/mc.py:329-332, 1003-1016)
@@ -265,7 +274,7 @@ int xo_device_count(void);
  * Writes up to n values in the order: xo_pf, xo_layer, xo_source_line,
  * xo_source_gaussian, xo_detector, xo_fluence, xo_mcml_cfg, xo_material,
  * xo_voxel_cfg, xo_mcvox_cfg, xo_run_report, xo_batch_instance,
- * xo_batch_opts. Returns the count written. */
+ * xo_batch_opts, xo_serve_opts, xo_serve_stats. Returns the count written. */
 int xo_struct_sizes(uint32_t *out, int n);
 
 /* ---------------- seeds (host) ---------------- */
@@ -353,6 +362,83 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *instances,
  * (seed, p), x a valid state (0 <= c < a-1, x < 2^32-1) from a second hash.
  * Host copy of the device function, for checks. */
 int xo_packet_seed(uint64_t seed, uint64_t p, uint64_t *x, uint32_t *a);
+
+/* ---------------- serving (device) ---------------- */
+
+/* A server keeps simulation requests resident on one device and runs them
+ * by continuous batching. No counterpart in the reference, which runs one
+ * Mc.run at a time per OpenCL queue (xopto/mcml/mc.py:730-1018); every
+ * request's integers equal those of the corresponding xo_batch_run /
+ * keyed xo_mcvox_run, whatever else is served with it. */
+typedef struct xo_server xo_server;
+
+typedef struct xo_serve_opts {
+    uint64_t accum_capacity;    /* u64 accumulator bins of all requests      */
+    uint64_t lut_capacity;      /* LUT floats of all requests                */
+    uint64_t voxel_capacity;    /* voxels (one byte each) of all requests    */
+    uint64_t tick_packets;      /* layered packets per tick, shared fairly   */
+    uint64_t vox_tick_packets;  /* packets per voxel request per tick        */
+    uint32_t instance_slots;    /* layered requests resident at once         */
+    uint32_t max_items;         /* work-queue items per tick                 */
+    uint32_t vox_lanes;         /* lane cap of a voxel launch                */
+    uint32_t chunk;             /* packets per work-queue item               */
+    uint32_t block;             /* threads per block of the batch kernel     */
+    uint32_t blocks_per_cu;     /* 0: as many as fit                          */
+    float w_min;                /* roulette threshold (layered requests)     */
+    float lottery_chance;
+    int32_t use_lottery;
+    int32_t use_graph;          /* replay a captured graph per tick          */
+} xo_serve_opts; /* 80 B */
+
+typedef struct xo_serve_stats {
+    uint64_t admitted, completed, cancelled;
+    uint64_t ticks;             /* ticks retired                             */
+    uint64_t packets;           /* packets of retired ticks                  */
+    uint64_t steps;             /* transport iterations of retired ticks     */
+    uint64_t accum_used, lut_used, voxel_used; /* arena occupancy now        */
+    double busy_ms;             /* device time of the layered launches       */
+} xo_serve_stats; /* 80 B */
+
+void xo_serve_default_opts(xo_serve_opts *opts);
+
+/* Allocate the arenas and tick buffers on ctx's device (opts NULL: the
+ * defaults). The server uses ctx's seed table; ctx must outlive it.
+ * Returns NULL on error. */
+xo_server *xo_serve_create(xo_ctx *ctx, const xo_serve_opts *opts);
+void xo_serve_destroy(xo_server *server);
+
+/* Admit a layered request (detector offsets relative to inst->accum_base,
+ * accum_count bins; lut/nlut its own LUT array). Returns 0 and the request
+ * id, 1 when the arenas or slots are full right now (retry after results
+ * are taken), negative on invalid input. */
+int xo_serve_admit_mcml(xo_server *server, const xo_batch_instance *inst,
+                        const float *lut, uint32_t nlut, uint64_t *id);
+
+/* Admit a voxelized request: cfg as for xo_mcvox_run, voxels int32[z][y][x],
+ * naccum accumulator bins, npackets packets keyed by seed. Returns as
+ * xo_serve_admit_mcml. */
+int xo_serve_admit_mcvox(xo_server *server, const xo_mcvox_cfg *cfg,
+                         const int32_t *voxels, uint64_t nvoxels, const float *lut,
+                         uint32_t nlut, uint64_t naccum, uint64_t npackets,
+                         uint64_t seed, uint64_t *id);
+
+/* Retire finished ticks and queue new ones (at most two in flight);
+ * never blocks on the device. *active: requests not yet complete. */
+int xo_serve_step(xo_server *server, uint32_t *active);
+
+/* Block until every queued tick has finished, then retire them. */
+int xo_serve_wait(xo_server *server);
+
+/* Ids of requests completed since the last poll (up to max). */
+int xo_serve_poll(xo_server *server, uint64_t *ids, uint32_t max, uint32_t *n);
+
+/* Copy a completed request's naccum bins to the host and release it. */
+int xo_serve_result(xo_server *server, uint64_t id, uint64_t *accum, uint64_t naccum);
+
+/* Drop a request; its memory is released once no tick holds its work. */
+int xo_serve_cancel(xo_server *server, uint64_t id);
+
+int xo_serve_get_stats(xo_server *server, xo_serve_stats *stats);
 
 #ifdef __cplusplus
 }

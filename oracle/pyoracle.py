@@ -38,12 +38,13 @@ def lib():
         l.oracle_mcml_run.argtypes = [
             POINTER(N.XoMcmlCfg), POINTER(c_float), c_uint32,
             POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int, c_int,
-            POINTER(c_uint64), c_uint64, POINTER(c_uint64)]
+            POINTER(c_uint64), c_uint64, POINTER(c_uint64), POINTER(c_uint32)]
         l.oracle_mcvox_run.restype = c_int
         l.oracle_mcvox_run.argtypes = [
             POINTER(N.XoMcvoxCfg), POINTER(c_int32), POINTER(c_float),
             c_uint32, POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int,
-            c_int, POINTER(c_uint64), c_uint64, POINTER(c_uint64)]
+            c_int, POINTER(c_uint64), c_uint64, POINTER(c_uint64),
+            POINTER(c_uint32)]
         l.oracle_rng.restype = c_int
         l.oracle_rng.argtypes = [POINTER(c_uint64), c_uint32, c_uint32,
                                  POINTER(c_float)]
@@ -135,8 +136,16 @@ def rng(x: int, a: int, n: int):
     return out, int(xs.value)
 
 
+def _keyed_table(keyed):
+    if not keyed:
+        return None
+    t = np.ascontiguousarray(N.safeprime_table(N.XO_NUM_PRIMES)[1:])
+    return t
+
+
 def mcml_run(cfg, lut, rng_x, rng_a, npackets: int, naccum: int,
-             math_mode: int = MATH_LIBM, nthreads: int = 0):
+             math_mode: int = MATH_LIBM, nthreads: int = 0,
+             keyed: bool = False):
     """Static-schedule run on len(rng_x) lanes. Returns
     (accum, counts[steps, boundary, crossings, deposits], rng_x_after)."""
     x = np.ascontiguousarray(rng_x, dtype=np.uint64).copy()
@@ -147,18 +156,21 @@ def mcml_run(cfg, lut, rng_x, rng_a, npackets: int, naccum: int,
     if lut is not None and len(lut):
         lut = np.ascontiguousarray(lut, dtype=np.float32)
         lut_p = _p(lut, c_float)
+    table = _keyed_table(keyed)
     rc = lib().oracle_mcml_run(ctypes.byref(cfg), lut_p, c_uint32(x.size),
                                _p(x, c_uint64), _p(a, c_uint32),
                                c_uint64(int(npackets)), c_int(math_mode),
                                c_int(nthreads), _p(accum, c_uint64),
-                               c_uint64(naccum), _p(counts, c_uint64))
+                               c_uint64(naccum), _p(counts, c_uint64),
+                               None if table is None else _p(table, c_uint32))
     if rc != 0:
         raise RuntimeError('oracle_mcml_run failed ({})'.format(rc))
     return accum, counts, x
 
 
 def mcvox_run(cfg, voxels, lut, rng_x, rng_a, npackets: int, naccum: int,
-              math_mode: int = MATH_LIBM, nthreads: int = 0):
+              math_mode: int = MATH_LIBM, nthreads: int = 0,
+              keyed: bool = False):
     x = np.ascontiguousarray(rng_x, dtype=np.uint64).copy()
     a = np.ascontiguousarray(rng_a, dtype=np.uint32)
     vox = np.ascontiguousarray(voxels, dtype=np.int32)
@@ -168,12 +180,14 @@ def mcvox_run(cfg, voxels, lut, rng_x, rng_a, npackets: int, naccum: int,
     if lut is not None and len(lut):
         lut = np.ascontiguousarray(lut, dtype=np.float32)
         lut_p = _p(lut, c_float)
+    table = _keyed_table(keyed)
     rc = lib().oracle_mcvox_run(ctypes.byref(cfg), _p(vox, c_int32), lut_p,
                                 c_uint32(x.size), _p(x, c_uint64),
                                 _p(a, c_uint32), c_uint64(int(npackets)),
                                 c_int(math_mode), c_int(nthreads),
                                 _p(accum, c_uint64), c_uint64(naccum),
-                                _p(counts, c_uint64))
+                                _p(counts, c_uint64),
+                                None if table is None else _p(table, c_uint32))
     if rc != 0:
         raise RuntimeError('oracle_mcvox_run failed ({})'.format(rc))
     return accum, counts, x

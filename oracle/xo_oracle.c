@@ -12,6 +12,11 @@
  * the configurations use, each function citing the reference lines it
  * follows. Packet state is binary32, accumulators are exact uint64 sums.
  *
+ * keyed_table (may be NULL): with the seed-table multipliers of rows
+ * 1..499999, packet p of a run draws from xo_packet_state(cfg->packet_seed,
+ * cfg->first_packet + p) instead of its lane's stream (the engine's
+ * XO_FLAG_PACKET_KEYED runs).
+ *
  * Deliberate, documented differences from the reference text:
  *  - packets are assigned statically (lane g runs packets g, g+L, ...) instead
  *    of through the global atomic counter; every lane still owns one MWC
@@ -442,7 +447,8 @@ static inline uint64_t o_lane_packets(uint64_t n, uint32_t lanes, uint32_t g) {
 int oracle_mcml_run(const xo_mcml_cfg *cfg, const float *lut,
                     uint32_t lanes, uint64_t *rng_x, const uint32_t *rng_a,
                     uint64_t npackets, int math_mode, int nthreads,
-                    uint64_t *accum, uint64_t naccum, uint64_t *counts_out) {
+                    uint64_t *accum, uint64_t naccum, uint64_t *counts_out,
+                    const uint32_t *keyed_table) {
     if (cfg->num_layers < 3 || cfg->num_layers > XO_MAX_LAYERS || lanes == 0)
         return -1;
     memset(accum, 0, naccum * sizeof(uint64_t));
@@ -455,9 +461,14 @@ int oracle_mcml_run(const xo_mcml_cfg *cfg, const float *lut,
         s.x = rng_x[g];
         s.a = rng_a[g];
         uint64_t n = o_lane_packets(npackets, lanes, (uint32_t)g);
-        for (uint64_t k = 0; k < n; ++k)
+        for (uint64_t k = 0; k < n; ++k) {
+            if (keyed_table) /* packet g + k*lanes of the keyed run */
+                xo_packet_state(cfg->packet_seed,
+                                cfg->first_packet + (uint64_t)g + k * lanes,
+                                keyed_table, &s.x, &s.a);
             o_mcml_packet(cfg, lut, &s, accum, &c, math_mode);
-        rng_x[g] = s.x;
+        }
+        if (!keyed_table) rng_x[g] = s.x;
         st += c.steps;
         sb += c.boundary;
     }
@@ -751,7 +762,7 @@ int oracle_mcvox_run(const xo_mcvox_cfg *cfg, const int32_t *voxels,
                      const float *lut, uint32_t lanes, uint64_t *rng_x,
                      const uint32_t *rng_a, uint64_t npackets, int math_mode,
                      int nthreads, uint64_t *accum, uint64_t naccum,
-                     uint64_t *counts_out) {
+                     uint64_t *counts_out, const uint32_t *keyed_table) {
     if (cfg->num_materials < 1 || cfg->num_materials > XO_MAX_MATERIALS ||
         lanes == 0)
         return -1;
@@ -766,9 +777,14 @@ int oracle_mcvox_run(const xo_mcvox_cfg *cfg, const int32_t *voxels,
         s.x = rng_x[g];
         s.a = rng_a[g];
         uint64_t n = o_lane_packets(npackets, lanes, (uint32_t)g);
-        for (uint64_t k = 0; k < n; ++k)
+        for (uint64_t k = 0; k < n; ++k) {
+            if (keyed_table)
+                xo_packet_state(cfg->packet_seed,
+                                cfg->first_packet + (uint64_t)g + k * lanes,
+                                keyed_table, &s.x, &s.a);
             o_mcvox_packet(&V, &s, accum, &c, math_mode);
-        rng_x[g] = s.x;
+        }
+        if (!keyed_table) rng_x[g] = s.x;
         st += c.steps;
         sb += c.boundary;
         sc += c.crossings;

@@ -34,6 +34,7 @@ XO_SRC_LINE, XO_SRC_GAUSSIAN = 0, 1
 XO_METHOD_AW = 0
 XO_FLAG_STATIC_SCHEDULE = 1 << 0
 XO_FLAG_NO_LDS_BINS = 1 << 1
+XO_FLAG_PACKET_KEYED = 1 << 2
 
 f3 = c_float * 3
 
@@ -83,7 +84,8 @@ class XoMcmlCfg(ctypes.Structure):
                 ('reserved', c_uint32), ('source', XoSourceLine),
                 ('top', XoDetector), ('bottom', XoDetector),
                 ('specular', XoDetector),
-                ('layers', XoLayer * XO_MAX_LAYERS)]
+                ('layers', XoLayer * XO_MAX_LAYERS),
+                ('packet_seed', c_uint64), ('first_packet', c_uint64)]
 
 
 class XoMaterial(ctypes.Structure):
@@ -105,7 +107,8 @@ class XoMcvoxCfg(ctypes.Structure):
                 ('line', XoSourceLine), ('gaussian', XoSourceGaussian),
                 ('fluence', XoFluence), ('top', XoDetector),
                 ('bottom', XoDetector), ('specular', XoDetector),
-                ('materials', XoMaterial * XO_MAX_MATERIALS)]
+                ('materials', XoMaterial * XO_MAX_MATERIALS),
+                ('packet_seed', c_uint64), ('first_packet', c_uint64)]
 
 
 class XoBatchInstance(ctypes.Structure):
@@ -125,6 +128,28 @@ class XoBatchOpts(ctypes.Structure):
                 ('use_lottery', c_int32), ('reserved', c_uint32)]
 
 
+class XoServeOpts(ctypes.Structure):
+    _fields_ = [('accum_capacity', c_uint64), ('lut_capacity', c_uint64),
+                ('voxel_capacity', c_uint64), ('tick_packets', c_uint64),
+                ('vox_tick_packets', c_uint64), ('instance_slots', c_uint32),
+                ('max_items', c_uint32), ('vox_lanes', c_uint32),
+                ('chunk', c_uint32), ('block', c_uint32),
+                ('blocks_per_cu', c_uint32), ('w_min', c_float),
+                ('lottery_chance', c_float), ('use_lottery', c_int32),
+                ('use_graph', c_int32)]
+
+
+class XoServeStats(ctypes.Structure):
+    _fields_ = [('admitted', c_uint64), ('completed', c_uint64),
+                ('cancelled', c_uint64), ('ticks', c_uint64),
+                ('packets', c_uint64), ('steps', c_uint64),
+                ('accum_used', c_uint64), ('lut_used', c_uint64),
+                ('voxel_used', c_uint64), ('busy_ms', c_double)]
+
+    def todict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class XoRunReport(ctypes.Structure):
     _fields_ = [('packets', c_uint64), ('steps', c_uint64),
                 ('boundary_steps', c_uint64), ('voxel_crossings', c_uint64),
@@ -140,7 +165,7 @@ class XoRunReport(ctypes.Structure):
 STRUCTS_IN_ABI_ORDER = [XoPf, XoLayer, XoSourceLine, XoSourceGaussian,
                         XoDetector, XoFluence, XoMcmlCfg, XoMaterial,
                         XoVoxelCfg, XoMcvoxCfg, XoRunReport, XoBatchInstance,
-                        XoBatchOpts]
+                        XoBatchOpts, XoServeOpts, XoServeStats]
 
 # Every entry point of include/xomc.h: name -> (restype, argtypes)
 PROTOTYPES = {
@@ -176,6 +201,24 @@ PROTOTYPES = {
                              c_uint64, POINTER(c_float), c_uint32, c_uint64,
                              c_uint32, c_uint32, POINTER(c_uint64), c_uint64,
                              POINTER(XoRunReport)]),
+    'xo_serve_default_opts': (None, [POINTER(XoServeOpts)]),
+    'xo_serve_create': (c_void_p, [c_void_p, POINTER(XoServeOpts)]),
+    'xo_serve_destroy': (None, [c_void_p]),
+    'xo_serve_admit_mcml': (c_int, [c_void_p, POINTER(XoBatchInstance),
+                                    POINTER(c_float), c_uint32,
+                                    POINTER(c_uint64)]),
+    'xo_serve_admit_mcvox': (c_int, [c_void_p, POINTER(XoMcvoxCfg),
+                                     POINTER(c_int32), c_uint64,
+                                     POINTER(c_float), c_uint32, c_uint64,
+                                     c_uint64, c_uint64, POINTER(c_uint64)]),
+    'xo_serve_step': (c_int, [c_void_p, POINTER(c_uint32)]),
+    'xo_serve_wait': (c_int, [c_void_p]),
+    'xo_serve_poll': (c_int, [c_void_p, POINTER(c_uint64), c_uint32,
+                              POINTER(c_uint32)]),
+    'xo_serve_result': (c_int, [c_void_p, c_uint64, POINTER(c_uint64),
+                                c_uint64]),
+    'xo_serve_cancel': (c_int, [c_void_p, c_uint64]),
+    'xo_serve_get_stats': (c_int, [c_void_p, POINTER(XoServeStats)]),
 }
 
 _lib = None

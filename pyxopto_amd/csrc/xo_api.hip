@@ -49,45 +49,10 @@ void clear_error() { g_error.clear(); }
 
 } // namespace xo
 
-#define XO_HIP(call)                                                                  \
-    do {                                                                              \
-        hipError_t e_ = (call);                                                       \
-        if (e_ != hipSuccess) {                                                       \
-            xo::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),      \
-                          __FILE__, __LINE__);                                        \
-            return -1;                                                                \
-        }                                                                             \
-    } while (0)
 
-struct xo_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t max_lanes = 0;
-    uint32_t cus = 0;
-    uint64_t *rng_x = nullptr;
-    uint32_t *rng_a = nullptr;
-    unsigned long long *counters = nullptr; /* [0] packet counter, [1..] stats */
-    unsigned long long *accum = nullptr;
-    size_t accum_cap = 0;
-    float *lut = nullptr;
-    size_t lut_cap = 0;
-    uint8_t *voxels = nullptr;
-    size_t voxels_cap = 0;
-    int32_t vox_shape[3] = {0, 0, 0};
-    bool vox_valid = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    /* batch buffers */
-    uint32_t *table = nullptr;          /* seed multipliers, rows 1..      */
-    void *batch_layers = nullptr;
-    size_t batch_layers_cap = 0;
-    void *batch_hdr = nullptr;
-    size_t batch_hdr_cap = 0;
-    void *batch_items = nullptr;
-    size_t batch_items_cap = 0;
-    std::mutex mu;
-};
+#include "xo_ctx.h"
 
-namespace {
+namespace xo {
 
 int ensure_buffer(void **ptr, size_t *cap, size_t bytes) {
     if (bytes <= *cap && *ptr) return 0;
@@ -132,6 +97,20 @@ int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint
     return 0;
 }
 
+/* The seed-table multipliers (rows 1..) on the device, for keyed packets. */
+int ensure_table(xo_ctx *ctx) {
+    if (ctx->table) return 0;
+    std::vector<uint32_t> fora(XO_NUM_PRIMES);
+    if (xo::table_get(XO_NUM_PRIMES, fora.data()) != 0) {
+        xo::set_error("safe-prime table unavailable");
+        return -1;
+    }
+    XO_HIP(hipMalloc(&ctx->table, sizeof(uint32_t) * XO_SEED_TABLE_ROWS));
+    XO_HIP(hipMemcpy(ctx->table, fora.data() + 1, sizeof(uint32_t) * XO_SEED_TABLE_ROWS,
+                     hipMemcpyHostToDevice));
+    return 0;
+}
+
 uint32_t choose_lanes(xo_ctx *ctx, uint32_t requested, int blocks_per_cu) {
     uint32_t lanes = requested;
     if (lanes == 0) {
@@ -148,7 +127,14 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
         .count();
 }
 
-} // namespace
+} // namespace xo
+
+using xo::check_detector;
+using xo::check_pf;
+using xo::choose_lanes;
+using xo::ensure_buffer;
+using xo::ensure_table;
+using xo::ms_since;
 
 extern "C" {
 
@@ -170,7 +156,8 @@ int xo_struct_sizes(uint32_t *out, int n) {
         (uint32_t)sizeof(xo_mcml_cfg),    (uint32_t)sizeof(xo_material),
         (uint32_t)sizeof(xo_voxel_cfg),   (uint32_t)sizeof(xo_mcvox_cfg),
         (uint32_t)sizeof(xo_run_report),  (uint32_t)sizeof(xo_batch_instance),
-        (uint32_t)sizeof(xo_batch_opts)};
+        (uint32_t)sizeof(xo_batch_opts),  (uint32_t)sizeof(xo_serve_opts),
+        (uint32_t)sizeof(xo_serve_stats)};
     int count = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (n < count) count = n;
     for (int i = 0; i < count; ++i) out[i] = sizes[i];
@@ -357,9 +344,11 @@ int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t 
     if (ensure_buffer((void **)&ctx->accum, &ctx->accum_cap, naccum * 8)) return -1;
     if (nlut && ensure_buffer((void **)&ctx->lut, &ctx->lut_cap, (size_t)nlut * 4)) return -1;
 
+    if ((flags & XO_FLAG_PACKET_KEYED) && ensure_table(ctx)) return -1;
     xo::McmlParams P;
     std::memset(&P, 0, sizeof(P));
     P.cfg = *cfg;
+    P.table = ctx->table;
     P.lut = nlut ? ctx->lut : nullptr;
     P.rng_x = ctx->rng_x;
     P.rng_a = ctx->rng_a;
@@ -486,9 +475,11 @@ int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
     if (ensure_buffer((void **)&ctx->accum, &ctx->accum_cap, naccum * 8)) return -1;
     if (nlut && ensure_buffer((void **)&ctx->lut, &ctx->lut_cap, (size_t)nlut * 4)) return -1;
 
+    if ((flags & XO_FLAG_PACKET_KEYED) && ensure_table(ctx)) return -1;
     xo::McvoxParams P;
     std::memset(&P, 0, sizeof(P));
     P.cfg = *cfg;
+    P.table = ctx->table;
     P.voxels = ctx->voxels;
     P.lut = nlut ? ctx->lut : nullptr;
     P.rng_x = ctx->rng_x;
@@ -624,16 +615,7 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     XO_HIP(hipSetDevice(ctx->device));
-    if (!ctx->table) {
-        std::vector<uint32_t> fora(XO_NUM_PRIMES);
-        if (xo::table_get(XO_NUM_PRIMES, fora.data()) != 0) {
-            xo::set_error("xo_batch_run: safe-prime table unavailable");
-            return -1;
-        }
-        XO_HIP(hipMalloc(&ctx->table, sizeof(uint32_t) * XO_SEED_TABLE_ROWS));
-        XO_HIP(hipMemcpy(ctx->table, fora.data() + 1, sizeof(uint32_t) * XO_SEED_TABLE_ROWS,
-                         hipMemcpyHostToDevice));
-    }
+    if (ensure_table(ctx)) return -1;
     const size_t lbytes = layers.size() * sizeof(xo::LayerL);
     const size_t hbytes = hdr.size() * sizeof(xo::BatchHdr);
     const size_t ibytes = (items.empty() ? 1 : items.size()) * sizeof(xo::BatchItem);

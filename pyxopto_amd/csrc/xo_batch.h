@@ -44,6 +44,7 @@ struct BatchParams {
     unsigned int *item_counter;
     unsigned long long *accum;
     unsigned long long *stats;
+    const uint32_t *nitems_ptr;  /* if set, the item count is read here    */
     uint32_t nitems;
     float w_min;
     float chance;
@@ -53,6 +54,8 @@ struct BatchParams {
 hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,
                         hipStream_t stream);
 hipError_t occupancy_batch(uint32_t block, int *blocks_per_cu);
+/* Zero n 32-bit words (a kernel, so it can be recorded in a graph). */
+hipError_t launch_zero_u32(uint32_t *ptr, uint32_t n, hipStream_t stream);
 
 /* Host conversion of one layer into the device layout. */
 inline LayerL layer_to_device(const xo_layer &L) {

@@ -38,6 +38,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
     __shared__ BatchBlockStats sstats;
     if (threadIdx.x < XO_STAT_COUNT) sstats.v[threadIdx.x] = 0ull;
     __syncthreads();
+    const uint32_t nitems = P.nitems_ptr ? *P.nitems_ptr : P.nitems;
 
     const Bins bins{nullptr, P.accum};
     const uint32_t lane = lane_id();
@@ -74,7 +75,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
                 uint32_t it = 0;
                 if (lane == 0) it = atomicAdd(P.item_counter, 1u);
                 it = (uint32_t)__builtin_amdgcn_readlane((int)it, 0);
-                if (it >= P.nitems) {
+                if (it >= nitems) {
                     queue_empty = true;
                     break;
                 }
@@ -182,6 +183,17 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
 hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,
                         hipStream_t stream) {
     hipLaunchKernelGGL(batch_kernel, dim3(blocks), dim3(block), 0, stream, P);
+    return hipGetLastError();
+}
+
+__global__ void zero_u32_kernel(uint32_t *ptr, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ptr[i] = 0u;
+}
+
+hipError_t launch_zero_u32(uint32_t *ptr, uint32_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, ptr, n);
     return hipGetLastError();
 }
 

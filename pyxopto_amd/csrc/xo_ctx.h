@@ -1,0 +1,68 @@
+/*
+ * xo_ctx.h — the context object behind the opaque xo_ctx handle and the
+ * host helpers shared by the translation units of libxomc.so.
+ */
+#ifndef XO_CTX_H
+#define XO_CTX_H
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <mutex>
+
+#include "../../include/xomc.h"
+#include "xo_internal.h"
+
+/* Return -1 from the enclosing function, with the message set, when a HIP
+ * call fails. */
+#define XO_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            xo::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),      \
+                          __FILE__, __LINE__);                                        \
+            return -1;                                                                \
+        }                                                                             \
+    } while (0)
+
+struct xo_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t max_lanes = 0;
+    uint32_t cus = 0;
+    uint64_t *rng_x = nullptr;
+    uint32_t *rng_a = nullptr;
+    unsigned long long *counters = nullptr; /* [0] packet counter, [1..] stats */
+    unsigned long long *accum = nullptr;
+    size_t accum_cap = 0;
+    float *lut = nullptr;
+    size_t lut_cap = 0;
+    uint8_t *voxels = nullptr;
+    size_t voxels_cap = 0;
+    int32_t vox_shape[3] = {0, 0, 0};
+
+    bool vox_valid = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    /* batch buffers */
+    uint32_t *table = nullptr;          /* seed multipliers, rows 1..      */
+    void *batch_layers = nullptr;
+    size_t batch_layers_cap = 0;
+    void *batch_hdr = nullptr;
+    size_t batch_hdr_cap = 0;
+    void *batch_items = nullptr;
+    size_t batch_items_cap = 0;
+    std::mutex mu;
+};
+
+namespace xo {
+
+int ensure_buffer(void **ptr, size_t *cap, size_t bytes);
+int ensure_table(xo_ctx *ctx);
+int check_detector(const char *name, const xo_detector &d, uint64_t naccum, uint64_t *end);
+int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint32_t idx);
+uint32_t choose_lanes(xo_ctx *ctx, uint32_t requested, int blocks_per_cu);
+double ms_since(std::chrono::steady_clock::time_point t0);
+
+} // namespace xo
+
+#endif /* XO_CTX_H */

@@ -34,6 +34,7 @@ namespace xo {
 
 struct McmlParams {
     xo_mcml_cfg cfg;
+    const uint32_t *table; /* seed-table multipliers (XO_FLAG_PACKET_KEYED) */
     const float *lut;
     uint64_t *rng_x;
     const uint32_t *rng_a;
@@ -49,6 +50,7 @@ struct McmlParams {
 
 struct McvoxParams {
     xo_mcvox_cfg cfg;
+    const uint32_t *table; /* seed-table multipliers (XO_FLAG_PACKET_KEYED) */
     const uint8_t *voxels; /* material index per voxel, [z][y][x] */
     const float *lut;
     uint64_t *rng_x;

@@ -16,6 +16,7 @@
 
 #include "xo_device.h"
 #include "xo_kernels.h"
+#include "xo_seedhash.h"
 
 namespace xo {
 
@@ -79,12 +80,15 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     const uint32_t gid = blockIdx.x * XO_BLOCK + tid;
     const bool lane_ok = gid < P.lanes;
     const bool static_sched = (P.flags & XO_FLAG_STATIC_SCHEDULE) != 0u;
+    const bool keyed = (P.flags & XO_FLAG_PACKET_KEYED) != 0u;
     uint64_t x = 0;
     uint32_t a = 0;
     uint64_t my_n = 0, my_k = 0;
     if (lane_ok) {
-        x = P.rng_x[gid];
-        a = P.rng_a[gid];
+        if (!keyed) {
+            x = P.rng_x[gid];
+            a = P.rng_a[gid];
+        }
         if (static_sched)
             my_n = P.npackets / P.lanes + ((uint64_t)gid < P.npackets % P.lanes ? 1u : 0u);
     }
@@ -107,9 +111,11 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         /* ---- refill lanes whose packet ended ---- */
         bool want = active && !alive;
         bool launch = false;
+        uint64_t pkt_index = 0;
         if (static_sched) {
             if (want) {
                 if (my_k < my_n) {
+                    pkt_index = (uint64_t)gid + my_k * (uint64_t)P.lanes;
                     ++my_k;
                     launch = true;
                 } else {
@@ -126,14 +132,19 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 base = readlane_u64(base, leader);
                 if (want) {
                     uint64_t idx = base + prefix_count(m);
-                    if (idx < P.npackets)
+                    if (idx < P.npackets) {
                         launch = true;
-                    else
+                        pkt_index = idx;
+                    } else {
                         active = false;
+                    }
                 }
             }
         }
         if (launch) {
+            if (keyed)
+                xo_packet_state(P.cfg.packet_seed, P.cfg.first_packet + pkt_index, P.table, &x,
+                                &a);
             /* Line source, xopto/mcml/mcsource/line.py:95-110 */
             const xo_source_line &src = P.cfg.source;
             w = 1.0f - src.reflectance;
@@ -203,7 +214,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     }
 
     if (lane_ok) {
-        P.rng_x[gid] = x;
+        if (!keyed) P.rng_x[gid] = x;
         if (st[XO_STAT_PACKETS]) st[XO_STAT_THREADS] = 1;
     }
     stats_flush(&sstats, st, P.stats);
@@ -313,12 +324,15 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     const uint32_t gid = blockIdx.x * XO_BLOCK + tid;
     const bool lane_ok = gid < P.lanes;
     const bool static_sched = (P.flags & XO_FLAG_STATIC_SCHEDULE) != 0u;
+    const bool keyed = (P.flags & XO_FLAG_PACKET_KEYED) != 0u;
     uint64_t x = 0;
     uint32_t a = 0;
     uint64_t my_n = 0, my_k = 0;
     if (lane_ok) {
-        x = P.rng_x[gid];
-        a = P.rng_a[gid];
+        if (!keyed) {
+            x = P.rng_x[gid];
+            a = P.rng_a[gid];
+        }
         if (static_sched)
             my_n = P.npackets / P.lanes + ((uint64_t)gid < P.npackets % P.lanes ? 1u : 0u);
     }
@@ -342,9 +356,11 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     for (;;) {
         bool want = active && !alive;
         bool launch = false;
+        uint64_t pkt_index = 0;
         if (static_sched) {
             if (want) {
                 if (my_k < my_n) {
+                    pkt_index = (uint64_t)gid + my_k * (uint64_t)P.lanes;
                     ++my_k;
                     launch = true;
                 } else {
@@ -361,14 +377,19 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 base = readlane_u64(base, leader);
                 if (want) {
                     uint64_t idx = base + prefix_count(m);
-                    if (idx < P.npackets)
+                    if (idx < P.npackets) {
                         launch = true;
-                    else
+                        pkt_index = idx;
+                    } else {
                         active = false;
+                    }
                 }
             }
         }
         if (launch) {
+            if (keyed)
+                xo_packet_state(P.cfg.packet_seed, P.cfg.first_packet + pkt_index, P.table, &x,
+                                &a);
             if (line_src) {
                 /* xopto/mcvox/mcsource/line.py:84-104 */
                 const xo_source_line &src = P.cfg.line;
@@ -575,7 +596,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     }
 
     if (lane_ok) {
-        P.rng_x[gid] = x;
+        if (!keyed) P.rng_x[gid] = x;
         if (st[XO_STAT_PACKETS]) st[XO_STAT_THREADS] = 1;
     }
     stats_flush(&sstats, st, P.stats);
