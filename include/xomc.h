@@ -410,7 +410,8 @@ void xo_serve_destroy(xo_server *server);
 /* Admit a layered request (detector offsets relative to inst->accum_base,
  * accum_count bins; lut/nlut its own LUT array). Returns 0 and the request
  * id, 1 when the arenas or slots are full right now (retry after results
- * are taken), negative on invalid input. */
+ * are taken), negative on invalid input or on a request larger than the
+ * server's arenas. */
 int xo_serve_admit_mcml(xo_server *server, const xo_batch_instance *inst,
                         const float *lut, uint32_t nlut, uint64_t *id);
 

@@ -189,6 +189,23 @@ int build_graph(xo_server *S, int k) {
     return 0;
 }
 
+/* A request that can never fit is an error, not a wait. */
+int check_capacity(xo_server *S, const char *fn, uint64_t naccum, uint64_t nlut,
+                   uint64_t nvox) {
+    if (naccum > S->accum_arena.capacity() || nlut > S->lut_arena.capacity() ||
+        nvox > S->vox_arena.capacity()) {
+        xo::set_error("%s: request needs %llu bins, %llu LUT floats, %llu voxels; the server "
+                      "holds %llu, %llu, %llu",
+                      fn, (unsigned long long)naccum, (unsigned long long)nlut,
+                      (unsigned long long)nvox,
+                      (unsigned long long)S->accum_arena.capacity(),
+                      (unsigned long long)S->lut_arena.capacity(),
+                      (unsigned long long)S->vox_arena.capacity());
+        return -1;
+    }
+    return 0;
+}
+
 void free_request_memory(xo_server *S, Request &R) {
     S->accum_arena.release(R.accum_off, R.accum_n);
     S->lut_arena.release(R.lut_off, R.lut_n);
@@ -286,7 +303,9 @@ int dispatch_tick(xo_server *S, int k) {
     TickSlot &T = S->ticks[k];
     std::vector<uint64_t> ml, vx;
     for (uint64_t id : S->order) {
-        Request &R = S->reqs[id];
+        auto it = S->reqs.find(id);
+        if (it == S->reqs.end()) continue;
+        const Request &R = it->second;
         if (R.cancelled || R.next >= R.npackets) continue;
         (R.kind == KIND_MCML ? ml : vx).push_back(id);
     }
@@ -549,6 +568,7 @@ int xo_serve_admit_mcml(xo_server *S, const xo_batch_instance *inst, const float
         if (xo::check_detector(names[k], det[k], naccum, &end)) return -1;
     }
     std::lock_guard<std::mutex> lock(S->mu);
+    if (check_capacity(S, "xo_serve_admit_mcml", naccum, nlut, 0)) return -1;
     XO_CHECK(hipSetDevice(S->ctx->device));
     if (S->free_slots.empty()) return 1;
     Request R;
@@ -603,7 +623,12 @@ int xo_serve_admit_mcml(xo_server *S, const xo_batch_instance *inst, const float
                                  S->s_io));
     XO_CHECK(hipStreamSynchronize(S->s_io));
     *id_out = R.id;
-    S->order.push_back(R.id);
+    if (R.npackets == 0) {
+        R.completed = true; /* nothing to run: complete at once */
+        S->done.push_back(R.id);
+    } else {
+        S->order.push_back(R.id);
+    }
     S->reqs[R.id] = R;
     S->stats.admitted++;
     return 0;
@@ -659,6 +684,7 @@ int xo_serve_admit_mcvox(xo_server *S, const xo_mcvox_cfg *cfg, const int32_t *v
         packed[i] = (uint8_t)voxels[i];
     }
     std::lock_guard<std::mutex> lock(S->mu);
+    if (check_capacity(S, "xo_serve_admit_mcvox", naccum, nlut, nvoxels)) return -1;
     XO_CHECK(hipSetDevice(S->ctx->device));
     Request R;
     R.kind = KIND_MCVOX;
@@ -689,7 +715,12 @@ int xo_serve_admit_mcvox(xo_server *S, const xo_mcvox_cfg *cfg, const int32_t *v
                                  S->s_io));
     XO_CHECK(hipStreamSynchronize(S->s_io));
     *id_out = R.id;
-    S->order.push_back(R.id);
+    if (R.npackets == 0) {
+        R.completed = true; /* nothing to run: complete at once */
+        S->done.push_back(R.id);
+    } else {
+        S->order.push_back(R.id);
+    }
     S->reqs[R.id] = R;
     S->stats.admitted++;
     return 0;

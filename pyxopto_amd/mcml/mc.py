@@ -106,21 +106,29 @@ class Mc(McWorker):
     def run(self, nphotons: int, out=None, wgsize: int = None,
             maxthreads: int = None, copyseeds: bool = False,
             exportsrc=None, verbose: bool = False,
-            static_schedule: bool = False):
+            static_schedule: bool = False, packet_seed: int = None):
         """Simulate nphotons packets; returns (trace, fluence, detectors).
 
         maxthreads caps the lanes (default: the persistent grid of the GPU).
         static_schedule=True makes lane g run packets g, g+L, ... so that the
         result is a deterministic function of the seeds and the lane count.
+        packet_seed: draw packet p from the generator keyed on
+        (packet_seed, p) instead of the per-lane streams, so the result does
+        not depend on the lanes or schedule (the batch and serving engines
+        key every packet this way).
         """
         if out is None:
             out = (None, None, None)
         nphotons = self._check_nphotons(nphotons)
         cfg, lut, naccum = self.pack()
+        flags = self._run_flags(static_schedule)
+        if packet_seed is not None:
+            cfg.packet_seed = int(packet_seed) & 0xFFFFFFFFFFFFFFFF
+            cfg.first_packet = 0
+            flags |= N.XO_FLAG_PACKET_KEYED
         ctx = self._context()
         lanes = 0 if maxthreads is None else int(maxthreads)
-        accum, rep = ctx.run_mcml(cfg, lut, nphotons, naccum, lanes,
-                                  self._run_flags(static_schedule))
+        accum, rep = ctx.run_mcml(cfg, lut, nphotons, naccum, lanes, flags)
         self._record_report(rep, nphotons)
         self._last_accum = accum
         detectors_res = out[2]

@@ -102,21 +102,40 @@ class Mc(McWorker):
 
     def run(self, nphotons: int, out=None, wgsize: int = None,
             maxthreads: int = None, copyseeds: bool = False, exportsrc=None,
-            verbose: bool = False, static_schedule: bool = False):
-        """Simulate nphotons packets; returns (trace, fluence, detectors)."""
+            verbose: bool = False, static_schedule: bool = False,
+            packet_seed: int = None):
+        """Simulate nphotons packets; returns (trace, fluence, detectors).
+        packet_seed: draw packet p from the generator keyed on
+        (packet_seed, p), independent of lanes and schedule."""
         if out is None:
             out = (None, None, None)
         nphotons = self._check_nphotons(nphotons)
         cfg, lut, naccum = self.pack()
+        flags = self._run_flags(static_schedule)
+        if packet_seed is not None:
+            cfg.packet_seed = int(packet_seed) & 0xFFFFFFFFFFFFFFFF
+            cfg.first_packet = 0
+            flags |= N.XO_FLAG_PACKET_KEYED
         upload = self._voxels.update_required() or \
             self._uploaded_shape != self._voxels.shape
         voxels = self._voxels.material if upload else None
         ctx = self._context()
         lanes = 0 if maxthreads is None else int(maxthreads)
         accum, rep = ctx.run_mcvox(cfg, voxels, lut, nphotons, naccum, lanes,
-                                   self._run_flags(static_schedule))
+                                   flags)
         self._uploaded_shape = self._voxels.shape
         self._record_report(rep, nphotons)
+        _, fluence_res, detectors_res = self.unpack(accum, nphotons, out)
+        if verbose:
+            print('McKernel processed {:,d} packets: {:.3f} ms'.format(
+                nphotons, rep.kernel_ms))
+        return None, fluence_res, detectors_res
+
+    def unpack(self, accum, nphotons: int, out=None):
+        """(None, fluence, detectors) from the integer accumulator array
+        of a run packed by the last pack() call."""
+        if out is None:
+            out = (None, None, None)
         fluence_res = out[1]
         if self._fluence is not None:
             if fluence_res is None:
@@ -133,7 +152,4 @@ class Mc(McWorker):
                 off, n = self.allocation(det)
                 getattr(detectors_res, name).update_data(
                     self, [accum[off:off + n]], nphotons)
-        if verbose:
-            print('McKernel processed {:,d} packets: {:.3f} ms'.format(
-                nphotons, rep.kernel_ms))
         return None, fluence_res, detectors_res

@@ -13,6 +13,10 @@
   n 1.0 with no absorption or scattering.
 - ``layered_batch``: 4096 layered instances with 1-5 turbid layers and
   uneven packet budgets for the batched engine (see its docstring).
+- ``skin_two_layer``: 64^3 voxels of 31.25 um (a 2 mm cube), epidermis
+  100 um over dermis (the skin_vessel_200 materials without the vessel),
+  Gaussian beam sigma 50 um, deposition in every voxel (64^3 u64 bins,
+  2 MiB); the voxel requests of the serving workload, 1e6 packets each.
 """
 import os
 
@@ -68,6 +72,35 @@ def skin_vessel(n: int = 200, voxel: float = 10e-6,
     return sim
 
 
+def skin_two_layer(n: int = 64, size: float = 2e-3,
+                   rnginit: int = 0x5EED000000000001,
+                   epidermis: float = 100e-6):
+    half = 0.5 * size
+    M = mcvox.mcmaterial
+    materials = M.Materials([
+        M.Material(n=1.0, mua=0.0, mus=0.0, pf=mcvox.mcpf.Hg(0.8)),
+        M.Material(n=1.34, mua=1600.0, mus=4e4, pf=mcvox.mcpf.Hg(0.8)),
+        M.Material(n=1.40, mua=50.0, mus=2.5e4, pf=mcvox.mcpf.Hg(0.9))])
+    G = mcvox.mcgeometry
+    voxels = G.Voxels(G.Axis(-half, half, n), G.Axis(-half, half, n),
+                      G.Axis(0.0, size, n))
+    fluence = mcvox.mcfluence.Fluence(G.Axis(-half, half, n),
+                                      G.Axis(-half, half, n),
+                                      G.Axis(0.0, size, n))
+    D = mcvox.mcdetector
+    det = D.Detectors(top=D.Radial(D.RadialAxis(0.0, half, 100)),
+                      bottom=D.Total(), specular=D.Total())
+    sim = mcvox.Mc(voxels, materials, mcvox.mcsource.GaussianBeam(50e-6), det,
+                   fluence=fluence, rnginit=rnginit)
+    z, _, _ = sim.voxels.meshgrid()
+    mat = sim.voxels.material
+    mat[...] = 2
+    mat[z <= epidermis] = 1
+    sim.voxels.update()
+    return sim
+
+
+VOXEL_REQUEST_PACKETS = 1_000_000
 BATCH_INSTANCES = 4096
 
 
