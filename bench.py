@@ -31,6 +31,16 @@ SLOTS_INTERIOR = 85 - 6 + 6 * 4         # 103
 SLOTS_BOUNDARY = 43 - 6 + 6 * 4         # 61
 SLOTS_VOXEL_EXTRA = 12 - 3 + 3 * 4      # 21
 
+SERVING_QPS = 400.0                     # requests/s of the serving run
+
+# HBM bytes per launch of the dominant kernel: rocprofv3 FETCH_SIZE and
+# WRITE_SIZE of one launch, each scaled by what the counter reads for a
+# 1 GiB device-to-device copy (profiles/batch_kernel_layered_batch_4096.md,
+# profiles/mcvox_kernel_skin_vessel_200.md). Offline figures: measured for
+# the kernels as they are in this tree.
+TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 5.327e10,
+                            'skin_vessel_200': 1.703e12}
+
 METRIC = ('photon packets/s (1 GPU) batched vs plain; p50/p99 solve latency '
           'at fixed QPS')
 
@@ -43,16 +53,22 @@ def algorithmic_slots(rep, voxel: bool) -> float:
     return float(slots)
 
 
-def make_workload(name: str, packets):
+def make_workload(name: str, packets, launch=None, lut_processes=None):
     from pyxopto_amd import workloads
     if name == 'layered_batch_4096':
         from pyxopto_amd.mcml.batch import Batch
         scale = 1.0 if not packets else packets / 8.77e8
-        insts = workloads.layered_batch(budget_scale=scale)
+        insts = workloads.layered_batch(
+            budget_scale=scale, processes=lut_processes,
+            cache_dir=os.path.join(REPO, 'pyxopto_amd', 'data'))
         batch = Batch(insts, rnginit=0x5EED000000000001)
         packed = batch.pack()
-        from pyxopto_amd.mcml.autotune import tuned_opts
-        opts = tuned_opts(batch, packed)
+        if launch:
+            chunk, block, bpc = (int(v) for v in launch.split(':'))
+            opts = batch.opts(chunk, block, bpc)
+        else:
+            from pyxopto_amd.mcml.autotune import tuned_opts
+            opts = tuned_opts(batch, packed)
         arr, lut, naccum = packed
         npk = sum(int(i.nphotons) for i in insts)
         ctx = batch.context
@@ -67,6 +83,7 @@ def make_workload(name: str, packets):
                   'launch': {'chunk': opts.chunk, 'block': opts.block,
                              'blocks_per_cu': opts.blocks_per_cu}}
         batch.bench_plain = lambda: plain_rate(batch, insts)
+        batch.bench_insts = insts
         return batch, npk, step, False, config
     if name == 'skin_vessel_200':
         sim = workloads.skin_vessel()
@@ -97,6 +114,86 @@ def make_workload(name: str, packets):
                   'detectors': 'radial 100 bins + total R/T'}
         return sim, npk, step, False, config
     raise SystemExit('unknown workload ' + name)
+
+
+def ceilings(ctx):
+    """Measured ceilings on this device: streaming copy bandwidth (1 GiB,
+    read + write) and random-address u64 atomic adds over a 64 MiB array."""
+    nbytes = 1 << 30
+    ms = ctx.probe_copy(nbytes, reps=10)
+    natom = 1 << 28
+    ams, total = ctx.probe_atomic_u64(64 << 20, natom, seed=7)
+    if total != natom:
+        raise RuntimeError('atomic probe lost updates: {} of {}'.format(
+            total, natom))
+    return {'copy_GBps': 2 * nbytes / (ms * 1e-3) / 1e9,
+            'atomic_u64_per_s': natom / (ams * 1e-3),
+            'atomic_u64_GBps': 8 * natom / (ams * 1e-3) / 1e9,
+            'probe': 'copy 1 GiB x10; 2^28 u64 atomics, 64 MiB array'}
+
+
+def serving(insts, qps: float, duration: float, seed: int = 20261203,
+            voxel_fraction: float = 0.1, saturation_requests: int = 400):
+    """Poisson arrivals at a fixed rate for `duration` seconds: 90 % of the
+    requests are instances of the layered batch (in a seeded random order),
+    10 % are 64^3 two-layer voxel simulations of 1e6 packets. Requests are
+    packed before the run; the clock runs from the first submit to the last
+    result. Latency is submit to result. Before it, a burst of requests
+    submitted at once measures the capacity of the same mix."""
+    import numpy as np
+    from pyxopto_amd import serve, workloads
+    from pyxopto_amd.mcml.batch import Batch
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(insts))
+    packed = []
+    for i in order:
+        arr, lut, _ = Batch([insts[i]], rnginit=0x5EED000000000001).pack()
+        packed.append((arr[0], lut))
+    vox = workloads.skin_two_layer()
+    vcfg, vlut, vnaccum = vox.pack()
+    vmat = vox.voxels.material
+    nvox = workloads.VOXEL_REQUEST_PACKETS
+    is_vox = rng.random(1 << 20) < voxel_fraction
+
+    def make(srv, k):
+        if is_vox[k % is_vox.size]:
+            return srv.submit_packed_voxel(vcfg, vmat, vlut, vnaccum, nvox,
+                                           seed=0xA5A5 + k)
+        inst, lut = packed[k % len(packed)]
+        return srv.submit_packed_layered(inst, lut)
+
+    with serve.Server() as srv:
+        # first use of the server (graph, tables) outside the clock
+        warm = [make(srv, k) for k in range(4)]
+        srv.drain(120)
+        # capacity: a burst of requests submitted at once
+        t0 = time.perf_counter()
+        burst = [make(srv, (1 << 19) + k) for k in range(saturation_requests)]
+        srv.drain(600)
+        t_burst = time.perf_counter() - t0
+        s0 = srv.stats()
+        reqs, span = serve.poisson_load(srv, lambda sv, k: make(sv, k + 4),
+                                        qps, duration, seed=seed)
+        s1 = srv.stats()
+    del warm
+    burst_packets = sum(r.npackets for r in burst)
+    packets = sum(r.npackets for r in reqs)
+    out = {'capacity': {'requests': len(burst),
+                        'requests_per_s': len(burst) / t_burst,
+                        'packets_per_s': burst_packets / t_burst},
+           'qps': qps, 'arrivals_s': duration, 'requests': len(reqs),
+           'voxel_requests': sum(r.kind == 1 for r in reqs),
+           'packets': packets, 'packets_per_s': packets / span,
+           'steps_per_s': (s1['steps'] - s0['steps']) / span,
+           'ticks': s1['ticks'] - s0['ticks'],
+           'mix': '90% layered-batch instances, 10% 64^3 two-layer voxel '
+                  'requests of 1e6 packets'}
+    out.update(serve.latency_summary(reqs))
+    for kind, name in ((0, 'layered'), (1, 'voxel')):
+        lat = serve.latency_summary([r for r in reqs if r.kind == kind])
+        out[name + '_p50_ms'] = lat['p50_ms']
+        out[name + '_p99_ms'] = lat['p99_ms']
+    return out
 
 
 def plain_rate(batch, insts, every: int = 16):
@@ -182,6 +279,17 @@ def main():
                              'slab_1mm'])
     ap.add_argument('--packets', type=int, default=None)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-plain', action='store_true',
+                    help='skip the one-launch-per-instance comparison')
+    ap.add_argument('--no-serving', action='store_true',
+                    help='skip the fixed-QPS serving run')
+    ap.add_argument('--qps', type=float, default=SERVING_QPS)
+    ap.add_argument('--serving-seconds', type=float, default=8.0)
+    ap.add_argument('--lut-processes', type=int, default=None,
+                    help='LUT builder processes (1 under a profiler: no '
+                         'fork after the GPU is initialised)')
+    ap.add_argument('--launch', default=None,
+                    help='CHUNK:BLOCK:BLOCKS_PER_CU instead of the autotuner')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this engine has no multi-GPU '
@@ -190,7 +298,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    sim, npk, step, voxel, config = make_workload(args.workload, args.packets)
+    sim, npk, step, voxel, config = make_workload(
+        args.workload, args.packets, args.launch, args.lut_processes)
     print('bench: workload {} packets/step {} warmup {} steps {}'.format(
         config['workload'], npk, args.warmup, args.steps), file=sys.stderr,
         flush=True)
@@ -234,19 +343,29 @@ def main():
             'peak': PEAK_SLOTS / 1e12,
             'unit': 'T lane-op slots/s',
             'frac': achieved / (PEAK_SLOTS / 1e12),
-            'traffic': None,
+            'traffic': TRAFFIC_BYTES_PER_LAUNCH.get(config['workload']),
         },
     }
+    ceil = ceilings(sim.context)
+    out['measured_ceilings'] = ceil
+    out['roofline']['hbm_copy_GBps'] = ceil['copy_GBps']
     if voxel:
         dep = sum(r.deposits for r in reps)
         out['atomic_deposits_per_s'] = dep / kernel_s
         out['atomic_GBps'] = dep * 8 / kernel_s / 1e9
+        out['atomic_frac_of_probe'] = dep / kernel_s / ceil['atomic_u64_per_s']
         out['voxel_crossings_per_s'] = sum(r.voxel_crossings
                                            for r in reps) / kernel_s
-    if hasattr(sim, 'bench_plain'):
+    if hasattr(sim, 'bench_plain') and not args.no_plain:
         plain = sim.bench_plain()
         out['plain_one_launch_per_instance'] = plain
         out['batched_vs_plain'] = value / plain['packets_per_s']
+    if hasattr(sim, 'bench_insts') and not args.no_serving:
+        print('bench: serving at {} requests/s for {} s'.format(
+            args.qps, args.serving_seconds), file=sys.stderr, flush=True)
+        out['serving'] = serving(sim.bench_insts, args.qps,
+                                 args.serving_seconds)
+        print('bench: serving done', file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(sim, voxel)
     print(json.dumps(out), flush=True)

@@ -441,6 +441,21 @@ int xo_serve_cancel(xo_server *server, uint64_t id);
 
 int xo_serve_get_stats(xo_server *server, xo_serve_stats *stats);
 
+/* ---------------- measured ceilings (device) ---------------- */
+
+/* Device-to-device copy of `bytes` (16-byte vectors, one grid-stride
+ * kernel), reps times after one warm-up: average ms per copy. The HBM
+ * bandwidth the memory-bound figures are compared against. No counterpart
+ * in the reference. */
+int xo_probe_copy(xo_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms_per_copy);
+
+/* natomics u64 atomic adds of 1 at hashed random addresses of a zeroed
+ * array of array_bytes: kernel ms, and the array's sum (== natomics when
+ * every atomic landed). The HBM/L2 atomic rate the deposit accumulators
+ * are compared against. No counterpart in the reference. */
+int xo_probe_atomic_u64(xo_ctx *ctx, uint64_t array_bytes, uint64_t natomics,
+                        uint64_t seed, double *ms, uint64_t *checksum);
+
 #ifdef __cplusplus
 }
 #endif

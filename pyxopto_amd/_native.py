@@ -219,6 +219,10 @@ PROTOTYPES = {
                                 c_uint64]),
     'xo_serve_cancel': (c_int, [c_void_p, c_uint64]),
     'xo_serve_get_stats': (c_int, [c_void_p, POINTER(XoServeStats)]),
+    'xo_probe_copy': (c_int, [c_void_p, c_uint64, c_uint32,
+                              POINTER(c_double)]),
+    'xo_probe_atomic_u64': (c_int, [c_void_p, c_uint64, c_uint64, c_uint64,
+                                    POINTER(c_double), POINTER(c_uint64)]),
 }
 
 _lib = None
@@ -425,6 +429,25 @@ class Context:
                                  _ptr(accum, c_uint64), c_uint64(naccum),
                                  ctypes.byref(rep)), 'xo_batch_run')
         return accum, rep
+
+    def probe_copy(self, nbytes: int, reps: int = 10) -> float:
+        """Average ms of one device-to-device copy of nbytes."""
+        ms = c_double()
+        check(lib().xo_probe_copy(self._h, c_uint64(int(nbytes)),
+                                  c_uint32(reps), ctypes.byref(ms)),
+              'xo_probe_copy')
+        return ms.value
+
+    def probe_atomic_u64(self, array_bytes: int, natomics: int,
+                         seed: int = 1):
+        """(kernel ms, array sum) of natomics random u64 atomic adds."""
+        ms, total = c_double(), c_uint64()
+        check(lib().xo_probe_atomic_u64(self._h, c_uint64(int(array_bytes)),
+                                        c_uint64(int(natomics)),
+                                        c_uint64(int(seed)), ctypes.byref(ms),
+                                        ctypes.byref(total)),
+              'xo_probe_atomic_u64')
+        return ms.value, int(total.value)
 
     def run_mcvox(self, cfg: XoMcvoxCfg, voxels, lut, npackets: int,
                   naccum: int, lanes: int = 0, flags: int = 0):

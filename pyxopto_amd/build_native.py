@@ -29,7 +29,8 @@ HIP_FLAGS = [
     '-Wall', '-Wno-unused-function',
 ]
 
-HIP_SOURCES = ['xo_kernels.hip', 'xo_batch.hip', 'xo_api.hip', 'xo_serve.hip']
+HIP_SOURCES = ['xo_kernels.hip', 'xo_batch.hip', 'xo_api.hip', 'xo_serve.hip',
+               'xo_probe.hip']
 CXX_SOURCES = ['xo_seeds.cpp']
 
 

@@ -1,0 +1,138 @@
+/*
+ * xo_probe.hip — the two measured ceilings the transport kernels are
+ * compared against: a streaming device-to-device copy (HBM bandwidth) and
+ * random-address u64 atomic adds over a large array (the HBM/L2 atomic
+ * rate the fluence and detector deposits run into). No counterpart in the
+ * reference.
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+
+#include "../../include/xomc.h"
+#include "xo_ctx.h"
+#include "xo_seedhash.h"
+
+namespace {
+
+__global__ void __launch_bounds__(256) copy_kernel(const uint4 *__restrict__ src,
+                                                   uint4 *__restrict__ dst, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
+/* Thread t adds 1 at (hash(seed, t, k) mod nelem) for k < per_thread, for
+ * the first total atomics overall. */
+__global__ void __launch_bounds__(256) atomic_kernel(unsigned long long *__restrict__ arr,
+                                                     uint64_t nelem, uint64_t total,
+                                                     uint32_t per_thread, uint64_t seed) {
+    const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t k = 0; k < per_thread; ++k) {
+        const uint64_t j = (uint64_t)k * nthreads + t;
+        if (j >= total) break;
+        const uint64_t h = xo_mix64(seed + j * 0x9E3779B97F4A7C15ull);
+        atomicAdd(&arr[h % nelem], 1ull);
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int xo_probe_copy(xo_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms_per_copy) {
+    xo::clear_error();
+    if (!ctx || !ms_per_copy || bytes < 16 || reps == 0) {
+        xo::set_error("xo_probe_copy: invalid argument");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    const uint64_t n = bytes / 16;
+    void *src = nullptr, *dst = nullptr;
+    XO_HIP(hipMalloc(&src, n * 16));
+    if (hipMalloc(&dst, n * 16) != hipSuccess) {
+        (void)hipFree(src);
+        xo::set_error("xo_probe_copy: out of device memory");
+        return -1;
+    }
+    int rc = 0;
+    const uint32_t blocks = ctx->cus * 8;
+    hipError_t e = hipMemsetAsync(src, 1, n * 16, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                           (const uint4 *)src, (uint4 *)dst, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                           (const uint4 *)src, (uint4 *)dst, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (e != hipSuccess) {
+        xo::set_error("xo_probe_copy: %s", hipGetErrorString(e));
+        rc = -1;
+    }
+    *ms_per_copy = (double)ms / reps;
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return rc;
+}
+
+int xo_probe_atomic_u64(xo_ctx *ctx, uint64_t array_bytes, uint64_t natomics, uint64_t seed,
+                        double *ms, uint64_t *checksum) {
+    xo::clear_error();
+    if (!ctx || !ms || !checksum || array_bytes < 8 || natomics == 0) {
+        xo::set_error("xo_probe_atomic_u64: invalid argument");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    const uint64_t nelem = array_bytes / 8;
+    unsigned long long *arr = nullptr;
+    XO_HIP(hipMalloc(&arr, nelem * 8));
+    const uint32_t blocks = ctx->cus * 8;
+    const uint64_t nthreads = (uint64_t)blocks * 256;
+    const uint64_t per = (natomics + nthreads - 1) / nthreads;
+    int rc = 0;
+    hipError_t e = hipMemsetAsync(arr, 0, nelem * 8, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(atomic_kernel, dim3(blocks), dim3(256), 0, ctx->stream, arr, nelem,
+                           natomics, (uint32_t)per, seed);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    float kms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1);
+    /* every atomic landed: the array sums to natomics */
+    uint64_t sum = 0;
+    if (e == hipSuccess) {
+        const uint64_t chunk = 1u << 20;
+        uint64_t *host = new uint64_t[chunk];
+        for (uint64_t off = 0; off < nelem && e == hipSuccess; off += chunk) {
+            uint64_t m = nelem - off < chunk ? nelem - off : chunk;
+            e = hipMemcpy(host, arr + off, m * 8, hipMemcpyDeviceToHost);
+            for (uint64_t i = 0; i < m && e == hipSuccess; ++i) sum += host[i];
+        }
+        delete[] host;
+    }
+    if (e != hipSuccess) {
+        xo::set_error("xo_probe_atomic_u64: %s", hipGetErrorString(e));
+        rc = -1;
+    }
+    *ms = kms;
+    *checksum = sum;
+    (void)hipFree(arr);
+    return rc;
+}
+
+} // extern "C"

@@ -105,6 +105,7 @@ struct Request {
 
 constexpr int kSlots = 2;
 constexpr int kVoxPerTick = 8;
+constexpr int kVoxStreams = 4; /* voxel launches of a tick overlap each other */
 constexpr int kCounterWords = 1 + XO_STAT_COUNT; /* u64 words per counter block */
 
 struct TickSlot {
@@ -116,7 +117,8 @@ struct TickSlot {
     unsigned long long *vox_counters = nullptr; /* kVoxPerTick blocks */
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t ev_ml = nullptr, ev_vox = nullptr, ev_start = nullptr;
+    hipEvent_t ev_ml = nullptr, ev_start = nullptr;
+    hipEvent_t ev_vox[kVoxStreams] = {};
     bool busy = false;
     uint32_t nitems = 0;
     std::vector<uint64_t> requests; /* ids with work in this tick */
@@ -129,7 +131,8 @@ struct TickSlot {
 struct xo_server {
     xo_ctx *ctx = nullptr;
     xo_serve_opts opts;
-    hipStream_t s_ml = nullptr, s_vox = nullptr, s_io = nullptr;
+    hipStream_t s_ml = nullptr, s_io = nullptr;
+    hipStream_t s_vox[kVoxStreams] = {};
     Arena accum_arena, lut_arena, vox_arena;
     unsigned long long *accum = nullptr;
     float *lut = nullptr;
@@ -222,13 +225,18 @@ int retire_ticks(xo_server *S, bool wait) {
         if (!T.busy) continue;
         if (wait) {
             XO_CHECK(hipEventSynchronize(T.ev_ml));
-            XO_CHECK(hipEventSynchronize(T.ev_vox));
+            for (int v = 0; v < kVoxStreams; ++v) XO_CHECK(hipEventSynchronize(T.ev_vox[v]));
         } else {
+            bool ready = true;
             hipError_t a = hipEventQuery(T.ev_ml);
-            hipError_t b = hipEventQuery(T.ev_vox);
-            if (a == hipErrorNotReady || b == hipErrorNotReady) continue;
-            if (a != hipSuccess) return fail_hip("hipEventQuery", a);
-            if (b != hipSuccess) return fail_hip("hipEventQuery", b);
+            if (a == hipErrorNotReady) ready = false;
+            else if (a != hipSuccess) return fail_hip("hipEventQuery", a);
+            for (int v = 0; v < kVoxStreams && ready; ++v) {
+                hipError_t b = hipEventQuery(T.ev_vox[v]);
+                if (b == hipErrorNotReady) ready = false;
+                else if (b != hipSuccess) return fail_hip("hipEventQuery", b);
+            }
+            if (!ready) continue;
         }
         unsigned long long c[kCounterWords * (1 + kVoxPerTick)];
         XO_CHECK(hipMemcpy(c, T.ml_counters, sizeof(unsigned long long) * kCounterWords,
@@ -267,8 +275,8 @@ int retire_ticks(xo_server *S, bool wait) {
     return 0;
 }
 
-int launch_vox(xo_server *S, TickSlot &T, Request &R, uint64_t first, uint64_t count,
-               unsigned long long *counters) {
+int launch_vox(xo_server *S, Request &R, uint64_t first, uint64_t count,
+               unsigned long long *counters, hipStream_t stream) {
     xo::McvoxParams P;
     std::memset(&P, 0, sizeof(P));
     P.cfg = R.vcfg;
@@ -292,9 +300,8 @@ int launch_vox(xo_server *S, TickSlot &T, Request &R, uint64_t first, uint64_t c
     uint64_t lanes = std::min<uint64_t>(std::min<uint64_t>(count, resident), S->opts.vox_lanes);
     lanes = std::max<uint64_t>(lanes, 64u);
     P.lanes = (uint32_t)lanes;
-    XO_CHECK(xo::launch_zero_u32((uint32_t *)counters, 2 * kCounterWords, S->s_vox));
-    XO_CHECK(xo::launch_mcvox(P, (uint32_t)((lanes + XO_BLOCK - 1) / XO_BLOCK), S->s_vox));
-    (void)T;
+    XO_CHECK(xo::launch_zero_u32((uint32_t *)counters, 2 * kCounterWords, stream));
+    XO_CHECK(xo::launch_mcvox(P, (uint32_t)((lanes + XO_BLOCK - 1) / XO_BLOCK), stream));
     return 0;
 }
 
@@ -370,8 +377,9 @@ int dispatch_tick(xo_server *S, int k) {
         if ((int)T.vox_ids.size() >= kVoxPerTick) break;
         Request &R = S->reqs[id];
         uint64_t q = std::min<uint64_t>(S->opts.vox_tick_packets, R.npackets - R.next);
-        if (launch_vox(S, T, R, R.next, q,
-                       T.vox_counters + kCounterWords * T.vox_ids.size()))
+        /* a request's launches stay on one stream, in packet order */
+        if (launch_vox(S, R, R.next, q, T.vox_counters + kCounterWords * T.vox_ids.size(),
+                       S->s_vox[id % kVoxStreams]))
             return -1;
         R.next += q;
         R.inflight++;
@@ -379,7 +387,7 @@ int dispatch_tick(xo_server *S, int k) {
         T.vox_ids.push_back(id);
         T.packets += q;
     }
-    XO_CHECK(hipEventRecord(T.ev_vox, S->s_vox));
+    for (int v = 0; v < kVoxStreams; ++v) XO_CHECK(hipEventRecord(T.ev_vox[v], S->s_vox[v]));
     T.busy = true;
     return 0;
 }
@@ -423,7 +431,8 @@ void xo_serve_destroy(xo_server *S) {
     if (!S) return;
     (void)hipSetDevice(S->ctx->device);
     if (S->s_ml) (void)hipStreamSynchronize(S->s_ml);
-    if (S->s_vox) (void)hipStreamSynchronize(S->s_vox);
+    for (auto s : S->s_vox)
+        if (s) (void)hipStreamSynchronize(s);
     for (auto &T : S->ticks) {
         if (T.exec) (void)hipGraphExecDestroy(T.exec);
         if (T.graph) (void)hipGraphDestroy(T.graph);
@@ -434,7 +443,8 @@ void xo_serve_destroy(xo_server *S) {
         if (T.ml_counters) (void)hipFree(T.ml_counters);
         if (T.vox_counters) (void)hipFree(T.vox_counters);
         if (T.ev_ml) (void)hipEventDestroy(T.ev_ml);
-        if (T.ev_vox) (void)hipEventDestroy(T.ev_vox);
+        for (auto ev : T.ev_vox)
+            if (ev) (void)hipEventDestroy(ev);
         if (T.ev_start) (void)hipEventDestroy(T.ev_start);
     }
     if (S->accum) (void)hipFree(S->accum);
@@ -443,7 +453,8 @@ void xo_serve_destroy(xo_server *S) {
     if (S->layers) (void)hipFree(S->layers);
     if (S->hdr) (void)hipFree(S->hdr);
     if (S->s_ml) (void)hipStreamDestroy(S->s_ml);
-    if (S->s_vox) (void)hipStreamDestroy(S->s_vox);
+    for (auto s : S->s_vox)
+        if (s) (void)hipStreamDestroy(s);
     if (S->s_io) (void)hipStreamDestroy(S->s_io);
     delete S;
 }
@@ -480,8 +491,9 @@ xo_server *xo_serve_create(xo_ctx *ctx, const xo_serve_opts *opts) {
     }
     if ((e = hipStreamCreateWithFlags(&S->s_ml, hipStreamNonBlocking)) != hipSuccess)
         return bad("stream", e);
-    if ((e = hipStreamCreateWithFlags(&S->s_vox, hipStreamNonBlocking)) != hipSuccess)
-        return bad("stream", e);
+    for (auto &sv : S->s_vox)
+        if ((e = hipStreamCreateWithFlags(&sv, hipStreamNonBlocking)) != hipSuccess)
+            return bad("stream", e);
     if ((e = hipStreamCreateWithFlags(&S->s_io, hipStreamNonBlocking)) != hipSuccess)
         return bad("stream", e);
     if ((e = hipMalloc(&S->accum, 8 * o.accum_capacity)) != hipSuccess) return bad("accum", e);
@@ -519,7 +531,9 @@ xo_server *xo_serve_create(xo_ctx *ctx, const xo_serve_opts *opts) {
         if ((e = hipMalloc(&T.vox_counters, 8 * kCounterWords * kVoxPerTick)) != hipSuccess)
             return bad("counters", e);
         if ((e = hipEventCreate(&T.ev_ml)) != hipSuccess) return bad("event", e);
-        if ((e = hipEventCreate(&T.ev_vox)) != hipSuccess) return bad("event", e);
+        for (auto &ev : T.ev_vox)
+            if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+                return bad("event", e);
         if ((e = hipEventCreate(&T.ev_start)) != hipSuccess) return bad("event", e);
         if (o.use_graph && build_graph(S, k)) {
             xo_serve_destroy(S);

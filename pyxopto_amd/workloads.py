@@ -110,8 +110,36 @@ def _lut_params(args):
     return hostpf.MHg(g, b).mclut(2000)
 
 
+def _luts_cached(lut_jobs, cache_file, processes):
+    """LUT parameters and tables of the (g, beta) pairs, from cache_file
+    when it holds exactly these pairs (an .npz of plain arrays)."""
+    keys = np.array(lut_jobs, dtype=np.float64).reshape(-1, 2)
+    if cache_file and os.path.isfile(cache_file):
+        with np.load(cache_file, allow_pickle=False) as z:
+            if z['keys'].shape == keys.shape and np.array_equal(z['keys'],
+                                                                 keys):
+                return [(list(p), t) for p, t in zip(z['params'],
+                                                     z['tables'])]
+    if processes == 1 or not lut_jobs:
+        results = [_lut_params(j) for j in lut_jobs]
+    else:
+        import multiprocessing as mp
+        procs = processes or min(16, len(os.sched_getaffinity(0)))
+        with mp.get_context('fork').Pool(procs) as pool:
+            results = pool.map(_lut_params, lut_jobs, chunksize=16)
+    if cache_file and results:
+        os.makedirs(os.path.dirname(cache_file), exist_ok=True)
+        tmp = cache_file + '.tmp.npz'
+        np.savez(tmp, keys=keys,
+                 params=np.array([r[0] for r in results], dtype=np.float64),
+                 tables=np.array([r[1] for r in results]))
+        os.replace(tmp, cache_file)
+    return results
+
+
 def layered_batch(n: int = BATCH_INSTANCES, seed: int = 20261202,
-                  budget_scale: float = 1.0, processes: int = None):
+                  budget_scale: float = 1.0, processes: int = None,
+                  cache_dir: str = None):
     """``n`` layered instances drawn from a seeded generator: 1-5 turbid
     layers each (thickness log-uniform 0.05-5 mm, mua log-uniform
     0.01-50 /cm, mus log-uniform 10-500 /cm, n uniform 1.33-1.50) between
@@ -120,7 +148,9 @@ def layered_batch(n: int = BATCH_INSTANCES, seed: int = 20261202,
     uniform 0.6-0.95, beta uniform 0-1 per layer); packet budget
     log-uniform 1e4-1e6 (times budget_scale); 100-bin radial reflectance
     0-5 mm plus total transmittance and specular reflectance each.
-    Returns a list of batch.Instance."""
+    The lookup tables are built by a pool of `processes` forked workers
+    (1: in this process) or read back from an .npz in cache_dir written by
+    an earlier call. Returns a list of batch.Instance."""
     from .mcml import batch as mbatch
     rng = np.random.default_rng(seed)
     specs = []
@@ -143,14 +173,10 @@ def layered_batch(n: int = BATCH_INSTANCES, seed: int = 20261202,
             lut_jobs.extend((float(g[k]), float(beta[k])) for k in range(nl))
     luts = {}
     if lut_jobs:
-        if processes == 1:
-            results = [_lut_params(j) for j in lut_jobs]
-        else:
-            import multiprocessing as mp
-            procs = processes or min(16, len(os.sched_getaffinity(0)))
-            with mp.get_context('fork').Pool(procs) as pool:
-                results = pool.map(_lut_params, lut_jobs, chunksize=16)
-        luts = dict(zip(lut_jobs, results))
+        cache_file = None if cache_dir is None else os.path.join(
+            cache_dir, 'layered_batch_luts_{}_{}.npz'.format(n, seed))
+        luts = dict(zip(lut_jobs, _luts_cached(lut_jobs, cache_file,
+                                               processes)))
     L = mcml.mclayer
     D = mcml.mcdetector
     out = []
