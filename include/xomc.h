@@ -456,6 +456,41 @@ int xo_probe_copy(xo_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms_per_cop
 int xo_probe_atomic_u64(xo_ctx *ctx, uint64_t array_bytes, uint64_t natomics,
                         uint64_t seed, double *ms, uint64_t *checksum);
 
+/* ---------------- reflectance surrogate (device) ---------------- */
+
+/* A fully connected ReLU network (widths dims[0..nlayers]) trained and
+ * evaluated with bf16 MFMA products and fp32 accumulation; Adam on fp32
+ * master weights. No counterpart in the reference, whose regression
+ * utilities call TensorFlow (xopto/util/regression/ann.py). Parameters
+ * are packed W_1, b_1, ..., W_n, b_n with W_l row-major [in][out]. */
+typedef struct xo_mlp xo_mlp;
+
+xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers,
+                      uint32_t max_batch);
+void xo_mlp_destroy(xo_mlp *mlp);
+uint64_t xo_mlp_param_count(xo_mlp *mlp);
+/* Set (and reset the optimizer state) / read the fp32 parameters. */
+int xo_mlp_set_params(xo_mlp *mlp, const float *params, uint64_t n);
+int xo_mlp_get_params(xo_mlp *mlp, float *params, uint64_t n);
+/* y[n][dims[nlayers]] = network(x[n][dims[0]]); kernel_ms may be NULL. */
+int xo_mlp_forward(xo_mlp *mlp, const float *x, uint64_t n, float *y, double *kernel_ms);
+/* Mean squared error of one batch (n <= max_batch rows) and its gradient
+ * with respect to every parameter (packed like the parameters). */
+int xo_mlp_gradient(xo_mlp *mlp, const float *x, const float *t, uint32_t n,
+                    float *grad, float *loss);
+/* Keep a training set on the device. */
+int xo_mlp_set_data(xo_mlp *mlp, const float *x, const float *t, uint32_t n);
+/* `steps` Adam steps (beta 0.9, 0.999, eps 1e-8) on minibatches of `batch`
+ * rows drawn with replacement by a hash of (seed, step, row); one captured
+ * graph per step when use_graph. losses[s]: the minibatch loss of step s. */
+int xo_mlp_train(xo_mlp *mlp, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
+                 int use_graph, float *losses, double *kernel_ms);
+/* C = op(A) op(B) with A and B rounded to bf16, fp32 accumulation: the
+ * product the network runs on, exposed for tests. */
+int xo_gemm_bf16(xo_ctx *ctx, int trans_a, int trans_b, uint32_t M, uint32_t N,
+                 uint32_t K, const float *A, uint32_t lda, const float *B, uint32_t ldb,
+                 float *C, uint32_t ldc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -219,6 +219,24 @@ PROTOTYPES = {
                                 c_uint64]),
     'xo_serve_cancel': (c_int, [c_void_p, c_uint64]),
     'xo_serve_get_stats': (c_int, [c_void_p, POINTER(XoServeStats)]),
+    'xo_mlp_create': (c_void_p, [c_void_p, POINTER(c_uint32), c_uint32,
+                                 c_uint32]),
+    'xo_mlp_destroy': (None, [c_void_p]),
+    'xo_mlp_param_count': (c_uint64, [c_void_p]),
+    'xo_mlp_set_params': (c_int, [c_void_p, POINTER(c_float), c_uint64]),
+    'xo_mlp_get_params': (c_int, [c_void_p, POINTER(c_float), c_uint64]),
+    'xo_mlp_forward': (c_int, [c_void_p, POINTER(c_float), c_uint64,
+                               POINTER(c_float), POINTER(c_double)]),
+    'xo_mlp_gradient': (c_int, [c_void_p, POINTER(c_float), POINTER(c_float),
+                                c_uint32, POINTER(c_float), POINTER(c_float)]),
+    'xo_mlp_set_data': (c_int, [c_void_p, POINTER(c_float), POINTER(c_float),
+                                c_uint32]),
+    'xo_mlp_train': (c_int, [c_void_p, c_uint32, c_uint32, c_float, c_uint64,
+                             c_int, POINTER(c_float), POINTER(c_double)]),
+    'xo_gemm_bf16': (c_int, [c_void_p, c_int, c_int, c_uint32, c_uint32,
+                             c_uint32, POINTER(c_float), c_uint32,
+                             POINTER(c_float), c_uint32, POINTER(c_float),
+                             c_uint32]),
     'xo_probe_copy': (c_int, [c_void_p, c_uint64, c_uint32,
                               POINTER(c_double)]),
     'xo_probe_atomic_u64': (c_int, [c_void_p, c_uint64, c_uint64, c_uint64,

@@ -18,7 +18,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), 'include',
 
 def test_library_exports_every_declared_symbol():
     text = open(HEADER).read()
-    declared = set(re.findall(r'^[a-z_ ]+\**\s*\**(xo_[a-z0-9_]+)\s*\(', text,
+    declared = set(re.findall(r'^[a-z0-9_ ]+\**\s*\**(xo_[a-z0-9_]+)\s*\(', text,
                               re.M))
     assert len(declared) >= 15
     lib = ctypes.CDLL(N.LIB_PATH)
