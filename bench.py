@@ -31,7 +31,7 @@ SLOTS_INTERIOR = 85 - 6 + 6 * 4         # 103
 SLOTS_BOUNDARY = 43 - 6 + 6 * 4         # 61
 SLOTS_VOXEL_EXTRA = 12 - 3 + 3 * 4      # 21
 
-SERVING_QPS = 400.0                     # requests/s of the serving run
+SERVING_QPS = 250.0                     # requests/s of the serving run
 
 # HBM bytes per launch of the dominant kernel: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE of one launch, each scaled by what the counter reads for a
@@ -133,7 +133,8 @@ def ceilings(ctx):
 
 
 def serving(insts, qps: float, duration: float, seed: int = 20261203,
-            voxel_fraction: float = 0.1, saturation_requests: int = 400):
+            voxel_fraction: float = 0.1, saturation_requests: int = 400,
+            tick_packets: int = None):
     """Poisson arrivals at a fixed rate for `duration` seconds: 90 % of the
     requests are instances of the layered batch (in a seeded random order),
     10 % are 64^3 two-layer voxel simulations of 1e6 packets. Requests are
@@ -162,7 +163,8 @@ def serving(insts, qps: float, duration: float, seed: int = 20261203,
         inst, lut = packed[k % len(packed)]
         return srv.submit_packed_layered(inst, lut)
 
-    with serve.Server() as srv:
+    opts = {} if tick_packets is None else {'tick_packets': tick_packets}
+    with serve.Server(**opts) as srv:
         # first use of the server (graph, tables) outside the clock
         warm = [make(srv, k) for k in range(4)]
         srv.drain(120)
@@ -194,6 +196,38 @@ def serving(insts, qps: float, duration: float, seed: int = 20261203,
         out[name + '_p50_ms'] = lat['p50_ms']
         out[name + '_p99_ms'] = lat['p99_ms']
     return out
+
+
+def surrogate_rates(ctx, rows: int = 65536, batch: int = 4096,
+                    steps: int = 50):
+    """MFMA throughput of the surrogate network (5-256-256-256-100, bf16
+    products, fp32 accumulation): inference over `rows` synthetic inputs
+    and `steps` captured Adam steps on minibatches of `batch`."""
+    import numpy as np
+    from pyxopto_amd import surrogate as S
+    dims = S.DIMS
+    net = S.Network(dims, ctx, max_batch=rows)
+    net.set_params(S.init_params(dims, seed=1))
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, (rows, dims[0])).astype(np.float32)
+    t = rng.normal(0, 1, (rows, dims[-1])).astype(np.float32)
+    net.forward(x)
+    _, ms = net.forward(x)
+    flops_row = 2 * sum(i * o for i, o in zip(dims[:-1], dims[1:]))
+    net.set_data(x, t)
+    net.train(2, batch, 1e-3, seed=3)
+    _, tms = net.train(steps, batch, 1e-3, seed=3)
+    net.close()
+    infer = flops_row * rows / (ms * 1e-3) / 1e12
+    train = 3 * flops_row * batch * steps / (tms * 1e-3) / 1e12
+    return {'network': '5-256-256-256-100 relu, bf16 MFMA, fp32 accumulate',
+            'inference_rows_per_s': rows / (ms * 1e-3),
+            'inference_TFLOPs': infer,
+            'train_step_ms': tms / steps, 'train_batch': batch,
+            'train_TFLOPs': train,
+            'roofline': {'bound': 'mfma', 'peak': 2500.0, 'unit': 'TFLOP/s',
+                         'frac_inference': infer / 2500.0,
+                         'frac_train': train / 2500.0}}
 
 
 def plain_rate(batch, insts, every: int = 16):
@@ -283,7 +317,11 @@ def main():
                     help='skip the one-launch-per-instance comparison')
     ap.add_argument('--no-serving', action='store_true',
                     help='skip the fixed-QPS serving run')
+    ap.add_argument('--no-surrogate', action='store_true',
+                    help='skip the surrogate network throughput')
     ap.add_argument('--qps', type=float, default=SERVING_QPS)
+    ap.add_argument('--tick-packets', type=int, default=None,
+                    help='layered packets per serving tick')
     ap.add_argument('--serving-seconds', type=float, default=8.0)
     ap.add_argument('--lut-processes', type=int, default=None,
                     help='LUT builder processes (1 under a profiler: no '
@@ -360,11 +398,14 @@ def main():
         plain = sim.bench_plain()
         out['plain_one_launch_per_instance'] = plain
         out['batched_vs_plain'] = value / plain['packets_per_s']
+    if not args.no_surrogate:
+        out['surrogate'] = surrogate_rates(sim.context)
     if hasattr(sim, 'bench_insts') and not args.no_serving:
         print('bench: serving at {} requests/s for {} s'.format(
             args.qps, args.serving_seconds), file=sys.stderr, flush=True)
         out['serving'] = serving(sim.bench_insts, args.qps,
-                                 args.serving_seconds)
+                                 args.serving_seconds,
+                                 tick_packets=args.tick_packets)
         print('bench: serving done', file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(sim, voxel)

@@ -16,6 +16,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -67,6 +68,7 @@ struct GemmArgs {
 };
 
 constexpr int TM = 64, TN = 64, TK = 32, KPAD = 8;
+constexpr uint32_t kMaxSplits = 32;
 
 template <bool TA, bool TB, int EPI>
 __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
@@ -81,7 +83,12 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < g.K; k0 += TK) {
+    /* split K: block z takes K range [z kc, (z + 1) kc) and, for
+     * EPI_STORE, writes its partial product to C + z M ldc */
+    const int kc = (g.K + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int kbeg = (int)blockIdx.z * kc;
+    const int kend = kbeg + kc < g.K ? kbeg + kc : g.K;
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
         /* stage op(A)[m0:+64][k0:+32] as sA[m][k], op(B)[k0:+32][n0:+64]
          * as sB[n][k]; each thread moves 8 elements of each */
         for (int e = tid; e < TM * TK; e += 256) {
@@ -89,7 +96,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
             if (TA) { c = e / TM; r = e % TM; } else { r = e / TK; c = e % TK; }
             const int m = m0 + r, k = k0 + c;
             uint16_t v = 0;
-            if (m < g.M && k < g.K) v = TA ? g.A[(size_t)k * g.lda + m] : g.A[(size_t)m * g.lda + k];
+            if (m < g.M && k < kend) v = TA ? g.A[(size_t)k * g.lda + m] : g.A[(size_t)m * g.lda + k];
             sA[r][c] = v;
         }
         for (int e = tid; e < TN * TK; e += 256) {
@@ -97,7 +104,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
             if (TB) { r = e / TK; c = e % TK; } else { c = e / TN; r = e % TN; }
             const int n = n0 + r, k = k0 + c;
             uint16_t v = 0;
-            if (n < g.N && k < g.K) v = TB ? g.B[(size_t)n * g.ldb + k] : g.B[(size_t)k * g.ldb + n];
+            if (n < g.N && k < kend) v = TB ? g.B[(size_t)n * g.ldb + k] : g.B[(size_t)k * g.ldb + n];
             sB[r][c] = v;
         }
         __syncthreads();
@@ -123,7 +130,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
                 if (m >= g.M || n >= g.N) continue;
                 float v = acc[i][j][r];
                 if (EPI == EPI_STORE) {
-                    g.C[(size_t)m * g.ldc + n] = v;
+                    g.C[((size_t)blockIdx.z * g.M + m) * g.ldc + n] = v;
                 } else if (EPI == EPI_BIAS) {
                     g.C[(size_t)m * g.ldc + n] = v + g.bias[n];
                 } else if (EPI == EPI_BIAS_RELU) {
@@ -137,10 +144,20 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
 }
 
 template <bool TA, bool TB, int EPI>
-hipError_t gemm(const GemmArgs &g, hipStream_t s) {
-    dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM);
+hipError_t gemm(const GemmArgs &g, hipStream_t s, int splits = 1) {
+    dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, splits);
     hipLaunchKernelGGL((gemm_kernel<TA, TB, EPI>), grid, dim3(256), 0, s, g);
     return hipGetLastError();
+}
+
+/* C[i] = sum over z of P[z][i], z in order (deterministic). */
+__global__ void sum_splits_kernel(const float *P, uint32_t splits, uint64_t n, float *C) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (uint32_t z = 0; z < splits; ++z) s += P[(size_t)z * n + i];
+        C[i] = s;
+    }
 }
 
 __global__ void to_bf16_kernel(const float *x, uint16_t *y, uint64_t n) {
@@ -184,13 +201,23 @@ __global__ void loss_kernel(const float *Y, const float *T, uint32_t count, floa
     if (threadIdx.x == 0) atomicAdd(ring ? loss + (*step % ring) : loss, part[0] * scale);
 }
 
-/* db[n] = sum_m bf16 D[m][n]. */
-__global__ void colsum_kernel(const uint16_t *D, uint32_t M, uint32_t N, float *db) {
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    float s = 0.f;
-    for (uint32_t m = 0; m < M; ++m) s += bf2f(D[(size_t)m * N + n]);
-    db[n] = s;
+/* db[n] = sum_m bf16 D[m][n]: 64 columns per block, four row groups with
+ * eight partial sums each, combined in a fixed order. */
+__global__ void __launch_bounds__(256) colsum_kernel(const uint16_t *D, uint32_t M, uint32_t N,
+                                                     float *db) {
+    __shared__ float part[4][64];
+    const uint32_t c = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const uint32_t n = blockIdx.x * 64 + c;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (n < N) {
+        uint32_t m = grp;
+        for (; m + 28 < M; m += 32)
+            for (int u = 0; u < 8; ++u) acc[u] += bf2f(D[(size_t)(m + 4 * u) * N + n]);
+        for (; m < M; m += 4) acc[0] += bf2f(D[(size_t)m * N + n]);
+    }
+    part[grp][c] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    __syncthreads();
+    if (grp == 0 && n < N) db[n] = (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
 }
 
 __global__ void fill_f32_kernel(float *p, uint32_t n, float v) {
@@ -245,6 +272,7 @@ struct xo_mlp {
     std::vector<uint16_t *> h;        /* activations of hidden layers */
     uint16_t *d0 = nullptr, *d1 = nullptr; /* [B][max width] gradients */
     float *y = nullptr;               /* [B][out] */
+    float *work = nullptr;            /* split-K partial products */
     float *loss = nullptr;            /* per training step */
     uint32_t loss_cap = 0;
     uint32_t *step = nullptr;
@@ -326,10 +354,21 @@ int backward(xo_mlp *N, uint32_t B, float *loss, uint32_t ring) {
         g.M = in;
         g.N = out;
         g.K = (int)B;
-        g.C = N->grad + N->woff[l];
+        /* few output tiles and a long K (the batch): split K over blocks
+         * into the workspace, then add the partial products in order */
+        const int tiles = ((in + TM - 1) / TM) * ((out + TN - 1) / TN);
+        int splits = (int)std::min<uint32_t>(kMaxSplits, std::max<uint32_t>(1u, 256u / tiles));
+        splits = std::min(splits, std::max(1, (int)B / 128));
+        g.C = splits > 1 ? N->work : N->grad + N->woff[l];
         g.ldc = out;
-        MLP_CHECK((gemm<true, false, EPI_STORE>(g, N->stream)));
-        hipLaunchKernelGGL(colsum_kernel, dim3((out + 255) / 256), dim3(256), 0, N->stream, dcur,
+        MLP_CHECK((gemm<true, false, EPI_STORE>(g, N->stream, splits)));
+        if (splits > 1) {
+            const uint64_t n = (uint64_t)in * out;
+            hipLaunchKernelGGL(sum_splits_kernel, dim3(256), dim3(256), 0, N->stream, N->work,
+                               (uint32_t)splits, n, N->grad + N->woff[l]);
+            MLP_CHECK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(colsum_kernel, dim3((out + 63) / 64), dim3(256), 0, N->stream, dcur,
                            B, (uint32_t)out, N->grad + N->boff[l]);
         MLP_CHECK(hipGetLastError());
         if (l == 0) break;
@@ -407,8 +446,8 @@ void xo_mlp_destroy(xo_mlp *N) {
     if (N->exec) (void)hipGraphExecDestroy(N->exec);
     if (N->graph) (void)hipGraphDestroy(N->graph);
     void *bufs[] = {N->w, N->m, N->v, N->grad, N->wbf, N->is_matrix, N->bf_index, N->xb,
-                    N->xf, N->tb, N->d0, N->d1, N->y, N->loss, N->step, N->data_x,
-                    N->data_t};
+                    N->xf, N->tb, N->d0, N->d1, N->y, N->work, N->loss, N->step,
+                    N->data_x, N->data_t};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto p : N->h)
@@ -452,6 +491,9 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
         return bad("stream", e);
     const int W = widest(N);
     const uint64_t P = N->nparams;
+    uint64_t max_matrix = 0;
+    for (uint32_t l = 0; l < nlayers; ++l)
+        max_matrix = std::max<uint64_t>(max_matrix, (uint64_t)dims[l] * dims[l + 1]);
     struct {
         void **p;
         size_t bytes;
@@ -463,6 +505,7 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
         {(void **)&N->tb, 4ull * max_batch * dims[nlayers]},
         {(void **)&N->d0, 2ull * max_batch * W}, {(void **)&N->d1, 2ull * max_batch * W},
         {(void **)&N->y, 4ull * max_batch * dims[nlayers]}, {(void **)&N->step, 4},
+        {(void **)&N->work, 4ull * kMaxSplits * max_matrix},
     };
     for (auto &a : allocs)
         if ((e = hipMalloc(a.p, a.bytes)) != hipSuccess) return bad("allocation", e);

@@ -4,13 +4,15 @@
  * Requests (a layered instance, or a voxelized simulation) are admitted
  * when their accumulators, lookup tables and voxel grid fit in fixed device
  * arenas; their accumulators stay resident until the result is taken.
- * Work is dispatched in ticks: each tick gives every active request a fair
- * share of a packet quantum — layered requests as items of one batch_kernel
+ * Layered requests run in ticks: each tick gives every active layered
+ * request a fair share of a packet quantum as items of one batch_kernel
  * launch (a hipGraph captured once per tick slot: counter reset + launch,
- * with the item count read from device memory), voxel requests as keyed
- * mcvox launches on a second stream. Two tick slots alternate, so the next
- * tick is queued while the current one runs and newly admitted requests
- * join the next tick: a short request never waits for a long one to finish.
+ * with the item count read from device memory). Two tick slots alternate,
+ * so the next tick is queued while the current one runs and newly admitted
+ * requests join the next tick: a short request never waits for a long one
+ * to finish. Voxel requests run beside the ticks as keyed mcvox launches of
+ * at most vox_tick_packets packets, on four streams, at most two queued
+ * per request and sixteen in flight.
  *
  * Every packet draws from the generator keyed on (request seed, packet
  * index), so a request's integers do not depend on what else is served.
@@ -104,9 +106,20 @@ struct Request {
 };
 
 constexpr int kSlots = 2;
-constexpr int kVoxPerTick = 8;
-constexpr int kVoxStreams = 4; /* voxel launches of a tick overlap each other */
+constexpr int kVoxSlots = 16;  /* voxel launches in flight at once */
+constexpr int kVoxStreams = 4; /* voxel launches of different requests overlap */
+constexpr int kVoxPerRequest = 2; /* queued launches per voxel request */
 constexpr int kCounterWords = 1 + XO_STAT_COUNT; /* u64 words per counter block */
+
+/* One voxel launch in flight: voxel requests run beside the layered ticks,
+ * each on its own stream, so a long voxel launch never holds a tick back. */
+struct VoxSlot {
+    bool busy = false;
+    uint64_t id = 0;
+    uint64_t packets = 0;
+    hipEvent_t ev = nullptr;
+    unsigned long long *counters = nullptr;
+};
 
 struct TickSlot {
     xo::BatchItem *items_host = nullptr;  /* pinned */
@@ -114,15 +127,12 @@ struct TickSlot {
     xo::BatchItem *items_dev = nullptr;
     uint32_t *nitems_dev = nullptr;
     unsigned long long *ml_counters = nullptr; /* [item counter | stats] */
-    unsigned long long *vox_counters = nullptr; /* kVoxPerTick blocks */
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev_ml = nullptr, ev_start = nullptr;
-    hipEvent_t ev_vox[kVoxStreams] = {};
     bool busy = false;
     uint32_t nitems = 0;
     std::vector<uint64_t> requests; /* ids with work in this tick */
-    std::vector<uint64_t> vox_ids;  /* the voxel launches, in counter order */
     uint64_t packets = 0;
 };
 
@@ -142,6 +152,8 @@ struct xo_server {
     std::vector<uint32_t> free_slots;
     TickSlot ticks[kSlots];
     int next_tick = 0;
+    VoxSlot vox_slots[kVoxSlots];
+    unsigned long long *vox_counters = nullptr; /* kVoxSlots blocks */
     uint32_t ml_blocks = 0;
     std::unordered_map<uint64_t, Request> reqs;
     std::deque<uint64_t> order; /* active requests, admission order */
@@ -217,60 +229,64 @@ void free_request_memory(xo_server *S, Request &R) {
     R.accum_n = R.lut_n = R.vox_n = 0;
 }
 
-/* Retire finished ticks: fold their counters into the requests, mark
- * requests complete. Non-blocking unless wait is set. */
-int retire_ticks(xo_server *S, bool wait) {
+/* One unit of a request's work (a tick's items or a voxel launch) has
+ * finished: complete the request when nothing of it is left. */
+void work_done(xo_server *S, uint64_t id) {
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return;
+    Request &R = it->second;
+    if (R.inflight) R.inflight--;
+    if (R.inflight == 0 && (R.next >= R.npackets || R.cancelled) && !R.completed) {
+        R.completed = true;
+        if (R.cancelled) {
+            free_request_memory(S, R);
+            S->reqs.erase(it);
+        } else {
+            S->done.push_back(id);
+        }
+    }
+}
+
+/* Retire finished ticks and voxel launches: fold their counters into the
+ * statistics and the requests. Non-blocking unless wait is set. */
+int retire(xo_server *S, bool wait) {
     for (int k = 0; k < kSlots; ++k) {
         TickSlot &T = S->ticks[k];
         if (!T.busy) continue;
         if (wait) {
             XO_CHECK(hipEventSynchronize(T.ev_ml));
-            for (int v = 0; v < kVoxStreams; ++v) XO_CHECK(hipEventSynchronize(T.ev_vox[v]));
         } else {
-            bool ready = true;
             hipError_t a = hipEventQuery(T.ev_ml);
-            if (a == hipErrorNotReady) ready = false;
-            else if (a != hipSuccess) return fail_hip("hipEventQuery", a);
-            for (int v = 0; v < kVoxStreams && ready; ++v) {
-                hipError_t b = hipEventQuery(T.ev_vox[v]);
-                if (b == hipErrorNotReady) ready = false;
-                else if (b != hipSuccess) return fail_hip("hipEventQuery", b);
-            }
-            if (!ready) continue;
+            if (a == hipErrorNotReady) continue;
+            if (a != hipSuccess) return fail_hip("hipEventQuery", a);
         }
-        unsigned long long c[kCounterWords * (1 + kVoxPerTick)];
-        XO_CHECK(hipMemcpy(c, T.ml_counters, sizeof(unsigned long long) * kCounterWords,
-                           hipMemcpyDeviceToHost));
-        if (!T.vox_ids.empty())
-            XO_CHECK(hipMemcpy(c + kCounterWords, T.vox_counters,
-                               sizeof(unsigned long long) * kCounterWords * T.vox_ids.size(),
-                               hipMemcpyDeviceToHost));
+        unsigned long long c[kCounterWords];
+        XO_CHECK(hipMemcpy(c, T.ml_counters, sizeof(c), hipMemcpyDeviceToHost));
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, T.ev_start, T.ev_ml) == hipSuccess)
             S->stats.busy_ms += ms;
         S->stats.ticks++;
         S->stats.packets += T.packets;
         S->stats.steps += c[1 + XO_STAT_STEPS];
-        for (size_t v = 0; v < T.vox_ids.size(); ++v)
-            S->stats.steps += c[kCounterWords * (1 + v) + 1 + XO_STAT_STEPS];
-        for (uint64_t id : T.requests) {
-            auto it = S->reqs.find(id);
-            if (it == S->reqs.end()) continue;
-            Request &R = it->second;
-            if (R.inflight) R.inflight--;
-            if (R.inflight == 0 && (R.next >= R.npackets || R.cancelled) && !R.completed) {
-                R.completed = true;
-                if (R.cancelled) {
-                    free_request_memory(S, R);
-                    S->reqs.erase(it);
-                } else {
-                    S->done.push_back(id);
-                }
-            }
-        }
+        for (uint64_t id : T.requests) work_done(S, id);
         T.busy = false;
         T.requests.clear();
-        T.vox_ids.clear();
+    }
+    for (auto &V : S->vox_slots) {
+        if (!V.busy) continue;
+        if (wait) {
+            XO_CHECK(hipEventSynchronize(V.ev));
+        } else {
+            hipError_t a = hipEventQuery(V.ev);
+            if (a == hipErrorNotReady) continue;
+            if (a != hipSuccess) return fail_hip("hipEventQuery", a);
+        }
+        unsigned long long c[kCounterWords];
+        XO_CHECK(hipMemcpy(c, V.counters, sizeof(c), hipMemcpyDeviceToHost));
+        S->stats.packets += V.packets;
+        S->stats.steps += c[1 + XO_STAT_STEPS];
+        V.busy = false;
+        work_done(S, V.id);
     }
     return 0;
 }
@@ -308,19 +324,19 @@ int launch_vox(xo_server *S, Request &R, uint64_t first, uint64_t count,
 /* Form and launch one tick in slot k. Returns 1 if there was no work. */
 int dispatch_tick(xo_server *S, int k) {
     TickSlot &T = S->ticks[k];
-    std::vector<uint64_t> ml, vx;
+    std::vector<uint64_t> ml;
     for (uint64_t id : S->order) {
         auto it = S->reqs.find(id);
         if (it == S->reqs.end()) continue;
         const Request &R = it->second;
-        if (R.cancelled || R.next >= R.npackets) continue;
-        (R.kind == KIND_MCML ? ml : vx).push_back(id);
+        if (R.cancelled || R.next >= R.npackets || R.kind != KIND_MCML) continue;
+        ml.push_back(id);
     }
-    if (ml.empty() && vx.empty()) return 1;
+    if (ml.empty()) return 1;
     T.nitems = 0;
     T.packets = 0;
     const uint64_t chunk = S->opts.chunk;
-    if (!ml.empty()) {
+    {
         uint64_t share = S->opts.tick_packets / ml.size();
         share = std::max<uint64_t>(share, chunk);
         for (uint64_t id : ml) {
@@ -372,23 +388,38 @@ int dispatch_tick(xo_server *S, int k) {
         XO_CHECK(xo::launch_zero_u32((uint32_t *)T.ml_counters, 2 * kCounterWords, S->s_ml));
     }
     XO_CHECK(hipEventRecord(T.ev_ml, S->s_ml));
-    T.vox_ids.clear();
-    for (uint64_t id : vx) {
-        if ((int)T.vox_ids.size() >= kVoxPerTick) break;
-        Request &R = S->reqs[id];
-        uint64_t q = std::min<uint64_t>(S->opts.vox_tick_packets, R.npackets - R.next);
-        /* a request's launches stay on one stream, in packet order */
-        if (launch_vox(S, R, R.next, q, T.vox_counters + kCounterWords * T.vox_ids.size(),
-                       S->s_vox[id % kVoxStreams]))
-            return -1;
-        R.next += q;
-        R.inflight++;
-        T.requests.push_back(id);
-        T.vox_ids.push_back(id);
-        T.packets += q;
-    }
-    for (int v = 0; v < kVoxStreams; ++v) XO_CHECK(hipEventRecord(T.ev_vox[v], S->s_vox[v]));
     T.busy = true;
+    return 0;
+}
+
+/* Queue voxel launches: up to kVoxPerRequest per request, each of at most
+ * vox_tick_packets packets, on the request's stream (so its launches run
+ * in packet order), while launch slots are free. */
+int dispatch_vox(xo_server *S) {
+    for (uint64_t id : S->order) {
+        auto it = S->reqs.find(id);
+        if (it == S->reqs.end()) continue;
+        Request &R = it->second;
+        if (R.kind != KIND_MCVOX) continue;
+        while (!R.cancelled && R.next < R.npackets && R.inflight < kVoxPerRequest) {
+            VoxSlot *V = nullptr;
+            for (auto &slot : S->vox_slots)
+                if (!slot.busy) {
+                    V = &slot;
+                    break;
+                }
+            if (!V) return 0;
+            const uint64_t q = std::min<uint64_t>(S->opts.vox_tick_packets, R.npackets - R.next);
+            hipStream_t st = S->s_vox[id % kVoxStreams];
+            if (launch_vox(S, R, R.next, q, V->counters, st)) return -1;
+            XO_CHECK(hipEventRecord(V->ev, st));
+            V->busy = true;
+            V->id = id;
+            V->packets = q;
+            R.next += q;
+            R.inflight++;
+        }
+    }
     return 0;
 }
 
@@ -441,12 +472,12 @@ void xo_serve_destroy(xo_server *S) {
         if (T.items_dev) (void)hipFree(T.items_dev);
         if (T.nitems_dev) (void)hipFree(T.nitems_dev);
         if (T.ml_counters) (void)hipFree(T.ml_counters);
-        if (T.vox_counters) (void)hipFree(T.vox_counters);
         if (T.ev_ml) (void)hipEventDestroy(T.ev_ml);
-        for (auto ev : T.ev_vox)
-            if (ev) (void)hipEventDestroy(ev);
         if (T.ev_start) (void)hipEventDestroy(T.ev_start);
     }
+    for (auto &V : S->vox_slots)
+        if (V.ev) (void)hipEventDestroy(V.ev);
+    if (S->vox_counters) (void)hipFree(S->vox_counters);
     if (S->accum) (void)hipFree(S->accum);
     if (S->lut) (void)hipFree(S->lut);
     if (S->vox) (void)hipFree(S->vox);
@@ -528,17 +559,20 @@ xo_server *xo_serve_create(xo_ctx *ctx, const xo_serve_opts *opts) {
             return bad("count", e);
         if ((e = hipMalloc(&T.ml_counters, 8 * kCounterWords)) != hipSuccess)
             return bad("counters", e);
-        if ((e = hipMalloc(&T.vox_counters, 8 * kCounterWords * kVoxPerTick)) != hipSuccess)
-            return bad("counters", e);
         if ((e = hipEventCreate(&T.ev_ml)) != hipSuccess) return bad("event", e);
-        for (auto &ev : T.ev_vox)
-            if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
-                return bad("event", e);
         if ((e = hipEventCreate(&T.ev_start)) != hipSuccess) return bad("event", e);
         if (o.use_graph && build_graph(S, k)) {
             xo_serve_destroy(S);
             return nullptr;
         }
+    }
+    if ((e = hipMalloc(&S->vox_counters, 8 * kCounterWords * kVoxSlots)) != hipSuccess)
+        return bad("counters", e);
+    for (int v = 0; v < kVoxSlots; ++v) {
+        S->vox_slots[v].counters = S->vox_counters + (size_t)kCounterWords * v;
+        if ((e = hipEventCreateWithFlags(&S->vox_slots[v].ev, hipEventDisableTiming)) !=
+            hipSuccess)
+            return bad("event", e);
     }
     return S;
 }
@@ -748,7 +782,7 @@ int xo_serve_step(xo_server *S, uint32_t *active) {
     }
     std::lock_guard<std::mutex> lock(S->mu);
     XO_CHECK(hipSetDevice(S->ctx->device));
-    if (retire_ticks(S, false)) return -1;
+    if (retire(S, false)) return -1;
     for (int tries = 0; tries < kSlots; ++tries) {
         TickSlot &T = S->ticks[S->next_tick];
         if (T.busy) break;
@@ -757,6 +791,7 @@ int xo_serve_step(xo_server *S, uint32_t *active) {
         if (rc == 1) break;
         S->next_tick = (S->next_tick + 1) % kSlots;
     }
+    if (dispatch_vox(S)) return -1;
     prune_order(S);
     if (active) {
         uint32_t n = 0;
@@ -771,7 +806,7 @@ int xo_serve_wait(xo_server *S) {
     if (!S) return -1;
     std::lock_guard<std::mutex> lock(S->mu);
     XO_CHECK(hipSetDevice(S->ctx->device));
-    return retire_ticks(S, true);
+    return retire(S, true);
 }
 
 int xo_serve_poll(xo_server *S, uint64_t *ids, uint32_t max, uint32_t *n) {
