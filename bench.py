@@ -64,8 +64,8 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
         batch = Batch(insts, rnginit=0x5EED000000000001)
         packed = batch.pack()
         if launch:
-            chunk, block, bpc = (int(v) for v in launch.split(':'))
-            opts = batch.opts(chunk, block, bpc)
+            fields = [int(v) for v in launch.split(':')]
+            opts = batch.opts(*fields)
         else:
             from pyxopto_amd.mcml.autotune import tuned_opts
             opts = tuned_opts(batch, packed)
@@ -81,7 +81,8 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
                   'phase_functions': 'HG or 2000-point LUT of MHG',
                   'detectors': '100-bin radial + total R/T per instance',
                   'launch': {'chunk': opts.chunk, 'block': opts.block,
-                             'blocks_per_cu': opts.blocks_per_cu}}
+                             'blocks_per_cu': opts.blocks_per_cu,
+                             'refill_min': opts.refill_min or 4}}
         batch.bench_plain = lambda: plain_rate(batch, insts)
         batch.bench_insts = insts
         return batch, npk, step, False, config
@@ -327,7 +328,8 @@ def main():
                     help='LUT builder processes (1 under a profiler: no '
                          'fork after the GPU is initialised)')
     ap.add_argument('--launch', default=None,
-                    help='CHUNK:BLOCK:BLOCKS_PER_CU instead of the autotuner')
+                    help='CHUNK:BLOCK:BLOCKS_PER_CU[:REFILL_MIN] instead of the '
+                         'autotuner')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this engine has no multi-GPU '

@@ -254,7 +254,8 @@ typedef struct xo_batch_opts {
     float w_min;                /* roulette threshold for every instance     */
     float lottery_chance;
     int32_t use_lottery;
-    uint32_t reserved;
+    uint32_t refill_min;        /* a wave hands out new packets once this
+                                   many of its lanes are idle (0: 4)          */
 } xo_batch_opts;
 
 /* ---------------- library ---------------- */
@@ -375,7 +376,8 @@ typedef struct xo_server xo_server;
 typedef struct xo_serve_opts {
     uint64_t accum_capacity;    /* u64 accumulator bins of all requests      */
     uint64_t lut_capacity;      /* LUT floats of all requests                */
-    uint64_t voxel_capacity;    /* voxels (one byte each) of all requests    */
+    uint64_t voxel_capacity;    /* bytes of voxel grids (one byte per voxel,
+                                   grids padded to multiples of 4 per axis)  */
     uint64_t tick_packets;      /* layered packets per tick, shared fairly   */
     uint64_t vox_tick_packets;  /* packets per voxel request per tick        */
     uint32_t instance_slots;    /* layered requests resident at once         */

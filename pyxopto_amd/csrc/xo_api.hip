@@ -122,6 +122,36 @@ uint32_t choose_lanes(xo_ctx *ctx, uint32_t requested, int blocks_per_cu) {
     return lanes;
 }
 
+uint64_t brick_bytes(const int32_t shape[3]) {
+    const uint64_t nbx = ((uint64_t)shape[0] + 3) / 4, nby = ((uint64_t)shape[1] + 3) / 4,
+                   nbz = ((uint64_t)shape[2] + 3) / 4;
+    return nbx * nby * nbz * 64u;
+}
+
+int pack_voxels(const int32_t *voxels, const int32_t shape[3], uint32_t num_materials,
+                std::vector<uint8_t> &out, const char *fn) {
+    const uint32_t nx = (uint32_t)shape[0], ny = (uint32_t)shape[1], nz = (uint32_t)shape[2];
+    const uint32_t nbx = (nx + 3) / 4, nby = (ny + 3) / 4;
+    out.assign(brick_bytes(shape), 0u);
+    for (uint32_t z = 0; z < nz; ++z)
+        for (uint32_t y = 0; y < ny; ++y) {
+            const int32_t *row = voxels + ((uint64_t)z * ny + y) * nx;
+            const uint64_t base = ((uint64_t)(z >> 2) * nby + (y >> 2)) * nbx;
+            const uint32_t inner = ((z & 3u) << 4) | ((y & 3u) << 2);
+            for (uint32_t x = 0; x < nx; ++x) {
+                const int32_t m = row[x];
+                if (m < 0 || (uint32_t)m >= num_materials) {
+                    xo::set_error("%s: voxel (x %u, y %u, z %u) has material index %d "
+                                  "(valid 0..%u)",
+                                  fn, x, y, z, m, num_materials - 1);
+                    return -1;
+                }
+                out[(base + (x >> 2)) * 64u + (inner | (x & 3u))] = (uint8_t)m;
+            }
+        }
+    return 0;
+}
+
 double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
         .count();
@@ -451,19 +481,12 @@ int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
                           (unsigned long long)nvoxels, (unsigned long long)nvox);
             return -1;
         }
-        std::vector<uint8_t> packed(nvox);
-        for (uint64_t i = 0; i < nvox; ++i) {
-            int32_t m = voxels[i];
-            if (m < 0 || (uint32_t)m >= cfg->num_materials) {
-                xo::set_error("xo_mcvox_run: voxel %llu has material index %d (valid 0..%u)",
-                              (unsigned long long)i, m, cfg->num_materials - 1);
-                return -1;
-            }
-            packed[i] = (uint8_t)m;
-        }
+        std::vector<uint8_t> packed;
+        if (xo::pack_voxels(voxels, shape, cfg->num_materials, packed, "xo_mcvox_run"))
+            return -1;
         ctx->vox_valid = false;
-        if (ensure_buffer((void **)&ctx->voxels, &ctx->voxels_cap, nvox)) return -1;
-        XO_HIP(hipMemcpyAsync(ctx->voxels, packed.data(), nvox, hipMemcpyHostToDevice,
+        if (ensure_buffer((void **)&ctx->voxels, &ctx->voxels_cap, packed.size())) return -1;
+        XO_HIP(hipMemcpyAsync(ctx->voxels, packed.data(), packed.size(), hipMemcpyHostToDevice,
                               ctx->stream));
         XO_HIP(hipStreamSynchronize(ctx->stream));
         std::memcpy(ctx->vox_shape, shape, sizeof(ctx->vox_shape));
@@ -651,6 +674,7 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
     P.w_min = opts->w_min;
     P.chance = opts->lottery_chance;
     P.use_lottery = opts->use_lottery;
+    P.refill_min = opts->refill_min ? opts->refill_min : 4u;
     int bpc = 1;
     XO_HIP(xo::occupancy_batch(block, &bpc));
     if (opts->blocks_per_cu && (int)opts->blocks_per_cu < bpc) bpc = (int)opts->blocks_per_cu;

@@ -49,6 +49,7 @@ struct BatchParams {
     float w_min;
     float chance;
     int32_t use_lottery;
+    uint32_t refill_min;         /* idle lanes that trigger a refill (>= 1) */
 };
 
 hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,

@@ -62,15 +62,19 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
     uint32_t st[XO_STAT_COUNT] = {0, 0, 0, 0, 0, 0, 0};
 
     for (;;) {
-        /* ---- hand out packets to the lanes that need one ---- */
+        /* ---- hand out packets to the lanes that need one, once at least
+         * refill_min of them (or every lane) wait: one refill serves many
+         * lanes, the rest of the time no lane runs the launch code ---- */
         const bool want = !alive && !queue_empty;
         const uint64_t m = __ballot(want);
         const uint32_t nwant = (uint32_t)__builtin_popcountll(m);
+        const uint32_t nalive = (uint32_t)__builtin_popcountll(__ballot(alive));
         const uint32_t rank = prefix_count(m);
         uint32_t assigned = 0;
         bool launch = false;
         uint64_t my_p = 0;
-        while (assigned < nwant) {
+        const bool refill = nwant >= P.refill_min || nalive == 0;
+        while (refill && assigned < nwant) {
             if (cur_next >= cur_end) {
                 uint32_t it = 0;
                 if (lane == 0) it = atomicAdd(P.item_counter, 1u);
@@ -83,6 +87,14 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
                 cur_inst = item.inst;
                 cur_next = item.first;
                 cur_end = item.first + item.count;
+                /* every packet of an instance deposits the same specular
+                 * integer at the same bin: one atomic for the whole item */
+                if (lane == 0) {
+                    const BatchHdr &H = P.hdr[cur_inst];
+                    detector_deposit_n(H.specular, bins, H.src_pos[0], H.src_pos[1],
+                                       H.dir_reflected[0], H.dir_reflected[1],
+                                       H.dir_reflected[2], H.reflectance, item.count);
+                }
             }
             const uint64_t avail = cur_end - cur_next;
             const uint32_t left = nwant - assigned;
@@ -113,14 +125,15 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
             dx = H.dir_sample[0];
             dy = H.dir_sample[1];
             dz = H.dir_sample[2];
-            detector_deposit(H.specular, bins, px, py, H.dir_reflected[0],
-                             H.dir_reflected[1], H.dir_reflected[2], H.reflectance);
             layer = 1;
             pkt_steps = 0;
             alive = true;
             st[XO_STAT_PACKETS]++;
         }
-        if (!__any(alive)) break;
+        if (!__any(alive)) {
+            if (queue_empty) break;
+            continue;
+        }
         if (!alive) continue;
 
         /* ---- one transport step, mcml.template.c:514-788 ---- */

@@ -9,6 +9,7 @@
 
 #include <chrono>
 #include <mutex>
+#include <vector>
 
 #include "../../include/xomc.h"
 #include "xo_internal.h"
@@ -62,6 +63,16 @@ int check_detector(const char *name, const xo_detector &d, uint64_t naccum, uint
 int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint32_t idx);
 uint32_t choose_lanes(xo_ctx *ctx, uint32_t requested, int blocks_per_cu);
 double ms_since(std::chrono::steady_clock::time_point t0);
+
+/* Device layout of a voxel grid: uint8 material indices in 4x4x4 bricks of
+ * 64 bytes, bricks in [z][y][x] order, voxel (x, y, z) at byte
+ * 64 brick(x>>2, y>>2, z>>2) + 16 (z&3) + 4 (y&3) + (x&3). shape is
+ * {nx, ny, nz}; the host array is int32[nz][ny][nx]. */
+uint64_t brick_bytes(const int32_t shape[3]);
+/* Returns -1 (message set) if a voxel's material is outside
+ * [0, num_materials). */
+int pack_voxels(const int32_t *voxels, const int32_t shape[3], uint32_t num_materials,
+                std::vector<uint8_t> &out, const char *fn);
 
 } // namespace xo
 

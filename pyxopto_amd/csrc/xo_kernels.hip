@@ -238,11 +238,19 @@ XO_DEV bool vox_valid(const VoxGeom &g, int32_t ix, int32_t iy, int32_t iz) {
            iz < g.shape[2];
 }
 
+/* The grid is stored in 4x4x4 bricks of 64 bytes (xo_ctx.h,
+ * xo::pack_voxels): a packet crossing into a neighbour voxel along any axis
+ * stays in the same brick three times in four, so most gathers hit a cache
+ * line the lane (or its wave) has just read. */
 XO_DEV uint32_t vox_material(const uint8_t *vox, const VoxGeom &g, int32_t ix,
                              int32_t iy, int32_t iz) {
-    size_t flat = ((size_t)iz * (size_t)g.shape[1] + (size_t)iy) * (size_t)g.shape[0] +
-                  (size_t)ix;
-    return vox[flat];
+    const uint32_t nbx = ((uint32_t)g.shape[0] + 3u) >> 2;
+    const uint32_t nby = ((uint32_t)g.shape[1] + 3u) >> 2;
+    const size_t brick = ((size_t)((uint32_t)iz >> 2) * nby + ((uint32_t)iy >> 2)) * nbx +
+                         ((uint32_t)ix >> 2);
+    const uint32_t off = (((uint32_t)iz & 3u) << 4) | (((uint32_t)iy & 3u) << 2) |
+                         ((uint32_t)ix & 3u);
+    return vox[brick * 64u + off];
 }
 
 /* mcsim_sample_box_intersect, mcvox.template.c:94-162 */

@@ -36,7 +36,9 @@ XO_HD void xo_packet_state(uint64_t seed, uint64_t p, const uint32_t *table,
     uint64_t z = seed + (p + 1u) * XO_GOLDEN64;
     uint64_t h1 = xo_mix64(z);
     uint64_t h2 = xo_mix64(z + XO_GOLDEN64);
-    uint32_t m = table[h1 % XO_SEED_TABLE_ROWS];
+    /* row = floor(hi32(h1) * rows / 2^32): one multiply (Lemire's
+     * multiply-shift range reduction) instead of a 64-bit remainder */
+    uint32_t m = table[(uint32_t)(((h1 >> 32) * (uint64_t)XO_SEED_TABLE_ROWS) >> 32)];
     uint32_t lo = (uint32_t)h2;
     if (lo == 0u || lo == 0xFFFFFFFFu) lo = 1u;
     uint32_t c = (uint32_t)(((uint64_t)(uint32_t)(h2 >> 32) * (uint64_t)(m - 1u)) >> 32);

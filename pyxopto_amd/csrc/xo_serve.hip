@@ -193,6 +193,7 @@ int build_graph(xo_server *S, int k) {
     P.w_min = S->opts.w_min;
     P.chance = S->opts.lottery_chance;
     P.use_lottery = S->opts.use_lottery;
+    P.refill_min = 4u;
     XO_CHECK(hipStreamBeginCapture(S->s_ml, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = xo::launch_zero_u32((uint32_t *)T.ml_counters, 2 * kCounterWords, S->s_ml);
     hipError_t e2 = xo::launch_batch(P, S->ml_blocks, S->opts.block, S->s_ml);
@@ -382,6 +383,7 @@ int dispatch_tick(xo_server *S, int k) {
             P.w_min = S->opts.w_min;
             P.chance = S->opts.lottery_chance;
             P.use_lottery = S->opts.use_lottery;
+            P.refill_min = 4u;
             XO_CHECK(xo::launch_batch(P, S->ml_blocks, S->opts.block, S->s_ml));
         }
     } else {
@@ -722,17 +724,12 @@ int xo_serve_admit_mcvox(xo_server *S, const xo_mcvox_cfg *cfg, const int32_t *v
             return -1;
         }
     }
-    std::vector<uint8_t> packed(nvoxels);
-    for (uint64_t i = 0; i < nvoxels; ++i) {
-        if (voxels[i] < 0 || (uint32_t)voxels[i] >= cfg->num_materials) {
-            xo::set_error("xo_serve_admit_mcvox: voxel %llu has material %d",
-                          (unsigned long long)i, voxels[i]);
-            return -1;
-        }
-        packed[i] = (uint8_t)voxels[i];
-    }
+    std::vector<uint8_t> packed;
+    if (xo::pack_voxels(voxels, shape, cfg->num_materials, packed, "xo_serve_admit_mcvox"))
+        return -1;
+    const uint64_t nbytes = packed.size();
     std::lock_guard<std::mutex> lock(S->mu);
-    if (check_capacity(S, "xo_serve_admit_mcvox", naccum, nlut, nvoxels)) return -1;
+    if (check_capacity(S, "xo_serve_admit_mcvox", naccum, nlut, nbytes)) return -1;
     XO_CHECK(hipSetDevice(S->ctx->device));
     Request R;
     R.kind = KIND_MCVOX;
@@ -743,18 +740,18 @@ int xo_serve_admit_mcvox(xo_server *S, const xo_mcvox_cfg *cfg, const int32_t *v
         return 1;
     }
     R.lut_n = nlut;
-    if (!S->vox_arena.alloc(nvoxels, &R.vox_off)) {
+    if (!S->vox_arena.alloc(nbytes, &R.vox_off)) {
         S->accum_arena.release(R.accum_off, naccum);
         S->lut_arena.release(R.lut_off, nlut);
         return 1;
     }
-    R.vox_n = nvoxels;
+    R.vox_n = nbytes;
     R.vcfg = *cfg;
     R.vcfg.packet_seed = seed;
     R.lds_bins = end <= XO_LDS_BINS_MAX ? (uint32_t)end : 0u;
     R.npackets = npackets;
     R.id = S->next_id++;
-    XO_CHECK(hipMemcpyAsync(S->vox + R.vox_off, packed.data(), nvoxels, hipMemcpyHostToDevice,
+    XO_CHECK(hipMemcpyAsync(S->vox + R.vox_off, packed.data(), nbytes, hipMemcpyHostToDevice,
                             S->s_io));
     if (nlut)
         XO_CHECK(hipMemcpyAsync(S->lut + R.lut_off, lut, 4ull * nlut, hipMemcpyHostToDevice,
