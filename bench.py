@@ -231,6 +231,49 @@ def surrogate_rates(ctx, rows: int = 65536, batch: int = 4096,
                          'frac_train': train / 2500.0}}
 
 
+PUBLISHED_FIBER_S = 5.7   # RTX A6000, test/performance.py:134-135 (4e9 packets)
+
+
+def published_benchmark(packets: int = None):
+    """The reference's published benchmark (test/performance.py with
+    validate.SingleLayerUniformFiberRadial): 400 runs of 1e7 packets of a
+    single layer under a multimode fiber, timed from simulator construction
+    to the last result as the reference does (t2 - t1). Plain: one Mc.run per
+    grid point, as the reference. Batched: the 400 runs as one batch."""
+    from pyxopto_amd import workloads as W
+    from pyxopto_amd.mcml.batch import Batch
+    npk = packets or W.FIBER_PACKETS
+    mua, musr = W.fiber_grid()
+    # plain, as the reference test loops
+    t0 = time.perf_counter()
+    sim = W.fiber_sim()
+    for m, s in zip(mua, musr):
+        sim.layers[1].mua = m
+        sim.layers[1].mus = s / (1.0 - 0.85)
+        sim.run(npk)
+    t_plain = time.perf_counter() - t0
+    # batched
+    t0 = time.perf_counter()
+    b = Batch(W.fiber_batch(packets=npk), rnginit=0x5EED000000000001)
+    b._ctx = sim.context
+    res, rep = b.run(b.opts(chunk=8192, block=256))
+    t_batch = time.perf_counter() - t0
+    total = 400 * npk
+    return {'workload': '400 runs x {} packets, UniformFiberNI NA 0.22 '
+                        '200 um, 8 mm layer n 1.33 HG 0.85, rmax 5 mm'
+                        .format(npk),
+            'published_s': PUBLISHED_FIBER_S * npk / 1e7,
+            'published_device': 'NVIDIA RTX A6000',
+            'plain_s': t_plain, 'batched_s': t_batch,
+            'plain_packets_per_s': total / t_plain,
+            'batched_packets_per_s': total / t_batch,
+            'batched_kernel_s': rep.kernel_ms / 1e3,
+            'speedup_vs_published_plain': PUBLISHED_FIBER_S * npk / 1e7 /
+            t_plain,
+            'speedup_vs_published_batched': PUBLISHED_FIBER_S * npk / 1e7 /
+            t_batch}
+
+
 def plain_rate(batch, insts, every: int = 16):
     """The same instances one launch each (Mc.run: xo_mcml_run with
     lane-owned generators), on every 16th instance: packets/s."""
@@ -318,6 +361,8 @@ def main():
                     help='skip the one-launch-per-instance comparison')
     ap.add_argument('--no-serving', action='store_true',
                     help='skip the fixed-QPS serving run')
+    ap.add_argument('--no-published', action='store_true',
+                    help='skip the reference\'s published fiber benchmark')
     ap.add_argument('--no-surrogate', action='store_true',
                     help='skip the surrogate network throughput')
     ap.add_argument('--qps', type=float, default=SERVING_QPS)
@@ -400,6 +445,9 @@ def main():
         plain = sim.bench_plain()
         out['plain_one_launch_per_instance'] = plain
         out['batched_vs_plain'] = value / plain['packets_per_s']
+    if hasattr(sim, 'bench_insts') and not args.no_published:
+        print('bench: published fiber benchmark', file=sys.stderr, flush=True)
+        out['published_benchmark'] = published_benchmark()
     if not args.no_surrogate:
         out['surrogate'] = surrogate_rates(sim.context)
     if hasattr(sim, 'bench_insts') and not args.no_serving:

@@ -64,7 +64,8 @@ enum xo_det_kind {
 /* Photon packet source kinds. */
 enum xo_src_kind {
     XO_SRC_LINE = 0,     /* xopto/mcml/mcsource/line.py:95-110, mcvox/mcsource/line.py:84-104 */
-    XO_SRC_GAUSSIAN = 1  /* xopto/mcvox/mcsource/gaussianbeam.py:112-189 */
+    XO_SRC_GAUSSIAN = 1, /* xopto/mcvox/mcsource/gaussianbeam.py:112-189 */
+    XO_SRC_FIBER_NI = 2  /* xopto/mcml/mcsource/fiberni.py:237-290 (layered media) */
 };
 
 /* Stepping method (xopto/mcbase/mcoptions.py:269-318). Only albedo weight. */
@@ -123,6 +124,18 @@ typedef struct xo_source_line {
     float reserved[3];
 } xo_source_line; /* 64 B */
 
+/* Optical fiber in contact with the sample, uniform emission over the core
+ * and over the acceptance cone; McSource of UniformFiberNI
+ * (xopto/mcml/mcsource/fiberni.py:184-231, packed by cl_pack :341-375). */
+typedef struct xo_source_fiber {
+    float position[3];          /* core centre (z = 0)                        */
+    float radius;               /* core radius                                */
+    float cos_min;              /* cosine of the acceptance angle in air:
+                                   sqrt(1 - NA^2)                             */
+    float n;                    /* refractive index of the core               */
+    float reserved[2];
+} xo_source_fiber; /* 32 B */
+
 /* Gaussian beam; McSource of xopto/mcvox/mcsource/gaussianbeam.py:33-85. */
 typedef struct xo_source_gaussian {
     float transformation[9];    /* row-major 3x3, beam frame -> sample frame */
@@ -167,12 +180,14 @@ typedef struct xo_mcml_cfg {
     float w_min;                /* roulette threshold (1e-4)                  */
     float lottery_chance;       /* roulette survival probability (0.1)        */
     int32_t use_lottery;
-    uint32_t reserved;
+    int32_t source_kind;        /* XO_SRC_LINE (source) or XO_SRC_FIBER_NI
+                                   (fiber)                                    */
     xo_source_line source;
     xo_detector top, bottom, specular;
     xo_layer layers[XO_MAX_LAYERS];
     uint64_t packet_seed;       /* XO_FLAG_PACKET_KEYED only                  */
     uint64_t first_packet;      /* XO_FLAG_PACKET_KEYED only                  */
+    xo_source_fiber fiber;
 } xo_mcml_cfg;
 
 /* A voxel material; McMaterial of xopto/mcbase/mcmaterial.py:37-63. */
@@ -237,13 +252,14 @@ typedef struct xo_batch_instance {
     float rmax;
     uint32_t accum_base;        /* this instance's bins: [base, base+count)  */
     uint32_t accum_count;
-    uint32_t reserved;
+    int32_t source_kind;        /* XO_SRC_LINE or XO_SRC_FIBER_NI            */
     uint64_t npackets;
     uint64_t seed;
     xo_source_line source;
     xo_detector top, bottom, specular; /* offsets are absolute              */
     xo_layer layers[XO_BATCH_MAX_LAYERS];
-} xo_batch_instance; /* 888 B */
+    xo_source_fiber fiber;
+} xo_batch_instance; /* 920 B */
 
 /* Launch shape of a batch run (chosen by the autotuner). */
 typedef struct xo_batch_opts {
@@ -275,7 +291,8 @@ int xo_device_count(void);
  * Writes up to n values in the order: xo_pf, xo_layer, xo_source_line,
  * xo_source_gaussian, xo_detector, xo_fluence, xo_mcml_cfg, xo_material,
  * xo_voxel_cfg, xo_mcvox_cfg, xo_run_report, xo_batch_instance,
- * xo_batch_opts, xo_serve_opts, xo_serve_stats. Returns the count written. */
+ * xo_batch_opts, xo_serve_opts, xo_serve_stats, xo_source_fiber. Returns the
+ * count written. */
 int xo_struct_sizes(uint32_t *out, int n);
 
 /* ---------------- seeds (host) ---------------- */

@@ -327,6 +327,71 @@ static void o_mcml_launch(const xo_mcml_cfg *cfg, o_pkt *s, uint64_t *acc,
     s->layer = 1;
 }
 
+/* reflectance_cos2, mcbase.template.c:1303-1346: Fresnel reflectance from
+ * the cosine of the transmitted angle. */
+static float o_reflectance_cos2(float n1, float n2, float cos2) {
+    float R = 1.0f;
+    cos2 = fabsf(cos2);
+    if (n1 == n2) return 0.0f;
+    float sin2 = sqrtf(1.0f - cos2 * cos2);
+    if (cos2 >= O_FP_COS_0) sin2 = 0.0f;
+    float sin1 = (n2 / n1) * sin2;
+    if (sin1 < 1.0f) {
+        float cos1 = sqrtf(1.0f - sin1 * sin1);
+        float n1_d_n2 = n1 / n2;
+        float n_cos1 = n1_d_n2 * cos1;
+        float n_cos2 = n1_d_n2 * cos2;
+        float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
+        Rs *= Rs;
+        float Rp = (n_cos2 - cos1) / (n_cos2 + cos1);
+        Rp *= Rp;
+        R = 0.5f * (Rp + Rs);
+        if (cos1 <= O_FP_COS_90 || sin2 == 1.0f) return 1.0f;
+    }
+    return R;
+}
+
+/* UniformFiberNI launch, xopto/mcml/mcsource/fiberni.py:237-290: a point
+ * uniform over the core, cos(theta) uniform over the acceptance cone,
+ * refraction into layer 1, weight 1 - R; the specular part leaves along
+ * refract((dx, dy, -dz), (0, 0, -1), n_sample, n_core)
+ * (mcbase.template.c:1443-1462). */
+static void o_fiber_launch(const xo_mcml_cfg *cfg, o_pkt *s, uint64_t *acc,
+                           int mm) {
+    const xo_source_fiber *src = &cfg->fiber;
+    const float n1 = cfg->layers[1].n;
+    float sin_fi, cos_fi;
+    float r = sqrtf(o_rand(&s->x, s->a)) * src->radius;
+    o_sincos_azimuth(o_rand(&s->x, s->a), &sin_fi, &cos_fi, mm);
+    s->p[0] = src->position[0] + r * cos_fi;
+    s->p[1] = src->position[1] + r * sin_fi;
+    s->p[2] = 0.0f;
+    o_sincos_azimuth(o_rand(&s->x, s->a), &sin_fi, &cos_fi, mm);
+    float cos_theta = 1.0f - o_rand(&s->x, s->a) * (1.0f - src->cos_min);
+    float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+    sin_theta = sin_theta / n1;
+    cos_theta = sqrtf(1.0f - sin_theta * sin_theta);
+    s->d[0] = cos_fi * sin_theta;
+    s->d[1] = sin_fi * sin_theta;
+    s->d[2] = cos_theta;
+    float R = o_reflectance_cos2(src->n, n1, s->d[2]);
+    s->w = 1.0f - R;
+    if (cfg->specular.kind != XO_DET_NONE) {
+        float p[3] = {s->d[0], s->d[1], -s->d[2]};
+        float nrm[3] = {0.0f, 0.0f, -1.0f};
+        float cos1 = p[0] * nrm[0] + p[1] * nrm[1] + p[2] * nrm[2];
+        float n1_d_n2 = n1 / src->n;
+        float sin2_squared = n1_d_n2 * n1_d_n2 * (1.0f - cos1 * cos1);
+        float k = ((cos1 >= 0.0f) ? 1.0f : -1.0f) *
+                  (n1_d_n2 * fabsf(cos1) - sqrtf(1.0f - sin2_squared));
+        float rd[3];
+        for (int i = 0; i < 3; ++i) rd[i] = n1_d_n2 * p[i] - k * nrm[i];
+        o_detector_deposit(&cfg->specular, acc, s->p[0], s->p[1], rd[0], rd[1],
+                           rd[2], R, mm);
+    }
+    s->layer = 1;
+}
+
 /* mcsim_boundary, mcml.template.c:80-203 */
 static void o_mcml_boundary(const xo_mcml_cfg *cfg, o_pkt *s, int32_t next) {
     const xo_layer *cur = &cfg->layers[s->layer];
@@ -377,8 +442,13 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
                           uint64_t *acc, o_counts *cnt, int mm) {
     const int32_t nl = (int32_t)cfg->num_layers;
     const float rmax2 = cfg->rmax * cfg->rmax;
-    const float *sp = cfg->source.position;
-    o_mcml_launch(cfg, s, acc, mm);
+    const float *sp = (cfg->source_kind == XO_SRC_FIBER_NI)
+                          ? cfg->fiber.position
+                          : cfg->source.position;
+    if (cfg->source_kind == XO_SRC_FIBER_NI)
+        o_fiber_launch(cfg, s, acc, mm);
+    else
+        o_mcml_launch(cfg, s, acc, mm);
     uint32_t pkt_steps = 0;
     for (;;) {
         const xo_layer *L = &cfg->layers[s->layer];
@@ -827,6 +897,8 @@ int oracle_batch_run(const xo_batch_instance *inst, uint32_t ninst,
         c->lottery_chance = lottery_chance;
         c->use_lottery = use_lottery;
         c->source = inst[i].source;
+        c->source_kind = inst[i].source_kind;
+        c->fiber = inst[i].fiber;
         c->top = inst[i].top;
         c->bottom = inst[i].bottom;
         c->specular = inst[i].specular;

@@ -30,7 +30,7 @@ XO_BATCH_MAX_LAYERS = 8
 
 XO_PF_HG, XO_PF_MHG, XO_PF_LUT = 0, 1, 2
 XO_DET_NONE, XO_DET_TOTAL, XO_DET_RADIAL = 0, 1, 2
-XO_SRC_LINE, XO_SRC_GAUSSIAN = 0, 1
+XO_SRC_LINE, XO_SRC_GAUSSIAN, XO_SRC_FIBER_NI = 0, 1, 2
 XO_METHOD_AW = 0
 XO_FLAG_STATIC_SCHEDULE = 1 << 0
 XO_FLAG_NO_LDS_BINS = 1 << 1
@@ -77,15 +77,22 @@ class XoFluence(ctypes.Structure):
                 ('shape', c_uint32 * 3), ('offset', c_uint32), ('k', c_int32)]
 
 
+class XoSourceFiber(ctypes.Structure):
+    _fields_ = [('position', c_float * 3), ('radius', c_float),
+                ('cos_min', c_float), ('n', c_float),
+                ('reserved', c_float * 2)]
+
+
 class XoMcmlCfg(ctypes.Structure):
     _fields_ = [('num_layers', c_uint32), ('pf_kind', c_int32),
                 ('method', c_int32), ('rmax', c_float), ('w_min', c_float),
                 ('lottery_chance', c_float), ('use_lottery', c_int32),
-                ('reserved', c_uint32), ('source', XoSourceLine),
+                ('source_kind', c_int32), ('source', XoSourceLine),
                 ('top', XoDetector), ('bottom', XoDetector),
                 ('specular', XoDetector),
                 ('layers', XoLayer * XO_MAX_LAYERS),
-                ('packet_seed', c_uint64), ('first_packet', c_uint64)]
+                ('packet_seed', c_uint64), ('first_packet', c_uint64),
+                ('fiber', XoSourceFiber)]
 
 
 class XoMaterial(ctypes.Structure):
@@ -114,11 +121,12 @@ class XoMcvoxCfg(ctypes.Structure):
 class XoBatchInstance(ctypes.Structure):
     _fields_ = [('num_layers', c_uint32), ('pf_kind', c_int32),
                 ('rmax', c_float), ('accum_base', c_uint32),
-                ('accum_count', c_uint32), ('reserved', c_uint32),
+                ('accum_count', c_uint32), ('source_kind', c_int32),
                 ('npackets', c_uint64), ('seed', c_uint64),
                 ('source', XoSourceLine), ('top', XoDetector),
                 ('bottom', XoDetector), ('specular', XoDetector),
-                ('layers', XoLayer * XO_BATCH_MAX_LAYERS)]
+                ('layers', XoLayer * XO_BATCH_MAX_LAYERS),
+                ('fiber', XoSourceFiber)]
 
 
 class XoBatchOpts(ctypes.Structure):
@@ -165,7 +173,8 @@ class XoRunReport(ctypes.Structure):
 STRUCTS_IN_ABI_ORDER = [XoPf, XoLayer, XoSourceLine, XoSourceGaussian,
                         XoDetector, XoFluence, XoMcmlCfg, XoMaterial,
                         XoVoxelCfg, XoMcvoxCfg, XoRunReport, XoBatchInstance,
-                        XoBatchOpts, XoServeOpts, XoServeStats]
+                        XoBatchOpts, XoServeOpts, XoServeStats,
+                        XoSourceFiber]
 
 # Every entry point of include/xomc.h: name -> (restype, argtypes)
 PROTOTYPES = {

@@ -28,7 +28,8 @@ static_assert(sizeof(xo_detector) == 48, "xo_detector layout");
 static_assert(sizeof(xo_fluence) == 48, "xo_fluence layout");
 static_assert(sizeof(xo_material) == 64, "xo_material layout");
 static_assert(sizeof(xo_voxel_cfg) == 48, "xo_voxel_cfg layout");
-static_assert(sizeof(xo_batch_instance) == 888, "xo_batch_instance layout");
+static_assert(sizeof(xo_source_fiber) == 32, "xo_source_fiber layout");
+static_assert(sizeof(xo_batch_instance) == 920, "xo_batch_instance layout");
 static_assert(sizeof(xo::McvoxParams) <= 4096, "kernel argument too large");
 static_assert(sizeof(xo::McmlParams) <= 4096, "kernel argument too large");
 
@@ -187,7 +188,7 @@ int xo_struct_sizes(uint32_t *out, int n) {
         (uint32_t)sizeof(xo_voxel_cfg),   (uint32_t)sizeof(xo_mcvox_cfg),
         (uint32_t)sizeof(xo_run_report),  (uint32_t)sizeof(xo_batch_instance),
         (uint32_t)sizeof(xo_batch_opts),  (uint32_t)sizeof(xo_serve_opts),
-        (uint32_t)sizeof(xo_serve_stats)};
+        (uint32_t)sizeof(xo_serve_stats), (uint32_t)sizeof(xo_source_fiber)};
     int count = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (n < count) count = n;
     for (int i = 0; i < count; ++i) out[i] = sizes[i];
@@ -351,6 +352,10 @@ int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t 
     }
     if (cfg->method != XO_METHOD_AW) {
         xo::set_error("xo_mcml_run: only the albedo-weight method is implemented");
+        return -1;
+    }
+    if (cfg->source_kind != XO_SRC_LINE && cfg->source_kind != XO_SRC_FIBER_NI) {
+        xo::set_error("xo_mcml_run: unknown source kind %d", cfg->source_kind);
         return -1;
     }
     if (naccum == 0) {
@@ -610,21 +615,12 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
         for (uint32_t l = 0; l < XO_BATCH_MAX_LAYERS; ++l)
             layers[(size_t)i * XO_BATCH_MAX_LAYERS + l] =
                 xo::layer_to_device(I.layers[l < I.num_layers ? l : I.num_layers - 1]);
-        xo::BatchHdr &H = hdr[i];
-        std::memset(&H, 0, sizeof(H));
-        H.num_layers = (int32_t)I.num_layers;
-        H.pf_kind = I.pf_kind;
-        H.rmax2 = I.rmax * I.rmax;
-        H.reflectance = I.source.reflectance;
-        for (int k = 0; k < 3; ++k) {
-            H.src_pos[k] = I.source.position[k];
-            H.dir_sample[k] = I.source.direction_sample[k];
-            H.dir_reflected[k] = I.source.direction_reflected[k];
+        if (I.source_kind != XO_SRC_LINE && I.source_kind != XO_SRC_FIBER_NI) {
+            xo::set_error("instance %u: unknown source kind %d", i, I.source_kind);
+            return -1;
         }
-        H.seed = I.seed;
-        H.top = I.top;
-        H.bottom = I.bottom;
-        H.specular = I.specular;
+        const xo_detector det[3] = {I.top, I.bottom, I.specular};
+        hdr[i] = xo::batch_header(I, det, 0u);
         for (uint64_t first = 0; first < I.npackets; first += chunk) {
             uint64_t cnt = I.npackets - first < chunk ? I.npackets - first : chunk;
             items.push_back(xo::BatchItem{i, (uint32_t)cnt, first});

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
+
 #include "../../include/xomc.h"
 #include "xo_device.h"
 #include "xo_kernels.h"
@@ -20,13 +22,42 @@ struct BatchHdr {
     int32_t pf_kind;
     float rmax2;
     float reflectance;
-    float src_pos[3];
+    float src_pos[3];           /* Line: surface point; fiber: core centre */
     float dir_sample[3];
     float dir_reflected[3];
-    uint32_t pad0;
+    int32_t src_kind;           /* XO_SRC_LINE or XO_SRC_FIBER_NI */
     uint64_t seed;
     xo_detector top, bottom, specular;
+    xo_source_fiber fiber;
 };
+
+/* Host: the per-instance header of a batch instance; accum_shift is added
+ * to every detector offset. */
+inline BatchHdr batch_header(const xo_batch_instance &I, const xo_detector det[3],
+                             uint32_t accum_shift) {
+    BatchHdr H;
+    std::memset(&H, 0, sizeof(H));
+    H.num_layers = (int32_t)I.num_layers;
+    H.pf_kind = I.pf_kind;
+    H.rmax2 = I.rmax * I.rmax;
+    H.src_kind = I.source_kind;
+    H.fiber = I.fiber;
+    const bool fiber = I.source_kind == XO_SRC_FIBER_NI;
+    H.reflectance = fiber ? 0.0f : I.source.reflectance;
+    for (int k = 0; k < 3; ++k) {
+        H.src_pos[k] = fiber ? I.fiber.position[k] : I.source.position[k];
+        H.dir_sample[k] = I.source.direction_sample[k];
+        H.dir_reflected[k] = I.source.direction_reflected[k];
+    }
+    H.seed = I.seed;
+    H.top = det[0];
+    H.bottom = det[1];
+    H.specular = det[2];
+    H.top.offset += accum_shift;
+    H.bottom.offset += accum_shift;
+    H.specular.offset += accum_shift;
+    return H;
+}
 
 /* One work-queue item: packets [first, first + count) of one instance. */
 struct BatchItem {

@@ -91,9 +91,10 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
                  * integer at the same bin: one atomic for the whole item */
                 if (lane == 0) {
                     const BatchHdr &H = P.hdr[cur_inst];
-                    detector_deposit_n(H.specular, bins, H.src_pos[0], H.src_pos[1],
-                                       H.dir_reflected[0], H.dir_reflected[1],
-                                       H.dir_reflected[2], H.reflectance, item.count);
+                    if (H.src_kind == XO_SRC_LINE)
+                        detector_deposit_n(H.specular, bins, H.src_pos[0], H.src_pos[1],
+                                           H.dir_reflected[0], H.dir_reflected[1],
+                                           H.dir_reflected[2], H.reflectance, item.count);
                 }
             }
             const uint64_t avail = cur_end - cur_next;
@@ -117,14 +118,24 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
             spy = H.src_pos[1];
             spz = H.src_pos[2];
             xo_packet_state(H.seed, my_p, P.table, &x, &a);
-            /* Line source, xopto/mcml/mcsource/line.py:95-110 */
-            w = 1.0f - H.reflectance;
-            px = spx;
-            py = spy;
-            pz = spz;
-            dx = H.dir_sample[0];
-            dy = H.dir_sample[1];
-            dz = H.dir_sample[2];
+            if (H.src_kind == XO_SRC_FIBER_NI) {
+                const float n1 = layers[1].n;
+                const float R = fiber_ni_launch(H.fiber, n1, x, a, px, py, pz, dx, dy, dz, w);
+                if (H.specular.kind != XO_DET_NONE) {
+                    float rx, ry, rz;
+                    fiber_specular_direction(dx, dy, dz, n1, H.fiber.n, rx, ry, rz);
+                    detector_deposit(H.specular, bins, px, py, rx, ry, rz, R);
+                }
+            } else {
+                /* Line source, xopto/mcml/mcsource/line.py:95-110 */
+                w = 1.0f - H.reflectance;
+                px = spx;
+                py = spy;
+                pz = spz;
+                dx = H.dir_sample[0];
+                dy = H.dir_sample[1];
+                dz = H.dir_sample[2];
+            }
             layer = 1;
             pkt_steps = 0;
             alive = true;

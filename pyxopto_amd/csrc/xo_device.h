@@ -226,6 +226,72 @@ XO_DEV float reflectance(float n1, float n2, float cos1, float cc) {
 
 XO_DEV float fsign(float v) { return (v >= 0.0f) ? 1.0f : -1.0f; }
 
+/* reflectance_cos2 (mcbase.template.c:1303-1346): Fresnel reflectance from
+ * the cosine of the transmitted angle. */
+XO_DEV float reflectance_cos2(float n1, float n2, float cos2) {
+    float R = 1.0f;
+    cos2 = fabsf(cos2);
+    if (n1 == n2) return 0.0f;
+    float sin2 = __builtin_sqrtf(1.0f - cos2 * cos2);
+    if (cos2 >= kCos0) sin2 = 0.0f;
+    float sin1 = (n2 / n1) * sin2;
+    if (sin1 < 1.0f) {
+        float cos1 = __builtin_sqrtf(1.0f - sin1 * sin1);
+        float n1_d_n2 = n1 / n2;
+        float n_cos1 = n1_d_n2 * cos1;
+        float n_cos2 = n1_d_n2 * cos2;
+        float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
+        Rs *= Rs;
+        float Rp = (n_cos2 - cos1) / (n_cos2 + cos1);
+        Rp *= Rp;
+        R = 0.5f * (Rp + Rs);
+        if (cos1 <= kCos90 || sin2 == 1.0f) return 1.0f;
+    }
+    return R;
+}
+
+/* UniformFiberNI launch (xopto/mcml/mcsource/fiberni.py:237-290): a point
+ * uniform over the core, a direction uniform in cos(theta) over the
+ * acceptance cone, refracted into the sample (n_sample, layer 1), weight
+ * 1 - R with R the Fresnel reflectance core -> sample. Returns R. */
+XO_DEV float fiber_ni_launch(const xo_source_fiber &src, float n_sample, uint64_t &x,
+                             uint32_t a, float &px, float &py, float &pz, float &dx,
+                             float &dy, float &dz, float &w) {
+    float sin_fi, cos_fi;
+    const float r = __builtin_sqrtf(rng_next(x, a)) * src.radius;
+    xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+    px = src.position[0] + r * cos_fi;
+    py = src.position[1] + r * sin_fi;
+    pz = 0.0f;
+    xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+    float cos_theta = 1.0f - rng_next(x, a) * (1.0f - src.cos_min);
+    float sin_theta = __builtin_sqrtf(1.0f - cos_theta * cos_theta);
+    sin_theta = sin_theta / n_sample;
+    cos_theta = __builtin_sqrtf(1.0f - sin_theta * sin_theta);
+    dx = cos_fi * sin_theta;
+    dy = sin_fi * sin_theta;
+    dz = cos_theta;
+    const float R = reflectance_cos2(src.n, n_sample, dz);
+    w = 1.0f - R;
+    return R;
+}
+
+/* Direction of the specularly reflected part of a fiber packet: the
+ * sample direction mirrored at the surface, (dx, dy, -dz), refracted from
+ * the sample (n1) into the core (n2) across the normal (0, 0, -1)
+ * (refract, mcbase.template.c:1443-1462). */
+XO_DEV void fiber_specular_direction(float dx, float dy, float dz, float n1, float n2,
+                                     float &rx, float &ry, float &rz) {
+    const float p0 = dx, p1 = dy, p2 = -dz;
+    const float cos1 = p0 * 0.0f + p1 * 0.0f + p2 * -1.0f;
+    const float n1_d_n2 = n1 / n2;
+    const float sin2_squared = n1_d_n2 * n1_d_n2 * (1.0f - cos1 * cos1);
+    const float k = fsign(cos1) * (n1_d_n2 * fabsf(cos1) - __builtin_sqrtf(1.0f - sin2_squared));
+    rx = n1_d_n2 * p0 - k * 0.0f;
+    ry = n1_d_n2 * p1 - k * 0.0f;
+    rz = n1_d_n2 * p2 - k * -1.0f;
+}
+
 /* Roulette (mcml.template.c:738-753); returns true if the packet ends. */
 XO_DEV bool roulette(float &w, uint64_t &x, uint32_t a, float w_min,
                      float chance, int use_lottery) {

@@ -94,9 +94,11 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     }
 
     const float rmax2 = P.cfg.rmax * P.cfg.rmax;
-    const float spx = P.cfg.source.position[0];
-    const float spy = P.cfg.source.position[1];
-    const float spz = P.cfg.source.position[2];
+    const bool fiber = P.cfg.source_kind == XO_SRC_FIBER_NI;
+    const float *sp = fiber ? P.cfg.fiber.position : P.cfg.source.position;
+    const float spx = sp[0];
+    const float spy = sp[1];
+    const float spz = sp[2];
     const float w_min = P.cfg.w_min, chance = P.cfg.lottery_chance;
     const int use_lottery = P.cfg.use_lottery;
 
@@ -145,18 +147,29 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             if (keyed)
                 xo_packet_state(P.cfg.packet_seed, P.cfg.first_packet + pkt_index, P.table, &x,
                                 &a);
-            /* Line source, xopto/mcml/mcsource/line.py:95-110 */
-            const xo_source_line &src = P.cfg.source;
-            w = 1.0f - src.reflectance;
-            px = src.position[0];
-            py = src.position[1];
-            pz = src.position[2];
-            dx = src.direction_sample[0];
-            dy = src.direction_sample[1];
-            dz = src.direction_sample[2];
-            detector_deposit(P.cfg.specular, bins, px, py, src.direction_reflected[0],
-                             src.direction_reflected[1], src.direction_reflected[2],
-                             src.reflectance);
+            if (fiber) {
+                const float n1 = sl[1].n;
+                const float R = fiber_ni_launch(P.cfg.fiber, n1, x, a, px, py, pz, dx, dy,
+                                                dz, w);
+                if (P.cfg.specular.kind != XO_DET_NONE) {
+                    float rx, ry, rz;
+                    fiber_specular_direction(dx, dy, dz, n1, P.cfg.fiber.n, rx, ry, rz);
+                    detector_deposit(P.cfg.specular, bins, px, py, rx, ry, rz, R);
+                }
+            } else {
+                /* Line source, xopto/mcml/mcsource/line.py:95-110 */
+                const xo_source_line &src = P.cfg.source;
+                w = 1.0f - src.reflectance;
+                px = src.position[0];
+                py = src.position[1];
+                pz = src.position[2];
+                dx = src.direction_sample[0];
+                dy = src.direction_sample[1];
+                dz = src.direction_sample[2];
+                detector_deposit(P.cfg.specular, bins, px, py, src.direction_reflected[0],
+                                 src.direction_reflected[1], src.direction_reflected[2],
+                                 src.reflectance);
+            }
             layer = 1;
             alive = true;
             pkt_steps = 0;

@@ -592,6 +592,10 @@ int xo_serve_admit_mcml(xo_server *S, const xo_batch_instance *inst, const float
                       XO_BATCH_MAX_LAYERS);
         return -1;
     }
+    if (I.source_kind != XO_SRC_LINE && I.source_kind != XO_SRC_FIBER_NI) {
+        xo::set_error("xo_serve_admit_mcml: unknown source kind %d", I.source_kind);
+        return -1;
+    }
     for (uint32_t l = 0; l < I.num_layers; ++l)
         if (xo::check_pf(I.pf_kind, I.layers[l].pf, nlut, "layer", l)) return -1;
     if (I.pf_kind == XO_PF_LUT && !lut) {
@@ -641,24 +645,7 @@ int xo_serve_admit_mcml(xo_server *S, const xo_batch_instance *inst, const float
         lay.pf.offset += (uint32_t)R.lut_off;
         L[l] = xo::layer_to_device(lay);
     }
-    xo::BatchHdr H;
-    std::memset(&H, 0, sizeof(H));
-    H.num_layers = (int32_t)I.num_layers;
-    H.pf_kind = I.pf_kind;
-    H.rmax2 = I.rmax * I.rmax;
-    H.reflectance = I.source.reflectance;
-    for (int k = 0; k < 3; ++k) {
-        H.src_pos[k] = I.source.position[k];
-        H.dir_sample[k] = I.source.direction_sample[k];
-        H.dir_reflected[k] = I.source.direction_reflected[k];
-    }
-    H.seed = I.seed;
-    H.top = det[0];
-    H.bottom = det[1];
-    H.specular = det[2];
-    H.top.offset += (uint32_t)R.accum_off;
-    H.bottom.offset += (uint32_t)R.accum_off;
-    H.specular.offset += (uint32_t)R.accum_off;
+    const xo::BatchHdr H = xo::batch_header(I, det, (uint32_t)R.accum_off);
     /* Uploads go on the I/O stream, which holds no tick, so admission does
      * not wait for the running tick; the slot and arena ranges are unused by
      * any tick in flight, and the host synchronizes before the request can

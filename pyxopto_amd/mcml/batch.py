@@ -91,7 +91,7 @@ class Batch(McWorker):
                 t.layers[l] = cfg.layers[l]
             t.pf_kind = inst.layers.pf_kind
             t.rmax = inst.rmax
-            inst.source.pack(packer, t.source)
+            inst.source.pack_cfg(packer, t)
             base = self._alloc_offset
             if inst.detectors is not None:
                 inst.detectors.pack(packer, t)

@@ -15,6 +15,7 @@ import numpy as np
 from .. import _native as N
 from ..mcbase import mcdetector, mcoptions, mcpf
 from ..mcbase.mcutil import axis as _axis
+from ..mcbase.mcutil import fiber as fiberutil
 from ..mcbase.mcworker import McWorker
 from . import mclayer, mcsource
 
@@ -22,7 +23,8 @@ from . import mclayer, mcsource
 mcdetector.Axis = _axis.Axis
 mcdetector.RadialAxis = _axis.RadialAxis
 
-__all__ = ['Mc', 'mclayer', 'mcsource', 'mcdetector', 'mcpf', 'mcoptions']
+__all__ = ['Mc', 'mclayer', 'mcsource', 'mcdetector', 'mcpf', 'mcoptions',
+           'fiberutil']
 
 
 class Mc(McWorker):
@@ -45,9 +47,10 @@ class Mc(McWorker):
         self._pf_type = type(self._layers[1].pf)
         self._source = source
         self._source_type = type(source)
-        if not isinstance(source, mcsource.Line):
-            raise NotImplementedError('Only the Line source is implemented '
-                                      'for layered media.')
+        if not isinstance(source, (mcsource.Line, mcsource.UniformFiberNI)):
+            raise NotImplementedError('Only the Line and UniformFiberNI '
+                                      'sources are implemented for layered '
+                                      'media.')
         self._detectors = detectors
         self._detector_types = detectors.types() if detectors else None
         self._rmax = float('inf')
@@ -93,7 +96,7 @@ class Mc(McWorker):
         cfg.pf_kind = self._layers.pf_kind
         cfg.rmax = self._rmax
         self._layers.pack(self, cfg)
-        self._source.pack(self, cfg.source)
+        self._source.pack_cfg(self, cfg)
         if self._detectors is not None:
             if self._detectors.types() != self._detector_types:
                 raise ValueError('Detector types must not change between '

@@ -1,11 +1,13 @@
 """Photon packet sources of the layered simulator: Line
-(xopto/mcml/mcsource/line.py:30-252)."""
+(xopto/mcml/mcsource/line.py:30-252) and UniformFiberNI
+(xopto/mcml/mcsource/fiberni.py:180-420)."""
 from typing import Tuple
 
 import numpy as np
 
 from .. import _native as N
 from ..mcbase.mcutil import boundary
+from ..mcbase.mcutil.fiber import MultimodeFiber
 
 
 class Line:
@@ -33,6 +35,12 @@ class Line:
                         'Source position.')
     direction = property(lambda self: self._direction, _set_direction, None,
                          'Source direction.')
+
+    def pack_cfg(self, mc, cfg):
+        """Fill the source fields of an xo_mcml_cfg or xo_batch_instance."""
+        cfg.source_kind = self.kind
+        self.pack(mc, cfg.source)
+        return cfg
 
     def pack(self, mc, target: N.XoSourceLine) -> N.XoSourceLine:
         """Project to the sample surface, refract, specular reflectance
@@ -68,3 +76,65 @@ class Line:
     def __repr__(self):
         return 'Line(position=({}, {}, {}), direction=({}, {}, {}))'.format(
             *self._position, *self._direction)
+
+
+class UniformFiberNI:
+    """A multimode fiber in contact with the sample (normal incidence):
+    packets start uniformly over the core with cos(theta) uniform over the
+    acceptance cone, refracted into the sample; the Fresnel-reflected part
+    goes to the specular detector (xopto/mcml/mcsource/fiberni.py:180-420).
+    """
+    kind = N.XO_SRC_FIBER_NI
+
+    def __init__(self, fiber: MultimodeFiber,
+                 position: Tuple[float, float, float] = (0.0, 0.0, 0.0)):
+        self._fiber = fiber
+        self._position = np.zeros(3)
+        self.position = position
+
+    def _set_position(self, v):
+        self._position[:] = v
+        self._position[2] = 0.0
+
+    def _set_fiber(self, fib):
+        self._fiber = fib
+
+    fiber = property(lambda self: self._fiber, _set_fiber, None,
+                     'Multimode optical fiber.')
+    position = property(lambda self: self._position, _set_position, None,
+                        'Source position (z is always 0).')
+
+    def pack_cfg(self, mc, cfg):
+        """cl_pack of fiberni.py:341-375 into cfg.fiber."""
+        cfg.source_kind = self.kind
+        f = cfg.fiber
+        f.n = self._fiber.ncore
+        f.cos_min = (1.0 - self._fiber.na ** 2) ** 0.5
+        for i in range(3):
+            f.position[i] = self._position[i]
+        f.radius = self._fiber.dcore * 0.5
+        return cfg
+
+    def update(self, other):
+        if isinstance(other, UniformFiberNI):
+            self.fiber = other.fiber
+            self.position = other.position
+        elif isinstance(other, dict):
+            self.fiber = other.get('fiber', self.fiber)
+            self.position = other.get('position', self.position)
+
+    def todict(self):
+        return {'fiber': self._fiber.todict(),
+                'position': self._position.tolist(),
+                'type': 'UniformFiberNI'}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'UniformFiberNI':
+        data = dict(data)
+        data.pop('type', None)
+        fiber = MultimodeFiber.fromdict(data.pop('fiber'))
+        return cls(fiber, **data)
+
+    def __repr__(self):
+        return 'UniformFiberNI(fiber={}, position=({}, {}, {}))'.format(
+            self._fiber, *self._position)

@@ -13,6 +13,8 @@
   n 1.0 with no absorption or scattering.
 - ``layered_batch``: 4096 layered instances with 1-5 turbid layers and
   uneven packet budgets for the batched engine (see its docstring).
+- ``fiber_*``: the reference's published benchmark (400 single-layer runs
+  under a multimode fiber, 1e7 packets each), see FIBER_* below.
 - ``skin_two_layer``: 64^3 voxels of 31.25 um (a 2 mm cube), epidermis
   100 um over dermis (the skin_vessel_200 materials without the vessel),
   Gaussian beam sigma 50 um, deposition in every voxel (64^3 u64 bins,
@@ -98,6 +100,66 @@ def skin_two_layer(n: int = 64, size: float = 2e-3,
     mat[z <= epidermis] = 1
     sim.voxels.update()
     return sim
+
+
+# The reference's published benchmark (test/performance.py:36, 115-130 with
+# xopto/mcml/test/validate.py:348-511): a 20 x 20 grid of mua 1-2500 /m and
+# mus' 500-6000 /m of one 8 mm layer (n 1.33, HG g 0.85) under a 200 um
+# core, NA 0.22 fiber (n 1.452, also the medium above), radial detector
+# 0-1.7 mm in 340 bins collecting within the fiber acceptance, rmax 5 mm,
+# 1e7 packets per grid point.
+FIBER_GRID_MUA = np.linspace(1.0, 2500.0, 20)
+FIBER_GRID_MUSR = np.linspace(500.0, 6000.0, 20)
+FIBER_PACKETS = 10_000_000
+FIBER_SDS = np.array([220e-6, 440e-6, 660e-6, 880e-6, 1100e-6])
+
+
+def fiber_layers(mua: float = 0.0, musr: float = 0.0, g: float = 0.85):
+    L = mcml.mclayer
+    pf = mcml.mcpf.Hg(g)
+    return L.Layers([
+        L.Layer(d=np.inf, n=1.452, mua=0.0, mus=0.0, pf=pf),
+        L.Layer(d=8e-3, n=1.33, mua=mua, mus=musr / (1.0 - g), pf=pf),
+        L.Layer(d=np.inf, n=1.33, mua=0.0, mus=0.0, pf=pf)])
+
+
+def fiber_source():
+    fib = mcml.fiberutil.MultimodeFiber(dcore=200e-6, dcladding=200e-6,
+                                        ncore=1.452, na=0.22)
+    return mcml.mcsource.UniformFiberNI(fib)
+
+
+def fiber_detectors():
+    D = mcml.mcdetector
+    cosmin = (1.0 - (0.22 / 1.452) ** 2) ** 0.5
+    return D.Detectors(top=D.Radial(D.RadialAxis(0.0, 1.7e-3, 340),
+                                    cosmin=cosmin))
+
+
+def fiber_grid():
+    """(mua, mus') of the 400 runs, in the reference's order (mua major)."""
+    mua, musr = np.meshgrid(FIBER_GRID_MUA, FIBER_GRID_MUSR, indexing='ij')
+    return mua.ravel(), musr.ravel()
+
+
+def fiber_sim(rnginit: int = 0x5EED000000000001):
+    """The benchmark's simulator (mua, mus' set per run, as the reference
+    test does)."""
+    sim = mcml.Mc(fiber_layers(), fiber_source(), fiber_detectors(),
+                  rnginit=rnginit)
+    sim.rmax = 5e-3
+    return sim
+
+
+def fiber_batch(packets: int = FIBER_PACKETS, seed: int = 20261204):
+    """The 400 runs of the benchmark as instances of one batch."""
+    from .mcml import batch as mbatch
+    out = []
+    for k, (mua, musr) in enumerate(zip(*fiber_grid())):
+        out.append(mbatch.Instance(fiber_layers(mua, musr), fiber_source(),
+                                   fiber_detectors(), nphotons=packets,
+                                   seed=seed * 1000 + k, rmax=5e-3))
+    return out
 
 
 VOXEL_REQUEST_PACKETS = 1_000_000

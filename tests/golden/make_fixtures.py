@@ -13,6 +13,11 @@ Outputs (data only; no reference source is copied):
       the primitive values (float, int, str, bytes) and containers are
       rebuilt; the ndarray payload is read from its raw bytes with the
       dtype string ('f8', little endian) and shape recorded next to it.
+  cuda_singlelayer_uniformfiber.npz, cuda_doublelayer_uniformfiber.npz
+      The fiber-collected reflectance of the reference's uniform-fiber
+      tests (20 x 20 grid of mua, mus' of the varied layer, 5 source-
+      detector separations) and the parameters of those runs, decoded the
+      same way from the .pkl files of the same folder.
   seeds.npz
       Rows 0..4095 and a SHA-256 of the multiplier column of
       xopto/data/primes/safeprimes_base32_500k.npz (np.load without
@@ -141,6 +146,43 @@ def make_fixture_radial(ref_root):
     return out
 
 
+def make_fixture_fiber(ref_root, name):
+    """The CUDAMCML-derived fiber reflectance of the reference's uniform
+    fiber tests (xopto/mcml/test/validate.py:348-511 single layer,
+    :692-850 double layer): reflectance[mua][musr][sds] and the parameters
+    of the run, decoded statically like the radial fixture."""
+    src = os.path.join(ref_root, 'xopto', 'mcml', 'test', 'reference',
+                       name + '.pkl')
+    d = _static_pickle_values(src)
+    layers = d['layers']
+    out = {
+        'reflectance': np.asarray(d['reflectance'], dtype=np.float64),
+        'layer_n': np.array([l['basic']['n'] for l in layers]),
+        'layer_d': np.array([l['basic']['d'] for l in layers]),
+        'layer_g': np.array([l['pf']['pfparams'][0] for l in layers]),
+        'source_na': d['source']['na'],
+        'source_dcore': d['source']['dcore'],
+        'source_dcladding': d['source']['dcladding'],
+        'source_ncore': d['source']['ncore'],
+        'axis_start': d['detector']['axis']['start'],
+        'axis_stop': d['detector']['axis']['stop'],
+        'axis_n': d['detector']['axis']['n'],
+        'cosmin': d['detector']['cosmin'],
+        'sds': np.array(d['detector']['fibers']['sds'], dtype=np.float64),
+        'fiber_dcore': d['detector']['fibers']['dcore'],
+        'rmax': d['mc']['rmax'],
+    }
+    lut = d['lut']
+    if isinstance(lut, list):           # double layer: fixed top layer
+        out['layer1_mua'] = float(np.asarray(lut[0]['mua']).ravel()[0])
+        out['layer1_musr'] = float(np.asarray(lut[0]['musr']).ravel()[0])
+        lut = lut[1]
+    out['lut_mua'] = np.asarray(lut['mua'], dtype=np.float64)
+    out['lut_musr'] = np.asarray(lut['musr'], dtype=np.float64)
+    np.savez(os.path.join(HERE, name + '.npz'), **out)
+    return out
+
+
 def make_seeds(ref_root):
     table = np.load(os.path.join(ref_root, 'xopto', 'data', 'primes',
                                  'safeprimes_base32_500k.npz'))['data']
@@ -193,5 +235,8 @@ if __name__ == '__main__':
     fx = make_fixture_radial(ref)
     print('radial fixture:', fx['reflectance'].shape, fx['layer_mua'],
           fx['layer_mus'], fx['rmax'])
+    for name in ('cuda_singlelayer_uniformfiber', 'cuda_doublelayer_uniformfiber'):
+        fx = make_fixture_fiber(ref, name)
+        print(name, fx['reflectance'].shape, fx['sds'], fx['layer_n'])
     make_seeds(ref)
     print('seeds written')
