@@ -400,7 +400,9 @@ typedef struct xo_serve_opts {
     uint32_t instance_slots;    /* layered requests resident at once         */
     uint32_t max_items;         /* work-queue items per tick                 */
     uint32_t vox_lanes;         /* lane cap of a voxel launch                */
-    uint32_t chunk;             /* packets per work-queue item               */
+    uint32_t chunk;             /* at most this many packets per work-queue
+                                   item (fewer when a tick would otherwise
+                                   leave waves of the grid without an item)  */
     uint32_t block;             /* threads per block of the batch kernel     */
     uint32_t blocks_per_cu;     /* 0: as many as fit                          */
     float w_min;                /* roulette threshold (layered requests)     */

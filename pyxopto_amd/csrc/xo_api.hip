@@ -6,6 +6,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -631,6 +633,15 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
             return -1;
         }
     }
+
+    /* Waves claim consecutive items: grouping the items of instances that
+     * run the same code (phase-function kind, source, layer count) keeps the
+     * lanes of a wave on one path when it moves from item to item. The
+     * order does not change any result. */
+    std::stable_sort(items.begin(), items.end(),
+                     [inst](const xo::BatchItem &p, const xo::BatchItem &q) {
+                         return xo::item_class(inst[p.inst]) < xo::item_class(inst[q.inst]);
+                     });
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     XO_HIP(hipSetDevice(ctx->device));

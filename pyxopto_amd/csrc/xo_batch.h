@@ -31,6 +31,12 @@ struct BatchHdr {
     xo_source_fiber fiber;
 };
 
+/* Host: instances of one class run the same branches of the transport
+ * step (phase-function kind, source kind, number of layers). */
+inline int item_class(const xo_batch_instance &I) {
+    return (I.pf_kind * 4 + I.source_kind) * 16 + (int)I.num_layers;
+}
+
 /* Host: the per-instance header of a batch instance; accum_shift is added
  * to every detector offset. */
 inline BatchHdr batch_header(const xo_batch_instance &I, const xo_detector det[3],

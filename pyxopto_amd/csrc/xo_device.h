@@ -366,6 +366,66 @@ XO_DEV uint64_t readlane_u64(uint64_t v, uint32_t lane) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+/* Dynamic schedule of the single-simulation kernels: a wave takes packet
+ * indices from the launch's global counter in blocks of `claim` with one
+ * atomic and hands them to its idle lanes in lane order, refilling only
+ * once refill_min lanes (or all of its live lanes) wait. One counter shared
+ * by every wave of the grid would otherwise see an atomic every couple of
+ * transport steps per wave. Wave-uniform state. */
+struct WaveDispenser {
+    uint64_t next = 0, end = 0;
+    bool empty = false;
+};
+
+/* Returns true (and the index in pkt) for a lane that gets a packet; sets
+ * retire for a lane that wants one when the counter is exhausted; claimed
+ * (wave-uniform) is the number of packets of the blocks taken. */
+XO_DEV bool dispense(WaveDispenser &q, bool want, bool alive, unsigned long long *counter,
+                     uint64_t npackets, uint32_t claim, uint32_t refill_min, uint64_t &pkt,
+                     bool &retire, uint64_t &claimed) {
+    claimed = 0;
+    retire = false;
+    const uint64_t m = __ballot(want);
+    const uint32_t nwant = (uint32_t)__builtin_popcountll(m);
+    if (nwant == 0) return false;
+    const uint32_t nalive = (uint32_t)__builtin_popcountll(__ballot(alive));
+    if (nwant < refill_min && nalive != 0) return false;
+    const uint32_t rank = prefix_count(m);
+    const uint32_t lane = lane_id();
+    const uint32_t leader = (uint32_t)__builtin_ctzll(__ballot(true));
+    uint32_t assigned = 0;
+    bool got = false;
+    while (assigned < nwant) {
+        if (q.next >= q.end) {
+            if (q.empty) break;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(counter, (unsigned long long)claim);
+            base = readlane_u64(base, leader);
+            if (base >= npackets) {
+                q.empty = true;
+                break;
+            }
+            q.next = base;
+            q.end = base + claim < npackets ? base + claim : npackets;
+            claimed += q.end - q.next;
+        }
+        const uint64_t avail = q.end - q.next;
+        const uint32_t left = nwant - assigned;
+        const uint32_t take = avail < (uint64_t)left ? (uint32_t)avail : left;
+        if (want && rank >= assigned && rank < assigned + take) {
+            got = true;
+            pkt = q.next + (rank - assigned);
+        }
+        q.next += take;
+        assigned += take;
+    }
+    if (want && !got && q.empty) retire = true;
+    return got;
+}
+
+constexpr uint32_t kClaim = 128;    /* packets per counter atomic */
+constexpr uint32_t kRefillMin = 4;  /* idle lanes that trigger a refill */
+
 } // namespace xo
 
 #endif /* XO_DEVICE_H */

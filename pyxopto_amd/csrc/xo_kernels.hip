@@ -103,6 +103,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     const int use_lottery = P.cfg.use_lottery;
 
     bool active = lane_ok;
+    WaveDispenser wq;
     bool alive = false;
     float px = 0.f, py = 0.f, pz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, w = 0.f;
     int32_t layer = 1;
@@ -125,22 +126,19 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 }
             }
         } else {
-            uint64_t m = __ballot(want);
-            if (m) {
-                uint32_t leader = (uint32_t)__builtin_ctzll(m);
-                uint64_t base = 0;
-                if (lane_id() == leader)
-                    base = atomicAdd(P.counter, (unsigned long long)__builtin_popcountll(m));
-                base = readlane_u64(base, leader);
-                if (want) {
-                    uint64_t idx = base + prefix_count(m);
-                    if (idx < P.npackets) {
-                        launch = true;
-                        pkt_index = idx;
-                    } else {
-                        active = false;
-                    }
-                }
+            bool retire = false;
+            uint64_t claimed = 0;
+            launch = dispense(wq, want, alive, P.counter, P.npackets, kClaim, kRefillMin,
+                              pkt_index, retire, claimed);
+            if (retire) active = false;
+            /* every Line packet deposits the same specular integer at the
+             * same bin: one atomic per claimed block */
+            if (claimed && !fiber && lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
+                const xo_source_line &src = P.cfg.source;
+                detector_deposit_n(P.cfg.specular, bins, src.position[0], src.position[1],
+                                   src.direction_reflected[0], src.direction_reflected[1],
+                                   src.direction_reflected[2], src.reflectance,
+                                   (uint32_t)claimed);
             }
         }
         if (launch) {
@@ -166,9 +164,10 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 dx = src.direction_sample[0];
                 dy = src.direction_sample[1];
                 dz = src.direction_sample[2];
-                detector_deposit(P.cfg.specular, bins, px, py, src.direction_reflected[0],
-                                 src.direction_reflected[1], src.direction_reflected[2],
-                                 src.reflectance);
+                if (static_sched)
+                    detector_deposit(P.cfg.specular, bins, px, py, src.direction_reflected[0],
+                                     src.direction_reflected[1], src.direction_reflected[2],
+                                     src.reflectance);
             }
             layer = 1;
             alive = true;
@@ -366,6 +365,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     const int use_lottery = P.cfg.use_lottery;
 
     bool active = lane_ok;
+    WaveDispenser wq;
     bool alive = false;
     float p[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f};
     float w = 0.f;
@@ -389,22 +389,17 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 }
             }
         } else {
-            uint64_t m = __ballot(want);
-            if (m) {
-                uint32_t leader = (uint32_t)__builtin_ctzll(m);
-                uint64_t base = 0;
-                if (lane_id() == leader)
-                    base = atomicAdd(P.counter, (unsigned long long)__builtin_popcountll(m));
-                base = readlane_u64(base, leader);
-                if (want) {
-                    uint64_t idx = base + prefix_count(m);
-                    if (idx < P.npackets) {
-                        launch = true;
-                        pkt_index = idx;
-                    } else {
-                        active = false;
-                    }
-                }
+            bool retire = false;
+            uint64_t claimed = 0;
+            launch = dispense(wq, want, alive, P.counter, P.npackets, kClaim, kRefillMin,
+                              pkt_index, retire, claimed);
+            if (retire) active = false;
+            if (claimed && line_src && lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
+                const xo_source_line &src = P.cfg.line;
+                detector_deposit_n(P.cfg.specular, bins, src.position[0], src.position[1],
+                                   src.direction_reflected[0], src.direction_reflected[1],
+                                   src.direction_reflected[2], src.reflectance,
+                                   (uint32_t)claimed);
             }
         }
         if (launch) {
@@ -419,9 +414,10 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                     p[k] = src.position[k];
                     d[k] = src.direction_sample[k];
                 }
-                detector_deposit(P.cfg.specular, bins, p[0], p[1],
-                                 src.direction_reflected[0], src.direction_reflected[1],
-                                 src.direction_reflected[2], src.reflectance);
+                if (static_sched)
+                    detector_deposit(P.cfg.specular, bins, p[0], p[1],
+                                     src.direction_reflected[0], src.direction_reflected[1],
+                                     src.direction_reflected[2], src.reflectance);
             } else {
                 /* xopto/mcvox/mcsource/gaussianbeam.py:112-189 */
                 const xo_source_gaussian &src = P.cfg.gaussian;

@@ -100,6 +100,7 @@ struct Request {
     uint64_t lut_off = 0, lut_n = 0;
     uint64_t vox_off = 0, vox_n = 0;
     uint32_t slot = 0;
+    int cls = 0; /* xo::item_class of a layered request */
     uint32_t lds_bins = 0;
     xo_mcvox_cfg vcfg;
     uint64_t steps = 0, boundary = 0, crossings = 0, deposits = 0;
@@ -334,9 +335,18 @@ int dispatch_tick(xo_server *S, int k) {
         ml.push_back(id);
     }
     if (ml.empty()) return 1;
+    /* requests of one class next to each other (same branches, see
+     * xo::item_class): waves moving between their items stay coherent */
+    std::stable_sort(ml.begin(), ml.end(), [S](uint64_t p, uint64_t q) {
+        return S->reqs[p].cls < S->reqs[q].cls;
+    });
     T.nitems = 0;
     T.packets = 0;
-    const uint64_t chunk = S->opts.chunk;
+    /* at most opts.chunk packets per item, but enough items for every wave
+     * of the persistent grid to get one */
+    const uint64_t waves = (uint64_t)S->ml_blocks * (S->opts.block / 64u);
+    const uint64_t chunk = std::max<uint64_t>(
+        256u, std::min<uint64_t>(S->opts.chunk, S->opts.tick_packets / std::max<uint64_t>(waves, 1)));
     {
         uint64_t share = S->opts.tick_packets / ml.size();
         share = std::max<uint64_t>(share, chunk);
@@ -448,7 +458,7 @@ void xo_serve_default_opts(xo_serve_opts *o) {
     o->voxel_capacity = 256ull << 20;  /* 256 MiB of uint8 voxels */
     o->instance_slots = 16384;
     o->max_items = 65536;
-    o->tick_packets = 8000000;
+    o->tick_packets = 16000000;
     o->vox_tick_packets = 500000;
     o->vox_lanes = 131072;
     o->chunk = 2048;
@@ -636,6 +646,7 @@ int xo_serve_admit_mcml(xo_server *S, const xo_batch_instance *inst, const float
     R.lut_n = nlut;
     R.slot = S->free_slots.back();
     S->free_slots.pop_back();
+    R.cls = xo::item_class(I);
     R.npackets = I.npackets;
     R.id = S->next_id++;
     /* rebase into the arenas */
