@@ -160,6 +160,100 @@ __global__ void sum_splits_kernel(const float *P, uint32_t splits, uint64_t n, f
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Fused inference: every layer of a row block in one launch            */
+/* ------------------------------------------------------------------ */
+/* A 256-thread block takes 64 rows through the whole network: the
+ * activations stay in one LDS tile (64 x 256 bf16), each layer is a 64 x N
+ * product of that tile with the layer's weights read from L2 as bf16
+ * fragments (transposed copies [N][K], so a lane's 8 consecutive k are one
+ * 16-byte load), bias + ReLU applied in registers and the result written
+ * back over the tile. Networks: input width <= 32, hidden widths 256,
+ * output width <= 128, at most 8 layers. */
+constexpr int FBM = 64, FHID = 256, FSTRIDE = FHID + 8, FMAXL = 8;
+
+struct FusedArgs {
+    const float *X;
+    float *Y;
+    int M, nin, nout, nlayers;
+    const uint16_t *wt[FMAXL];  /* [Npad][Kpad]: Kpad 32 (first) or 256,
+                                   Npad 256 (hidden) or 128 (last) */
+    const float *bias[FMAXL];
+};
+
+__global__ void __launch_bounds__(256) mlp_fused_kernel(const FusedArgs f) {
+    __shared__ __attribute__((aligned(16))) uint16_t hb[FBM * FSTRIDE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row0 = blockIdx.x * FBM;
+    for (int e = tid; e < FBM * 32; e += 256) {
+        const int m = e >> 5, c = e & 31;
+        const float v = (row0 + m < f.M && c < f.nin) ? f.X[(size_t)(row0 + m) * f.nin + c] : 0.f;
+        hb[m * FSTRIDE + c] = f2bf(v);
+    }
+    __syncthreads();
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    for (int l = 0; l < f.nlayers; ++l) {
+        const bool last = l + 1 == f.nlayers;
+        const int K = l == 0 ? 32 : FHID;
+        const uint16_t *W = f.wt[l];
+        floatx4 acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        /* hidden: wave w owns columns 64 w .. 64 w + 63; last: 16-column
+         * tiles w and w + 4 of the 128 padded columns */
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            bf16x8 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                a[i] = *reinterpret_cast<const bf16x8 *>(&hb[(i * 16 + fr) * FSTRIDE + k0 + fk]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = last ? ((wave + 4 * (j & 1)) * 16 + fr) : (wave * 64 + j * 16 + fr);
+                b[j] = *reinterpret_cast<const bf16x8 *>(&W[(size_t)n * K + k0 + fk]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (!last || j < 2)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j],
+                                                                           0, 0, 0);
+        }
+        __syncthreads(); /* every wave has read the tile */
+        const float *bias = f.bias[l];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = i * 16 + (lane >> 4) * 4 + r;
+                    if (!last) {
+                        const int n = wave * 64 + j * 16 + fr;
+                        const float z = acc[i][j][r] + bias[n];
+                        hb[m * FSTRIDE + n] = f2bf(z > 0.f ? z : 0.f);
+                    } else if (j < 2) {
+                        const int n = (wave + 4 * j) * 16 + fr;
+                        if (n < f.nout && row0 + m < f.M)
+                            f.Y[(size_t)(row0 + m) * f.nout + n] = acc[i][j][r] + bias[n];
+                    }
+                }
+        __syncthreads(); /* the tile holds this layer's output */
+    }
+}
+
+/* dst[n][k] = bf16(w[k][n]) for k < in, n < out; zero elsewhere in
+ * [npad][kpad]. */
+__global__ void transpose_pad_kernel(const float *w, int in, int out, int kpad, int npad,
+                                     uint16_t *dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kpad * npad) return;
+    const int n = i / kpad, k = i % kpad;
+    dst[i] = (k < in && n < out) ? f2bf(w[(size_t)k * out + n]) : (uint16_t)0;
+}
+
 __global__ void to_bf16_kernel(const float *x, uint16_t *y, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -272,6 +366,9 @@ struct xo_mlp {
     std::vector<uint16_t *> h;        /* activations of hidden layers */
     uint16_t *d0 = nullptr, *d1 = nullptr; /* [B][max width] gradients */
     float *y = nullptr;               /* [B][out] */
+    uint16_t *wt = nullptr;           /* padded transposed bf16 weights */
+    std::vector<uint64_t> wt_off;
+    bool fusable = false, wt_dirty = true;
     float *work = nullptr;            /* split-K partial products */
     float *loss = nullptr;            /* per training step */
     uint32_t loss_cap = 0;
@@ -426,6 +523,37 @@ int refresh_bf16(xo_mlp *N) {
     return 0;
 }
 
+int refresh_transposed(xo_mlp *N) {
+    for (int l = 0; l < N->nlayers; ++l) {
+        const int kpad = l == 0 ? 32 : FHID, npad = l + 1 == N->nlayers ? 128 : FHID;
+        hipLaunchKernelGGL(transpose_pad_kernel, dim3((kpad * npad + 255) / 256), dim3(256), 0,
+                           N->stream, N->w + N->woff[l], N->dims[l], N->dims[l + 1], kpad, npad,
+                           N->wt + N->wt_off[l]);
+        MLP_CHECK(hipGetLastError());
+    }
+    N->wt_dirty = false;
+    return 0;
+}
+
+/* Fused forward pass of B rows in N->xf (fp32) into N->y. */
+int forward_fused(xo_mlp *N, uint32_t B) {
+    FusedArgs f;
+    std::memset(&f, 0, sizeof(f));
+    f.X = N->xf;
+    f.Y = N->y;
+    f.M = (int)B;
+    f.nin = N->dims[0];
+    f.nout = N->dims[N->nlayers];
+    f.nlayers = N->nlayers;
+    for (int l = 0; l < N->nlayers; ++l) {
+        f.wt[l] = N->wt + N->wt_off[l];
+        f.bias[l] = N->w + N->boff[l];
+    }
+    hipLaunchKernelGGL(mlp_fused_kernel, dim3((B + FBM - 1) / FBM), dim3(256), 0, N->stream, f);
+    MLP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int upload_rows(xo_mlp *N, const float *x, uint32_t B) {
     MLP_CHECK(hipMemcpyAsync(N->xf, x, sizeof(float) * B * N->dims[0], hipMemcpyHostToDevice,
                              N->stream));
@@ -447,7 +575,7 @@ void xo_mlp_destroy(xo_mlp *N) {
     if (N->graph) (void)hipGraphDestroy(N->graph);
     void *bufs[] = {N->w, N->m, N->v, N->grad, N->wbf, N->is_matrix, N->bf_index, N->xb,
                     N->xf, N->tb, N->d0, N->d1, N->y, N->work, N->loss, N->step,
-                    N->data_x, N->data_t};
+                    N->data_x, N->data_t, N->wt};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto p : N->h)
@@ -515,6 +643,20 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
             return bad("allocation", e);
         N->h.push_back(p);
     }
+    /* the fused inference kernel covers input <= 32, hidden 256, output
+     * <= 128, at most 8 layers */
+    N->fusable = nlayers >= 2 && nlayers <= (uint32_t)FMAXL && dims[0] <= 32 &&
+                 dims[nlayers] <= 128;
+    for (uint32_t l = 1; l < nlayers; ++l) N->fusable = N->fusable && dims[l] == (uint32_t)FHID;
+    if (N->fusable) {
+        uint64_t total = 0;
+        for (uint32_t l = 0; l < nlayers; ++l) {
+            N->wt_off.push_back(total);
+            const uint64_t kpad = l == 0 ? 32 : FHID, npad = l + 1 == nlayers ? 128 : FHID;
+            total += kpad * npad;
+        }
+        if ((e = hipMalloc(&N->wt, 2 * total)) != hipSuccess) return bad("allocation", e);
+    }
     /* which parameters are matrix entries, and where their bf16 copy is */
     std::vector<uint8_t> ism(P, 0);
     std::vector<uint64_t> idx(P, 0);
@@ -553,6 +695,7 @@ int xo_mlp_set_params(xo_mlp *N, const float *params, uint64_t n) {
     MLP_CHECK(hipMemsetAsync(N->v, 0, 4 * n, N->stream));
     MLP_CHECK(hipMemsetAsync(N->step, 0, 4, N->stream));
     if (refresh_bf16(N)) return -1;
+    N->wt_dirty = true;
     MLP_CHECK(hipStreamSynchronize(N->stream));
     return 0;
 }
@@ -585,11 +728,15 @@ int xo_mlp_forward(xo_mlp *N, const float *x, uint64_t n, float *y, double *kern
     MLP_CHECK(hipEventCreate(&e1));
     double total = 0.0;
     int rc = 0;
+    if (N->fusable && N->wt_dirty && refresh_transposed(N)) rc = -1;
     for (uint64_t r = 0; r < n && rc == 0; r += N->max_batch) {
         const uint32_t B = (uint32_t)(n - r < N->max_batch ? n - r : N->max_batch);
-        if (upload_rows(N, x + r * nin, B)) { rc = -1; break; }
+        if (N->fusable) {
+            if (hipMemcpyAsync(N->xf, x + r * nin, sizeof(float) * B * nin,
+                               hipMemcpyHostToDevice, N->stream) != hipSuccess) { rc = -1; break; }
+        } else if (upload_rows(N, x + r * nin, B)) { rc = -1; break; }
         if (hipEventRecord(e0, N->stream) != hipSuccess) { rc = -1; break; }
-        if (forward(N, B)) { rc = -1; break; }
+        if (N->fusable ? forward_fused(N, B) : forward(N, B)) { rc = -1; break; }
         if (hipEventRecord(e1, N->stream) != hipSuccess) { rc = -1; break; }
         if (hipMemcpyAsync(y + r * nout, N->y, 4ull * B * nout, hipMemcpyDeviceToHost,
                            N->stream) != hipSuccess ||
@@ -674,6 +821,7 @@ int xo_mlp_train(xo_mlp *N, uint32_t steps, uint32_t batch, float lr, uint64_t s
     }
     if (steps == 0) return 0;
     std::lock_guard<std::mutex> lock(N->mu);
+    N->wt_dirty = true;
     MLP_CHECK(hipSetDevice(N->ctx->device));
     if (steps > N->loss_cap) {
         if (N->loss) (void)hipFree(N->loss);

@@ -81,6 +81,27 @@ def test_gradient_matches_cpu_restatement(ctx):
     net.close()
 
 
+@pytest.mark.parametrize('rows', [1, 63, 64, 1000, 4096])
+def test_fused_inference_equals_layer_by_layer(ctx, rows):
+    """forward() runs the fused kernel (activations kept in LDS); the
+    gradient path runs one product per layer. Same products in the same
+    order: the network outputs agree to the bit, which the gradient's loss
+    against the fused outputs shows as exactly zero."""
+    dims = S.DIMS
+    net = S.Network(dims, ctx, max_batch=4096)
+    rng = np.random.default_rng(rows)
+    p = M.init_params(dims, seed=rows + 1)
+    p[-100:] = rng.normal(0, 0.1, 100)
+    net.set_params(p)
+    x = rng.uniform(-1, 1, (rows, 5)).astype(np.float32)
+    y, _ = net.forward(x)
+    loss, _ = net.gradient(x, y)
+    assert loss == 0.0
+    yref, _ = M.forward(p, x, dims, emulate_bf16=True)
+    assert np.linalg.norm(y - yref) / np.linalg.norm(yref) < 1e-3
+    net.close()
+
+
 def test_captured_steps_equal_eager_steps(ctx):
     dims = (5, 64, 64, 10)
     rng = np.random.default_rng(7)
