@@ -38,8 +38,8 @@ SERVING_QPS = 250.0                     # requests/s of the serving run
 # 1 GiB device-to-device copy (profiles/batch_kernel_layered_batch_4096.md,
 # profiles/mcvox_kernel_skin_vessel_200.md). Offline figures: measured for
 # the kernels as they are in this tree.
-TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 5.327e10,
-                            'skin_vessel_200': 1.703e12}
+TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 1.561e11,
+                            'skin_vessel_200': 7.344e11}
 
 METRIC = ('photon packets/s (1 GPU) batched vs plain; p50/p99 solve latency '
           'at fixed QPS')
@@ -127,10 +127,13 @@ def ceilings(ctx):
     if total != natom:
         raise RuntimeError('atomic probe lost updates: {} of {}'.format(
             total, natom))
-    return {'copy_GBps': 2 * nbytes / (ms * 1e-3) / 1e9,
+    vms, vops = ctx.probe_valu(20000)
+    return {'valu_fma_lane_ops_per_s': vops / (vms * 1e-3),
+            'copy_GBps': 2 * nbytes / (ms * 1e-3) / 1e9,
             'atomic_u64_per_s': natom / (ams * 1e-3),
             'atomic_u64_GBps': 8 * natom / (ams * 1e-3) / 1e9,
-            'probe': 'copy 1 GiB x10; 2^28 u64 atomics, 64 MiB array'}
+            'probe': 'fp32 FMA chains 8 x 20000 per lane; copy 1 GiB x10; '
+                     '2^28 u64 atomics, 64 MiB array'}
 
 
 def serving(insts, qps: float, duration: float, seed: int = 20261203,

@@ -477,6 +477,12 @@ int xo_probe_copy(xo_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms_per_cop
 int xo_probe_atomic_u64(xo_ctx *ctx, uint64_t array_bytes, uint64_t natomics,
                         uint64_t seed, double *ms, uint64_t *checksum);
 
+/* Eight independent fp32 fused multiply-add chains per lane, iters rounds,
+ * 32 blocks of 256 threads per CU: kernel ms and the lane-operations done.
+ * The VALU issue ceiling the transport kernels are compared against. No
+ * counterpart in the reference. */
+int xo_probe_valu(xo_ctx *ctx, uint32_t iters, double *ms, double *lane_ops);
+
 /* ---------------- reflectance surrogate (device) ---------------- */
 
 /* A fully connected ReLU network (widths dims[0..nlayers]) trained and

@@ -250,6 +250,8 @@ PROTOTYPES = {
                               POINTER(c_double)]),
     'xo_probe_atomic_u64': (c_int, [c_void_p, c_uint64, c_uint64, c_uint64,
                                     POINTER(c_double), POINTER(c_uint64)]),
+    'xo_probe_valu': (c_int, [c_void_p, c_uint32, POINTER(c_double),
+                              POINTER(c_double)]),
 }
 
 _lib = None
@@ -464,6 +466,13 @@ class Context:
                                   c_uint32(reps), ctypes.byref(ms)),
               'xo_probe_copy')
         return ms.value
+
+    def probe_valu(self, iters: int = 20000):
+        """(kernel ms, lane-operations) of the fp32 FMA throughput probe."""
+        ms, ops = c_double(), c_double()
+        check(lib().xo_probe_valu(self._h, c_uint32(iters), ctypes.byref(ms),
+                                  ctypes.byref(ops)), 'xo_probe_valu')
+        return ms.value, ops.value
 
     def probe_atomic_u64(self, array_bytes: int, natomics: int,
                          seed: int = 1):

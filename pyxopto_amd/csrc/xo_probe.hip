@@ -1,6 +1,7 @@
 /*
- * xo_probe.hip — the two measured ceilings the transport kernels are
- * compared against: a streaming device-to-device copy (HBM bandwidth) and
+ * xo_probe.hip — the measured ceilings the kernels are compared against:
+ * fp32 fused multiply-add throughput (the VALU issue rate of the transport
+ * kernels), a streaming device-to-device copy (HBM bandwidth) and
  * random-address u64 atomic adds over a large array (the HBM/L2 atomic
  * rate the fluence and detector deposits run into). No counterpart in the
  * reference.
@@ -38,9 +39,56 @@ __global__ void __launch_bounds__(256) atomic_kernel(unsigned long long *__restr
     }
 }
 
+/* Eight independent fused multiply-add chains per lane, `iters` rounds:
+ * 8 iters lane-ops per thread, no memory traffic but the final store. */
+__global__ void __launch_bounds__(256) valu_kernel(float *out, uint32_t iters, float a) {
+    float v[8];
+    for (int k = 0; k < 8; ++k) v[k] = (float)(threadIdx.x + k) * 1e-3f;
+    for (uint32_t i = 0; i < iters; ++i)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = __builtin_fmaf(v[k], a, 0.5f);
+    float s = 0.f;
+    for (int k = 0; k < 8; ++k) s += v[k];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 } // namespace
 
 extern "C" {
+
+int xo_probe_valu(xo_ctx *ctx, uint32_t iters, double *ms, double *lane_ops) {
+    xo::clear_error();
+    if (!ctx || !ms || !lane_ops || iters == 0) {
+        xo::set_error("xo_probe_valu: invalid argument");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    const uint32_t blocks = ctx->cus * 32;
+    float *out = nullptr;
+    XO_HIP(hipMalloc(&out, sizeof(float) * blocks * 256));
+    hipLaunchKernelGGL(valu_kernel, dim3(blocks), dim3(256), 0, ctx->stream, out, iters / 8 + 1,
+                       0.999f);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(valu_kernel, dim3(blocks), dim3(256), 0, ctx->stream, out, iters,
+                           0.999f);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    float kms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1);
+    (void)hipFree(out);
+    if (e != hipSuccess) {
+        xo::set_error("xo_probe_valu: %s", hipGetErrorString(e));
+        return -1;
+    }
+    *ms = kms;
+    *lane_ops = 8.0 * (double)iters * blocks * 256.0;
+    return 0;
+}
 
 int xo_probe_copy(xo_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms_per_copy) {
     xo::clear_error();
