@@ -68,8 +68,21 @@ enum xo_src_kind {
     XO_SRC_FIBER_NI = 2  /* xopto/mcml/mcsource/fiberni.py:237-290 (layered media) */
 };
 
-/* Stepping method (xopto/mcbase/mcoptions.py:269-318). Only albedo weight. */
-enum xo_method { XO_METHOD_AW = 0 };
+/* Stepping method (xopto/mcbase/mcoptions.py:269-318; the branches of
+ * mcml.template.c:529-755). Layered runs take all three; voxelized runs and
+ * batches take albedo weight. */
+enum xo_method {
+    XO_METHOD_AW = 0,  /* albedo weight                 */
+    XO_METHOD_AR = 1,  /* albedo rejection              */
+    XO_METHOD_MBL = 2  /* microscopic Beer-Lambert      */
+};
+
+/* Fluence accumulator geometry (xo_fluence.enabled). */
+enum xo_fluence_kind {
+    XO_FLUENCE_NONE = 0,
+    XO_FLUENCE_XYZ = 1, /* Fluence, xopto/mcbase/mcfluence/fluence.py       */
+    XO_FLUENCE_RZ = 2   /* FluenceRz, xopto/mcbase/mcfluence/fluencerz.py   */
+};
 
 /* Run flags. */
 enum xo_run_flags {
@@ -158,10 +171,12 @@ typedef struct xo_detector {
     int32_t log_scale;
 } xo_detector; /* 48 B */
 
-/* Cartesian fluence/deposition accumulator; McFluence of
- * xopto/mcbase/mcfluence/fluence.py:35-65. Deposition mode only. */
+/* Deposition accumulator; McFluence of xopto/mcbase/mcfluence/fluence.py:
+ * 35-65 (XO_FLUENCE_XYZ: top_left, inv_step and shape per x, y, z) or of
+ * fluencerz.py (XO_FLUENCE_RZ: top_left = centre, inv_step[0] = 1/dr,
+ * inv_step[2] = 1/dz, shape = {n_r, 1, n_z}). Deposition mode only. */
 typedef struct xo_fluence {
-    int32_t enabled;
+    int32_t enabled;            /* enum xo_fluence_kind                       */
     float inv_step[3];
     float top_left[3];
     uint32_t shape[3];
@@ -188,6 +203,7 @@ typedef struct xo_mcml_cfg {
     uint64_t packet_seed;       /* XO_FLAG_PACKET_KEYED only                  */
     uint64_t first_packet;      /* XO_FLAG_PACKET_KEYED only                  */
     xo_source_fiber fiber;
+    xo_fluence fluence;         /* bins after every detector bin              */
 } xo_mcml_cfg;
 
 /* A voxel material; McMaterial of xopto/mcbase/mcmaterial.py:37-63. */

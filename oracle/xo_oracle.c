@@ -307,6 +307,45 @@ static inline int o_roulette(o_pkt *s, float w_min, float chance,
     return 0;
 }
 
+/* Fluence deposit (deposition mode): the Cartesian voxel of
+ * mcsim_fluence_deposit_at (xopto/mcbase/mcfluence/fluence.py:102-147) or
+ * the (r, z) cell of fluencerz.py:112-160. */
+static void o_fluence_deposit(const xo_fluence *f, uint64_t *acc,
+                              const float p[3], float w, uint64_t *deposits) {
+    uint64_t index;
+    if (f->enabled == XO_FLUENCE_XYZ) {
+        float fx = (p[0] - f->top_left[0]) * f->inv_step[0];
+        float fy = (p[1] - f->top_left[1]) * f->inv_step[1];
+        float fz = (p[2] - f->top_left[2]) * f->inv_step[2];
+        if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)f->shape[0] &&
+              fy < (float)f->shape[1] && fz < (float)f->shape[2]))
+            return;
+        uint64_t ix = o_uint(fx), iy = o_uint(fy), iz = o_uint(fz);
+        index = (iz * f->shape[1] + iy) * f->shape[0] + ix;
+    } else if (f->enabled == XO_FLUENCE_RZ) {
+        float dx = p[0] - f->top_left[0];
+        float dy = p[1] - f->top_left[1];
+        float r = sqrtf(dx * dx + dy * dy);
+        float dz = p[2] - f->top_left[2];
+        float fr = r * f->inv_step[0];
+        float fz = dz * f->inv_step[2];
+        if (!(fr >= 0.0f && fz >= 0.0f && fr < (float)f->shape[0] && fz < (float)f->shape[2]))
+            return;
+        index = (uint64_t)o_uint(fz) * f->shape[0] + o_uint(fr);
+    } else {
+        return;
+    }
+    uint32_t ui = o_uint(w * (float)f->k + 0.5f);
+    if (ui > 0u) {
+        o_accu(acc, f->offset + index, ui);
+        (*deposits)++;
+    }
+}
+
+static inline float o_exp(float x, int mm) {
+    return (mm == XO_OMATH_KERNEL) ? xo_expf(x) : expf(x);
+}
+
 /* ------------------------------------------------------------------ */
 /* mcml                                                               */
 /* ------------------------------------------------------------------ */
@@ -454,7 +493,11 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
         const xo_layer *L = &cfg->layers[s->layer];
         int done = 0;
         cnt->steps++;
-        float step = -o_log(o_rand(&s->x, s->a), mm) * L->inv_mut;
+        float step;
+        if (cfg->method == XO_METHOD_MBL) /* mcml.template.c:529-531 */
+            step = (-o_log(o_rand(&s->x, s->a), mm)) / L->mus;
+        else
+            step = -o_log(o_rand(&s->x, s->a), mm) * L->inv_mut;
         step = fminf(step, XO_FLT_MAX);
         int32_t next = s->layer;
         if (fmaf(step, s->d[2], s->p[2]) < L->top) {
@@ -472,6 +515,23 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
         if (s->layer < next) s->p[2] = L->bottom;
         if (s->layer > next) s->p[2] = L->top;
 
+        const int method = cfg->method;
+        if (method == XO_METHOD_MBL) {
+            /* absorption along the whole step, mcml.template.c:584-606 */
+            float mua = L->mua;
+            float fraction = 1.0f - o_exp(-(mua * step), mm);
+            float deposit = fraction * s->w;
+            s->w -= deposit;
+            if (cfg->fluence.enabled) {
+                float back = (mua != 0.0f)
+                                 ? step - (-o_log(1.0f - o_rand(&s->x, s->a) * fraction,
+                                                  mm)) / mua
+                                 : 0.0f;
+                float q[3] = {s->p[0] - back * s->d[0], s->p[1] - back * s->d[1],
+                              s->p[2] - back * s->d[2]};
+                o_fluence_deposit(&cfg->fluence, acc, q, deposit, &cnt->deposits);
+            }
+        }
         if (next != s->layer) {
             cnt->boundary++;
             o_mcml_boundary(cfg, s, next);
@@ -484,17 +544,35 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
                                        s->d[0], s->d[1], s->d[2], s->w, mm);
                 done = 1;
             }
-        } else {
-            float deposit = s->w * L->mua_inv_mut;
+        } else if (method == XO_METHOD_AR &&
+                   o_rand(&s->x, s->a) < L->mua_inv_mut) {
+            /* albedo rejection, mcml.template.c:696-708 */
+            float deposit = s->w;
             s->w -= deposit;
+            if (cfg->fluence.enabled)
+                o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, &cnt->deposits);
+            done = 1;
+        } else {
+            if (method == XO_METHOD_AW) {
+                float deposit = s->w * L->mua_inv_mut;
+                s->w -= deposit;
+                if (cfg->fluence.enabled)
+                    o_fluence_deposit(&cfg->fluence, acc, s->p, deposit,
+                                      &cnt->deposits);
+            }
             float xi_fi;
             float ct = o_pf_sample(cfg->pf_kind, &L->pf, lut, &s->x, s->a,
                                    &xi_fi, mm);
             o_scatter_direction(&s->d[0], &s->d[1], &s->d[2], ct, xi_fi, mm);
-            if (o_roulette(s, cfg->w_min, cfg->lottery_chance,
-                           cfg->use_lottery))
+            if (method == XO_METHOD_AW &&
+                o_roulette(s, cfg->w_min, cfg->lottery_chance, cfg->use_lottery))
                 done = 1;
         }
+        /* microscopic Beer-Lambert: roulette after the boundary or the
+         * scattering, whether or not the packet left (mcml.template.c:652-667) */
+        if (method == XO_METHOD_MBL &&
+            o_roulette(s, cfg->w_min, cfg->lottery_chance, cfg->use_lottery))
+            done = 1;
         float ex = s->p[0] - sp[0], ey = s->p[1] - sp[1], ez = s->p[2] - sp[2];
         if (ex * ex + ey * ey + ez * ez > rmax2) done = 1;
         /* guard against packets that can never end (XO_MAX_PACKET_STEPS) */
@@ -522,9 +600,9 @@ int oracle_mcml_run(const xo_mcml_cfg *cfg, const float *lut,
     if (cfg->num_layers < 3 || cfg->num_layers > XO_MAX_LAYERS || lanes == 0)
         return -1;
     memset(accum, 0, naccum * sizeof(uint64_t));
-    uint64_t st = 0, sb = 0;
+    uint64_t st = 0, sb = 0, sd = 0;
     if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 16) reduction(+ : st, sb)
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : st, sb, sd)
     for (int64_t g = 0; g < (int64_t)lanes; ++g) {
         o_pkt s;
         o_counts c = {0, 0, 0, 0, 0};
@@ -541,12 +619,13 @@ int oracle_mcml_run(const xo_mcml_cfg *cfg, const float *lut,
         if (!keyed_table) rng_x[g] = s.x;
         st += c.steps;
         sb += c.boundary;
+        sd += c.deposits;
     }
     if (counts_out) {
         counts_out[0] = st;
         counts_out[1] = sb;
         counts_out[2] = 0;
-        counts_out[3] = 0;
+        counts_out[3] = sd;
     }
     return 0;
 }
@@ -750,23 +829,6 @@ static void o_mcvox_boundary(const o_vox *V, o_pkt *s, const float dist[3],
 }
 
 /* mcsim_fluence_deposit_at, deposition mode (fluence.py:102-147) */
-static void o_fluence_deposit(const xo_fluence *f, uint64_t *acc,
-                              const float p[3], float w, uint64_t *deposits) {
-    float fx = (p[0] - f->top_left[0]) * f->inv_step[0];
-    float fy = (p[1] - f->top_left[1]) * f->inv_step[1];
-    float fz = (p[2] - f->top_left[2]) * f->inv_step[2];
-    if (fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)f->shape[0] &&
-        fy < (float)f->shape[1] && fz < (float)f->shape[2]) {
-        uint64_t ix = o_uint(fx), iy = o_uint(fy), iz = o_uint(fz);
-        uint64_t index = (iz * f->shape[1] + iy) * f->shape[0] + ix;
-        uint32_t ui = o_uint(w * (float)f->k + 0.5f);
-        if (ui > 0u) {
-            o_accu(acc, f->offset + index, ui);
-            (*deposits)++;
-        }
-    }
-}
-
 /* One packet: the while(!done) loop of mcvox.template.c:748-1012 (albedo
  * weight). */
 static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
@@ -943,6 +1005,8 @@ int oracle_math(int which, uint32_t n, const float *in, float *out,
             xo_sincos_turns(in[i], &out[i], &out2[i]);
         } else if (which == 2) {
             out[i] = xo_cbrtf(in[i]);
+        } else if (which == 3) {
+            out[i] = xo_expf(in[i]);
         } else {
             return -1;
         }

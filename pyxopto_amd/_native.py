@@ -31,6 +31,8 @@ XO_BATCH_MAX_LAYERS = 8
 XO_PF_HG, XO_PF_MHG, XO_PF_LUT = 0, 1, 2
 XO_DET_NONE, XO_DET_TOTAL, XO_DET_RADIAL = 0, 1, 2
 XO_SRC_LINE, XO_SRC_GAUSSIAN, XO_SRC_FIBER_NI = 0, 1, 2
+XO_METHOD_AW, XO_METHOD_AR, XO_METHOD_MBL = 0, 1, 2
+XO_FLUENCE_NONE, XO_FLUENCE_XYZ, XO_FLUENCE_RZ = 0, 1, 2
 XO_METHOD_AW = 0
 XO_FLAG_STATIC_SCHEDULE = 1 << 0
 XO_FLAG_NO_LDS_BINS = 1 << 1
@@ -92,7 +94,7 @@ class XoMcmlCfg(ctypes.Structure):
                 ('specular', XoDetector),
                 ('layers', XoLayer * XO_MAX_LAYERS),
                 ('packet_seed', c_uint64), ('first_packet', c_uint64),
-                ('fiber', XoSourceFiber)]
+                ('fiber', XoSourceFiber), ('fluence', XoFluence)]
 
 
 class XoMaterial(ctypes.Structure):

@@ -86,6 +86,25 @@ int check_detector(const char *name, const xo_detector &d, uint64_t naccum,
     return 0;
 }
 
+/* The fluence bins must follow the detector bins (which may be staged in
+ * LDS) inside the accumulator. */
+int check_fluence(const char *fn, const xo_fluence &f, uint64_t det_end, uint64_t naccum) {
+    if (f.enabled == XO_FLUENCE_NONE) return 0;
+    if (f.enabled != XO_FLUENCE_XYZ && f.enabled != XO_FLUENCE_RZ) {
+        xo::set_error("%s: unknown fluence kind %d", fn, f.enabled);
+        return -1;
+    }
+    const uint64_t nf = (uint64_t)f.shape[0] * f.shape[1] * f.shape[2];
+    if (nf == 0 || f.offset < det_end || (uint64_t)f.offset + nf > naccum) {
+        xo::set_error("%s: fluence bins [%u, +%llu) must follow the detector bins (%llu) "
+                      "inside the accumulator (%llu)",
+                      fn, f.offset, (unsigned long long)nf, (unsigned long long)det_end,
+                      (unsigned long long)naccum);
+        return -1;
+    }
+    return 0;
+}
+
 int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint32_t idx) {
     if (pf_kind == XO_PF_LUT) {
         if (pf.size < 2 || (uint64_t)pf.offset + pf.size > nlut) {
@@ -163,6 +182,7 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 } // namespace xo
 
 using xo::check_detector;
+using xo::check_fluence;
 using xo::check_pf;
 using xo::choose_lanes;
 using xo::ensure_buffer;
@@ -352,8 +372,9 @@ int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t 
                       XO_MAX_LAYERS);
         return -1;
     }
-    if (cfg->method != XO_METHOD_AW) {
-        xo::set_error("xo_mcml_run: only the albedo-weight method is implemented");
+    if (cfg->method != XO_METHOD_AW && cfg->method != XO_METHOD_AR &&
+        cfg->method != XO_METHOD_MBL) {
+        xo::set_error("xo_mcml_run: unknown stepping method %d", cfg->method);
         return -1;
     }
     if (cfg->source_kind != XO_SRC_LINE && cfg->source_kind != XO_SRC_FIBER_NI) {
@@ -375,6 +396,7 @@ int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t 
         check_detector("bottom", cfg->bottom, naccum, &det_end) ||
         check_detector("specular", cfg->specular, naccum, &det_end))
         return -1;
+    if (check_fluence("xo_mcml_run", cfg->fluence, det_end, naccum)) return -1;
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     XO_HIP(hipSetDevice(ctx->device));

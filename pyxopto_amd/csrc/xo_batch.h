@@ -112,7 +112,8 @@ inline LayerL layer_to_device(const xo_layer &L) {
     o.lc = L.pf.c;
     o.loff = L.pf.offset;
     o.lsize_m1 = L.pf.size > 0u ? L.pf.size - 1u : 0u;
-    o.pad0 = o.pad1 = 0u;
+    o.mua = L.mua;
+    o.mus = L.mus;
     return o;
 }
 

@@ -61,6 +61,7 @@ int ensure_buffer(void **ptr, size_t *cap, size_t bytes);
 int ensure_table(xo_ctx *ctx);
 int check_detector(const char *name, const xo_detector &d, uint64_t naccum, uint64_t *end);
 int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint32_t idx);
+int check_fluence(const char *fn, const xo_fluence &f, uint64_t det_end, uint64_t naccum);
 uint32_t choose_lanes(xo_ctx *ctx, uint32_t requested, int blocks_per_cu);
 double ms_since(std::chrono::steady_clock::time_point t0);
 

@@ -29,7 +29,8 @@ struct alignas(16) LayerL {
     float top, bottom, inv_mut, mua_inv_mut;
     float n, cc_top, cc_bottom, g;
     float beta, la, lb, lc;
-    uint32_t loff, lsize_m1, pad0, pad1;
+    uint32_t loff, lsize_m1;
+    float mua, mus; /* microscopic Beer-Lambert steps */
 };
 
 /* Material as staged in LDS. */
@@ -124,6 +125,40 @@ XO_DEV void detector_deposit_n(const xo_detector &d, const Bins &bins, float px,
         else
             atomicAdd(&bins.glob[d.offset + index], v);
     }
+}
+
+/* Fluence deposit at a point (deposition mode): the Cartesian voxel of
+ * mcsim_fluence_deposit_at (fluence.py:102-147) or the (r, z) cell of
+ * fluencerz.py:112-160. Returns true if a non-zero integer was added. */
+XO_DEV bool fluence_deposit(const xo_fluence &F, unsigned long long *accum, float px,
+                            float py, float pz, float w) {
+    uint64_t index;
+    if (F.enabled == XO_FLUENCE_XYZ) {
+        const float fx = (px - F.top_left[0]) * F.inv_step[0];
+        const float fy = (py - F.top_left[1]) * F.inv_step[1];
+        const float fz = (pz - F.top_left[2]) * F.inv_step[2];
+        if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)F.shape[0] &&
+              fy < (float)F.shape[1] && fz < (float)F.shape[2]))
+            return false;
+        const uint64_t ix = f2u_sat(fx), iy = f2u_sat(fy), iz = f2u_sat(fz);
+        index = (iz * F.shape[1] + iy) * F.shape[0] + ix;
+    } else if (F.enabled == XO_FLUENCE_RZ) {
+        const float ddx = px - F.top_left[0];
+        const float ddy = py - F.top_left[1];
+        const float r = __builtin_sqrtf(ddx * ddx + ddy * ddy);
+        const float ddz = pz - F.top_left[2];
+        const float fr = r * F.inv_step[0];
+        const float fz = ddz * F.inv_step[2];
+        if (!(fr >= 0.0f && fz >= 0.0f && fr < (float)F.shape[0] && fz < (float)F.shape[2]))
+            return false;
+        index = (uint64_t)f2u_sat(fz) * F.shape[0] + f2u_sat(fr);
+    } else {
+        return false;
+    }
+    const uint32_t ui = f2u_sat(w * (float)F.k + 0.5f);
+    if (ui == 0u) return false;
+    atomicAdd(&accum[F.offset + index], (unsigned long long)ui);
+    return true;
 }
 
 /* Phase-function parameters of one layer/material, read from LDS. */

@@ -38,7 +38,7 @@ XO_DEV void stats_flush(BlockStats *ss, const uint32_t *mine,
         atomicAdd(&gstats[threadIdx.x], ss->v[threadIdx.x]);
 }
 
-template <int PF>
+template <int PF, int METHOD>
 __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ BlockStats sstats;
@@ -66,7 +66,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         o.lc = L.pf.c;
         o.loff = L.pf.offset;
         o.lsize_m1 = L.pf.size > 0u ? L.pf.size - 1u : 0u;
-        o.pad0 = o.pad1 = 0u;
+        o.mua = L.mua;
+        o.mus = L.mus;
         sl[tid] = o;
     }
     for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK) sbins[i] = 0ull;
@@ -101,6 +102,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     const float spz = sp[2];
     const float w_min = P.cfg.w_min, chance = P.cfg.lottery_chance;
     const int use_lottery = P.cfg.use_lottery;
+    const xo_fluence &F = P.cfg.fluence;
+    const bool use_fluence = F.enabled != XO_FLUENCE_NONE;
 
     bool active = lane_ok;
     WaveDispenser wq;
@@ -179,7 +182,11 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         /* ---- one iteration of while(!done), mcml.template.c:514-788 ---- */
         const LayerL &L = sl[layer];
         st[XO_STAT_STEPS]++;
-        float step = -xo_logf(rng_next(x, a)) * L.inv_mut;
+        float step;
+        if (METHOD == XO_METHOD_MBL)
+            step = (-xo_logf(rng_next(x, a))) / L.mus; /* mcml.template.c:529-531 */
+        else
+            step = -xo_logf(rng_next(x, a)) * L.inv_mut;
         step = fminf(step, XO_FLT_MAX);
         int32_t next = layer;
         if (__builtin_fmaf(step, dz, pz) < L.top) {
@@ -197,6 +204,21 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         if (layer > next) pz = L.top;
 
         bool done = false;
+        if (METHOD == XO_METHOD_MBL) {
+            /* absorption along the whole step, mcml.template.c:584-606 */
+            const float mua = L.mua;
+            const float fraction = 1.0f - xo_expf(-(mua * step));
+            const float deposit = fraction * w;
+            w -= deposit;
+            if (use_fluence) {
+                const float back =
+                    (mua != 0.0f) ? step - (-xo_logf(1.0f - rng_next(x, a) * fraction)) / mua
+                                  : 0.0f;
+                if (fluence_deposit(F, P.accum, px - back * dx, py - back * dy, pz - back * dz,
+                                    deposit))
+                    st[XO_STAT_DEPOSITS]++;
+            }
+        }
         if (next != layer) {
             st[XO_STAT_BOUNDARY]++;
             mcml_boundary(sl, layer, next, dx, dy, dz, x, a);
@@ -207,15 +229,34 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                     detector_deposit(P.cfg.bottom, bins, px, py, dx, dy, dz, w);
                 done = true;
             }
-        } else {
-            float deposit = w * L.mua_inv_mut;
+        } else if (METHOD == XO_METHOD_AR &&
+                   rng_next(x, a) < L.mua_inv_mut) {
+            /* albedo rejection: the whole weight is absorbed here,
+             * mcml.template.c:696-708 */
+            const float deposit = w;
             w -= deposit;
+            if (use_fluence && fluence_deposit(F, P.accum, px, py, pz, deposit))
+                st[XO_STAT_DEPOSITS]++;
+            done = true;
+        } else {
+            if (METHOD == XO_METHOD_AW) {
+                const float deposit = w * L.mua_inv_mut;
+                w -= deposit;
+                if (use_fluence && fluence_deposit(F, P.accum, px, py, pz, deposit))
+                    st[XO_STAT_DEPOSITS]++;
+            }
             PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
             float xi_fi;
             float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
             scatter_direction(dx, dy, dz, ct, xi_fi);
-            if (roulette(w, x, a, w_min, chance, use_lottery)) done = true;
+            if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
+                done = true;
         }
+        /* microscopic Beer-Lambert: roulette after the boundary or the
+         * scattering, whether or not the packet has left (as the reference
+         * does, mcml.template.c:652-667) */
+        if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
+            done = true;
         float ex = px - spx, ey = py - spy, ez = pz - spz;
         if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
         if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
@@ -581,22 +622,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         } else {
             float deposit = w * M.mua_inv_mut;
             w -= deposit;
-            if (use_fluence) {
-                /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
-                float fx = (p[0] - F.top_left[0]) * F.inv_step[0];
-                float fy = (p[1] - F.top_left[1]) * F.inv_step[1];
-                float fz = (p[2] - F.top_left[2]) * F.inv_step[2];
-                if (fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)F.shape[0] &&
-                    fy < (float)F.shape[1] && fz < (float)F.shape[2]) {
-                    uint64_t ix = f2u_sat(fx), iy = f2u_sat(fy), iz = f2u_sat(fz);
-                    uint64_t index = (iz * F.shape[1] + iy) * F.shape[0] + ix;
-                    uint32_t ui = f2u_sat(deposit * (float)F.k + 0.5f);
-                    if (ui > 0u) {
-                        atomicAdd(&P.accum[F.offset + index], (unsigned long long)ui);
-                        st[XO_STAT_DEPOSITS]++;
-                    }
-                }
-            }
+            /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
+            if (use_fluence && fluence_deposit(F, P.accum, p[0], p[1], p[2], deposit))
+                st[XO_STAT_DEPOSITS]++;
             PfL pf{M.g, M.beta, M.la, M.lb, M.lc, M.loff, M.lsize_m1};
             float xi_fi;
             float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
@@ -634,22 +662,40 @@ size_t mcvox_lds_bytes(const McvoxParams &P) {
     return XO_MAX_MATERIALS * sizeof(MaterialL) + 8u * P.lds_bins + 4u * P.lds_lut;
 }
 
-hipError_t launch_mcml(const McmlParams &P, uint32_t blocks, hipStream_t stream) {
-    size_t lds = mcml_lds_bytes(P);
-    switch (P.cfg.pf_kind) {
-    case XO_PF_HG:
-        hipLaunchKernelGGL(mcml_kernel<XO_PF_HG>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+template <int PF>
+static hipError_t launch_mcml_pf(const McmlParams &P, uint32_t blocks, size_t lds,
+                                 hipStream_t stream) {
+    switch (P.cfg.method) {
+    case XO_METHOD_AW:
+        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AW>), dim3(blocks), dim3(XO_BLOCK), lds,
+                           stream, P);
         break;
-    case XO_PF_MHG:
-        hipLaunchKernelGGL(mcml_kernel<XO_PF_MHG>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+    case XO_METHOD_AR:
+        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AR>), dim3(blocks), dim3(XO_BLOCK), lds,
+                           stream, P);
         break;
-    case XO_PF_LUT:
-        hipLaunchKernelGGL(mcml_kernel<XO_PF_LUT>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+    case XO_METHOD_MBL:
+        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_MBL>), dim3(blocks), dim3(XO_BLOCK), lds,
+                           stream, P);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_mcml(const McmlParams &P, uint32_t blocks, hipStream_t stream) {
+    size_t lds = mcml_lds_bytes(P);
+    switch (P.cfg.pf_kind) {
+    case XO_PF_HG:
+        return launch_mcml_pf<XO_PF_HG>(P, blocks, lds, stream);
+    case XO_PF_MHG:
+        return launch_mcml_pf<XO_PF_MHG>(P, blocks, lds, stream);
+    case XO_PF_LUT:
+        return launch_mcml_pf<XO_PF_LUT>(P, blocks, lds, stream);
+    default:
+        return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_mcvox(const McvoxParams &P, uint32_t blocks, hipStream_t stream) {
@@ -671,16 +717,18 @@ hipError_t launch_mcvox(const McvoxParams &P, uint32_t blocks, hipStream_t strea
 }
 
 hipError_t occupancy_mcml(int pf_kind, size_t lds, int *blocks_per_cu) {
+    /* the albedo-weight build stands for all three methods (same
+     * registers within a few) */
     switch (pf_kind) {
     case XO_PF_MHG:
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_kernel<XO_PF_MHG>,
-                                                            XO_BLOCK, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            blocks_per_cu, mcml_kernel<XO_PF_MHG, XO_METHOD_AW>, XO_BLOCK, lds);
     case XO_PF_LUT:
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_kernel<XO_PF_LUT>,
-                                                            XO_BLOCK, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            blocks_per_cu, mcml_kernel<XO_PF_LUT, XO_METHOD_AW>, XO_BLOCK, lds);
     default:
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcml_kernel<XO_PF_HG>,
-                                                            XO_BLOCK, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            blocks_per_cu, mcml_kernel<XO_PF_HG, XO_METHOD_AW>, XO_BLOCK, lds);
     }
 }
 

@@ -98,6 +98,36 @@ XO_HD float xo_logf(float x) {
     return XO_FMA(fe, 0.693359375f, r);
 }
 
+/* Exponential: n = round(x / ln 2), r = x - n ln 2 in two Cody-Waite
+ * parts, Cephes' degree-5 polynomial (S. L. Moshier, Cephes Math Library,
+ * expf.c) for e^r, scaled by 2^n through the exponent field (two steps
+ * below 2^-126 so results stay correct down to the subnormals). */
+XO_HD float xo_expf(float x) {
+    if (x != x) return x;
+    if (x > 88.7228391f) return XO_INF_F;
+    if (x < -103.972084f) return 0.0f;
+    float n = XO_RINT(x * 1.44269504088896341f);
+    float r = XO_FMA(n, -0.693359375f, x);
+    r = XO_FMA(n, 2.12194440e-4f, r);
+    float p = 1.9875691500e-4f;
+    p = XO_FMA(p, r, 1.3981999507e-3f);
+    p = XO_FMA(p, r, 8.3334519073e-3f);
+    p = XO_FMA(p, r, 4.1665795894e-2f);
+    p = XO_FMA(p, r, 1.6666665459e-1f);
+    p = XO_FMA(p, r, 5.0000001201e-1f);
+    float y = XO_FMA(p, r * r, r) + 1.0f;
+    int32_t ni = (int32_t)n;
+    if (ni > 127) {
+        y = y * 2.0f;
+        ni -= 1;
+    }
+    if (ni < -126) {
+        y = y * xo_u2f((uint32_t)(ni + 126 + 127) << 23);
+        return y * 1.17549435e-38f; /* 2^-126 */
+    }
+    return y * xo_u2f((uint32_t)(ni + 127) << 23);
+}
+
 /* sin(2*pi*t) and cos(2*pi*t). */
 XO_HD void xo_sincos_turns(float t, float *s_out, float *c_out) {
     float q = XO_RINT(4.0f * t);

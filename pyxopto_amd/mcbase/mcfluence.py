@@ -1,7 +1,10 @@
-"""Cartesian deposition accumulator (xopto/mcbase/mcfluence/fluence.py).
+"""Deposition accumulators: Cartesian (xopto/mcbase/mcfluence/fluence.py)
+and cylindrical r-z (xopto/mcbase/mcfluence/fluencerz.py).
 
-Deposition mode: each absorbed weight is added to the voxel that contains
-the packet; ``data`` is raw/(nphotons*dx*dy*dz) (fluence.py:343-347)."""
+Deposition mode: each absorbed weight is added to the voxel (or ring cell)
+that contains the packet; ``data`` is raw/(nphotons*volume), volume
+dx*dy*dz (fluence.py:343-347) or pi (r1^2 - r0^2) dz (fluencerz.py:343-
+348)."""
 import numpy as np
 
 from .. import _native as N
@@ -58,7 +61,7 @@ class Fluence:
 
     def pack(self, mc, target: N.XoFluence):
         n = self._x_axis.n * self._y_axis.n * self._z_axis.n
-        target.enabled = 1
+        target.enabled = N.XO_FLUENCE_XYZ
         target.offset = mc.allocate_accumulator(self, n)
         for i, ax in enumerate((self._x_axis, self._y_axis, self._z_axis)):
             target.top_left[i] = ax.start
@@ -71,3 +74,83 @@ class Fluence:
         return {'type': 'Fluence', 'xaxis': self._x_axis.todict(),
                 'yaxis': self._y_axis.todict(), 'zaxis': self._z_axis.todict(),
                 'mode': self._mode}
+
+
+class FluenceRz:
+    """Deposition on a cylindrical grid: rings of radius raxis around the
+    axis through `center` (x, y), slices zaxis (fluencerz.py:169-450).
+    The raw data are [z][r] (the kernel's flat index is iz * n_r + ir);
+    the reference reshapes that flat array as (n_r, n_z), which only
+    matches its own index when n_r == n_z — here the array is (n_z, n_r)."""
+
+    def __init__(self, raxis, zaxis: Axis = None, center=(0.0, 0.0),
+                 mode: str = 'deposition'):
+        k = N.XO_ACCU_K
+        data, nphotons = None, 0
+        if isinstance(raxis, FluenceRz):
+            other = raxis
+            raxis, zaxis = Axis(other.raxis), Axis(other.zaxis)
+            center, mode, k, nphotons = (other.center, other.mode, other.k,
+                                         other.nphotons)
+            if other.raw is not None:
+                data = np.copy(other.raw)
+        if mode not in ('deposition',):
+            raise NotImplementedError(
+                'Only the "deposition" fluence mode is implemented!')
+        self._r_axis, self._z_axis = raxis, zaxis
+        self._center = np.array([float(center[0]), float(center[1])])
+        self._mode = mode
+        self._k = int(k)
+        self._data = data
+        self._nphotons = int(nphotons)
+
+    raxis = property(lambda self: self._r_axis)
+    zaxis = property(lambda self: self._z_axis)
+    center = property(lambda self: self._center)
+    mode = property(lambda self: self._mode)
+    k = property(lambda self: self._k)
+    nphotons = property(lambda self: self._nphotons)
+    raw = property(lambda self: self._data)
+    shape = property(lambda self: (self._z_axis.n, self._r_axis.n))
+    dz = property(lambda self: self._z_axis.step)
+    r = property(lambda self: self._r_axis.centers)
+    z = property(lambda self: self._z_axis.centers)
+
+    def _get_data(self):
+        e = self._r_axis.edges
+        area = np.pi * (e[1:] ** 2 - e[:-1] ** 2)
+        k = 1.0 / (max(self._nphotons, 1) * area * self.dz)
+        return self._data * k[np.newaxis, :]
+    data = property(_get_data, None, None, 'Deposition (1/m^3).')
+
+    def update_data(self, mc, accumulators, nphotons: int):
+        new = np.asarray(accumulators[0]).reshape(self.shape) * (1.0 / self._k)
+        if self._data is None:
+            self._data = new
+            self._nphotons = int(nphotons)
+        else:
+            self._data += new
+            self._nphotons += int(nphotons)
+
+    def pack(self, mc, target: N.XoFluence):
+        """cl_pack of fluencerz.py:425-450."""
+        target.enabled = N.XO_FLUENCE_RZ
+        target.offset = mc.allocate_accumulator(
+            self, self._r_axis.n * self._z_axis.n)
+        target.top_left[0] = self._center[0]
+        target.top_left[1] = self._center[1]
+        target.top_left[2] = self._z_axis.start
+        target.inv_step[0] = (1.0 / self._r_axis.step
+                              if self._r_axis.step != 0.0 else 0.0)
+        target.inv_step[1] = 0.0
+        target.inv_step[2] = 1.0 / self._z_axis.step
+        target.shape[0] = self._r_axis.n
+        target.shape[1] = 1
+        target.shape[2] = self._z_axis.n
+        target.k = self._k
+        return target
+
+    def todict(self):
+        return {'type': 'FluenceRz', 'raxis': self._r_axis.todict(),
+                'zaxis': self._z_axis.todict(),
+                'center': self._center.tolist(), 'mode': self._mode}

@@ -15,16 +15,21 @@ class McOption:
 
 
 class McMethod(McOption):
+    """Stepping method: albedo weight (0), albedo rejection (1) or
+    microscopic Beer-Lambert (2) (xopto/mcbase/mcoptions.py:269-318).
+    Layered simulations take all three; voxelized simulations and batches
+    take albedo weight."""
     name = 'method'
-    aw, ar, mbl = 0, 1, 2
 
-    def __init__(self, value=0):
-        if isinstance(value, str):
-            value = {'aw': 0, 'ar': 1, 'mbl': 2}[value]
-        if int(value) != 0:
-            raise NotImplementedError(
-                'Only the albedo-weight method (McMethod.aw) is implemented!')
+    def __init__(self, value: int = 0):
+        if isinstance(value, str) or value not in (0, 1, 2):
+            raise ValueError('Allowed values are 0, 1 or 2!')
         super().__init__(int(value))
+
+
+McMethod.albedo_weight = McMethod.aw = McMethod.default = McMethod(0)
+McMethod.albedo_rejection = McMethod.ar = McMethod(1)
+McMethod.microscopic_beer_lambert = McMethod.mbl = McMethod(2)
 
 
 class McMinimumPacketWeight(McOption):

@@ -77,6 +77,9 @@ class Batch(McWorker):
 
     def pack(self):
         """(XoBatchInstance array, lut, naccum)."""
+        if int(self._opts['method']) != N.XO_METHOD_AW:
+            raise NotImplementedError('Batches run the albedo-weight method '
+                                      'only.')
         self._reset_allocations()
         arr = (N.XoBatchInstance * len(self.instances))()
         self._packers = []

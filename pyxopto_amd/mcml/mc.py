@@ -14,17 +14,20 @@ import numpy as np
 
 from .. import _native as N
 from ..mcbase import mcdetector, mcoptions, mcpf
+from ..mcbase import mcfluence
 from ..mcbase.mcutil import axis as _axis
 from ..mcbase.mcutil import fiber as fiberutil
 from ..mcbase.mcworker import McWorker
 from . import mclayer, mcsource
+
+mcfluence.Axis = _axis.Axis
 
 # Module layout of xopto.mcml.mc: the plugin modules are reachable from here.
 mcdetector.Axis = _axis.Axis
 mcdetector.RadialAxis = _axis.RadialAxis
 
 __all__ = ['Mc', 'mclayer', 'mcsource', 'mcdetector', 'mcpf', 'mcoptions',
-           'fiberutil']
+           'mcfluence', 'fiberutil']
 
 
 class Mc(McWorker):
@@ -35,10 +38,10 @@ class Mc(McWorker):
         if trace is not None or surface is not None:
             raise NotImplementedError(
                 'Trace and surface layouts are not part of this engine.')
-        if fluence is not None:
-            raise NotImplementedError(
-                'Fluence accumulation is implemented for the voxelized '
-                'simulator (pyxopto_amd.mcvox); the layered kernel has none.')
+        if fluence is not None and not isinstance(
+                fluence, (mcfluence.Fluence, mcfluence.FluenceRz)):
+            raise TypeError('fluence must be a Fluence or FluenceRz object')
+        self._fluence = fluence
         super().__init__(options=options, rnginit=rnginit,
                          cl_devices=cl_devices,
                          cl_build_options=cl_build_options,
@@ -70,6 +73,8 @@ class Mc(McWorker):
 
     layers = property(lambda self: self._layers, None, None, 'Layer stack.')
     source = property(lambda self: self._source, None, None, 'Source.')
+    fluence = property(lambda self: self._fluence, None, None,
+                       'Fluence/deposition accumulator or None.')
     detectors = property(lambda self: self._detectors, None, None,
                          'Detectors.')
     options = property(lambda self: list(self._options), None, None,
@@ -102,6 +107,8 @@ class Mc(McWorker):
                 raise ValueError('Detector types must not change between '
                                  'simulation calls!')
             self._detectors.pack(self, cfg)
+        if self._fluence is not None:
+            self._fluence.pack(self, cfg.fluence)
         naccum = max(self._alloc_offset, 1)
         return cfg, self._lut_array(), naccum
 
@@ -148,7 +155,13 @@ class Mc(McWorker):
             print('McKernel processed {:,d} packets in {:,d} threads: '
                   '{:.3f} ms'.format(nphotons, rep.threads_used,
                                      rep.kernel_ms))
-        return None, out[1], detectors_res
+        fluence_res = out[1]
+        if self._fluence is not None:
+            if fluence_res is None:
+                fluence_res = type(self._fluence)(self._fluence)
+            off, n = self.allocation(self._fluence)
+            fluence_res.update_data(self, [accum[off:off + n]], nphotons)
+        return None, fluence_res, detectors_res
 
     def __repr__(self):
         return 'Mc({}, {}, {})'.format(self._layers, self._source,

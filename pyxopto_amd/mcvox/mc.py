@@ -83,6 +83,9 @@ class Mc(McWorker):
                              'change between simulation calls!')
         cfg = N.XoMcvoxCfg()
         self._apply_options(cfg)
+        if cfg.method != N.XO_METHOD_AW:
+            raise NotImplementedError('Voxelized simulations run the '
+                                      'albedo-weight method only.')
         cfg.pf_kind = self._materials.pf_kind
         cfg.rmax = self._rmax
         cfg.num_materials = len(self._materials)

@@ -131,7 +131,7 @@ def test_errors_mirror_reference():
         L.Layers([L.Layer(d=1, n=1, mua=0, mus=0, pf=mc.mcpf.Hg(0))])
     with pytest.raises(ValueError):
         sim.layer(1).pf = mc.mcpf.MHg(0.1, 0.2)
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError):
         mcoptions.McMethod('mbl')
     with pytest.raises(ValueError):
         mc.mcsource.Line(direction=(0, 0, 0))

@@ -58,3 +58,23 @@ def test_cbrtf_accuracy():
     assert O.kernel_math('cbrt', np.float32([0.0]))[0] == 0.0
     np.testing.assert_array_equal(O.kernel_math('cbrt', np.float32([8.0, 27.0])),
                                   np.float32([2.0, 3.0]))
+
+
+def test_expf_accuracy():
+    rng = np.random.default_rng(4)
+    x = np.concatenate([
+        rng.uniform(-103.0, 88.0, 1_000_000),
+        -rng.exponential(3.0, 1_000_000),          # exp(-mua * step)
+        [0.0, -0.0, 1.0, -1.0, 88.72, -87.3, -100.0, -103.9, 1e-8, -1e-8]
+    ]).astype(np.float32)
+    y = O.kernel_math('exp', x)
+    ref = np.exp(x.astype(np.float64))
+    normal = ref >= 2.0 ** -126
+    err = np.abs(y[normal] - ref[normal]) / _ulp(ref[normal].astype(np.float32))
+    assert err.max() <= 2.0, err.max()
+    # subnormal results: within one subnormal step
+    sub = ~normal
+    assert np.all(np.abs(y[sub] - ref[sub]) <= 2.0 ** -149 * 1.01)
+    e = O.kernel_math('exp', np.float32([0.0, np.inf, -np.inf, 89.0, -104.0]))
+    np.testing.assert_array_equal(e, np.float32([1.0, np.inf, 0.0, np.inf, 0.0]))
+    assert np.isnan(O.kernel_math('exp', np.float32([np.nan]))[0])
