@@ -829,8 +829,9 @@ static void o_mcvox_boundary(const o_vox *V, o_pkt *s, const float dist[3],
 }
 
 /* mcsim_fluence_deposit_at, deposition mode (fluence.py:102-147) */
-/* One packet: the while(!done) loop of mcvox.template.c:748-1012 (albedo
- * weight). */
+/* One packet: the while(!done) loop of mcvox.template.c:748-1012 with the
+ * three stepping methods (microscopic Beer-Lambert :763-881, albedo
+ * rejection :925-944, albedo weight :945-977). */
 static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
                            o_counts *cnt, int mm) {
     const xo_mcvox_cfg *cfg = V->cfg;
@@ -845,7 +846,12 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
         const xo_material *M = &cfg->materials[s->mi];
         int done = 0;
         cnt->steps++;
-        float step = -o_log(o_rand(&s->x, s->a), mm) * M->inv_mut;
+        const int method = cfg->method;
+        float step;
+        if (method == XO_METHOD_MBL) /* mcvox.template.c:763-766 */
+            step = (-o_log(o_rand(&s->x, s->a), mm)) / M->mus;
+        else
+            step = -o_log(o_rand(&s->x, s->a), mm) * M->inv_mut;
         step = fminf(step, XO_FLT_MAX);
         float dist[3];
         float d = o_intersect(vc, s, dist);
@@ -853,6 +859,23 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
         s->p[0] = fmaf(d_ok, s->d[0], s->p[0]);
         s->p[1] = fmaf(d_ok, s->d[1], s->p[1]);
         s->p[2] = fmaf(d_ok, s->d[2], s->p[2]);
+        if (method == XO_METHOD_MBL) {
+            /* absorption along the part of the step inside this voxel,
+             * mcvox.template.c:800-818 */
+            float mua = M->mua;
+            float fraction = 1.0f - o_exp(-(mua * d_ok), mm);
+            float deposit = fraction * s->w;
+            s->w -= deposit;
+            if (cfg->fluence.enabled) {
+                float back = (mua != 0.0f)
+                                 ? d_ok - (-o_log(1.0f - o_rand(&s->x, s->a) * fraction,
+                                                  mm)) / mua
+                                 : 0.0f;
+                float q[3] = {s->p[0] - back * s->d[0], s->p[1] - back * s->d[1],
+                              s->p[2] - back * s->d[2]};
+                o_fluence_deposit(&cfg->fluence, acc, q, deposit, &cnt->deposits);
+            }
+        }
         if (d < step) {
             cnt->boundary++;
             o_mcvox_boundary(V, s, dist, &cnt->crossings);
@@ -865,20 +888,35 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
                                        s->d[0], s->d[1], s->d[2], s->w, mm);
                 done = 1;
             }
-        } else {
-            float deposit = s->w * M->mua_inv_mut;
+        } else if (method == XO_METHOD_AR &&
+                   o_rand(&s->x, s->a) < M->mua_inv_mut) {
+            /* albedo rejection, mcvox.template.c:925-944 */
+            float deposit = s->w;
             s->w -= deposit;
             if (cfg->fluence.enabled)
-                o_fluence_deposit(&cfg->fluence, acc, s->p, deposit,
-                                  &cnt->deposits);
+                o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, &cnt->deposits);
+            done = 1;
+        } else {
+            if (method == XO_METHOD_AW) {
+                float deposit = s->w * M->mua_inv_mut;
+                s->w -= deposit;
+                if (cfg->fluence.enabled)
+                    o_fluence_deposit(&cfg->fluence, acc, s->p, deposit,
+                                      &cnt->deposits);
+            }
             float xi_fi;
             float ct = o_pf_sample(cfg->pf_kind, &M->pf, V->lut, &s->x, s->a,
                                    &xi_fi, mm);
             o_scatter_direction(&s->d[0], &s->d[1], &s->d[2], ct, xi_fi, mm);
-            if (o_roulette(s, cfg->w_min, cfg->lottery_chance,
-                           cfg->use_lottery))
+            if (method == XO_METHOD_AW &&
+                o_roulette(s, cfg->w_min, cfg->lottery_chance, cfg->use_lottery))
                 done = 1;
         }
+        /* microscopic Beer-Lambert: roulette after the boundary or the
+         * scattering, whether or not the packet left (mcvox.template.c:865-881) */
+        if (method == XO_METHOD_MBL &&
+            o_roulette(s, cfg->w_min, cfg->lottery_chance, cfg->use_lottery))
+            done = 1;
         float ex = s->p[0] - sp[0], ey = s->p[1] - sp[1], ez = s->p[2] - sp[2];
         if (ex * ex + ey * ey + ez * ez > rmax2 || s->w <= 0.0f) done = 1;
         /* guard against packets that can never end (XO_MAX_PACKET_STEPS) */

@@ -460,8 +460,9 @@ int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
                       XO_MAX_MATERIALS);
         return -1;
     }
-    if (cfg->method != XO_METHOD_AW) {
-        xo::set_error("xo_mcvox_run: only the albedo-weight method is implemented");
+    if (cfg->method != XO_METHOD_AW && cfg->method != XO_METHOD_AR &&
+        cfg->method != XO_METHOD_MBL) {
+        xo::set_error("xo_mcvox_run: unknown stepping method %d", cfg->method);
         return -1;
     }
     if (cfg->source_kind != XO_SRC_LINE && cfg->source_kind != XO_SRC_GAUSSIAN) {

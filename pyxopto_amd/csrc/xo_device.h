@@ -37,7 +37,8 @@ struct alignas(16) LayerL {
 struct alignas(16) MaterialL {
     float inv_mut, mua_inv_mut, n, g;
     float beta, la, lb, lc;
-    uint32_t loff, lsize_m1, pad0, pad1;
+    uint32_t loff, lsize_m1;
+    float mua, mus; /* microscopic Beer-Lambert steps */
 };
 
 /* fp_random_single (mcbase.template.c:1576-1594): one MWC step. The state

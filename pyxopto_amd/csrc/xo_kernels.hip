@@ -336,7 +336,7 @@ XO_DEV bool box_intersect(const VoxGeom &g, const float pos[3], const float dir[
     return tmax >= tmin && tmax >= 0.0f;
 }
 
-template <int PF>
+template <int PF, int METHOD>
 __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ BlockStats sstats;
@@ -360,7 +360,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         o.lc = M.pf.c;
         o.loff = M.pf.offset;
         o.lsize_m1 = M.pf.size > 0u ? M.pf.size - 1u : 0u;
-        o.pad0 = o.pad1 = 0u;
+        o.mua = M.mua;
+        o.mus = M.mus;
         sm[tid] = o;
     }
     for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK) sbins[i] = 0ull;
@@ -539,7 +540,11 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         /* ---- one iteration of while(!done), mcvox.template.c:748-1012 ---- */
         const MaterialL &M = sm[mi];
         st[XO_STAT_STEPS]++;
-        float step = -xo_logf(rng_next(x, a)) * M.inv_mut;
+        float step;
+        if (METHOD == XO_METHOD_MBL)
+            step = (-xo_logf(rng_next(x, a))) / M.mus; /* mcvox.template.c:763-766 */
+        else
+            step = -xo_logf(rng_next(x, a)) * M.inv_mut;
         step = fminf(step, XO_FLT_MAX);
         /* mcsim_intersect, mcvox.template.c:173-196 */
         float dist[3];
@@ -555,6 +560,22 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         p[2] = __builtin_fmaf(d_ok, d[2], p[2]);
 
         bool done = false;
+        if (METHOD == XO_METHOD_MBL) {
+            /* absorption along the step inside this voxel,
+             * mcvox.template.c:800-818 */
+            const float mua = M.mua;
+            const float fraction = 1.0f - xo_expf(-(mua * d_ok));
+            const float deposit = fraction * w;
+            w -= deposit;
+            if (use_fluence) {
+                const float back =
+                    (mua != 0.0f) ? d_ok - (-xo_logf(1.0f - rng_next(x, a) * fraction)) / mua
+                                  : 0.0f;
+                if (fluence_deposit(F, P.accum, p[0] - back * d[0], p[1] - back * d[1],
+                                    p[2] - back * d[2], deposit))
+                    st[XO_STAT_DEPOSITS]++;
+            }
+        }
         if (dd < step) {
             st[XO_STAT_BOUNDARY]++;
             /* mcsim_boundary, mcvox.template.c:275-388 */
@@ -619,18 +640,34 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                     done = true;
                 }
             }
-        } else {
-            float deposit = w * M.mua_inv_mut;
+        } else if (METHOD == XO_METHOD_AR && rng_next(x, a) < M.mua_inv_mut) {
+            /* albedo rejection: the whole weight is absorbed here,
+             * mcvox.template.c:925-944 */
+            const float deposit = w;
             w -= deposit;
-            /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
             if (use_fluence && fluence_deposit(F, P.accum, p[0], p[1], p[2], deposit))
                 st[XO_STAT_DEPOSITS]++;
+            done = true;
+        } else {
+            if (METHOD == XO_METHOD_AW) {
+                float deposit = w * M.mua_inv_mut;
+                w -= deposit;
+                /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
+                if (use_fluence && fluence_deposit(F, P.accum, p[0], p[1], p[2], deposit))
+                    st[XO_STAT_DEPOSITS]++;
+            }
             PfL pf{M.g, M.beta, M.la, M.lb, M.lc, M.loff, M.lsize_m1};
             float xi_fi;
             float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
             scatter_direction(d[0], d[1], d[2], ct, xi_fi);
-            if (roulette(w, x, a, w_min, chance, use_lottery)) done = true;
+            if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
+                done = true;
         }
+        /* microscopic Beer-Lambert: roulette after the boundary or the
+         * scattering, whether or not the packet has left
+         * (mcvox.template.c:865-881) */
+        if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
+            done = true;
         float ex = p[0] - spx, ey = p[1] - spy, ez = p[2] - spz;
         if (ex * ex + ey * ey + ez * ez > rmax2 || w <= 0.0f) done = true;
         if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
@@ -698,22 +735,40 @@ hipError_t launch_mcml(const McmlParams &P, uint32_t blocks, hipStream_t stream)
     }
 }
 
-hipError_t launch_mcvox(const McvoxParams &P, uint32_t blocks, hipStream_t stream) {
-    size_t lds = mcvox_lds_bytes(P);
-    switch (P.cfg.pf_kind) {
-    case XO_PF_HG:
-        hipLaunchKernelGGL(mcvox_kernel<XO_PF_HG>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+template <int PF>
+static hipError_t launch_mcvox_pf(const McvoxParams &P, uint32_t blocks, size_t lds,
+                                  hipStream_t stream) {
+    switch (P.cfg.method) {
+    case XO_METHOD_AW:
+        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AW>), dim3(blocks), dim3(XO_BLOCK), lds,
+                           stream, P);
         break;
-    case XO_PF_MHG:
-        hipLaunchKernelGGL(mcvox_kernel<XO_PF_MHG>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+    case XO_METHOD_AR:
+        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AR>), dim3(blocks), dim3(XO_BLOCK), lds,
+                           stream, P);
         break;
-    case XO_PF_LUT:
-        hipLaunchKernelGGL(mcvox_kernel<XO_PF_LUT>, dim3(blocks), dim3(XO_BLOCK), lds, stream, P);
+    case XO_METHOD_MBL:
+        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_MBL>), dim3(blocks), dim3(XO_BLOCK), lds,
+                           stream, P);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_mcvox(const McvoxParams &P, uint32_t blocks, hipStream_t stream) {
+    size_t lds = mcvox_lds_bytes(P);
+    switch (P.cfg.pf_kind) {
+    case XO_PF_HG:
+        return launch_mcvox_pf<XO_PF_HG>(P, blocks, lds, stream);
+    case XO_PF_MHG:
+        return launch_mcvox_pf<XO_PF_MHG>(P, blocks, lds, stream);
+    case XO_PF_LUT:
+        return launch_mcvox_pf<XO_PF_LUT>(P, blocks, lds, stream);
+    default:
+        return hipErrorInvalidValue;
+    }
 }
 
 hipError_t occupancy_mcml(int pf_kind, size_t lds, int *blocks_per_cu) {
@@ -733,16 +788,17 @@ hipError_t occupancy_mcml(int pf_kind, size_t lds, int *blocks_per_cu) {
 }
 
 hipError_t occupancy_mcvox(int pf_kind, size_t lds, int *blocks_per_cu) {
+    /* the albedo-weight build stands for all three methods */
     switch (pf_kind) {
     case XO_PF_MHG:
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcvox_kernel<XO_PF_MHG>,
-                                                            XO_BLOCK, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            blocks_per_cu, mcvox_kernel<XO_PF_MHG, XO_METHOD_AW>, XO_BLOCK, lds);
     case XO_PF_LUT:
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcvox_kernel<XO_PF_LUT>,
-                                                            XO_BLOCK, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            blocks_per_cu, mcvox_kernel<XO_PF_LUT, XO_METHOD_AW>, XO_BLOCK, lds);
     default:
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, mcvox_kernel<XO_PF_HG>,
-                                                            XO_BLOCK, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            blocks_per_cu, mcvox_kernel<XO_PF_HG, XO_METHOD_AW>, XO_BLOCK, lds);
     }
 }
 

@@ -697,7 +697,8 @@ int xo_serve_admit_mcvox(xo_server *S, const xo_mcvox_cfg *cfg, const int32_t *v
         return -1;
     }
     if (cfg->num_materials < 1 || cfg->num_materials > XO_MAX_MATERIALS ||
-        cfg->method != XO_METHOD_AW ||
+        (cfg->method != XO_METHOD_AW && cfg->method != XO_METHOD_AR &&
+         cfg->method != XO_METHOD_MBL) ||
         (cfg->source_kind != XO_SRC_LINE && cfg->source_kind != XO_SRC_GAUSSIAN)) {
         xo::set_error("xo_serve_admit_mcvox: invalid configuration");
         return -1;

@@ -17,8 +17,8 @@ class McOption:
 class McMethod(McOption):
     """Stepping method: albedo weight (0), albedo rejection (1) or
     microscopic Beer-Lambert (2) (xopto/mcbase/mcoptions.py:269-318).
-    Layered simulations take all three; voxelized simulations and batches
-    take albedo weight."""
+    Layered and voxelized simulations take all three; batches of layered
+    instances take albedo weight."""
     name = 'method'
 
     def __init__(self, value: int = 0):

@@ -28,6 +28,9 @@ class Mc(McWorker):
         if trace is not None or surface is not None:
             raise NotImplementedError(
                 'Trace and surface layouts are not part of this engine.')
+        if fluence is not None and not isinstance(
+                fluence, (mcfluence.Fluence, mcfluence.FluenceRz)):
+            raise TypeError('fluence must be a Fluence or FluenceRz object')
         super().__init__(options=options, rnginit=rnginit,
                          cl_devices=cl_devices,
                          cl_build_options=cl_build_options,
@@ -83,9 +86,6 @@ class Mc(McWorker):
                              'change between simulation calls!')
         cfg = N.XoMcvoxCfg()
         self._apply_options(cfg)
-        if cfg.method != N.XO_METHOD_AW:
-            raise NotImplementedError('Voxelized simulations run the '
-                                      'albedo-weight method only.')
         cfg.pf_kind = self._materials.pf_kind
         cfg.rmax = self._rmax
         cfg.num_materials = len(self._materials)
@@ -142,7 +142,7 @@ class Mc(McWorker):
         fluence_res = out[1]
         if self._fluence is not None:
             if fluence_res is None:
-                fluence_res = mcfluence.Fluence(self._fluence)
+                fluence_res = type(self._fluence)(self._fluence)
             off, n = self.allocation(self._fluence)
             fluence_res.update_data(self, [accum[off:off + n]], nphotons)
         detectors_res = out[2]

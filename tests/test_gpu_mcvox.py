@@ -15,8 +15,10 @@ from test_oracle_mcvox import column_sim, fixture_voxel_sim, two_layer_pair
 pytestmark = pytest.mark.gpu
 
 
-def skin_vessel_sim(n=40, size=0.8e-3, lut=False):
-    """Two-layer skin with a vessel (the mcvox 200^3 workload, reduced)."""
+def skin_vessel_sim(n=40, size=0.8e-3, lut=False, method=0, rz=False):
+    """Two-layer skin with a vessel (the mcvox 200^3 workload, reduced);
+    `method` is the McMethod value, `rz` swaps the Cartesian fluence for
+    an r-z one."""
     M = mc.mcmaterial
     if lut:
         from pyxopto_amd.mcbase import pf as hostpf
@@ -32,14 +34,19 @@ def skin_vessel_sim(n=40, size=0.8e-3, lut=False):
     G = mc.mcgeometry
     voxels = G.Voxels(G.Axis(-size / 2, size / 2, n),
                       G.Axis(-size / 2, size / 2, n), G.Axis(0.0, size, n))
-    fl = mc.mcfluence.Fluence(G.Axis(-size / 2, size / 2, n),
-                              G.Axis(-size / 2, size / 2, n),
-                              G.Axis(0.0, size, n))
+    if rz:
+        fl = mc.mcfluence.FluenceRz(G.Axis(0.0, size / 2, n // 2),
+                                    G.Axis(0.0, size, n))
+    else:
+        fl = mc.mcfluence.Fluence(G.Axis(-size / 2, size / 2, n),
+                                  G.Axis(-size / 2, size / 2, n),
+                                  G.Axis(0.0, size, n))
     D = mc.mcdetector
     det = D.Detectors(top=D.Radial(D.RadialAxis(0, size / 2, 20)),
                       bottom=D.Total(), specular=D.Total())
     sim = mc.Mc(voxels, materials, mc.mcsource.GaussianBeam(50e-6), det,
-                fluence=fl, rnginit=XINIT)
+                fluence=fl, options=[mc.mcoptions.McMethod(method)],
+                rnginit=XINIT)
     z, y, x = sim.voxels.meshgrid()
     mat = sim.voxels.material
     mat[...] = 2

@@ -1,8 +1,8 @@
-"""GPU parity of the layered kernel's stepping methods and fluence
-accumulators (xo_mcml_run): bit-exact with the oracle under the static
-schedule for albedo weight, albedo rejection and microscopic Beer-Lambert
-with Cartesian and r-z fluence, and statistical parity of the production
-(dynamic) schedule."""
+"""GPU parity of the stepping methods and fluence accumulators of the
+layered (xo_mcml_run) and voxelized (xo_mcvox_run) kernels: bit-exact with
+the oracle under the static schedule for albedo weight, albedo rejection and
+microscopic Beer-Lambert with Cartesian and r-z fluence, and statistical
+parity of the production (dynamic) schedule."""
 import numpy as np
 import pytest
 
@@ -11,6 +11,7 @@ from pyxopto_amd.mcml import mc
 from oracle import pyoracle as O
 
 from helpers import XINIT, batch_mean_se, oracle_batches, z_scores
+from test_gpu_mcvox import skin_vessel_sim
 
 pytestmark = pytest.mark.gpu
 
@@ -78,4 +79,50 @@ def test_methods_dynamic_schedule_statistics(method):
     assert np.abs(z).max() < 4.5, np.abs(z).max()
     for i in (60, 61):  # transmittance, specular (after 60 radial bins)
         assert abs(mg[i] - mo[i]) <= 3 * np.hypot(sg[i], so[i]) + 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize('method', [N.XO_METHOD_AR, N.XO_METHOD_MBL])
+@pytest.mark.parametrize('geometry', ['xyz', 'rz'])
+def test_voxel_methods_static_schedule_bit_exact(method, geometry):
+    print('case voxel', method, geometry, flush=True)
+    sim = skin_vessel_sim(method=method, rz=geometry == 'rz')
+    cfg, lut, naccum = sim.pack()
+    assert cfg.method == method
+    lanes, npk = 8192, 40000
+    ctx = N.Context(0, 0, XINIT)
+    gpu, rep = ctx.run_mcvox(cfg, sim.voxels.material, lut, npk, naccum,
+                             lanes, N.XO_FLAG_STATIC_SCHEDULE)
+    gx, _ = ctx.get_rng(lanes)
+    x, a = N.init_rng(lanes, XINIT)
+    ref, counts, rx = O.mcvox_run(cfg, sim.voxels.material, lut, x, a, npk,
+                                  naccum, O.MATH_KERNEL)
+    assert int(rep.steps) == int(counts[0])
+    assert int(rep.voxel_crossings) == int(counts[2])
+    assert int(rep.deposits) == int(counts[3])
+    np.testing.assert_array_equal(gpu, ref)
+    np.testing.assert_array_equal(gx, rx)
+    assert gpu[cfg.fluence.offset:].sum() > 0
+    ctx.close()
+
+
+def test_voxel_mbl_dynamic_schedule_statistics():
+    sim = skin_vessel_sim(n=20, method=N.XO_METHOD_MBL, rz=True)
+    cfg, lut, naccum = sim.pack()
+    vox = sim.voxels.material
+    k, npk = 40, 10000
+    ctx = N.Context(0, 0, XINIT)
+    g = np.array([ctx.run_mcvox(cfg, vox if i == 0 else None, lut, npk,
+                                naccum)[0] for i in range(k)],
+                 dtype=np.float64) / N.XO_ACCU_K / npk
+
+    def one(x, a):
+        acc, _, _ = O.mcvox_run(cfg, vox, lut, x, a, npk, naccum, O.MATH_LIBM)
+        return acc.astype(np.float64) / N.XO_ACCU_K / npk
+    o = oracle_batches(one, k, 256)
+    mg, sg = batch_mean_se(g)
+    mo, so = batch_mean_se(o)
+    ok = (g > 0).all(axis=0) & (o > 0).all(axis=0)
+    z = z_scores(mg[ok], sg[ok], mo[ok], so[ok])
+    assert np.abs(z).max() < 4.5, np.abs(z).max()
     ctx.close()

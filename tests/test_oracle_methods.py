@@ -109,3 +109,87 @@ def test_method_option_objects():
               options=[opts.McMethod.ar])
     with pytest.raises(NotImplementedError):
         b.pack()
+
+
+# ---- voxelized media (mcvox.template.c:763-977) ----
+def _vox_sim(method, mua, mus, geometry='xyz', nz=20, depth=2e-3, g=0.8,
+             n=1.0):
+    from pyxopto_amd.mcvox import mc as mcv
+    M, G = mcv.mcmaterial, mcv.mcgeometry
+    materials = M.Materials([
+        M.Material(n=1.0, mua=0.0, mus=0.0, pf=mcv.mcpf.Hg(g)),
+        M.Material(n=n, mua=mua, mus=mus, pf=mcv.mcpf.Hg(g))])
+    voxels = G.Voxels(G.Axis(-1.0, 1.0, 1), G.Axis(-1.0, 1.0, 1),
+                      G.Axis(0.0, depth, nz))
+    A = mcv.mcfluence.Axis
+    if geometry == 'xyz':
+        flu = mcv.mcfluence.Fluence(A(-1.0, 1.0, 1), A(-1.0, 1.0, 1),
+                                    A(0.0, depth, nz))
+    else:
+        flu = mcv.mcfluence.FluenceRz(A(0.0, 1.0, 1), A(0.0, depth, nz))
+    D = mcv.mcdetector
+    det = D.Detectors(top=D.Total(), bottom=D.Total(), specular=D.Total())
+    sim = mcv.Mc(voxels, materials, mcv.mcsource.Line(), det, fluence=flu,
+                 options=[mc.mcoptions.McMethod(method)],
+                 rnginit=0x5EED000000000001)
+    sim.voxels.material[:] = 1
+    return sim
+
+
+def _vox_batches(sim, npk, k=24, lanes=256):
+    cfg, lut, naccum = sim.pack()
+    vox = sim.voxels.material
+
+    def one(x, a):
+        acc, _, _ = O.mcvox_run(cfg, vox, lut, x, a, npk, naccum,
+                                O.MATH_LIBM)
+        return acc.astype(np.float64) / K / npk
+    return oracle_batches(one, k, lanes)
+
+
+@pytest.mark.parametrize('method', ['aw', 'ar', 'mbl'])
+@pytest.mark.parametrize('geometry', ['xyz', 'rz'])
+def test_voxel_nonscattering_column_beer_lambert(method, geometry):
+    mua, depth, nz = 1000.0, 2e-3, 20
+    sim = _vox_sim(METHODS[method], mua, 0.0, geometry, nz, depth)
+    cfg, _, _ = sim.pack()
+    assert cfg.method == METHODS[method]
+    b = _vox_batches(sim, 4000)
+    m, se = batch_mean_se(b)
+    top, bottom, spec = m[0], m[1], m[2]
+    dep = m[3:3 + nz]
+    z = np.linspace(0.0, depth, nz + 1)
+    expect = np.exp(-mua * z[:-1]) - np.exp(-mua * z[1:])
+    if method == 'mbl':
+        # each voxel takes exactly 1 - exp(-mua dz) of the weight left
+        np.testing.assert_allclose(dep, expect, rtol=2e-5, atol=2e-7)
+    else:
+        zs = (dep - expect) / np.maximum(se[3:3 + nz], 1e-12)
+        assert np.abs(zs).max() < 5.0, (method, zs)
+    assert abs(bottom - np.exp(-mua * depth)) < 5 * se[1] + 1e-5
+    assert top == 0.0 and spec == 0.0
+    assert abs(dep.sum() + bottom - 1.0) < 1e-4
+
+
+def test_voxel_methods_agree_with_each_other_and_the_layered_kernel():
+    """A scattering slab as a voxel column: R and T agree across the three
+    voxel methods and with the layered kernel's albedo-weight run."""
+    mua, mus, n, depth = 300.0, 2e4, 1.37, 1e-3
+    res = {}
+    for name, method in METHODS.items():
+        sim = _vox_sim(method, mua, mus, 'xyz', 10, depth, n=n)
+        b = _vox_batches(sim, 3000)
+        top, bottom, spec = b[:, 0], b[:, 1], b[:, 2]
+        total = top + bottom + spec + b[:, 3:].sum(axis=1)
+        assert np.all(np.abs(total - 1.0) < 2e-3), (name, total)
+        res[name] = (batch_mean_se(top), batch_mean_se(bottom))
+    lay = _batches(_sim(METHODS['aw'], mua, mus, n_slab=n, d=depth, nz=10),
+                   3000)
+    res['layered'] = (batch_mean_se(lay[:, 0]), batch_mean_se(lay[:, 1]))
+    names = list(res)
+    for i, a in enumerate(names):
+        for c in names[i + 1:]:
+            for j in range(2):
+                (m1, s1), (m2, s2) = res[a][j], res[c][j]
+                z = (m1 - m2) / np.sqrt(s1 ** 2 + s2 ** 2)
+                assert abs(z) < 4.0, (a, c, j, m1, m2, z)
