@@ -11,6 +11,7 @@ from typing import Tuple
 import numpy as np
 
 from .. import _native as N
+from .mcutil import axis as _axis
 from .mcutil.axis import Axis, RadialAxis
 
 
@@ -107,9 +108,18 @@ class Radial(Detector):
         return target
 
     def todict(self):
-        return {'type': 'Radial', 'raxis': self._r_axis.todict(),
-                'position': self._position.tolist(), 'cosmin': self._cosmin,
+        """Configuration without the accumulator data (radial.py:292-309)."""
+        return {'type': 'Radial', 'position': self._position.tolist(),
+                'r_axis': self._r_axis.todict(), 'cosmin': self._cosmin,
                 'direction': self._direction.tolist()}
+
+    @staticmethod
+    def fromdict(data: dict) -> 'Radial':
+        data = dict(data)
+        kind = data.pop('type')
+        if kind != 'Radial':
+            raise TypeError('Expected "Radial" type but got "{}"!'.format(kind))
+        return Radial(_axis.fromdict(data.pop('r_axis')), **data)
 
     def __repr__(self):
         return 'Radial({}, position={}, cosmin={})'.format(
@@ -152,8 +162,17 @@ class Total(Detector):
         return target
 
     def todict(self):
+        """Configuration without the accumulator data (total.py:197-212)."""
         return {'type': 'Total', 'cosmin': self._cosmin,
                 'direction': self._direction.tolist()}
+
+    @staticmethod
+    def fromdict(data: dict) -> 'Total':
+        data = dict(data)
+        kind = data.pop('type')
+        if kind != 'Total':
+            raise TypeError('Expected "Total" type but got "{}"!'.format(kind))
+        return Total(**data)
 
     def __repr__(self):
         return 'Total(cosmin={})'.format(self._cosmin)
@@ -201,6 +220,35 @@ class Detectors:
         for name, det in self.items():
             if det is not None and name in accumulators_by_name:
                 det.update_data(mc, [accumulators_by_name[name]], nphotons)
+
+    def todict(self) -> dict:
+        """base.py:551-561; an unused location is {'type':
+        'DetectorDefault'} (base.py:201-205)."""
+        out = {'type': 'Detectors'}
+        for name, det in self.items():
+            out[name] = det.todict() if det is not None else \
+                {'type': 'DetectorDefault'}
+        return out
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'Detectors':
+        data = dict(data)
+        kind = data.pop('type')
+        if kind != 'Detectors':
+            raise TypeError('Expected "Detectors" type but got "{}"!'.format(
+                kind))
+        kinds = {'Radial': Radial, 'Total': Total}
+        dets = {}
+        for name in ('top', 'bottom', 'specular'):
+            item = data.get(name)
+            if item is None or item.get('type') == 'DetectorDefault':
+                dets[name] = None
+            elif item.get('type') in kinds:
+                dets[name] = kinds[item['type']].fromdict(item)
+            else:
+                raise TypeError('Unsupported detector type "{}"!'.format(
+                    item.get('type')))
+        return cls(**dets)
 
     def __repr__(self):
         return 'Detectors(top={}, bottom={}, specular={})'.format(

@@ -8,7 +8,17 @@ dx*dy*dz (fluence.py:343-347) or pi (r1^2 - r0^2) dz (fluencerz.py:343-
 import numpy as np
 
 from .. import _native as N
+from .mcutil import axis as _axis
 from .mcutil.axis import Axis
+
+
+def _pop_type(cls, data: dict) -> dict:
+    data = dict(data)
+    kind = data.pop('type')
+    if kind != cls.__name__:
+        raise TypeError('Expected "{}" type but got "{}"!'.format(
+            cls.__name__, kind))
+    return data
 
 
 class Fluence:
@@ -71,9 +81,19 @@ class Fluence:
         return target
 
     def todict(self):
-        return {'type': 'Fluence', 'xaxis': self._x_axis.todict(),
-                'yaxis': self._y_axis.todict(), 'zaxis': self._z_axis.todict(),
-                'mode': self._mode}
+        """Configuration without the accumulator data (fluence.py:247-262)."""
+        return {'type': 'Fluence', 'mode': self._mode,
+                'xaxis': self._x_axis.todict(), 'yaxis': self._y_axis.todict(),
+                'zaxis': self._z_axis.todict()}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'Fluence':
+        """fluence.py:266-284"""
+        data = _pop_type(cls, data)
+        x = _axis.fromdict(data.pop('xaxis'))
+        y = _axis.fromdict(data.pop('yaxis'))
+        z = _axis.fromdict(data.pop('zaxis'))
+        return cls(x, y, z, **data)
 
 
 class FluenceRz:
@@ -151,6 +171,25 @@ class FluenceRz:
         return target
 
     def todict(self):
-        return {'type': 'FluenceRz', 'raxis': self._r_axis.todict(),
-                'zaxis': self._z_axis.todict(),
-                'center': self._center.tolist(), 'mode': self._mode}
+        """Configuration without the accumulator data
+        (fluencerz.py:254-269)."""
+        return {'type': 'FluenceRz', 'mode': self._mode,
+                'raxis': self._r_axis.todict(), 'zaxis': self._z_axis.todict(),
+                'center': self._center.tolist()}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'FluenceRz':
+        """fluencerz.py:273-290"""
+        data = _pop_type(cls, data)
+        r = _axis.fromdict(data.pop('raxis'))
+        z = _axis.fromdict(data.pop('zaxis'))
+        return cls(r, z, **data)
+
+
+def fromdict(data: dict):
+    """Fluence or FluenceRz from its todict() data."""
+    kinds = {'Fluence': Fluence, 'FluenceRz': FluenceRz}
+    if data.get('type') not in kinds:
+        raise TypeError('Unsupported fluence type "{}"!'.format(
+            data.get('type')))
+    return kinds[data['type']].fromdict(data)

@@ -2,6 +2,7 @@
 from typing import List
 
 from .. import _native as N
+from . import mcpf
 from .mcpf import PfBase
 
 
@@ -30,8 +31,19 @@ class Material:
         return target
 
     def todict(self):
-        return {'type': 'Material', 'n': self.n, 'mua': self.mua,
-                'mus': self.mus, 'pf': self.pf.todict()}
+        """mcmaterial.py:278-283"""
+        return {'n': self.n, 'mua': self.mua, 'mus': self.mus,
+                'pf': self.pf.todict(), 'type': 'Material'}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'Material':
+        data = dict(data)
+        kind = data.pop('type')
+        if kind != cls.__name__:
+            raise TypeError('Expected "{}" type but got "{}"!'.format(
+                cls.__name__, kind))
+        pf = mcpf.fromdict(data.pop('pf'))
+        return cls(pf=pf, **data)
 
     def __repr__(self):
         return 'Material(n={}, mua={}, mus={}, pf={})'.format(
@@ -64,3 +76,17 @@ class Materials:
         return iter(self._materials)
 
     pf_kind = property(lambda self: self._materials[0].pf.kind)
+
+    def todict(self) -> dict:
+        """mcmaterial.py:807-814"""
+        return {'type': 'Materials',
+                'materials': [m.todict() for m in self._materials]}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'Materials':
+        data = dict(data)
+        kind = data.pop('type')
+        if kind != cls.__name__:
+            raise TypeError('Expected "{}" type but got "{}"!'.format(
+                cls.__name__, kind))
+        return cls([Material.fromdict(m) for m in data.pop('materials')])

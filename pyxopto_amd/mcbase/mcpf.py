@@ -1,7 +1,9 @@
 """Device phase-function samplers as plugin objects: Hg
 (xopto/mcbase/mcpf/hg.py:25-140), MHg (mcpf/mhg.py), Lut and LutEx
 (mcpf/lut.py:30-300). Each packs its parameters into the xo_pf struct; the
-kernels specialise on the kind."""
+kernels specialise on the kind. todict/fromdict keep the reference's dict
+schema ({'type': class name, constructor arguments...}; hg.py:160-164,
+mhg.py:186-190, lut.py:258-263, 317-323, pfbase.py:36-48)."""
 import numpy as np
 
 from .. import _native as N
@@ -16,6 +18,16 @@ class PfBase:
 
     def todict(self) -> dict:
         raise NotImplementedError
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'PfBase':
+        data = dict(data)
+        kind = data.pop('type')
+        if kind != cls.__name__:
+            raise TypeError(
+                'Cannot initialize an instance of {:s} scattering phase '
+                'function from the data of "{}"!'.format(cls.__name__, kind))
+        return cls(**data)
 
 
 class Hg(PfBase):
@@ -40,7 +52,7 @@ class Hg(PfBase):
         return target
 
     def todict(self):
-        return {'type': 'Hg', 'g': self._g}
+        return {'g': self._g, 'type': 'Hg'}
 
     def __repr__(self):
         return 'Hg(g={})'.format(self._g)
@@ -65,7 +77,7 @@ class MHg(PfBase):
         return target
 
     def todict(self):
-        return {'type': 'MHg', 'g': self._g, 'b': self._b}
+        return {'g': self._g, 'b': self._b, 'type': 'MHg'}
 
     def __repr__(self):
         return 'MHg(g={}, b={})'.format(self._g, self._b)
@@ -97,8 +109,8 @@ class Lut(PfBase):
         return target
 
     def todict(self):
-        return {'type': 'Lut', 'params': self._params.tolist(),
-                'lut': self._lut.tolist()}
+        return {'params': self._params.tolist(), 'lut': self._lut.tolist(),
+                'type': 'Lut'}
 
     def __repr__(self):
         return 'Lut(params=({}, {}, {}), size={})'.format(
@@ -112,6 +124,7 @@ class LutEx(Lut):
         if isinstance(pftype, str):
             pftype = getattr(hostpf, pftype)
         self._pfargs = tuple(pfargs)
+        self._kwargs = dict(kwargs)
         self._pf_obj = pftype(*pfargs)
         params, lut = self._pf_obj.mclut(lutsize, **kwargs)
         super().__init__(params, lut)
@@ -119,5 +132,31 @@ class LutEx(Lut):
     pfargs = property(lambda self: self._pfargs, None, None,
                       'Phase function arguments.')
 
+    def todict(self):
+        return {'pftype': type(self._pf_obj).__name__,
+                'pfargs': list(self._pfargs), 'lutsize': int(self._lut.size),
+                'kwargs': dict(self._kwargs), 'type': 'LutEx'}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'LutEx':
+        data = dict(data)
+        kind = data.pop('type')
+        if kind != cls.__name__:
+            raise TypeError(
+                'Cannot initialize an instance of {:s} scattering phase '
+                'function from the data of "{}"!'.format(cls.__name__, kind))
+        kwargs = dict(data.pop('kwargs', None) or {})
+        return cls(data['pftype'], data['pfargs'], data['lutsize'], **kwargs)
+
     def pf(self):
         return self._pf_obj
+
+
+def fromdict(data: dict) -> PfBase:
+    """Phase function object of any kind from its todict() data."""
+    kinds = {'Hg': Hg, 'MHg': MHg, 'Lut': Lut, 'LutEx': LutEx}
+    kind = data.get('type')
+    if kind not in kinds:
+        raise TypeError('Scattering phase function "{}" not '
+                        'implemented'.format(kind))
+    return kinds[kind].fromdict(data)

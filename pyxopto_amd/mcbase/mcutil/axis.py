@@ -85,3 +85,14 @@ class RadialAxis(Axis):
         return self._true_centers
     centers = property(_get_centers, None, None,
                        'Area-corrected centres of the bins.')
+
+
+def fromdict(data: dict) -> Axis:
+    """Axis or RadialAxis from its todict() data (axis.py:125-146,
+    251-264). RadialAxis.todict returns its dict here; the reference's
+    returns None (axis.py:251-256 has no return statement)."""
+    kinds = {'Axis': Axis, 'RadialAxis': RadialAxis}
+    kind = data.get('type', 'Axis')
+    if kind not in kinds:
+        raise TypeError('Unsupported axis type "{}"!'.format(kind))
+    return kinds[kind].fromdict(data)

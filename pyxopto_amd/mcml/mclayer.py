@@ -5,6 +5,7 @@ import numpy as np
 
 from .. import _native as N
 from ..mcbase.mcmaterial import _mut_terms
+from ..mcbase import mcpf
 from ..mcbase.mcpf import PfBase
 from ..mcbase.mcutil import boundary
 
@@ -51,8 +52,18 @@ class Layer:
         return target
 
     def todict(self):
+        """layer.py:356-361"""
         return {'d': self._d, 'n': self._n, 'mua': self._mua,
                 'mus': self._mus, 'pf': self._pf.todict(), 'type': 'Layer'}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'Layer':
+        """layer.py:364-385"""
+        data = dict(data)
+        if data.pop('type') != 'Layer':
+            raise ValueError('Cannot create a Layer instance from the data!')
+        pf = mcpf.fromdict(data.pop('pf'))
+        return cls(pf=pf, **data)
 
     def __repr__(self):
         return 'Layer(d={}, n={}, mua={}, mus={}, pf={})'.format(
@@ -87,6 +98,20 @@ class Layers:
 
     def tolist(self) -> List[Layer]:
         return list(self._layers)
+
+    def todict(self) -> dict:
+        """layer.py:1009-1014"""
+        return {'layers': [layer.todict() for layer in self._layers],
+                'type': 'Layers'}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'Layers':
+        """layer.py:1028-1046"""
+        data = dict(data)
+        if data.pop('type') != 'Layers':
+            raise ValueError(
+                'Cannot create a Layers instance from the given data!')
+        return cls([Layer.fromdict(item) for item in data.pop('layers')])
 
     def __len__(self):
         return len(self._layers)

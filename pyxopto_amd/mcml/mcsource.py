@@ -70,8 +70,18 @@ class Line:
         return target
 
     def todict(self):
+        """mcml/mcsource/line.py:239-245"""
         return {'position': self._position.tolist(),
                 'direction': self._direction.tolist(), 'type': 'Line'}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'Line':
+        data = dict(data)
+        kind = data.pop('type')
+        if kind != cls.__name__:
+            raise TypeError('Expected "{}" type but got "{}"!'.format(
+                cls.__name__, kind))
+        return cls(**data)
 
     def __repr__(self):
         return 'Line(position=({}, {}, {}), direction=({}, {}, {}))'.format(
@@ -130,11 +140,24 @@ class UniformFiberNI:
 
     @classmethod
     def fromdict(cls, data: dict) -> 'UniformFiberNI':
+        """fiberni.py:406-420"""
         data = dict(data)
-        data.pop('type', None)
+        kind = data.pop('type')
+        if kind != cls.__name__:
+            raise TypeError('Expected "{}" type but got "{}"!'.format(
+                cls.__name__, kind))
         fiber = MultimodeFiber.fromdict(data.pop('fiber'))
         return cls(fiber, **data)
 
     def __repr__(self):
         return 'UniformFiberNI(fiber={}, position=({}, {}, {}))'.format(
             self._fiber, *self._position)
+
+
+def fromdict(data: dict):
+    """Line or UniformFiberNI from its todict() data."""
+    kinds = {'Line': Line, 'UniformFiberNI': UniformFiberNI}
+    if data.get('type') not in kinds:
+        raise TypeError('Unsupported source type "{}"!'.format(
+            data.get('type')))
+    return kinds[data['type']].fromdict(data)

@@ -9,6 +9,7 @@ from typing import Tuple
 import numpy as np
 
 from .. import _native as N
+from ..mcbase.mcutil import axis as _axis
 from ..mcbase.mcutil.axis import Axis
 
 
@@ -108,6 +109,19 @@ class Voxels:
         return target
 
     def todict(self):
+        """voxel.py:687-696; the material grid is not part of the dict."""
         return {'type': 'Voxels', 'xaxis': self._x_axis.todict(),
                 'yaxis': self._y_axis.todict(),
                 'zaxis': self._z_axis.todict()}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'Voxels':
+        """voxel.py:699-720"""
+        data = dict(data)
+        kind = data.pop('type')
+        if kind != cls.__name__:
+            raise TypeError('Expected data of instance Voxels '
+                            'but got "{}"!'.format(kind))
+        return cls(_axis.fromdict(data.pop('xaxis')),
+                   _axis.fromdict(data.pop('yaxis')),
+                   _axis.fromdict(data.pop('zaxis')))

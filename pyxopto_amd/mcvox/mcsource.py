@@ -18,6 +18,15 @@ def _unit(v):
     return v / norm
 
 
+def _pop_type(cls, data: dict) -> dict:
+    data = dict(data)
+    kind = data.pop('type')
+    if kind != cls.__name__:
+        raise TypeError('Expected "{}" type but got "{}"!'.format(
+            cls.__name__, kind))
+    return data
+
+
 class Line:
     kind = N.XO_SRC_LINE
 
@@ -60,8 +69,13 @@ class Line:
         return cfg
 
     def todict(self):
-        return {'type': 'Line', 'position': self._position.tolist(),
-                'direction': self._direction.tolist()}
+        """mcvox/mcsource/line.py:244-250"""
+        return {'position': self._position.tolist(),
+                'direction': self._direction.tolist(), 'type': 'Line'}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'Line':
+        return cls(**_pop_type(cls, data))
 
 
 class GaussianBeam:
@@ -99,6 +113,20 @@ class GaussianBeam:
         return cfg
 
     def todict(self):
-        return {'type': 'GaussianBeam', 'sigma': self._sigma.tolist(),
-                'clip': self._clip, 'position': self._position.tolist(),
-                'direction': self._direction.tolist()}
+        """gaussianbeam.py:340-349"""
+        return {'sigma': self._sigma.tolist(), 'clip': self._clip,
+                'position': self._position.tolist(),
+                'direction': self._direction.tolist(), 'type': 'GaussianBeam'}
+
+    @classmethod
+    def fromdict(cls, data: dict) -> 'GaussianBeam':
+        return cls(**_pop_type(cls, data))
+
+
+def fromdict(data: dict):
+    """Line or GaussianBeam from its todict() data."""
+    kinds = {'Line': Line, 'GaussianBeam': GaussianBeam}
+    if data.get('type') not in kinds:
+        raise TypeError('Unsupported source type "{}"!'.format(
+            data.get('type')))
+    return kinds[data['type']].fromdict(data)

@@ -191,6 +191,30 @@ def simulate(rows, packets: int, seed: int = 1,
     return np.array([r.top.reflectance for r in res]), dt
 
 
+def write_dataset(folder: str, rows, packets: int, seed: int = 1,
+                  context: N.Context = None) -> list:
+    """Simulate the feature rows (one batch launch) and write one file per
+    row in the layered dataset layout (analytical_pf.template.py:226-246;
+    ``dataset.mcml_record``), named by mua and mus' like the reference's
+    files. Returns the file names."""
+    import os
+    from . import dataset
+    from .mcml.batch import Batch
+    rows = np.atleast_2d(rows)
+    insts = [_instance(r, packets, (seed * 0x9E3779B97F4A7C15 + k + 1) &
+                       0xFFFFFFFFFFFFFFFF) for k, r in enumerate(rows)]
+    b = Batch(insts, rnginit=XINIT)
+    if context is not None:
+        b._ctx = context
+    res, _ = b.run(b.opts(chunk=2048, block=256))
+    names = []
+    for row, inst, det in zip(rows, insts, res):
+        mua, musr = np.exp(row[0]), np.exp(row[1])
+        name = os.path.join(folder, dataset.mcml_filename(mua, musr))
+        names.append(dataset.save(name, dataset.mcml_record(inst, det, 'aw')))
+    return names
+
+
 class ReflectanceSurrogate:
     """Network plus the feature and target scalings of one training box."""
 
