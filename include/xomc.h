@@ -69,8 +69,8 @@ enum xo_src_kind {
 };
 
 /* Stepping method (xopto/mcbase/mcoptions.py:269-318; the branches of
- * mcml.template.c:529-755). Layered runs take all three; voxelized runs and
- * batches take albedo weight. */
+ * mcml.template.c:529-755, mcvox.template.c:763-977). Layered and
+ * voxelized runs take all three; batches take albedo weight. */
 enum xo_method {
     XO_METHOD_AW = 0,  /* albedo weight                 */
     XO_METHOD_AR = 1,  /* albedo rejection              */
@@ -83,6 +83,31 @@ enum xo_fluence_kind {
     XO_FLUENCE_XYZ = 1, /* Fluence, xopto/mcbase/mcfluence/fluence.py       */
     XO_FLUENCE_RZ = 2   /* FluenceRz, xopto/mcbase/mcfluence/fluencerz.py   */
 };
+
+/* Photon packet trace options (MC_USE_TRACE; xopto/mcbase/mctrace.py:444-455):
+ * which states of a packet are logged. */
+enum xo_trace_options {
+    XO_TRACE_NONE = 0,
+    XO_TRACE_START = 1, /* the state right after the launch                */
+    XO_TRACE_END = 2,   /* the state when the packet terminates            */
+    XO_TRACE_ALL = 7    /* launch, every step and termination              */
+};
+
+/* Event flags of one transport step (MC_EVENT_*,
+ * xopto/mcbase/kernel/mcbase.template.h:664-681). */
+enum xo_event_flags {
+    XO_EVENT_REFLECTION = 1u,
+    XO_EVENT_REFRACTION = 2u,
+    XO_EVENT_BOUNDARY_HIT = 4u,
+    XO_EVENT_LAUNCH = 8u,
+    XO_EVENT_ABSORPTION = 16u,
+    XO_EVENT_SCATTERING = 32u,
+    XO_EVENT_TERMINATED = 64u,
+    XO_EVENT_ESCAPED = 128u
+};
+
+/* Floats per trace entry: x, y, z, px, py, pz, w, pl (TRACE_ENTRY_LEN). */
+#define XO_TRACE_ENTRY_LEN 8
 
 /* Run flags. */
 enum xo_run_flags {
@@ -206,6 +231,36 @@ typedef struct xo_mcml_cfg {
     xo_fluence fluence;         /* bins after every detector bin              */
 } xo_mcml_cfg;
 
+/* Photon packet trace of one run; McTrace of xopto/mcbase/mctrace.py:480-510
+ * with the compile-time options of Trace.cl_options (:590-595) as fields.
+ * Packet p's events go to entries [p][0 .. max_events-1] of the trace data;
+ * an event past the last entry overwrites the last one; the event count
+ * (which may exceed max_events) goes to the count array. */
+typedef struct xo_trace {
+    int32_t max_events;         /* entries per packet (>= 1)                  */
+    uint32_t options;           /* enum xo_trace_options                      */
+    uint32_t event_mask;        /* logged events (use_events only)            */
+    int32_t use_events;         /* 0: every event passes the mask             */
+    int32_t plon;               /* track the optical path length ('pl')       */
+    int32_t reserved[3];
+} xo_trace; /* 32 B */
+
+/* Sampling volume accumulator; McSamplingVolume of
+ * xopto/mcbase/mcsv.py:34-86. Bins [z][y][x] at accum[offset...]. */
+typedef struct xo_sampling_volume {
+    float top_left[3];
+    float voxel_size[3];
+    uint32_t shape[3];          /* x, y, z                                    */
+    float multiplier;           /* 1 / smallest voxel size                    */
+    uint32_t offset;
+    int32_t k;                  /* float to integer weight factor (0x7FFFFF)  */
+    int32_t end_field;          /* field of a packet's last trace entry read
+                                   as its terminal weight: 7 (pl, what the
+                                   reference reads, mcsv.template.c:278) or
+                                   6 (w)                                      */
+    int32_t reserved[3];
+} xo_sampling_volume; /* 64 B */
+
 /* A voxel material; McMaterial of xopto/mcbase/mcmaterial.py:37-63. */
 typedef struct xo_material {
     float n, mus, mua, inv_mut, mua_inv_mut;
@@ -307,8 +362,8 @@ int xo_device_count(void);
  * Writes up to n values in the order: xo_pf, xo_layer, xo_source_line,
  * xo_source_gaussian, xo_detector, xo_fluence, xo_mcml_cfg, xo_material,
  * xo_voxel_cfg, xo_mcvox_cfg, xo_run_report, xo_batch_instance,
- * xo_batch_opts, xo_serve_opts, xo_serve_stats, xo_source_fiber. Returns the
- * count written. */
+ * xo_batch_opts, xo_serve_opts, xo_serve_stats, xo_source_fiber, xo_trace,
+ * xo_sampling_volume. Returns the count written. */
 int xo_struct_sizes(uint32_t *out, int n);
 
 /* ---------------- seeds (host) ---------------- */
@@ -368,6 +423,31 @@ int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg,
                 const float *lut, uint32_t nlut,
                 uint64_t npackets, uint32_t lanes, uint32_t flags,
                 uint64_t *accum, uint64_t naccum, xo_run_report *report);
+
+/* xo_mcml_run with the photon packet trace (the MC_USE_TRACE build of
+ * McKernel, xopto/mcml/kernel/mcml.template.c:513-520, 764-810, and
+ * mcsim_trace_event / mcsim_trace_complete of xopto/mcbase/mctrace.py:
+ * 519-566). trace_data: host output of npackets * max_events *
+ * XO_TRACE_ENTRY_LEN floats; trace_n: host output of npackets event counts.
+ * Row p holds packet p (the packet index of the run). */
+int xo_mcml_run_trace(xo_ctx *ctx, const xo_mcml_cfg *cfg,
+                      const float *lut, uint32_t nlut,
+                      uint64_t npackets, uint32_t lanes, uint32_t flags,
+                      const xo_trace *trace, float *trace_data, int32_t *trace_n,
+                      uint64_t *accum, uint64_t naccum, xo_run_report *report);
+
+/* Sampling volume of a trace: the SamplingVolume kernel of
+ * xopto/mcbase/kernel/mcsv.template.c:236-420 called by Mc.sampling_volume
+ * (xopto/mcml/mc.py:1040-1200). Every packet's path between consecutive
+ * trace events is walked through the voxel grid; each voxel it crosses gets
+ * round(terminal weight * path length in the voxel * multiplier * k). A
+ * packet with no event is skipped. accum/naccum: host output (bins at
+ * sv->offset); total_weight: sum of round(terminal weight * k). */
+int xo_sampling_volume_run(xo_ctx *ctx, const xo_sampling_volume *sv,
+                           int32_t max_events, const float *trace_data,
+                           const int32_t *trace_n, uint64_t npackets,
+                           uint64_t *accum, uint64_t naccum,
+                           uint64_t *total_weight, xo_run_report *report);
 
 /* Voxelized medium: McKernel of xopto/mcvox/kernel/mcvox.template.c:548-1048
  * called by xopto/mcvox/mc.py:772-1068. voxels is int32[shape z][y][x]; pass

@@ -307,6 +307,51 @@ static inline int o_roulette(o_pkt *s, float w_min, float chance,
     return 0;
 }
 
+/* Photon packet trace (xopto/mcbase/mctrace.py:519-566); NULL when off. */
+typedef struct o_trace {
+    xo_trace cfg;
+    float *data;    /* [packet][max_events][XO_TRACE_ENTRY_LEN] */
+    int32_t *n;     /* [packet]                                  */
+} o_trace;
+
+typedef struct o_tstate {
+    uint64_t pkt;
+    uint32_t count;
+    uint32_t ev;    /* event flags of the current step           */
+    float pl;       /* optical path length                       */
+} o_tstate;
+
+/* mcsim_trace_event: log unless the event mask rejects this step; an event
+ * past the last entry overwrites it (mctrace.py:527-560). */
+static void o_trace_event(const o_trace *t, o_tstate *ts, const o_pkt *s) {
+    if (t->cfg.use_events && !(t->cfg.event_mask & ts->ev)) return;
+    uint32_t last = (uint32_t)t->cfg.max_events - 1u;
+    uint32_t slot = ts->count < last ? ts->count : last;
+    float *e = t->data +
+               (ts->pkt * (uint64_t)t->cfg.max_events + slot) * XO_TRACE_ENTRY_LEN;
+    e[0] = s->p[0];
+    e[1] = s->p[1];
+    e[2] = s->p[2];
+    e[3] = s->d[0];
+    e[4] = s->d[1];
+    e[5] = s->d[2];
+    e[6] = s->w;
+    e[7] = t->cfg.plon ? ts->pl : 0.0f;
+    ts->count++;
+}
+
+/* End of one transport step (mcml.template.c:765-781): termination flag,
+ * the logged state, the event count of a finished packet. */
+static void o_trace_step(const o_trace *t, o_tstate *ts, const o_pkt *s,
+                         int done) {
+    if (done) ts->ev |= XO_EVENT_TERMINATED;
+    if (t->cfg.options == XO_TRACE_ALL ||
+        ((t->cfg.options & XO_TRACE_END) && done))
+        o_trace_event(t, ts, s);
+    ts->ev = 0;
+    if (done) t->n[ts->pkt] = (int32_t)ts->count;
+}
+
 /* Fluence deposit (deposition mode): the Cartesian voxel of
  * mcsim_fluence_deposit_at (xopto/mcbase/mcfluence/fluence.py:102-147) or
  * the (r, z) cell of fluencerz.py:112-160. */
@@ -476,9 +521,11 @@ static void o_mcml_boundary(const xo_mcml_cfg *cfg, o_pkt *s, int32_t next) {
 }
 
 /* One packet from launch to termination: the while(!done) loop of
- * mcml.template.c:514-788 (albedo weight, MC_METHOD 0). */
+ * mcml.template.c:514-788 with the three stepping methods; tr (may be NULL)
+ * logs the packet as row pkt of the trace. */
 static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
-                          uint64_t *acc, o_counts *cnt, int mm) {
+                          uint64_t *acc, o_counts *cnt, int mm,
+                          const o_trace *tr, uint64_t pkt) {
     const int32_t nl = (int32_t)cfg->num_layers;
     const float rmax2 = cfg->rmax * cfg->rmax;
     const float *sp = (cfg->source_kind == XO_SRC_FIBER_NI)
@@ -488,6 +535,8 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
         o_fiber_launch(cfg, s, acc, mm);
     else
         o_mcml_launch(cfg, s, acc, mm);
+    o_tstate ts = {pkt, 0u, XO_EVENT_LAUNCH, 0.0f};
+    if (tr && (tr->cfg.options & XO_TRACE_START)) o_trace_event(tr, &ts, s);
     uint32_t pkt_steps = 0;
     for (;;) {
         const xo_layer *L = &cfg->layers[s->layer];
@@ -512,6 +561,8 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
         s->p[0] = fmaf(s->d[0], step, s->p[0]);
         s->p[1] = fmaf(s->d[1], step, s->p[1]);
         s->p[2] = fmaf(s->d[2], step, s->p[2]);
+        /* optical path length, mcml.template.c:570-574 */
+        if (tr && tr->cfg.plon) ts.pl += L->n * step;
         if (s->layer < next) s->p[2] = L->bottom;
         if (s->layer > next) s->p[2] = L->top;
 
@@ -522,6 +573,7 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
             float fraction = 1.0f - o_exp(-(mua * step), mm);
             float deposit = fraction * s->w;
             s->w -= deposit;
+            ts.ev |= XO_EVENT_ABSORPTION;
             if (cfg->fluence.enabled) {
                 float back = (mua != 0.0f)
                                  ? step - (-o_log(1.0f - o_rand(&s->x, s->a) * fraction,
@@ -534,7 +586,10 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
         }
         if (next != s->layer) {
             cnt->boundary++;
+            const int32_t before = s->layer;
             o_mcml_boundary(cfg, s, next);
+            ts.ev |= XO_EVENT_BOUNDARY_HIT | (s->layer != before ? XO_EVENT_REFRACTION
+                                                                 : XO_EVENT_REFLECTION);
             if (!(s->layer > 0 && s->layer < nl - 1)) {
                 if (s->layer <= 0)
                     o_detector_deposit(&cfg->top, acc, s->p[0], s->p[1],
@@ -549,6 +604,7 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
             /* albedo rejection, mcml.template.c:696-708 */
             float deposit = s->w;
             s->w -= deposit;
+            ts.ev |= XO_EVENT_ABSORPTION;
             if (cfg->fluence.enabled)
                 o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, &cnt->deposits);
             done = 1;
@@ -556,6 +612,7 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
             if (method == XO_METHOD_AW) {
                 float deposit = s->w * L->mua_inv_mut;
                 s->w -= deposit;
+                ts.ev |= XO_EVENT_ABSORPTION;
                 if (cfg->fluence.enabled)
                     o_fluence_deposit(&cfg->fluence, acc, s->p, deposit,
                                       &cnt->deposits);
@@ -564,6 +621,7 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
             float ct = o_pf_sample(cfg->pf_kind, &L->pf, lut, &s->x, s->a,
                                    &xi_fi, mm);
             o_scatter_direction(&s->d[0], &s->d[1], &s->d[2], ct, xi_fi, mm);
+            ts.ev |= XO_EVENT_SCATTERING;
             if (method == XO_METHOD_AW &&
                 o_roulette(s, cfg->w_min, cfg->lottery_chance, cfg->use_lottery))
                 done = 1;
@@ -574,12 +632,16 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
             o_roulette(s, cfg->w_min, cfg->lottery_chance, cfg->use_lottery))
             done = 1;
         float ex = s->p[0] - sp[0], ey = s->p[1] - sp[1], ez = s->p[2] - sp[2];
-        if (ex * ex + ey * ey + ez * ez > rmax2) done = 1;
+        if (ex * ex + ey * ey + ez * ez > rmax2) {
+            done = 1;
+            ts.ev |= XO_EVENT_ESCAPED;
+        }
         /* guard against packets that can never end (XO_MAX_PACKET_STEPS) */
         if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
             cnt->truncated++;
             done = 1;
         }
+        if (tr) o_trace_step(tr, &ts, s, done);
         if (done) return;
     }
 }
@@ -592,11 +654,43 @@ static inline uint64_t o_lane_packets(uint64_t n, uint32_t lanes, uint32_t g) {
  * updated in place (the reference writes the state back,
  * mcml.template.c:821). counts_out: steps, boundary steps, crossings,
  * deposits. */
+static int mcml_run(const xo_mcml_cfg *cfg, const float *lut, uint32_t lanes,
+                    uint64_t *rng_x, const uint32_t *rng_a, uint64_t npackets,
+                    int math_mode, int nthreads, uint64_t *accum, uint64_t naccum,
+                    uint64_t *counts_out, const uint32_t *keyed_table,
+                    const o_trace *tr);
+
 int oracle_mcml_run(const xo_mcml_cfg *cfg, const float *lut,
                     uint32_t lanes, uint64_t *rng_x, const uint32_t *rng_a,
                     uint64_t npackets, int math_mode, int nthreads,
                     uint64_t *accum, uint64_t naccum, uint64_t *counts_out,
                     const uint32_t *keyed_table) {
+    return mcml_run(cfg, lut, lanes, rng_x, rng_a, npackets, math_mode, nthreads,
+                    accum, naccum, counts_out, keyed_table, NULL);
+}
+
+/* oracle_mcml_run with the photon packet trace: trace_data holds npackets *
+ * max_events * XO_TRACE_ENTRY_LEN floats (zeroed here), trace_n npackets
+ * event counts; row p is packet p of the static schedule. */
+int oracle_mcml_run_trace(const xo_mcml_cfg *cfg, const float *lut,
+                          uint32_t lanes, uint64_t *rng_x, const uint32_t *rng_a,
+                          uint64_t npackets, int math_mode, int nthreads,
+                          uint64_t *accum, uint64_t naccum, uint64_t *counts_out,
+                          const uint32_t *keyed_table, const xo_trace *trace,
+                          float *trace_data, int32_t *trace_n) {
+    if (!trace || trace->max_events < 1) return -1;
+    o_trace t = {*trace, trace_data, trace_n};
+    memset(trace_data, 0, npackets * (uint64_t)trace->max_events *
+                              XO_TRACE_ENTRY_LEN * sizeof(float));
+    return mcml_run(cfg, lut, lanes, rng_x, rng_a, npackets, math_mode, nthreads,
+                    accum, naccum, counts_out, keyed_table, &t);
+}
+
+static int mcml_run(const xo_mcml_cfg *cfg, const float *lut, uint32_t lanes,
+                    uint64_t *rng_x, const uint32_t *rng_a, uint64_t npackets,
+                    int math_mode, int nthreads, uint64_t *accum, uint64_t naccum,
+                    uint64_t *counts_out, const uint32_t *keyed_table,
+                    const o_trace *tr) {
     if (cfg->num_layers < 3 || cfg->num_layers > XO_MAX_LAYERS || lanes == 0)
         return -1;
     memset(accum, 0, naccum * sizeof(uint64_t));
@@ -614,7 +708,8 @@ int oracle_mcml_run(const xo_mcml_cfg *cfg, const float *lut,
                 xo_packet_state(cfg->packet_seed,
                                 cfg->first_packet + (uint64_t)g + k * lanes,
                                 keyed_table, &s.x, &s.a);
-            o_mcml_packet(cfg, lut, &s, accum, &c, math_mode);
+            o_mcml_packet(cfg, lut, &s, accum, &c, math_mode, tr,
+                          (uint64_t)g + k * lanes);
         }
         if (!keyed_table) rng_x[g] = s.x;
         st += c.steps;
@@ -1013,7 +1108,7 @@ int oracle_batch_run(const xo_batch_instance *inst, uint32_t ninst,
             o_pkt s;
             o_counts c = {0, 0, 0, 0, 0};
             xo_packet_state(inst[i].seed, (uint64_t)p, table, &s.x, &s.a);
-            o_mcml_packet(&cfgs[i], lut, &s, accum, &c, math_mode);
+            o_mcml_packet(&cfgs[i], lut, &s, accum, &c, math_mode, NULL, 0);
             st += c.steps;
             sb += c.boundary;
         }
@@ -1079,5 +1174,121 @@ int oracle_reflectance(float n1, float n2, uint32_t n, const float *cos1,
                        float *out) {
     float cc = o_cos_critical(n1, n2);
     for (uint32_t i = 0; i < n; ++i) out[i] = o_reflectance(n1, n2, cos1[i], cc);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* sampling volume                                                    */
+/* ------------------------------------------------------------------ */
+static inline uint32_t o_weight(float v) { return o_uint(v + 0.5f); }
+
+static inline float o_distance(const float a[3], const float b[3]) {
+    /* mc_distance_point3f, mcbase.template.c:914-916, 1061-1064 */
+    float x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+    return sqrtf(x * x + y * y + z * z);
+}
+
+static inline void o_sv_deposit(const xo_sampling_volume *sv, uint64_t *acc,
+                                const int32_t v[3], uint32_t w) {
+    if (v[0] >= 0 && v[1] >= 0 && v[2] >= 0 && (uint32_t)v[0] < sv->shape[0] &&
+        (uint32_t)v[1] < sv->shape[1] && (uint32_t)v[2] < sv->shape[2])
+        acc[sv->offset + ((uint64_t)v[2] * sv->shape[1] + (uint64_t)v[1]) *
+                             sv->shape[0] + (uint64_t)v[0]] += w;
+}
+
+/* The SamplingVolume kernel of mcsv.template.c:236-420, one packet after
+ * the other: walk the path between consecutive trace events through the
+ * voxel lattice (sva_intersect :95-115, sva_next_voxel :130-152) and give
+ * each voxel round(terminal weight * path in the voxel * multiplier * k);
+ * the terminal weight is field end_field of the last valid entry (the
+ * reference reads field 7, :278). Packets without events are skipped.
+ * counts_out: [walk steps, truncated walks]. */
+int oracle_sampling_volume(const xo_sampling_volume *sv, int32_t max_events,
+                           const float *data, const int32_t *n,
+                           uint64_t npackets, uint64_t *accum, uint64_t naccum,
+                           uint64_t *total_weight, uint64_t *counts_out) {
+    if (max_events < 1 || (sv->end_field != 6 && sv->end_field != 7)) return -1;
+    memset(accum, 0, naccum * sizeof(uint64_t));
+    const float kf = (float)sv->k;
+    uint64_t total = 0, walked = 0, truncated = 0;
+    for (uint64_t p = 0; p < npackets; ++p) {
+        int32_t nv = n[p] < max_events ? n[p] : max_events;
+        if (nv <= 0) continue;
+        const float *base = data + p * (uint64_t)max_events * XO_TRACE_ENTRY_LEN;
+        float end_w = base[(uint64_t)(nv - 1) * XO_TRACE_ENTRY_LEN + sv->end_field];
+        total += o_weight(end_w * kf);
+        float p1[3], d1[3], p2[3], d2[3];
+        const float *e = base;
+        for (int k = 0; k < 3; ++k) {
+            p1[k] = e[k];
+            d1[k] = e[3 + k];
+        }
+        e = base + (uint64_t)(nv > 1 ? 1 : 0) * XO_TRACE_ENTRY_LEN;
+        for (int k = 0; k < 3; ++k) {
+            p2[k] = e[k];
+            d2[k] = e[3 + k];
+        }
+        float d12 = o_distance(p1, p2);
+        int32_t v[3];
+        for (int k = 0; k < 3; ++k)
+            v[k] = o_int((p1[k] - sv->top_left[k]) / sv->voxel_size[k]);
+        float l_voxel = 0.0f;
+        int32_t idx = 0;
+        uint32_t steps = 0;
+        for (;;) {
+            float dist[3];
+            for (int k = 0; k < 3; ++k) {
+                float side = (float)((int64_t)(d1[k] >= 0.0f) + v[k]);
+                dist[k] = sv->top_left[k] + side * sv->voxel_size[k] - p1[k];
+                dist[k] = (d1[k] != 0.0f) ? dist[k] / d1[k] : XO_INF_F;
+            }
+            float d_voxel = fminf(dist[2], fminf(dist[0], dist[1]));
+            float d_move = fminf(d_voxel, d12);
+            l_voxel += d_move;
+            int last = 0;
+            if (d_voxel < d12) {
+                o_sv_deposit(sv, accum, v,
+                             o_weight(end_w * l_voxel * sv->multiplier * kf));
+                for (int k = 0; k < 3; ++k) p1[k] += d1[k] * d_move;
+                d12 -= d_move;
+                int32_t nx = (dist[0] <= dist[1]) && (dist[0] <= dist[2]);
+                int32_t ny = nx ? 0 : ((dist[1] <= dist[0]) && (dist[1] <= dist[2]));
+                int32_t nz = !(nx + ny);
+                v[0] += d1[0] < 0.0f ? -nx : nx;
+                v[1] += d1[1] < 0.0f ? -ny : ny;
+                v[2] += d1[2] < 0.0f ? -nz : nz;
+                l_voxel = 0.0f;
+            } else {
+                for (int k = 0; k < 3; ++k) {
+                    p1[k] = p2[k];
+                    d1[k] = d2[k];
+                }
+                ++idx;
+                if (idx < nv - 1) {
+                    e = base + (uint64_t)(idx + 1) * XO_TRACE_ENTRY_LEN;
+                    for (int k = 0; k < 3; ++k) {
+                        p2[k] = e[k];
+                        d2[k] = e[3 + k];
+                    }
+                    d12 = o_distance(p1, p2);
+                } else {
+                    o_sv_deposit(sv, accum, v,
+                                 o_weight(end_w * l_voxel * sv->multiplier * kf));
+                    last = 1;
+                }
+            }
+            ++walked;
+            if (last) break;
+            if (++steps >= XO_MAX_PACKET_STEPS) {
+                ++truncated;
+                break;
+            }
+        }
+    }
+    *total_weight = total;
+    if (counts_out) {
+        counts_out[0] = walked;
+        counts_out[1] = truncated;
+    }
     return 0;
 }

@@ -33,7 +33,11 @@ XO_DET_NONE, XO_DET_TOTAL, XO_DET_RADIAL = 0, 1, 2
 XO_SRC_LINE, XO_SRC_GAUSSIAN, XO_SRC_FIBER_NI = 0, 1, 2
 XO_METHOD_AW, XO_METHOD_AR, XO_METHOD_MBL = 0, 1, 2
 XO_FLUENCE_NONE, XO_FLUENCE_XYZ, XO_FLUENCE_RZ = 0, 1, 2
-XO_METHOD_AW = 0
+XO_TRACE_NONE, XO_TRACE_START, XO_TRACE_END, XO_TRACE_ALL = 0, 1, 2, 7
+XO_EVENT_REFLECTION, XO_EVENT_REFRACTION, XO_EVENT_BOUNDARY_HIT = 1, 2, 4
+XO_EVENT_LAUNCH, XO_EVENT_ABSORPTION, XO_EVENT_SCATTERING = 8, 16, 32
+XO_EVENT_TERMINATED, XO_EVENT_ESCAPED = 64, 128
+XO_TRACE_ENTRY_LEN = 8
 XO_FLAG_STATIC_SCHEDULE = 1 << 0
 XO_FLAG_NO_LDS_BINS = 1 << 1
 XO_FLAG_PACKET_KEYED = 1 << 2
@@ -160,6 +164,19 @@ class XoServeStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class XoTrace(ctypes.Structure):
+    _fields_ = [('max_events', c_int32), ('options', c_uint32),
+                ('event_mask', c_uint32), ('use_events', c_int32),
+                ('plon', c_int32), ('reserved', c_int32 * 3)]
+
+
+class XoSamplingVolume(ctypes.Structure):
+    _fields_ = [('top_left', c_float * 3), ('voxel_size', c_float * 3),
+                ('shape', c_uint32 * 3), ('multiplier', c_float),
+                ('offset', c_uint32), ('k', c_int32), ('end_field', c_int32),
+                ('reserved', c_int32 * 3)]
+
+
 class XoRunReport(ctypes.Structure):
     _fields_ = [('packets', c_uint64), ('steps', c_uint64),
                 ('boundary_steps', c_uint64), ('voxel_crossings', c_uint64),
@@ -176,7 +193,7 @@ STRUCTS_IN_ABI_ORDER = [XoPf, XoLayer, XoSourceLine, XoSourceGaussian,
                         XoDetector, XoFluence, XoMcmlCfg, XoMaterial,
                         XoVoxelCfg, XoMcvoxCfg, XoRunReport, XoBatchInstance,
                         XoBatchOpts, XoServeOpts, XoServeStats,
-                        XoSourceFiber]
+                        XoSourceFiber, XoTrace, XoSamplingVolume]
 
 # Every entry point of include/xomc.h: name -> (restype, argtypes)
 PROTOTYPES = {
@@ -202,6 +219,18 @@ PROTOTYPES = {
                             c_uint32, c_uint64, c_uint32, c_uint32,
                             POINTER(c_uint64), c_uint64,
                             POINTER(XoRunReport)]),
+    'xo_mcml_run_trace': (c_int, [c_void_p, POINTER(XoMcmlCfg),
+                                  POINTER(c_float), c_uint32, c_uint64,
+                                  c_uint32, c_uint32, POINTER(XoTrace),
+                                  POINTER(c_float), POINTER(c_int32),
+                                  POINTER(c_uint64), c_uint64,
+                                  POINTER(XoRunReport)]),
+    'xo_sampling_volume_run': (c_int, [c_void_p, POINTER(XoSamplingVolume),
+                                       c_int32, POINTER(c_float),
+                                       POINTER(c_int32), c_uint64,
+                                       POINTER(c_uint64), c_uint64,
+                                       POINTER(c_uint64),
+                                       POINTER(XoRunReport)]),
     'xo_batch_run': (c_int, [c_void_p, POINTER(XoBatchInstance), c_uint32,
                              POINTER(c_float), c_uint32, POINTER(XoBatchOpts),
                              POINTER(c_uint64), c_uint64,
@@ -274,8 +303,8 @@ def _load():
         fn = getattr(lib, name)
         fn.restype = restype
         fn.argtypes = argtypes
-    sizes = (c_uint32 * 16)()
-    count = lib.xo_struct_sizes(sizes, 16)
+    sizes = (c_uint32 * 32)()
+    count = lib.xo_struct_sizes(sizes, 32)
     for i, struct in enumerate(STRUCTS_IN_ABI_ORDER[:count]):
         if ctypes.sizeof(struct) != sizes[i]:
             raise RuntimeError(
@@ -446,6 +475,47 @@ class Context:
                                 _ptr(accum, c_uint64), c_uint64(naccum),
                                 ctypes.byref(rep)), 'xo_mcml_run')
         return accum, rep
+
+    def run_mcml_trace(self, cfg: XoMcmlCfg, lut, npackets: int, naccum: int,
+                       trace: XoTrace, lanes: int = 0, flags: int = 0):
+        """run_mcml with the packet trace: (accum, report, data, n) with
+        data float32[npackets][max_events][8] and n int32[npackets]."""
+        npackets = int(npackets)
+        accum = np.zeros(naccum, dtype=np.uint64)
+        data = np.zeros((npackets, trace.max_events, XO_TRACE_ENTRY_LEN),
+                        dtype=np.float32)
+        n = np.zeros(npackets, dtype=np.int32)
+        lut_p, nlut = None, 0
+        if lut is not None and len(lut):
+            lut = np.ascontiguousarray(lut, dtype=np.float32)
+            lut_p, nlut = _ptr(lut, c_float), lut.size
+        rep = XoRunReport()
+        check(lib().xo_mcml_run_trace(
+            self._h, ctypes.byref(cfg), lut_p, c_uint32(nlut),
+            c_uint64(npackets), c_uint32(lanes), c_uint32(flags),
+            ctypes.byref(trace), _ptr(data, c_float), _ptr(n, c_int32),
+            _ptr(accum, c_uint64), c_uint64(naccum), ctypes.byref(rep)),
+            'xo_mcml_run_trace')
+        return accum, rep, data, n
+
+    def sampling_volume(self, sv: XoSamplingVolume, data, n, naccum: int):
+        """(accum, total_weight, report) of xo_sampling_volume_run over a
+        trace (data float32[packets][max_events][8], n int32[packets])."""
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        if data.ndim != 3 or data.shape[2] != XO_TRACE_ENTRY_LEN or \
+                data.shape[0] != n.size:
+            raise ValueError('trace data must be [packets][max_events][8] '
+                             'with one count per packet')
+        accum = np.zeros(naccum, dtype=np.uint64)
+        total = c_uint64(0)
+        rep = XoRunReport()
+        check(lib().xo_sampling_volume_run(
+            self._h, ctypes.byref(sv), c_int32(data.shape[1]),
+            _ptr(data, c_float), _ptr(n, c_int32), c_uint64(n.size),
+            _ptr(accum, c_uint64), c_uint64(naccum), ctypes.byref(total),
+            ctypes.byref(rep)), 'xo_sampling_volume_run')
+        return accum, int(total.value), rep
 
     def run_batch(self, instances, lut, opts: XoBatchOpts, naccum: int):
         """instances: ctypes array of XoBatchInstance."""

@@ -32,6 +32,8 @@ static_assert(sizeof(xo_material) == 64, "xo_material layout");
 static_assert(sizeof(xo_voxel_cfg) == 48, "xo_voxel_cfg layout");
 static_assert(sizeof(xo_source_fiber) == 32, "xo_source_fiber layout");
 static_assert(sizeof(xo_batch_instance) == 920, "xo_batch_instance layout");
+static_assert(sizeof(xo_trace) == 32, "xo_trace layout");
+static_assert(sizeof(xo_sampling_volume) == 64, "xo_sampling_volume layout");
 static_assert(sizeof(xo::McvoxParams) <= 4096, "kernel argument too large");
 static_assert(sizeof(xo::McmlParams) <= 4096, "kernel argument too large");
 
@@ -210,7 +212,8 @@ int xo_struct_sizes(uint32_t *out, int n) {
         (uint32_t)sizeof(xo_voxel_cfg),   (uint32_t)sizeof(xo_mcvox_cfg),
         (uint32_t)sizeof(xo_run_report),  (uint32_t)sizeof(xo_batch_instance),
         (uint32_t)sizeof(xo_batch_opts),  (uint32_t)sizeof(xo_serve_opts),
-        (uint32_t)sizeof(xo_serve_stats), (uint32_t)sizeof(xo_source_fiber)};
+        (uint32_t)sizeof(xo_serve_stats), (uint32_t)sizeof(xo_source_fiber),
+        (uint32_t)sizeof(xo_trace),       (uint32_t)sizeof(xo_sampling_volume)};
     int count = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (n < count) count = n;
     for (int i = 0; i < count; ++i) out[i] = sizes[i];
@@ -288,6 +291,8 @@ void xo_ctx_destroy(xo_ctx *ctx) {
     if (ctx->batch_layers) (void)hipFree(ctx->batch_layers);
     if (ctx->batch_hdr) (void)hipFree(ctx->batch_hdr);
     if (ctx->batch_items) (void)hipFree(ctx->batch_items);
+    if (ctx->trace_data) (void)hipFree(ctx->trace_data);
+    if (ctx->trace_n) (void)hipFree(ctx->trace_n);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -358,14 +363,38 @@ static void fill_report(xo_run_report *r, const unsigned long long *stats, uint3
     r->total_ms = total_ms;
 }
 
-int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t nlut,
-                uint64_t npackets, uint32_t lanes, uint32_t flags, uint64_t *accum,
-                uint64_t naccum, xo_run_report *report) {
+/* xo_mcml_run and xo_mcml_run_trace; trace NULL: no trace. */
+static int mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t nlut,
+                    uint64_t npackets, uint32_t lanes, uint32_t flags, const xo_trace *trace,
+                    float *trace_data, int32_t *trace_n, uint64_t *accum, uint64_t naccum,
+                    xo_run_report *report) {
     auto t0 = std::chrono::steady_clock::now();
     xo::clear_error();
     if (!ctx || !cfg || !accum) {
         xo::set_error("xo_mcml_run: null argument");
         return -1;
+    }
+    uint64_t trace_floats = 0;
+    if (trace) {
+        if (!trace_data || !trace_n) {
+            xo::set_error("xo_mcml_run_trace: null trace buffer");
+            return -1;
+        }
+        if (trace->max_events < 1 ||
+            (trace->options != XO_TRACE_START && trace->options != XO_TRACE_END &&
+             trace->options != (XO_TRACE_START | XO_TRACE_END) &&
+             trace->options != XO_TRACE_ALL)) {
+            xo::set_error("xo_mcml_run_trace: max_events %d / options %u invalid",
+                          trace->max_events, trace->options);
+            return -1;
+        }
+        const uint64_t per = (uint64_t)trace->max_events * XO_TRACE_ENTRY_LEN;
+        if (npackets > (UINT64_MAX / 4u) / per) {
+            xo::set_error("xo_mcml_run_trace: trace of %llu packets too large",
+                          (unsigned long long)npackets);
+            return -1;
+        }
+        trace_floats = npackets * per;
     }
     if (cfg->num_layers < 3 || cfg->num_layers > XO_MAX_LAYERS) {
         xo::set_error("xo_mcml_run: %u layers (3..%d supported)", cfg->num_layers,
@@ -402,11 +431,21 @@ int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t 
     XO_HIP(hipSetDevice(ctx->device));
     if (ensure_buffer((void **)&ctx->accum, &ctx->accum_cap, naccum * 8)) return -1;
     if (nlut && ensure_buffer((void **)&ctx->lut, &ctx->lut_cap, (size_t)nlut * 4)) return -1;
+    if (trace && npackets) {
+        if (ensure_buffer((void **)&ctx->trace_data, &ctx->trace_data_cap, trace_floats * 4) ||
+            ensure_buffer((void **)&ctx->trace_n, &ctx->trace_n_cap, npackets * 4))
+            return -1;
+    }
 
     if ((flags & XO_FLAG_PACKET_KEYED) && ensure_table(ctx)) return -1;
     xo::McmlParams P;
     std::memset(&P, 0, sizeof(P));
     P.cfg = *cfg;
+    if (trace) {
+        P.trace = *trace;
+        P.trace_data = ctx->trace_data;
+        P.trace_n = ctx->trace_n;
+    }
     P.table = ctx->table;
     P.lut = nlut ? ctx->lut : nullptr;
     P.rng_x = ctx->rng_x;
@@ -431,11 +470,20 @@ int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t 
     XO_HIP(hipMemsetAsync(ctx->accum, 0, naccum * 8, ctx->stream));
     XO_HIP(hipMemsetAsync(ctx->counters, 0, sizeof(unsigned long long) * (1 + XO_STAT_COUNT),
                           ctx->stream));
+    /* entries past a packet's last event read as zeros */
+    if (trace && npackets)
+        XO_HIP(hipMemsetAsync(ctx->trace_data, 0, trace_floats * 4, ctx->stream));
     XO_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (npackets > 0) XO_HIP(xo::launch_mcml(P, blocks, ctx->stream));
     XO_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     unsigned long long stats[1 + XO_STAT_COUNT];
     XO_HIP(hipMemcpyAsync(accum, ctx->accum, naccum * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (trace && npackets) {
+        XO_HIP(hipMemcpyAsync(trace_data, ctx->trace_data, trace_floats * 4,
+                              hipMemcpyDeviceToHost, ctx->stream));
+        XO_HIP(hipMemcpyAsync(trace_n, ctx->trace_n, npackets * 4, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
     XO_HIP(hipMemcpyAsync(stats, ctx->counters, sizeof(stats), hipMemcpyDeviceToHost,
                           ctx->stream));
     XO_HIP(hipStreamSynchronize(ctx->stream));
@@ -443,6 +491,26 @@ int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t 
     XO_HIP(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
     fill_report(report, stats + 1, P.lanes, kms, ms_since(t0));
     return 0;
+}
+
+int xo_mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t nlut,
+                uint64_t npackets, uint32_t lanes, uint32_t flags, uint64_t *accum,
+                uint64_t naccum, xo_run_report *report) {
+    return mcml_run(ctx, cfg, lut, nlut, npackets, lanes, flags, nullptr, nullptr, nullptr,
+                    accum, naccum, report);
+}
+
+int xo_mcml_run_trace(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t nlut,
+                      uint64_t npackets, uint32_t lanes, uint32_t flags, const xo_trace *trace,
+                      float *trace_data, int32_t *trace_n, uint64_t *accum, uint64_t naccum,
+                      xo_run_report *report) {
+    if (!trace) {
+        xo::clear_error();
+        xo::set_error("xo_mcml_run_trace: null trace");
+        return -1;
+    }
+    return mcml_run(ctx, cfg, lut, nlut, npackets, lanes, flags, trace, trace_data, trace_n,
+                    accum, naccum, report);
 }
 
 int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,

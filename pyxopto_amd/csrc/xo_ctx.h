@@ -52,6 +52,11 @@ struct xo_ctx {
     size_t batch_hdr_cap = 0;
     void *batch_items = nullptr;
     size_t batch_items_cap = 0;
+    /* trace and sampling-volume buffers */
+    float *trace_data = nullptr;
+    size_t trace_data_cap = 0;
+    int32_t *trace_n = nullptr;
+    size_t trace_n_cap = 0;
     std::mutex mu;
 };
 

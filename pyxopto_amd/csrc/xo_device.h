@@ -128,6 +128,30 @@ XO_DEV void detector_deposit_n(const xo_detector &d, const Bins &bins, float px,
     }
 }
 
+/* Photon packet trace state of one lane: the packet's row, its logged
+ * event count, this step's event flags and the optical path length. */
+struct TraceState {
+    uint64_t pkt;
+    uint32_t count;
+    uint32_t ev;
+    float pl;
+};
+
+/* mcsim_trace_event (xopto/mcbase/mctrace.py:527-560): unless the event
+ * mask rejects this step's events, write the packet state to entry
+ * min(count, max_events - 1) of its row and count it. */
+XO_DEV void trace_event(const xo_trace &T, float *data, TraceState &ts, float px, float py,
+                        float pz, float dx, float dy, float dz, float w) {
+    if (T.use_events && !(T.event_mask & ts.ev)) return;
+    const uint32_t last = (uint32_t)T.max_events - 1u;
+    const uint32_t slot = ts.count < last ? ts.count : last;
+    float4 *dst = reinterpret_cast<float4 *>(
+        data + (ts.pkt * (uint64_t)T.max_events + slot) * XO_TRACE_ENTRY_LEN);
+    dst[0] = make_float4(px, py, pz, dx);
+    dst[1] = make_float4(dy, dz, w, T.plon ? ts.pl : 0.0f);
+    ts.count++;
+}
+
 /* Fluence deposit at a point (deposition mode): the Cartesian voxel of
  * mcsim_fluence_deposit_at (fluence.py:102-147) or the (r, z) cell of
  * fluencerz.py:112-160. Returns true if a non-zero integer was added. */

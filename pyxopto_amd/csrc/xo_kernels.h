@@ -34,6 +34,9 @@ namespace xo {
 
 struct McmlParams {
     xo_mcml_cfg cfg;
+    xo_trace trace;        /* max_events 0: no trace                      */
+    float *trace_data;     /* [packet][max_events][XO_TRACE_ENTRY_LEN]    */
+    int32_t *trace_n;      /* [packet] event counts                       */
     const uint32_t *table; /* seed-table multipliers (XO_FLAG_PACKET_KEYED) */
     const float *lut;
     uint64_t *rng_x;

@@ -38,7 +38,7 @@ XO_DEV void stats_flush(BlockStats *ss, const uint32_t *mine,
         atomicAdd(&gstats[threadIdx.x], ss->v[threadIdx.x]);
 }
 
-template <int PF, int METHOD>
+template <int PF, int METHOD, bool TRACE>
 __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ BlockStats sstats;
@@ -112,6 +112,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     int32_t layer = 1;
     uint32_t st[XO_STAT_COUNT] = {0, 0, 0, 0, 0, 0, 0};
     uint32_t pkt_steps = 0;
+    TraceState ts{0, 0, 0, 0.0f};
+    const xo_trace &T = P.trace;
 
     for (;;) {
         /* ---- refill lanes whose packet ended ---- */
@@ -176,6 +178,12 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             alive = true;
             pkt_steps = 0;
             st[XO_STAT_PACKETS]++;
+            if (TRACE) {
+                /* mcml.template.c:513-520, 791-810 */
+                ts = TraceState{pkt_index, 0u, XO_EVENT_LAUNCH, 0.0f};
+                if (T.options & XO_TRACE_START)
+                    trace_event(T, P.trace_data, ts, px, py, pz, dx, dy, dz, w);
+            }
         }
         if (!__any(alive)) break;
         if (!alive) continue;
@@ -200,6 +208,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         px = __builtin_fmaf(dx, step, px);
         py = __builtin_fmaf(dy, step, py);
         pz = __builtin_fmaf(dz, step, pz);
+        /* optical path length, mcml.template.c:570-574 */
+        if (TRACE && T.plon) ts.pl += L.n * step;
         if (layer < next) pz = L.bottom;
         if (layer > next) pz = L.top;
 
@@ -210,6 +220,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             const float fraction = 1.0f - xo_expf(-(mua * step));
             const float deposit = fraction * w;
             w -= deposit;
+            if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
             if (use_fluence) {
                 const float back =
                     (mua != 0.0f) ? step - (-xo_logf(1.0f - rng_next(x, a) * fraction)) / mua
@@ -221,7 +232,11 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         }
         if (next != layer) {
             st[XO_STAT_BOUNDARY]++;
+            const int32_t before = layer;
             mcml_boundary(sl, layer, next, dx, dy, dz, x, a);
+            if (TRACE)
+                ts.ev |= XO_EVENT_BOUNDARY_HIT |
+                         (layer != before ? XO_EVENT_REFRACTION : XO_EVENT_REFLECTION);
             if (!(layer > 0 && layer < (int32_t)nl - 1)) {
                 if (layer <= 0)
                     detector_deposit(P.cfg.top, bins, px, py, dx, dy, dz, w);
@@ -235,6 +250,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
              * mcml.template.c:696-708 */
             const float deposit = w;
             w -= deposit;
+            if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
             if (use_fluence && fluence_deposit(F, P.accum, px, py, pz, deposit))
                 st[XO_STAT_DEPOSITS]++;
             done = true;
@@ -242,6 +258,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             if (METHOD == XO_METHOD_AW) {
                 const float deposit = w * L.mua_inv_mut;
                 w -= deposit;
+                if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                 if (use_fluence && fluence_deposit(F, P.accum, px, py, pz, deposit))
                     st[XO_STAT_DEPOSITS]++;
             }
@@ -249,6 +266,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             float xi_fi;
             float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
             scatter_direction(dx, dy, dz, ct, xi_fi);
+            if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
             if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
                 done = true;
         }
@@ -258,10 +276,21 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
             done = true;
         float ex = px - spx, ey = py - spy, ez = pz - spz;
-        if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
+        if (ex * ex + ey * ey + ez * ez > rmax2) {
+            done = true;
+            if (TRACE) ts.ev |= XO_EVENT_ESCAPED;
+        }
         if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
             done = true;
             st[XO_STAT_TRUNCATED]++;
+        }
+        if (TRACE) {
+            /* mcml.template.c:765-781 */
+            if (done) ts.ev |= XO_EVENT_TERMINATED;
+            if (T.options == XO_TRACE_ALL || ((T.options & XO_TRACE_END) && done))
+                trace_event(T, P.trace_data, ts, px, py, pz, dx, dy, dz, w);
+            ts.ev = 0u;
+            if (done) P.trace_n[ts.pkt] = (int32_t)ts.count;
         }
         if (done) alive = false;
     }
@@ -699,26 +728,33 @@ size_t mcvox_lds_bytes(const McvoxParams &P) {
     return XO_MAX_MATERIALS * sizeof(MaterialL) + 8u * P.lds_bins + 4u * P.lds_lut;
 }
 
-template <int PF>
+template <int PF, bool TRACE>
 static hipError_t launch_mcml_pf(const McmlParams &P, uint32_t blocks, size_t lds,
                                  hipStream_t stream) {
     switch (P.cfg.method) {
     case XO_METHOD_AW:
-        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AW>), dim3(blocks), dim3(XO_BLOCK), lds,
-                           stream, P);
+        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AW, TRACE>), dim3(blocks),
+                           dim3(XO_BLOCK), lds, stream, P);
         break;
     case XO_METHOD_AR:
-        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AR>), dim3(blocks), dim3(XO_BLOCK), lds,
-                           stream, P);
+        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AR, TRACE>), dim3(blocks),
+                           dim3(XO_BLOCK), lds, stream, P);
         break;
     case XO_METHOD_MBL:
-        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_MBL>), dim3(blocks), dim3(XO_BLOCK), lds,
-                           stream, P);
+        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_MBL, TRACE>), dim3(blocks),
+                           dim3(XO_BLOCK), lds, stream, P);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int PF>
+static hipError_t launch_mcml_pf(const McmlParams &P, uint32_t blocks, size_t lds,
+                                 hipStream_t stream) {
+    return P.trace.max_events > 0 ? launch_mcml_pf<PF, true>(P, blocks, lds, stream)
+                                  : launch_mcml_pf<PF, false>(P, blocks, lds, stream);
 }
 
 hipError_t launch_mcml(const McmlParams &P, uint32_t blocks, hipStream_t stream) {
@@ -777,13 +813,13 @@ hipError_t occupancy_mcml(int pf_kind, size_t lds, int *blocks_per_cu) {
     switch (pf_kind) {
     case XO_PF_MHG:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcml_kernel<XO_PF_MHG, XO_METHOD_AW>, XO_BLOCK, lds);
+            blocks_per_cu, mcml_kernel<XO_PF_MHG, XO_METHOD_AW, false>, XO_BLOCK, lds);
     case XO_PF_LUT:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcml_kernel<XO_PF_LUT, XO_METHOD_AW>, XO_BLOCK, lds);
+            blocks_per_cu, mcml_kernel<XO_PF_LUT, XO_METHOD_AW, false>, XO_BLOCK, lds);
     default:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcml_kernel<XO_PF_HG, XO_METHOD_AW>, XO_BLOCK, lds);
+            blocks_per_cu, mcml_kernel<XO_PF_HG, XO_METHOD_AW, false>, XO_BLOCK, lds);
     }
 }
 

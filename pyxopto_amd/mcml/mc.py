@@ -14,20 +14,21 @@ import numpy as np
 
 from .. import _native as N
 from ..mcbase import mcdetector, mcoptions, mcpf
-from ..mcbase import mcfluence
+from ..mcbase import mcfluence, mcsv, mctrace
 from ..mcbase.mcutil import axis as _axis
 from ..mcbase.mcutil import fiber as fiberutil
 from ..mcbase.mcworker import McWorker
 from . import mclayer, mcsource
 
 mcfluence.Axis = _axis.Axis
+mcsv.Axis = _axis.Axis
 
 # Module layout of xopto.mcml.mc: the plugin modules are reachable from here.
 mcdetector.Axis = _axis.Axis
 mcdetector.RadialAxis = _axis.RadialAxis
 
 __all__ = ['Mc', 'mclayer', 'mcsource', 'mcdetector', 'mcpf', 'mcoptions',
-           'mcfluence', 'fiberutil']
+           'mcfluence', 'mctrace', 'mcsv', 'fiberutil']
 
 
 class Mc(McWorker):
@@ -35,9 +36,12 @@ class Mc(McWorker):
                  trace=None, fluence=None, surface=None, types=None,
                  options=None, rnginit=None, cl_devices=None,
                  cl_build_options=None, cl_profiling: bool = False):
-        if trace is not None or surface is not None:
+        if surface is not None:
             raise NotImplementedError(
-                'Trace and surface layouts are not part of this engine.')
+                'Surface layouts are not part of this engine.')
+        if trace is not None and not isinstance(trace, mctrace.Trace):
+            raise TypeError('trace must be a Trace object')
+        self._trace = trace
         if fluence is not None and not isinstance(
                 fluence, (mcfluence.Fluence, mcfluence.FluenceRz)):
             raise TypeError('fluence must be a Fluence or FluenceRz object')
@@ -75,6 +79,8 @@ class Mc(McWorker):
     source = property(lambda self: self._source, None, None, 'Source.')
     fluence = property(lambda self: self._fluence, None, None,
                        'Fluence/deposition accumulator or None.')
+    trace = property(lambda self: self._trace, None, None,
+                     'Trace object if available.')
     detectors = property(lambda self: self._detectors, None, None,
                          'Detectors.')
     options = property(lambda self: list(self._options), None, None,
@@ -138,7 +144,16 @@ class Mc(McWorker):
             flags |= N.XO_FLAG_PACKET_KEYED
         ctx = self._context()
         lanes = 0 if maxthreads is None else int(maxthreads)
-        accum, rep = ctx.run_mcml(cfg, lut, nphotons, naccum, lanes, flags)
+        trace_res = out[0]
+        if self._trace is not None:
+            accum, rep, tdata, tn = ctx.run_mcml_trace(
+                cfg, lut, nphotons, naccum, self._trace.pack(), lanes, flags)
+            if trace_res is None:
+                trace_res = mctrace.Trace(self._trace)
+            trace_res.update_data(self, tdata, tn, nphotons)
+        else:
+            accum, rep = ctx.run_mcml(cfg, lut, nphotons, naccum, lanes,
+                                      flags)
         self._record_report(rep, nphotons)
         self._last_accum = accum
         detectors_res = out[2]
@@ -161,7 +176,28 @@ class Mc(McWorker):
                 fluence_res = type(self._fluence)(self._fluence)
             off, n = self.allocation(self._fluence)
             fluence_res.update_data(self, [accum[off:off + n]], nphotons)
-        return None, fluence_res, detectors_res
+        return trace_res, fluence_res, detectors_res
+
+    # ---------------- sampling volume (xopto/mcml/mc.py:1040-1200) ------
+    def sampling_volume(self, trace: mctrace.Trace, sv: mcsv.SamplingVolume,
+                        wgsize: int = None, maxthreads: int = None,
+                        exportsrc=None, verbose: bool = False):
+        """Add the sampling volume of `trace` to `sv` and return `sv`
+        (one xo_sampling_volume_run launch)."""
+        if self._trace is None:
+            raise RuntimeError(
+                'This Monte Carlo simulator was build without the Trace '
+                'functionality (the trace argument of Mc.__init__ was None)! '
+                'Sampling volume analysis requires trace functionality!')
+        data, n = trace.raw()
+        naccum = int(np.prod(sv.shape))
+        accum, total, rep = self._context().sampling_volume(
+            sv.pack(0), data, n, naccum)
+        sv.update_data(self, [accum], total_weight=total)
+        if verbose:
+            print('SamplingVolume processed {:,d} packet traces: {:.3f} '
+                  'ms'.format(len(n), rep.kernel_ms))
+        return sv
 
     def __repr__(self):
         return 'Mc({}, {}, {})'.format(self._layers, self._source,

@@ -28,8 +28,8 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes_match_library():
-    sizes = (ctypes.c_uint32 * 16)()
-    count = N.lib().xo_struct_sizes(sizes, 16)
+    sizes = (ctypes.c_uint32 * 64)()
+    count = N.lib().xo_struct_sizes(sizes, 64)
     assert count == len(N.STRUCTS_IN_ABI_ORDER)
     for i, s in enumerate(N.STRUCTS_IN_ABI_ORDER):
         assert ctypes.sizeof(s) == sizes[i], s.__name__
