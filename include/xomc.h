@@ -459,6 +459,16 @@ int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg,
                  uint64_t npackets, uint32_t lanes, uint32_t flags,
                  uint64_t *accum, uint64_t naccum, xo_run_report *report);
 
+/* xo_mcvox_run with the photon packet trace (the MC_USE_TRACE build of the
+ * voxel McKernel, xopto/mcvox/kernel/mcvox.template.c:750-756, 986-1034);
+ * trace buffers as for xo_mcml_run_trace. */
+int xo_mcvox_run_trace(xo_ctx *ctx, const xo_mcvox_cfg *cfg,
+                       const int32_t *voxels, uint64_t nvoxels,
+                       const float *lut, uint32_t nlut,
+                       uint64_t npackets, uint32_t lanes, uint32_t flags,
+                       const xo_trace *trace, float *trace_data, int32_t *trace_n,
+                       uint64_t *accum, uint64_t naccum, xo_run_report *report);
+
 /* Run a batch of layered-media instances in one persistent launch: a device
  * work queue of (instance, chunk of packets) items; lanes take packets of
  * whichever item their wave holds and the wave takes the next item as soon

@@ -45,6 +45,13 @@ def lib():
             POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int, c_int,
             POINTER(c_uint64), c_uint64, POINTER(c_uint64), POINTER(c_uint32),
             POINTER(N.XoTrace), POINTER(c_float), POINTER(c_int32)]
+        l.oracle_mcvox_run_trace.restype = c_int
+        l.oracle_mcvox_run_trace.argtypes = [
+            POINTER(N.XoMcvoxCfg), POINTER(c_int32), POINTER(c_float),
+            c_uint32, POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int,
+            c_int, POINTER(c_uint64), c_uint64, POINTER(c_uint64),
+            POINTER(c_uint32), POINTER(N.XoTrace), POINTER(c_float),
+            POINTER(c_int32)]
         l.oracle_sampling_volume.restype = c_int
         l.oracle_sampling_volume.argtypes = [
             POINTER(N.XoSamplingVolume), c_int32, POINTER(c_float),
@@ -250,6 +257,36 @@ def mcvox_run(cfg, voxels, lut, rng_x, rng_a, npackets: int, naccum: int,
     if rc != 0:
         raise RuntimeError('oracle_mcvox_run failed ({})'.format(rc))
     return accum, counts, x
+
+
+def mcvox_run_trace(cfg, voxels, lut, rng_x, rng_a, npackets: int,
+                    naccum: int, trace, math_mode: int = MATH_LIBM,
+                    nthreads: int = 0, keyed: bool = False):
+    """mcvox_run with the packet trace; returns (accum, counts,
+    rng_x_after, data, n) as mcml_run_trace."""
+    npackets = int(npackets)
+    x = np.ascontiguousarray(rng_x, dtype=np.uint64).copy()
+    a = np.ascontiguousarray(rng_a, dtype=np.uint32)
+    vox = np.ascontiguousarray(voxels, dtype=np.int32)
+    accum = np.zeros(naccum, dtype=np.uint64)
+    counts = np.zeros(4, dtype=np.uint64)
+    data = np.zeros((npackets, trace.max_events, N.XO_TRACE_ENTRY_LEN),
+                    dtype=np.float32)
+    n = np.zeros(npackets, dtype=np.int32)
+    lut_p = None
+    if lut is not None and len(lut):
+        lut = np.ascontiguousarray(lut, dtype=np.float32)
+        lut_p = _p(lut, c_float)
+    table = _keyed_table(keyed)
+    rc = lib().oracle_mcvox_run_trace(
+        ctypes.byref(cfg), _p(vox, c_int32), lut_p, c_uint32(x.size),
+        _p(x, c_uint64), _p(a, c_uint32), c_uint64(npackets), c_int(math_mode),
+        c_int(nthreads), _p(accum, c_uint64), c_uint64(naccum),
+        _p(counts, c_uint64), None if table is None else _p(table, c_uint32),
+        ctypes.byref(trace), _p(data, c_float), _p(n, c_int32))
+    if rc != 0:
+        raise RuntimeError('oracle_mcvox_run_trace failed ({})'.format(rc))
+    return accum, counts, x, data, n
 
 
 def batch_run(instances, lut, naccum: int, w_min=1e-4, lottery_chance=0.1,

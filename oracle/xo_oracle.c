@@ -928,7 +928,8 @@ static void o_mcvox_boundary(const o_vox *V, o_pkt *s, const float dist[3],
  * three stepping methods (microscopic Beer-Lambert :763-881, albedo
  * rejection :925-944, albedo weight :945-977). */
 static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
-                           o_counts *cnt, int mm) {
+                           o_counts *cnt, int mm, const o_trace *tr,
+                           uint64_t pkt) {
     const xo_mcvox_cfg *cfg = V->cfg;
     const xo_voxel_cfg *vc = &cfg->voxels;
     const float rmax2 = cfg->rmax * cfg->rmax;
@@ -936,6 +937,8 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
                           ? cfg->line.position
                           : cfg->gaussian.position;
     o_mcvox_launch(V, s, acc, mm);
+    o_tstate ts = {pkt, 0u, XO_EVENT_LAUNCH, 0.0f};
+    if (tr && (tr->cfg.options & XO_TRACE_START)) o_trace_event(tr, &ts, s);
     uint32_t pkt_steps = 0;
     for (;;) {
         const xo_material *M = &cfg->materials[s->mi];
@@ -954,6 +957,8 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
         s->p[0] = fmaf(d_ok, s->d[0], s->p[0]);
         s->p[1] = fmaf(d_ok, s->d[1], s->p[1]);
         s->p[2] = fmaf(d_ok, s->d[2], s->p[2]);
+        /* optical path length, mcvox.template.c:792-798 */
+        if (tr && tr->cfg.plon) ts.pl += M->n * d_ok;
         if (method == XO_METHOD_MBL) {
             /* absorption along the part of the step inside this voxel,
              * mcvox.template.c:800-818 */
@@ -961,6 +966,7 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
             float fraction = 1.0f - o_exp(-(mua * d_ok), mm);
             float deposit = fraction * s->w;
             s->w -= deposit;
+            ts.ev |= XO_EVENT_ABSORPTION;
             if (cfg->fluence.enabled) {
                 float back = (mua != 0.0f)
                                  ? d_ok - (-o_log(1.0f - o_rand(&s->x, s->a) * fraction,
@@ -973,7 +979,11 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
         }
         if (d < step) {
             cnt->boundary++;
+            const uint64_t crossed = cnt->crossings;
             o_mcvox_boundary(V, s, dist, &cnt->crossings);
+            ts.ev |= XO_EVENT_BOUNDARY_HIT | (cnt->crossings != crossed
+                                                  ? XO_EVENT_REFRACTION
+                                                  : XO_EVENT_REFLECTION);
             if (!o_valid(vc, s->vi)) {
                 if (s->vi[2] < 0)
                     o_detector_deposit(&cfg->top, acc, s->p[0], s->p[1],
@@ -988,6 +998,7 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
             /* albedo rejection, mcvox.template.c:925-944 */
             float deposit = s->w;
             s->w -= deposit;
+            ts.ev |= XO_EVENT_ABSORPTION;
             if (cfg->fluence.enabled)
                 o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, &cnt->deposits);
             done = 1;
@@ -995,6 +1006,7 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
             if (method == XO_METHOD_AW) {
                 float deposit = s->w * M->mua_inv_mut;
                 s->w -= deposit;
+                ts.ev |= XO_EVENT_ABSORPTION;
                 if (cfg->fluence.enabled)
                     o_fluence_deposit(&cfg->fluence, acc, s->p, deposit,
                                       &cnt->deposits);
@@ -1003,6 +1015,7 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
             float ct = o_pf_sample(cfg->pf_kind, &M->pf, V->lut, &s->x, s->a,
                                    &xi_fi, mm);
             o_scatter_direction(&s->d[0], &s->d[1], &s->d[2], ct, xi_fi, mm);
+            ts.ev |= XO_EVENT_SCATTERING;
             if (method == XO_METHOD_AW &&
                 o_roulette(s, cfg->w_min, cfg->lottery_chance, cfg->use_lottery))
                 done = 1;
@@ -1013,21 +1026,59 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
             o_roulette(s, cfg->w_min, cfg->lottery_chance, cfg->use_lottery))
             done = 1;
         float ex = s->p[0] - sp[0], ey = s->p[1] - sp[1], ez = s->p[2] - sp[2];
-        if (ex * ex + ey * ey + ez * ez > rmax2 || s->w <= 0.0f) done = 1;
+        if (ex * ex + ey * ey + ez * ez > rmax2 || s->w <= 0.0f) {
+            done = 1;
+            ts.ev |= XO_EVENT_ESCAPED;
+        }
         /* guard against packets that can never end (XO_MAX_PACKET_STEPS) */
         if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
             cnt->truncated++;
             done = 1;
         }
+        if (tr) o_trace_step(tr, &ts, s, done);
         if (done) return;
     }
 }
+
+static int mcvox_run(const xo_mcvox_cfg *cfg, const int32_t *voxels,
+                     const float *lut, uint32_t lanes, uint64_t *rng_x,
+                     const uint32_t *rng_a, uint64_t npackets, int math_mode,
+                     int nthreads, uint64_t *accum, uint64_t naccum,
+                     uint64_t *counts_out, const uint32_t *keyed_table,
+                     const o_trace *tr);
 
 int oracle_mcvox_run(const xo_mcvox_cfg *cfg, const int32_t *voxels,
                      const float *lut, uint32_t lanes, uint64_t *rng_x,
                      const uint32_t *rng_a, uint64_t npackets, int math_mode,
                      int nthreads, uint64_t *accum, uint64_t naccum,
                      uint64_t *counts_out, const uint32_t *keyed_table) {
+    return mcvox_run(cfg, voxels, lut, lanes, rng_x, rng_a, npackets, math_mode,
+                     nthreads, accum, naccum, counts_out, keyed_table, NULL);
+}
+
+/* oracle_mcvox_run with the photon packet trace (buffers as for
+ * oracle_mcml_run_trace). */
+int oracle_mcvox_run_trace(const xo_mcvox_cfg *cfg, const int32_t *voxels,
+                           const float *lut, uint32_t lanes, uint64_t *rng_x,
+                           const uint32_t *rng_a, uint64_t npackets,
+                           int math_mode, int nthreads, uint64_t *accum,
+                           uint64_t naccum, uint64_t *counts_out,
+                           const uint32_t *keyed_table, const xo_trace *trace,
+                           float *trace_data, int32_t *trace_n) {
+    if (!trace || trace->max_events < 1) return -1;
+    o_trace t = {*trace, trace_data, trace_n};
+    memset(trace_data, 0, npackets * (uint64_t)trace->max_events *
+                              XO_TRACE_ENTRY_LEN * sizeof(float));
+    return mcvox_run(cfg, voxels, lut, lanes, rng_x, rng_a, npackets, math_mode,
+                     nthreads, accum, naccum, counts_out, keyed_table, &t);
+}
+
+static int mcvox_run(const xo_mcvox_cfg *cfg, const int32_t *voxels,
+                     const float *lut, uint32_t lanes, uint64_t *rng_x,
+                     const uint32_t *rng_a, uint64_t npackets, int math_mode,
+                     int nthreads, uint64_t *accum, uint64_t naccum,
+                     uint64_t *counts_out, const uint32_t *keyed_table,
+                     const o_trace *tr) {
     if (cfg->num_materials < 1 || cfg->num_materials > XO_MAX_MATERIALS ||
         lanes == 0)
         return -1;
@@ -1047,7 +1098,8 @@ int oracle_mcvox_run(const xo_mcvox_cfg *cfg, const int32_t *voxels,
                 xo_packet_state(cfg->packet_seed,
                                 cfg->first_packet + (uint64_t)g + k * lanes,
                                 keyed_table, &s.x, &s.a);
-            o_mcvox_packet(&V, &s, accum, &c, math_mode);
+            o_mcvox_packet(&V, &s, accum, &c, math_mode, tr,
+                           (uint64_t)g + k * lanes);
         }
         if (!keyed_table) rng_x[g] = s.x;
         st += c.steps;

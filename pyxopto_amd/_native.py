@@ -225,6 +225,13 @@ PROTOTYPES = {
                                   POINTER(c_float), POINTER(c_int32),
                                   POINTER(c_uint64), c_uint64,
                                   POINTER(XoRunReport)]),
+    'xo_mcvox_run_trace': (c_int, [c_void_p, POINTER(XoMcvoxCfg),
+                                   POINTER(c_int32), c_uint64,
+                                   POINTER(c_float), c_uint32, c_uint64,
+                                   c_uint32, c_uint32, POINTER(XoTrace),
+                                   POINTER(c_float), POINTER(c_int32),
+                                   POINTER(c_uint64), c_uint64,
+                                   POINTER(XoRunReport)]),
     'xo_sampling_volume_run': (c_int, [c_void_p, POINTER(XoSamplingVolume),
                                        c_int32, POINTER(c_float),
                                        POINTER(c_int32), c_uint64,
@@ -558,7 +565,9 @@ class Context:
         return ms.value, int(total.value)
 
     def run_mcvox(self, cfg: XoMcvoxCfg, voxels, lut, npackets: int,
-                  naccum: int, lanes: int = 0, flags: int = 0):
+                  naccum: int, lanes: int = 0, flags: int = 0, trace=None):
+        """(accum, report), or (accum, report, data, n) with a trace
+        (XoTrace): data float32[npackets][max_events][8], n int32."""
         accum = np.zeros(naccum, dtype=np.uint64)
         vox_p, nvox = None, 0
         if voxels is not None:
@@ -569,6 +578,18 @@ class Context:
             lut = np.ascontiguousarray(lut, dtype=np.float32)
             lut_p, nlut = _ptr(lut, c_float), lut.size
         rep = XoRunReport()
+        if trace is not None:
+            npackets = int(npackets)
+            data = np.zeros((npackets, trace.max_events, XO_TRACE_ENTRY_LEN),
+                            dtype=np.float32)
+            n = np.zeros(npackets, dtype=np.int32)
+            check(lib().xo_mcvox_run_trace(
+                self._h, ctypes.byref(cfg), vox_p, c_uint64(nvox), lut_p,
+                c_uint32(nlut), c_uint64(npackets), c_uint32(lanes),
+                c_uint32(flags), ctypes.byref(trace), _ptr(data, c_float),
+                _ptr(n, c_int32), _ptr(accum, c_uint64), c_uint64(naccum),
+                ctypes.byref(rep)), 'xo_mcvox_run_trace')
+            return accum, rep, data, n
         check(lib().xo_mcvox_run(self._h, ctypes.byref(cfg), vox_p,
                                  c_uint64(nvox), lut_p, c_uint32(nlut),
                                  c_uint64(int(npackets)), c_uint32(lanes),

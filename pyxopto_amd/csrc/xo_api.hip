@@ -363,6 +363,51 @@ static void fill_report(xo_run_report *r, const unsigned long long *stats, uint3
     r->total_ms = total_ms;
 }
 
+/* Trace arguments of xo_mcml_run_trace / xo_mcvox_run_trace; *floats: the
+ * size of the trace data. */
+static int check_trace(const char *fn, const xo_trace *trace, const float *data,
+                       const int32_t *n, uint64_t npackets, uint64_t *floats) {
+    if (!data || !n) {
+        xo::set_error("%s: null trace buffer", fn);
+        return -1;
+    }
+    if (trace->max_events < 1 ||
+        (trace->options != XO_TRACE_START && trace->options != XO_TRACE_END &&
+         trace->options != (XO_TRACE_START | XO_TRACE_END) &&
+         trace->options != XO_TRACE_ALL)) {
+        xo::set_error("%s: max_events %d / options %u invalid", fn, trace->max_events,
+                      trace->options);
+        return -1;
+    }
+    const uint64_t per = (uint64_t)trace->max_events * XO_TRACE_ENTRY_LEN;
+    if (npackets > (UINT64_MAX / 4u) / per) {
+        xo::set_error("%s: trace of %llu packets too large", fn, (unsigned long long)npackets);
+        return -1;
+    }
+    *floats = npackets * per;
+    return 0;
+}
+
+/* Device trace buffers of one run, zeroed on the stream (entries past a
+ * packet's last event read as zeros). */
+static int trace_buffers(xo_ctx *ctx, uint64_t floats, uint64_t npackets) {
+    if (!npackets) return 0;
+    if (ensure_buffer((void **)&ctx->trace_data, &ctx->trace_data_cap, floats * 4) ||
+        ensure_buffer((void **)&ctx->trace_n, &ctx->trace_n_cap, npackets * 4))
+        return -1;
+    XO_HIP(hipMemsetAsync(ctx->trace_data, 0, floats * 4, ctx->stream));
+    return 0;
+}
+
+static int trace_download(xo_ctx *ctx, float *data, int32_t *n, uint64_t floats,
+                          uint64_t npackets) {
+    if (!npackets) return 0;
+    XO_HIP(hipMemcpyAsync(data, ctx->trace_data, floats * 4, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    XO_HIP(hipMemcpyAsync(n, ctx->trace_n, npackets * 4, hipMemcpyDeviceToHost, ctx->stream));
+    return 0;
+}
+
 /* xo_mcml_run and xo_mcml_run_trace; trace NULL: no trace. */
 static int mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint32_t nlut,
                     uint64_t npackets, uint32_t lanes, uint32_t flags, const xo_trace *trace,
@@ -375,27 +420,9 @@ static int mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint3
         return -1;
     }
     uint64_t trace_floats = 0;
-    if (trace) {
-        if (!trace_data || !trace_n) {
-            xo::set_error("xo_mcml_run_trace: null trace buffer");
-            return -1;
-        }
-        if (trace->max_events < 1 ||
-            (trace->options != XO_TRACE_START && trace->options != XO_TRACE_END &&
-             trace->options != (XO_TRACE_START | XO_TRACE_END) &&
-             trace->options != XO_TRACE_ALL)) {
-            xo::set_error("xo_mcml_run_trace: max_events %d / options %u invalid",
-                          trace->max_events, trace->options);
-            return -1;
-        }
-        const uint64_t per = (uint64_t)trace->max_events * XO_TRACE_ENTRY_LEN;
-        if (npackets > (UINT64_MAX / 4u) / per) {
-            xo::set_error("xo_mcml_run_trace: trace of %llu packets too large",
-                          (unsigned long long)npackets);
-            return -1;
-        }
-        trace_floats = npackets * per;
-    }
+    if (trace && check_trace("xo_mcml_run_trace", trace, trace_data, trace_n, npackets,
+                             &trace_floats))
+        return -1;
     if (cfg->num_layers < 3 || cfg->num_layers > XO_MAX_LAYERS) {
         xo::set_error("xo_mcml_run: %u layers (3..%d supported)", cfg->num_layers,
                       XO_MAX_LAYERS);
@@ -431,11 +458,7 @@ static int mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint3
     XO_HIP(hipSetDevice(ctx->device));
     if (ensure_buffer((void **)&ctx->accum, &ctx->accum_cap, naccum * 8)) return -1;
     if (nlut && ensure_buffer((void **)&ctx->lut, &ctx->lut_cap, (size_t)nlut * 4)) return -1;
-    if (trace && npackets) {
-        if (ensure_buffer((void **)&ctx->trace_data, &ctx->trace_data_cap, trace_floats * 4) ||
-            ensure_buffer((void **)&ctx->trace_n, &ctx->trace_n_cap, npackets * 4))
-            return -1;
-    }
+    if (trace && trace_buffers(ctx, trace_floats, npackets)) return -1;
 
     if ((flags & XO_FLAG_PACKET_KEYED) && ensure_table(ctx)) return -1;
     xo::McmlParams P;
@@ -470,20 +493,12 @@ static int mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint3
     XO_HIP(hipMemsetAsync(ctx->accum, 0, naccum * 8, ctx->stream));
     XO_HIP(hipMemsetAsync(ctx->counters, 0, sizeof(unsigned long long) * (1 + XO_STAT_COUNT),
                           ctx->stream));
-    /* entries past a packet's last event read as zeros */
-    if (trace && npackets)
-        XO_HIP(hipMemsetAsync(ctx->trace_data, 0, trace_floats * 4, ctx->stream));
     XO_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (npackets > 0) XO_HIP(xo::launch_mcml(P, blocks, ctx->stream));
     XO_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     unsigned long long stats[1 + XO_STAT_COUNT];
     XO_HIP(hipMemcpyAsync(accum, ctx->accum, naccum * 8, hipMemcpyDeviceToHost, ctx->stream));
-    if (trace && npackets) {
-        XO_HIP(hipMemcpyAsync(trace_data, ctx->trace_data, trace_floats * 4,
-                              hipMemcpyDeviceToHost, ctx->stream));
-        XO_HIP(hipMemcpyAsync(trace_n, ctx->trace_n, npackets * 4, hipMemcpyDeviceToHost,
-                              ctx->stream));
-    }
+    if (trace && trace_download(ctx, trace_data, trace_n, trace_floats, npackets)) return -1;
     XO_HIP(hipMemcpyAsync(stats, ctx->counters, sizeof(stats), hipMemcpyDeviceToHost,
                           ctx->stream));
     XO_HIP(hipStreamSynchronize(ctx->stream));
@@ -513,16 +528,22 @@ int xo_mcml_run_trace(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uin
                     accum, naccum, report);
 }
 
-int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
-                 uint64_t nvoxels, const float *lut, uint32_t nlut, uint64_t npackets,
-                 uint32_t lanes, uint32_t flags, uint64_t *accum, uint64_t naccum,
-                 xo_run_report *report) {
+/* xo_mcvox_run and xo_mcvox_run_trace; trace NULL: no trace. */
+static int mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
+                     uint64_t nvoxels, const float *lut, uint32_t nlut, uint64_t npackets,
+                     uint32_t lanes, uint32_t flags, const xo_trace *trace, float *trace_data,
+                     int32_t *trace_n, uint64_t *accum, uint64_t naccum,
+                     xo_run_report *report) {
     auto t0 = std::chrono::steady_clock::now();
     xo::clear_error();
     if (!ctx || !cfg || !accum) {
         xo::set_error("xo_mcvox_run: null argument");
         return -1;
     }
+    uint64_t trace_floats = 0;
+    if (trace && check_trace("xo_mcvox_run_trace", trace, trace_data, trace_n, npackets,
+                             &trace_floats))
+        return -1;
     if (cfg->num_materials < 1 || cfg->num_materials > XO_MAX_MATERIALS) {
         xo::set_error("xo_mcvox_run: %u materials (1..%d supported)", cfg->num_materials,
                       XO_MAX_MATERIALS);
@@ -596,10 +617,17 @@ int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
     if (ensure_buffer((void **)&ctx->accum, &ctx->accum_cap, naccum * 8)) return -1;
     if (nlut && ensure_buffer((void **)&ctx->lut, &ctx->lut_cap, (size_t)nlut * 4)) return -1;
 
+    if (trace && trace_buffers(ctx, trace_floats, npackets)) return -1;
+
     if ((flags & XO_FLAG_PACKET_KEYED) && ensure_table(ctx)) return -1;
     xo::McvoxParams P;
     std::memset(&P, 0, sizeof(P));
     P.cfg = *cfg;
+    if (trace) {
+        P.trace = *trace;
+        P.trace_data = ctx->trace_data;
+        P.trace_n = ctx->trace_n;
+    }
     P.table = ctx->table;
     P.voxels = ctx->voxels;
     P.lut = nlut ? ctx->lut : nullptr;
@@ -630,6 +658,7 @@ int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
     XO_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     unsigned long long stats[1 + XO_STAT_COUNT];
     XO_HIP(hipMemcpyAsync(accum, ctx->accum, naccum * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (trace && trace_download(ctx, trace_data, trace_n, trace_floats, npackets)) return -1;
     XO_HIP(hipMemcpyAsync(stats, ctx->counters, sizeof(stats), hipMemcpyDeviceToHost,
                           ctx->stream));
     XO_HIP(hipStreamSynchronize(ctx->stream));
@@ -637,6 +666,28 @@ int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
     XO_HIP(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
     fill_report(report, stats + 1, P.lanes, kms, ms_since(t0));
     return 0;
+}
+
+int xo_mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
+                 uint64_t nvoxels, const float *lut, uint32_t nlut, uint64_t npackets,
+                 uint32_t lanes, uint32_t flags, uint64_t *accum, uint64_t naccum,
+                 xo_run_report *report) {
+    return mcvox_run(ctx, cfg, voxels, nvoxels, lut, nlut, npackets, lanes, flags, nullptr,
+                     nullptr, nullptr, accum, naccum, report);
+}
+
+int xo_mcvox_run_trace(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels,
+                       uint64_t nvoxels, const float *lut, uint32_t nlut, uint64_t npackets,
+                       uint32_t lanes, uint32_t flags, const xo_trace *trace, float *trace_data,
+                       int32_t *trace_n, uint64_t *accum, uint64_t naccum,
+                       xo_run_report *report) {
+    if (!trace) {
+        xo::clear_error();
+        xo::set_error("xo_mcvox_run_trace: null trace");
+        return -1;
+    }
+    return mcvox_run(ctx, cfg, voxels, nvoxels, lut, nlut, npackets, lanes, flags, trace,
+                     trace_data, trace_n, accum, naccum, report);
 }
 
 int xo_packet_seed(uint64_t seed, uint64_t p, uint64_t *x, uint32_t *a) {

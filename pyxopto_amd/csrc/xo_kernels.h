@@ -53,6 +53,9 @@ struct McmlParams {
 
 struct McvoxParams {
     xo_mcvox_cfg cfg;
+    xo_trace trace;        /* max_events 0: no trace                      */
+    float *trace_data;     /* [packet][max_events][XO_TRACE_ENTRY_LEN]    */
+    int32_t *trace_n;      /* [packet] event counts                       */
     const uint32_t *table; /* seed-table multipliers (XO_FLAG_PACKET_KEYED) */
     const uint8_t *voxels; /* material index per voxel, [z][y][x] */
     const float *lut;

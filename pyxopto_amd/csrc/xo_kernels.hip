@@ -365,7 +365,7 @@ XO_DEV bool box_intersect(const VoxGeom &g, const float pos[3], const float dir[
     return tmax >= tmin && tmax >= 0.0f;
 }
 
-template <int PF, int METHOD>
+template <int PF, int METHOD, bool TRACE>
 __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ BlockStats sstats;
@@ -444,6 +444,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     uint32_t mi = 0;
     uint32_t st[XO_STAT_COUNT] = {0, 0, 0, 0, 0, 0, 0};
     uint32_t pkt_steps = 0;
+    TraceState ts{0, 0, 0, 0.0f};
+    const xo_trace &T = P.trace;
 
     for (;;) {
         bool want = active && !alive;
@@ -562,6 +564,12 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
             alive = true;
             pkt_steps = 0;
             st[XO_STAT_PACKETS]++;
+            if (TRACE) {
+                /* mcvox.template.c:750-756, 1015-1034 */
+                ts = TraceState{pkt_index, 0u, XO_EVENT_LAUNCH, 0.0f};
+                if (T.options & XO_TRACE_START)
+                    trace_event(T, P.trace_data, ts, p[0], p[1], p[2], d[0], d[1], d[2], w);
+            }
         }
         if (!__any(alive)) break;
         if (!alive) continue;
@@ -587,6 +595,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         p[0] = __builtin_fmaf(d_ok, d[0], p[0]);
         p[1] = __builtin_fmaf(d_ok, d[1], p[1]);
         p[2] = __builtin_fmaf(d_ok, d[2], p[2]);
+        /* optical path length, mcvox.template.c:792-798 */
+        if (TRACE && T.plon) ts.pl += M.n * d_ok;
 
         bool done = false;
         if (METHOD == XO_METHOD_MBL) {
@@ -596,6 +606,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
             const float fraction = 1.0f - xo_expf(-(mua * d_ok));
             const float deposit = fraction * w;
             w -= deposit;
+            if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
             if (use_fluence) {
                 const float back =
                     (mua != 0.0f) ? d_ok - (-xo_logf(1.0f - rng_next(x, a) * fraction)) / mua
@@ -655,6 +666,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                     }
                 }
             }
+            if (TRACE)
+                ts.ev |= XO_EVENT_BOUNDARY_HIT |
+                         (crossed ? XO_EVENT_REFRACTION : XO_EVENT_REFLECTION);
             if (crossed) {
                 vi[0] = nv0;
                 vi[1] = nv1;
@@ -674,6 +688,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
              * mcvox.template.c:925-944 */
             const float deposit = w;
             w -= deposit;
+            if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
             if (use_fluence && fluence_deposit(F, P.accum, p[0], p[1], p[2], deposit))
                 st[XO_STAT_DEPOSITS]++;
             done = true;
@@ -681,6 +696,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
             if (METHOD == XO_METHOD_AW) {
                 float deposit = w * M.mua_inv_mut;
                 w -= deposit;
+                if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                 /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
                 if (use_fluence && fluence_deposit(F, P.accum, p[0], p[1], p[2], deposit))
                     st[XO_STAT_DEPOSITS]++;
@@ -689,6 +705,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
             float xi_fi;
             float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
             scatter_direction(d[0], d[1], d[2], ct, xi_fi);
+            if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
             if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
                 done = true;
         }
@@ -698,10 +715,21 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
             done = true;
         float ex = p[0] - spx, ey = p[1] - spy, ez = p[2] - spz;
-        if (ex * ex + ey * ey + ez * ez > rmax2 || w <= 0.0f) done = true;
+        if (ex * ex + ey * ey + ez * ez > rmax2 || w <= 0.0f) {
+            done = true;
+            if (TRACE) ts.ev |= XO_EVENT_ESCAPED;
+        }
         if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
             done = true;
             st[XO_STAT_TRUNCATED]++;
+        }
+        if (TRACE) {
+            /* mcvox.template.c:989-1005 */
+            if (done) ts.ev |= XO_EVENT_TERMINATED;
+            if (T.options == XO_TRACE_ALL || ((T.options & XO_TRACE_END) && done))
+                trace_event(T, P.trace_data, ts, p[0], p[1], p[2], d[0], d[1], d[2], w);
+            ts.ev = 0u;
+            if (done) P.trace_n[ts.pkt] = (int32_t)ts.count;
         }
         if (done) alive = false;
     }
@@ -771,26 +799,33 @@ hipError_t launch_mcml(const McmlParams &P, uint32_t blocks, hipStream_t stream)
     }
 }
 
-template <int PF>
+template <int PF, bool TRACE>
 static hipError_t launch_mcvox_pf(const McvoxParams &P, uint32_t blocks, size_t lds,
                                   hipStream_t stream) {
     switch (P.cfg.method) {
     case XO_METHOD_AW:
-        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AW>), dim3(blocks), dim3(XO_BLOCK), lds,
-                           stream, P);
+        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AW, TRACE>), dim3(blocks),
+                           dim3(XO_BLOCK), lds, stream, P);
         break;
     case XO_METHOD_AR:
-        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AR>), dim3(blocks), dim3(XO_BLOCK), lds,
-                           stream, P);
+        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AR, TRACE>), dim3(blocks),
+                           dim3(XO_BLOCK), lds, stream, P);
         break;
     case XO_METHOD_MBL:
-        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_MBL>), dim3(blocks), dim3(XO_BLOCK), lds,
-                           stream, P);
+        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_MBL, TRACE>), dim3(blocks),
+                           dim3(XO_BLOCK), lds, stream, P);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int PF>
+static hipError_t launch_mcvox_pf(const McvoxParams &P, uint32_t blocks, size_t lds,
+                                  hipStream_t stream) {
+    return P.trace.max_events > 0 ? launch_mcvox_pf<PF, true>(P, blocks, lds, stream)
+                                  : launch_mcvox_pf<PF, false>(P, blocks, lds, stream);
 }
 
 hipError_t launch_mcvox(const McvoxParams &P, uint32_t blocks, hipStream_t stream) {
@@ -828,13 +863,13 @@ hipError_t occupancy_mcvox(int pf_kind, size_t lds, int *blocks_per_cu) {
     switch (pf_kind) {
     case XO_PF_MHG:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcvox_kernel<XO_PF_MHG, XO_METHOD_AW>, XO_BLOCK, lds);
+            blocks_per_cu, mcvox_kernel<XO_PF_MHG, XO_METHOD_AW, false>, XO_BLOCK, lds);
     case XO_PF_LUT:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcvox_kernel<XO_PF_LUT, XO_METHOD_AW>, XO_BLOCK, lds);
+            blocks_per_cu, mcvox_kernel<XO_PF_LUT, XO_METHOD_AW, false>, XO_BLOCK, lds);
     default:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcvox_kernel<XO_PF_HG, XO_METHOD_AW>, XO_BLOCK, lds);
+            blocks_per_cu, mcvox_kernel<XO_PF_HG, XO_METHOD_AW, false>, XO_BLOCK, lds);
     }
 }
 

@@ -6,6 +6,7 @@ import numpy as np
 
 from .. import _native as N
 from ..mcbase import mcdetector, mcfluence, mcmaterial, mcoptions, mcpf
+from ..mcbase import mcsv, mctrace
 from ..mcbase.mcutil import axis as _axis
 from ..mcbase.mcworker import McWorker
 from . import mcgeometry, mcsource
@@ -14,9 +15,10 @@ mcdetector.Axis = _axis.Axis
 mcdetector.RadialAxis = _axis.RadialAxis
 mcfluence.Axis = _axis.Axis
 mcgeometry.Axis = _axis.Axis
+mcsv.Axis = _axis.Axis
 
 __all__ = ['Mc', 'mcgeometry', 'mcsource', 'mcdetector', 'mcpf', 'mcoptions',
-           'mcfluence', 'mcmaterial']
+           'mcfluence', 'mcmaterial', 'mctrace', 'mcsv']
 
 
 class Mc(McWorker):
@@ -25,9 +27,12 @@ class Mc(McWorker):
                  fluence: mcfluence.Fluence = None, surface=None, types=None,
                  options=None, rnginit=None, cl_devices=None,
                  cl_build_options=None, cl_profiling: bool = False):
-        if trace is not None or surface is not None:
+        if surface is not None:
             raise NotImplementedError(
-                'Trace and surface layouts are not part of this engine.')
+                'Surface layouts are not part of this engine.')
+        if trace is not None and not isinstance(trace, mctrace.Trace):
+            raise TypeError('trace must be a Trace object')
+        self._trace = trace
         if fluence is not None and not isinstance(
                 fluence, (mcfluence.Fluence, mcfluence.FluenceRz)):
             raise TypeError('fluence must be a Fluence or FluenceRz object')
@@ -64,6 +69,8 @@ class Mc(McWorker):
     detectors = property(lambda self: self._detectors, None, None,
                          'Detectors.')
     fluence = property(lambda self: self._fluence, None, None, 'Fluence.')
+    trace = property(lambda self: self._trace, None, None,
+                     'Trace object if available.')
 
     def material(self, index: int) -> mcmaterial.Material:
         return self._materials[index]
@@ -124,15 +131,40 @@ class Mc(McWorker):
         voxels = self._voxels.material if upload else None
         ctx = self._context()
         lanes = 0 if maxthreads is None else int(maxthreads)
-        accum, rep = ctx.run_mcvox(cfg, voxels, lut, nphotons, naccum, lanes,
-                                   flags)
+        trace_res = out[0]
+        if self._trace is not None:
+            accum, rep, tdata, tn = ctx.run_mcvox(
+                cfg, voxels, lut, nphotons, naccum, lanes, flags,
+                trace=self._trace.pack())
+            if trace_res is None:
+                trace_res = mctrace.Trace(self._trace)
+            trace_res.update_data(self, tdata, tn, nphotons)
+        else:
+            accum, rep = ctx.run_mcvox(cfg, voxels, lut, nphotons, naccum,
+                                       lanes, flags)
         self._uploaded_shape = self._voxels.shape
         self._record_report(rep, nphotons)
         _, fluence_res, detectors_res = self.unpack(accum, nphotons, out)
         if verbose:
             print('McKernel processed {:,d} packets: {:.3f} ms'.format(
                 nphotons, rep.kernel_ms))
-        return None, fluence_res, detectors_res
+        return trace_res, fluence_res, detectors_res
+
+    def sampling_volume(self, trace, sv, wgsize: int = None,
+                        maxthreads: int = None, exportsrc=None,
+                        verbose: bool = False):
+        """Add the sampling volume of `trace` to `sv` and return `sv`
+        (xopto/mcvox/mc.py:1089-1240)."""
+        if self._trace is None:
+            raise RuntimeError(
+                'This Monte Carlo simulator was build without the Trace '
+                'functionality (the trace argument of Mc.__init__ was None)! '
+                'Sampling volume analysis requires trace functionality!')
+        data, n = trace.raw()
+        accum, total, rep = self._context().sampling_volume(
+            sv.pack(0), data, n, int(np.prod(sv.shape)))
+        sv.update_data(self, [accum], total_weight=total)
+        return sv
 
     def unpack(self, accum, nphotons: int, out=None):
         """(None, fluence, detectors) from the integer accumulator array

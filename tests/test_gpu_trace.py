@@ -150,3 +150,51 @@ def test_mc_trace_filter_and_sampling_volume():
     # the sampling volume of reflected packets sits under the source
     zprof = sv.data.sum(axis=(1, 2))
     assert zprof[0] > 0 and zprof[0] > zprof[-1]
+
+
+VOX_CASES = {
+    'vox_aw_all': (0, _xt(1200)),
+    'vox_mbl_all': (2, _xt(1200)),
+    'vox_ar_end': (1, _xt(1, N.XO_TRACE_END)),
+    'vox_aw_refraction_mask': (0, _xt(300, mask=N.XO_EVENT_REFRACTION)),
+}
+
+
+@pytest.mark.parametrize('name', list(VOX_CASES))
+def test_voxel_trace_static_schedule_bit_exact(name):
+    from test_gpu_mcvox import skin_vessel_sim
+    print('case', name, flush=True)
+    method, trace = VOX_CASES[name]
+    sim = skin_vessel_sim(n=20, method=method)
+    cfg, lut, naccum = sim.pack()
+    vox = sim.voxels.material
+    lanes, npk = 4096, 12000
+    ctx = N.Context(0, 0, XINIT)
+    acc, rep, data, n = ctx.run_mcvox(cfg, vox, lut, npk, naccum, lanes,
+                                      N.XO_FLAG_STATIC_SCHEDULE, trace=trace)
+    x, a = N.init_rng(lanes, XINIT)
+    racc, counts, rx, rdata, rn = O.mcvox_run_trace(
+        cfg, vox, lut, x, a, npk, naccum, trace, O.MATH_KERNEL)
+    assert int(rep.steps) == int(counts[0])
+    np.testing.assert_array_equal(n, rn)
+    np.testing.assert_array_equal(data, rdata)
+    np.testing.assert_array_equal(acc, racc)
+    ctx.close()
+
+
+def test_voxel_mc_trace_and_sampling_volume():
+    from test_gpu_mcvox import skin_vessel_sim
+    from pyxopto_amd.mcvox import mc as mcv
+    base = skin_vessel_sim(n=20)
+    sim = mcv.Mc(base.voxels, base.materials, base.source, base.detectors,
+                 trace=mcv.mctrace.Trace(500), rnginit=XINIT)
+    sim.voxels.material[...] = base.voxels.material
+    trace, _, det = sim.run(5000)
+    assert len(trace) == 5000
+    A = mcv.mcsv.Axis
+    sv = mcv.mcsv.SamplingVolume(A(-4e-4, 4e-4, 20), A(-4e-4, 4e-4, 20),
+                                 A(0.0, 8e-4, 20), weight_field='w')
+    sv = sim.sampling_volume(trace, sv)
+    assert sv.data.sum() > 0
+    assert sv.weight == pytest.approx(
+        float(np.sum(trace.terminal['w'], dtype=np.float64)), rel=1e-5)

@@ -309,3 +309,70 @@ def test_sampling_volume_of_a_simulated_trace_conserves_path():
     got = acc.sum() / (k * m)
     assert got == pytest.approx(expect, rel=2e-3)
     assert counts[1] == 0
+
+
+# ---------------------------------------------------------------- voxels
+def _vox_sim(method=0):
+    from pyxopto_amd.mcvox import mc as mcv
+    M, G = mcv.mcmaterial, mcv.mcgeometry
+    mats = M.Materials([
+        M.Material(n=1.0, mua=0.0, mus=0.0, pf=mcv.mcpf.Hg(0.8)),
+        M.Material(n=1.37, mua=300.0, mus=2e4, pf=mcv.mcpf.Hg(0.8)),
+        M.Material(n=1.37, mua=2000.0, mus=1e4, pf=mcv.mcpf.Hg(0.8))])
+    vox = G.Voxels(G.Axis(-1e-3, 1e-3, 8), G.Axis(-1e-3, 1e-3, 8),
+                   G.Axis(0.0, 1e-3, 8))
+    D = mcv.mcdetector
+    det = D.Detectors(top=D.Total(), bottom=D.Total(), specular=D.Total())
+    sim = mcv.Mc(vox, mats, mcv.mcsource.Line(), det,
+                 options=[mcv.mcoptions.McMethod(method)], rnginit=XINIT)
+    sim.voxels.material[:] = 1
+    sim.voxels.material[4:, 2:5, 2:5] = 2
+    return sim
+
+
+def _vox_run(sim, trace, npk=500, lanes=64):
+    cfg, lut, naccum = sim.pack()
+    x, a = N.init_rng(lanes, XINIT)
+    return O.mcvox_run_trace(cfg, sim.voxels.material, lut, x, a, npk, naccum,
+                             trace, O.MATH_LIBM)
+
+
+@pytest.mark.parametrize('method', [0, 1, 2])
+def test_voxel_trace_does_not_perturb_and_follows_the_path(method):
+    sim = _vox_sim(method)
+    cfg, lut, naccum = sim.pack()
+    x, a = N.init_rng(64, XINIT)
+    acc0, c0, x0 = O.mcvox_run(cfg, sim.voxels.material, lut, x, a, 500,
+                               naccum, O.MATH_LIBM)
+    acc, c, x1, data, n = _vox_run(sim, _xt())
+    np.testing.assert_array_equal(acc0, acc)
+    np.testing.assert_array_equal(c0, c)
+    np.testing.assert_array_equal(x0, x1)
+    assert int(n.astype(np.int64).sum()) == int(c[0]) + 500
+    for p in range(0, 500, 5):
+        e = data[p, :n[p]].astype(np.float64)
+        seg = e[1:, 0:3] - e[:-1, 0:3]
+        length = np.linalg.norm(seg, axis=1)
+        cross = np.linalg.norm(np.cross(seg, e[:-1, 3:6]), axis=1)
+        assert np.all(cross <= 1e-5 * np.maximum(length, 1e-9) + 1e-9)
+        # every material inside has n = 1.37, outside 1.0
+        zmid = 0.5 * (e[1:, 2] + e[:-1, 2])
+        nn = np.where((zmid > 0.0) & (zmid < 1e-3), 1.37, 1.0)
+        np.testing.assert_allclose(np.diff(e[:, 7]), nn * length, rtol=2e-4,
+                                   atol=1e-9)
+
+
+def test_voxel_event_masks():
+    sim = _vox_sim(0)
+    _, c, _, _, n = _vox_run(sim, _xt())
+    steps, boundary, crossings = int(c[0]), int(c[1]), int(c[2])
+    _, _, _, _, nb = _vox_run(sim, _xt(2000, mask=N.XO_EVENT_BOUNDARY_HIT))
+    _, _, _, _, nr = _vox_run(sim, _xt(2000, mask=N.XO_EVENT_REFRACTION))
+    _, _, _, _, nl = _vox_run(sim, _xt(2000, mask=N.XO_EVENT_REFLECTION))
+    _, _, _, _, ns = _vox_run(sim, _xt(2000, mask=N.XO_EVENT_SCATTERING))
+    assert int(nb.sum()) == boundary
+    # a crossed voxel face is a refraction (same material included), a
+    # face that turned the packet back a reflection
+    assert int(nr.sum()) == crossings
+    assert int(nl.sum()) == boundary - crossings
+    assert int(ns.sum()) == steps - boundary
