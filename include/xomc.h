@@ -65,7 +65,8 @@ enum xo_det_kind {
 enum xo_src_kind {
     XO_SRC_LINE = 0,     /* xopto/mcml/mcsource/line.py:95-110, mcvox/mcsource/line.py:84-104 */
     XO_SRC_GAUSSIAN = 1, /* xopto/mcvox/mcsource/gaussianbeam.py:112-189 */
-    XO_SRC_FIBER_NI = 2  /* xopto/mcml/mcsource/fiberni.py:237-290 (layered media) */
+    XO_SRC_FIBER_NI = 2, /* xopto/mcml/mcsource/fiberni.py:237-290 (layered media) */
+    XO_SRC_FIBER = 3     /* xopto/mcml/mcsource/fiber.py:252-330 (layered media)   */
 };
 
 /* Stepping method (xopto/mcbase/mcoptions.py:269-318; the branches of
@@ -209,6 +210,15 @@ typedef struct xo_fluence {
     int32_t k;
 } xo_fluence; /* 48 B */
 
+/* Axis of a fiber source that meets the surface at an angle (UniformFiber,
+ * xopto/mcml/mcsource/fiber.py:204-227): the rotation of the fiber frame
+ * (z along the fiber) onto the sample frame, row major, and the fiber
+ * direction (z > 0). Used with the core parameters of xo_source_fiber. */
+typedef struct xo_fiber_axis {
+    float transformation[9];
+    float direction[3];
+} xo_fiber_axis; /* 48 B */
+
 /* Everything one layered-media run needs besides the LUT array. Compile-time
  * options of the reference (McMinimumPacketWeight, McPacketLotteryChance,
  * McUseLottery: xopto/mcbase/mcoptions.py:600-670) are runtime fields here. */
@@ -220,8 +230,9 @@ typedef struct xo_mcml_cfg {
     float w_min;                /* roulette threshold (1e-4)                  */
     float lottery_chance;       /* roulette survival probability (0.1)        */
     int32_t use_lottery;
-    int32_t source_kind;        /* XO_SRC_LINE (source) or XO_SRC_FIBER_NI
-                                   (fiber)                                    */
+    int32_t source_kind;        /* XO_SRC_LINE (source), XO_SRC_FIBER_NI
+                                   (fiber) or XO_SRC_FIBER (fiber and
+                                   fiber_axis)                                */
     xo_source_line source;
     xo_detector top, bottom, specular;
     xo_layer layers[XO_MAX_LAYERS];
@@ -229,6 +240,7 @@ typedef struct xo_mcml_cfg {
     uint64_t first_packet;      /* XO_FLAG_PACKET_KEYED only                  */
     xo_source_fiber fiber;
     xo_fluence fluence;         /* bins after every detector bin              */
+    xo_fiber_axis fiber_axis;   /* XO_SRC_FIBER only                          */
 } xo_mcml_cfg;
 
 /* Photon packet trace of one run; McTrace of xopto/mcbase/mctrace.py:480-510
@@ -363,7 +375,7 @@ int xo_device_count(void);
  * xo_source_gaussian, xo_detector, xo_fluence, xo_mcml_cfg, xo_material,
  * xo_voxel_cfg, xo_mcvox_cfg, xo_run_report, xo_batch_instance,
  * xo_batch_opts, xo_serve_opts, xo_serve_stats, xo_source_fiber, xo_trace,
- * xo_sampling_volume. Returns the count written. */
+ * xo_sampling_volume, xo_fiber_axis. Returns the count written. */
 int xo_struct_sizes(uint32_t *out, int n);
 
 /* ---------------- seeds (host) ---------------- */

@@ -476,6 +476,44 @@ static void o_fiber_launch(const xo_mcml_cfg *cfg, o_pkt *s, uint64_t *acc,
     s->layer = 1;
 }
 
+/* UniformFiber launch, xopto/mcml/mcsource/fiber.py:265-330: the core
+ * point rotated by the fiber axis and projected along it onto z = 0, the
+ * direction rotated likewise. The reference tests the projected point's z
+ * (0) against the critical cosine before refracting (:305-309), so the
+ * direction is never refracted; the specular part leaves along it. */
+static void o_fiber_axis_launch(const xo_mcml_cfg *cfg, o_pkt *s,
+                                uint64_t *acc, int mm) {
+    const xo_source_fiber *src = &cfg->fiber;
+    const float *T = cfg->fiber_axis.transformation;
+    const float *ax = cfg->fiber_axis.direction;
+    const float n1 = cfg->layers[1].n;
+    float sin_fi, cos_fi;
+    float r = sqrtf(o_rand(&s->x, s->a)) * src->radius;
+    o_sincos_azimuth(o_rand(&s->x, s->a), &sin_fi, &cos_fi, mm);
+    float v[3] = {r * cos_fi, r * sin_fi, 0.0f}, m[3];
+    for (int i = 0; i < 3; ++i)
+        m[i] = T[3 * i] * v[0] + T[3 * i + 1] * v[1] + T[3 * i + 2] * v[2];
+    float k = (0.0f - m[2]) / ax[2];
+    m[0] += k * ax[0];
+    m[1] += k * ax[1];
+    s->p[0] = src->position[0] + m[0];
+    s->p[1] = src->position[1] + m[1];
+    s->p[2] = 0.0f;
+    o_sincos_azimuth(o_rand(&s->x, s->a), &sin_fi, &cos_fi, mm);
+    float cos_theta = 1.0f - o_rand(&s->x, s->a) * (1.0f - src->cos_min);
+    float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+    sin_theta = sin_theta / src->n;
+    cos_theta = sqrtf(1.0f - sin_theta * sin_theta);
+    float q[3] = {cos_fi * sin_theta, sin_fi * sin_theta, cos_theta};
+    for (int i = 0; i < 3; ++i)
+        s->d[i] = T[3 * i] * q[0] + T[3 * i + 1] * q[1] + T[3 * i + 2] * q[2];
+    float R = o_reflectance(src->n, n1, s->d[2], o_cos_critical(src->n, n1));
+    s->w = 1.0f - R;
+    o_detector_deposit(&cfg->specular, acc, s->p[0], s->p[1], s->d[0], s->d[1],
+                       s->d[2], R, mm);
+    s->layer = 1;
+}
+
 /* mcsim_boundary, mcml.template.c:80-203 */
 static void o_mcml_boundary(const xo_mcml_cfg *cfg, o_pkt *s, int32_t next) {
     const xo_layer *cur = &cfg->layers[s->layer];
@@ -528,11 +566,13 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
                           const o_trace *tr, uint64_t pkt) {
     const int32_t nl = (int32_t)cfg->num_layers;
     const float rmax2 = cfg->rmax * cfg->rmax;
-    const float *sp = (cfg->source_kind == XO_SRC_FIBER_NI)
-                          ? cfg->fiber.position
-                          : cfg->source.position;
+    const float *sp = (cfg->source_kind == XO_SRC_LINE)
+                          ? cfg->source.position
+                          : cfg->fiber.position;
     if (cfg->source_kind == XO_SRC_FIBER_NI)
         o_fiber_launch(cfg, s, acc, mm);
+    else if (cfg->source_kind == XO_SRC_FIBER)
+        o_fiber_axis_launch(cfg, s, acc, mm);
     else
         o_mcml_launch(cfg, s, acc, mm);
     o_tstate ts = {pkt, 0u, XO_EVENT_LAUNCH, 0.0f};

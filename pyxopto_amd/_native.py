@@ -30,7 +30,7 @@ XO_BATCH_MAX_LAYERS = 8
 
 XO_PF_HG, XO_PF_MHG, XO_PF_LUT = 0, 1, 2
 XO_DET_NONE, XO_DET_TOTAL, XO_DET_RADIAL = 0, 1, 2
-XO_SRC_LINE, XO_SRC_GAUSSIAN, XO_SRC_FIBER_NI = 0, 1, 2
+XO_SRC_LINE, XO_SRC_GAUSSIAN, XO_SRC_FIBER_NI, XO_SRC_FIBER = 0, 1, 2, 3
 XO_METHOD_AW, XO_METHOD_AR, XO_METHOD_MBL = 0, 1, 2
 XO_FLUENCE_NONE, XO_FLUENCE_XYZ, XO_FLUENCE_RZ = 0, 1, 2
 XO_TRACE_NONE, XO_TRACE_START, XO_TRACE_END, XO_TRACE_ALL = 0, 1, 2, 7
@@ -89,6 +89,10 @@ class XoSourceFiber(ctypes.Structure):
                 ('reserved', c_float * 2)]
 
 
+class XoFiberAxis(ctypes.Structure):
+    _fields_ = [('transformation', c_float * 9), ('direction', c_float * 3)]
+
+
 class XoMcmlCfg(ctypes.Structure):
     _fields_ = [('num_layers', c_uint32), ('pf_kind', c_int32),
                 ('method', c_int32), ('rmax', c_float), ('w_min', c_float),
@@ -98,7 +102,8 @@ class XoMcmlCfg(ctypes.Structure):
                 ('specular', XoDetector),
                 ('layers', XoLayer * XO_MAX_LAYERS),
                 ('packet_seed', c_uint64), ('first_packet', c_uint64),
-                ('fiber', XoSourceFiber), ('fluence', XoFluence)]
+                ('fiber', XoSourceFiber), ('fluence', XoFluence),
+                ('fiber_axis', XoFiberAxis)]
 
 
 class XoMaterial(ctypes.Structure):
@@ -193,7 +198,8 @@ STRUCTS_IN_ABI_ORDER = [XoPf, XoLayer, XoSourceLine, XoSourceGaussian,
                         XoDetector, XoFluence, XoMcmlCfg, XoMaterial,
                         XoVoxelCfg, XoMcvoxCfg, XoRunReport, XoBatchInstance,
                         XoBatchOpts, XoServeOpts, XoServeStats,
-                        XoSourceFiber, XoTrace, XoSamplingVolume]
+                        XoSourceFiber, XoTrace, XoSamplingVolume,
+                        XoFiberAxis]
 
 # Every entry point of include/xomc.h: name -> (restype, argtypes)
 PROTOTYPES = {

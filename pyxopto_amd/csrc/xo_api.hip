@@ -33,6 +33,7 @@ static_assert(sizeof(xo_voxel_cfg) == 48, "xo_voxel_cfg layout");
 static_assert(sizeof(xo_source_fiber) == 32, "xo_source_fiber layout");
 static_assert(sizeof(xo_batch_instance) == 920, "xo_batch_instance layout");
 static_assert(sizeof(xo_trace) == 32, "xo_trace layout");
+static_assert(sizeof(xo_fiber_axis) == 48, "xo_fiber_axis layout");
 static_assert(sizeof(xo_sampling_volume) == 64, "xo_sampling_volume layout");
 static_assert(sizeof(xo::McvoxParams) <= 4096, "kernel argument too large");
 static_assert(sizeof(xo::McmlParams) <= 4096, "kernel argument too large");
@@ -213,7 +214,8 @@ int xo_struct_sizes(uint32_t *out, int n) {
         (uint32_t)sizeof(xo_run_report),  (uint32_t)sizeof(xo_batch_instance),
         (uint32_t)sizeof(xo_batch_opts),  (uint32_t)sizeof(xo_serve_opts),
         (uint32_t)sizeof(xo_serve_stats), (uint32_t)sizeof(xo_source_fiber),
-        (uint32_t)sizeof(xo_trace),       (uint32_t)sizeof(xo_sampling_volume)};
+        (uint32_t)sizeof(xo_trace),       (uint32_t)sizeof(xo_sampling_volume),
+        (uint32_t)sizeof(xo_fiber_axis)};
     int count = (int)(sizeof(sizes) / sizeof(sizes[0]));
     if (n < count) count = n;
     for (int i = 0; i < count; ++i) out[i] = sizes[i];
@@ -433,8 +435,13 @@ static int mcml_run(xo_ctx *ctx, const xo_mcml_cfg *cfg, const float *lut, uint3
         xo::set_error("xo_mcml_run: unknown stepping method %d", cfg->method);
         return -1;
     }
-    if (cfg->source_kind != XO_SRC_LINE && cfg->source_kind != XO_SRC_FIBER_NI) {
+    if (cfg->source_kind != XO_SRC_LINE && cfg->source_kind != XO_SRC_FIBER_NI &&
+        cfg->source_kind != XO_SRC_FIBER) {
         xo::set_error("xo_mcml_run: unknown source kind %d", cfg->source_kind);
+        return -1;
+    }
+    if (cfg->source_kind == XO_SRC_FIBER && !(cfg->fiber_axis.direction[2] > 0.0f)) {
+        xo::set_error("xo_mcml_run: the fiber direction must point into the sample (z > 0)");
         return -1;
     }
     if (naccum == 0) {

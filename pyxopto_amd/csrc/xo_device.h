@@ -336,6 +336,45 @@ XO_DEV float fiber_ni_launch(const xo_source_fiber &src, float n_sample, uint64_
     return R;
 }
 
+/* Launch of UniformFiber (xopto/mcml/mcsource/fiber.py:265-330): a point
+ * uniform over the core, rotated into the sample frame by the fiber axis and
+ * projected along the fiber onto z = 0; cos(theta) uniform over the
+ * acceptance cone in air, scaled to the core index, rotated likewise. The
+ * reference tests the projected point's z (0) against the critical cosine
+ * before refracting, so the direction enters the sample unrefracted; the
+ * weight is 1 - the Fresnel reflectance at that direction. Returns that
+ * reflectance (the specular part, deposited along the launch direction). */
+XO_DEV float fiber_launch(const xo_source_fiber &src, const xo_fiber_axis &ax, float n_sample,
+                          uint64_t &x, uint32_t a, float &px, float &py, float &pz, float &dx,
+                          float &dy, float &dz, float &w) {
+    const float *T = ax.transformation;
+    float sin_fi, cos_fi;
+    const float r = __builtin_sqrtf(rng_next(x, a)) * src.radius;
+    xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+    const float s0 = r * cos_fi, s1 = r * sin_fi, s2 = 0.0f;
+    float m0 = T[0] * s0 + T[1] * s1 + T[2] * s2;
+    float m1 = T[3] * s0 + T[4] * s1 + T[5] * s2;
+    const float m2 = T[6] * s0 + T[7] * s1 + T[8] * s2;
+    const float k = (0.0f - m2) / ax.direction[2];
+    m0 += k * ax.direction[0];
+    m1 += k * ax.direction[1];
+    px = src.position[0] + m0;
+    py = src.position[1] + m1;
+    pz = 0.0f;
+    xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+    float cos_theta = 1.0f - rng_next(x, a) * (1.0f - src.cos_min);
+    float sin_theta = __builtin_sqrtf(1.0f - cos_theta * cos_theta);
+    sin_theta = sin_theta / src.n;
+    cos_theta = __builtin_sqrtf(1.0f - sin_theta * sin_theta);
+    const float q0 = cos_fi * sin_theta, q1 = sin_fi * sin_theta, q2 = cos_theta;
+    dx = T[0] * q0 + T[1] * q1 + T[2] * q2;
+    dy = T[3] * q0 + T[4] * q1 + T[5] * q2;
+    dz = T[6] * q0 + T[7] * q1 + T[8] * q2;
+    const float R = reflectance(src.n, n_sample, dz, cos_critical(src.n, n_sample));
+    w = 1.0f - R;
+    return R;
+}
+
 /* Direction of the specularly reflected part of a fiber packet: the
  * sample direction mirrored at the surface, (dx, dy, -dz), refracted from
  * the sample (n1) into the core (n2) across the normal (0, 0, -1)

@@ -95,8 +95,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     }
 
     const float rmax2 = P.cfg.rmax * P.cfg.rmax;
-    const bool fiber = P.cfg.source_kind == XO_SRC_FIBER_NI;
-    const float *sp = fiber ? P.cfg.fiber.position : P.cfg.source.position;
+    const int32_t skind = P.cfg.source_kind;
+    const bool line_src = skind == XO_SRC_LINE;
+    const float *sp = line_src ? P.cfg.source.position : P.cfg.fiber.position;
     const float spx = sp[0];
     const float spy = sp[1];
     const float spz = sp[2];
@@ -138,7 +139,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             if (retire) active = false;
             /* every Line packet deposits the same specular integer at the
              * same bin: one atomic per claimed block */
-            if (claimed && !fiber && lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
+            if (claimed && line_src &&
+                lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
                 const xo_source_line &src = P.cfg.source;
                 detector_deposit_n(P.cfg.specular, bins, src.position[0], src.position[1],
                                    src.direction_reflected[0], src.direction_reflected[1],
@@ -150,7 +152,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             if (keyed)
                 xo_packet_state(P.cfg.packet_seed, P.cfg.first_packet + pkt_index, P.table, &x,
                                 &a);
-            if (fiber) {
+            if (skind == XO_SRC_FIBER_NI) {
                 const float n1 = sl[1].n;
                 const float R = fiber_ni_launch(P.cfg.fiber, n1, x, a, px, py, pz, dx, dy,
                                                 dz, w);
@@ -159,6 +161,10 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                     fiber_specular_direction(dx, dy, dz, n1, P.cfg.fiber.n, rx, ry, rz);
                     detector_deposit(P.cfg.specular, bins, px, py, rx, ry, rz, R);
                 }
+            } else if (skind == XO_SRC_FIBER) {
+                const float R = fiber_launch(P.cfg.fiber, P.cfg.fiber_axis, sl[1].n, x, a, px,
+                                             py, pz, dx, dy, dz, w);
+                detector_deposit(P.cfg.specular, bins, px, py, dx, dy, dz, R);
             } else {
                 /* Line source, xopto/mcml/mcsource/line.py:95-110 */
                 const xo_source_line &src = P.cfg.source;

@@ -55,7 +55,8 @@ class Mc(McWorker):
         self._source = source
         self._source_type = type(source)
         if not isinstance(source, (mcsource.Line, mcsource.UniformFiberNI)):
-            raise NotImplementedError('Only the Line and UniformFiberNI '
+            raise NotImplementedError('Only the Line, UniformFiberNI and '
+                                      'UniformFiber '
                                       'sources are implemented for layered '
                                       'media.')
         self._detectors = detectors

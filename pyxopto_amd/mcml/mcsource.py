@@ -1,12 +1,13 @@
 """Photon packet sources of the layered simulator: Line
-(xopto/mcml/mcsource/line.py:30-252) and UniformFiberNI
-(xopto/mcml/mcsource/fiberni.py:180-420)."""
+(xopto/mcml/mcsource/line.py:30-252), UniformFiberNI
+(xopto/mcml/mcsource/fiberni.py:180-420) and UniformFiber
+(xopto/mcml/mcsource/fiber.py:190-497)."""
 from typing import Tuple
 
 import numpy as np
 
 from .. import _native as N
-from ..mcbase.mcutil import boundary
+from ..mcbase.mcutil import boundary, geometry
 from ..mcbase.mcutil.fiber import MultimodeFiber
 
 
@@ -154,9 +155,77 @@ class UniformFiberNI:
             self._fiber, *self._position)
 
 
+class UniformFiber(UniformFiberNI):
+    """A multimode fiber in contact with the sample along any direction
+    with z > 0, its tip cut parallel to the surface (an elliptical contact
+    area); launch angles uniform in cos(theta) over the acceptance cone,
+    inside the core (xopto/mcml/mcsource/fiber.py:190-497). As in the
+    reference, the launch direction is not refracted into the sample (its
+    refraction test compares the surface point's z, which is 0, with the
+    critical cosine, fiber.py:305-309); the Fresnel reflectance at that
+    direction goes to the specular detector."""
+    kind = N.XO_SRC_FIBER
+
+    def __init__(self, fiber: MultimodeFiber,
+                 position: Tuple[float, float, float] = (0.0, 0.0, 0.0),
+                 direction: Tuple[float, float, float] = (0.0, 0.0, 1.0)):
+        super().__init__(fiber, position)
+        self._direction = np.array([0.0, 0.0, 1.0])
+        self.direction = direction
+
+    def _set_direction(self, v):
+        v = np.asarray(v, dtype=np.float64)
+        norm = float(np.linalg.norm(v))
+        if norm == 0.0:
+            raise ValueError('The direction vector is singular!')
+        v = v / norm
+        if v[2] <= 0.0:
+            raise ValueError('Z component of the propagation direction '
+                             'must be positive!')
+        self._direction[:] = v
+
+    direction = property(lambda self: self._direction, _set_direction, None,
+                         'Fiber direction (z > 0).')
+
+    def pack_cfg(self, mc, cfg):
+        """cl_pack of fiber.py:429-465 into cfg.fiber and cfg.fiber_axis."""
+        super().pack_cfg(mc, cfg)
+        cfg.source_kind = self.kind
+        T = geometry.transform_base((0.0, 0.0, 1.0), self._direction)
+        ax = cfg.fiber_axis
+        for i in range(9):
+            ax.transformation[i] = T.flat[i]
+        for i in range(3):
+            ax.direction[i] = self._direction[i]
+        return cfg
+
+    def update(self, other):
+        if isinstance(other, UniformFiber):
+            self.fiber = other.fiber
+            self.position = other.position
+            self.direction = other.direction
+        elif isinstance(other, dict):
+            self.fiber = other.get('fiber', self.fiber)
+            self.position = other.get('position', self.position)
+            self.direction = other.get('direction', self.direction)
+
+    def todict(self):
+        """fiber.py:466-474"""
+        return {'fiber': self._fiber.todict(),
+                'position': self._position.tolist(),
+                'direction': self._direction.tolist(),
+                'type': 'UniformFiber'}
+
+    def __repr__(self):
+        return 'UniformFiber(fiber={}, position=({}, {}, {}), ' \
+               'direction=({}, {}, {}))'.format(self._fiber, *self._position,
+                                                *self._direction)
+
+
 def fromdict(data: dict):
-    """Line or UniformFiberNI from its todict() data."""
-    kinds = {'Line': Line, 'UniformFiberNI': UniformFiberNI}
+    """Line, UniformFiberNI or UniformFiber from its todict() data."""
+    kinds = {'Line': Line, 'UniformFiberNI': UniformFiberNI,
+             'UniformFiber': UniformFiber}
     if data.get('type') not in kinds:
         raise TypeError('Unsupported source type "{}"!'.format(
             data.get('type')))

@@ -56,6 +56,7 @@ REFERENCE_KEYS = {
     'FluenceRz': {'type', 'mode', 'raxis', 'zaxis', 'center'},
     'Line': {'position', 'direction', 'type'},
     'UniformFiberNI': {'fiber', 'position', 'type'},
+    'UniformFiber': {'fiber', 'position', 'direction', 'type'},
     'MultimodeFiber': {'dcore', 'dcladding', 'ncore', 'na', 'type'},
     'GaussianBeam': {'sigma', 'clip', 'position', 'direction', 'type'},
 }
@@ -93,6 +94,8 @@ def _objects():
                                center=(1e-4, -2e-4)),
         mc.mcsource.Line(position=(1e-4, 0.0, 0.0), direction=(0.1, 0, 1)),
         mc.mcsource.UniformFiberNI(fiber, position=(0.0, 1e-4, 0.0)),
+        mc.mcsource.UniformFiber(fiber, position=(0.0, 1e-4, 0.0),
+                                 direction=(0.2, 0.1, 1.0)),
         mcv.mcsource.Line(), mcv.mcsource.GaussianBeam(50e-6, clip=4.0),
         G.Voxels(G.Axis(-1e-3, 1e-3, 4), G.Axis(-1e-3, 1e-3, 4),
                  G.Axis(0.0, 2e-3, 8)),
@@ -104,7 +107,7 @@ def _objects():
     ]
 
 
-@pytest.mark.parametrize('k', range(16))
+@pytest.mark.parametrize('k', range(17))
 def test_todict_fromdict_round_trip_and_reference_keys(k):
     obj = _objects()[k]
     d = obj.todict()
@@ -118,6 +121,10 @@ def test_todict_fromdict_round_trip_and_reference_keys(k):
         np.testing.assert_allclose(d2['params'], d['params'])
         np.testing.assert_allclose(d2['lut'], d['lut'])
     else:
+        if type(obj) is mc.mcsource.UniformFiber:
+            # a unit vector normalised again may move by an ulp
+            np.testing.assert_allclose(d2.pop('direction'),
+                                       d.pop('direction'), rtol=1e-15)
         assert d2 == d
 
 

@@ -114,3 +114,26 @@ def test_double_layer_fiber_fixture_full_size():
             L.Layer(d=np.inf, n=float(n[3]), mua=0.0, mus=0.0, pf=pf[3])])
     err = _fixture_run('cuda_doublelayer_uniformfiber', layers)
     assert abs(err.mean()) < 0.005
+
+
+@pytest.mark.parametrize('direction', [(0.0, 0.0, 1.0), (0.3, -0.2, 0.9)])
+def test_uniform_fiber_static_schedule_bit_exact(direction):
+    fib = mc.fiberutil.MultimodeFiber(400e-6, 420e-6, 1.45, 0.22)
+    src = mc.mcsource.UniformFiber(fib, position=(1e-4, 0.0, 0.0),
+                                   direction=direction)
+    D = mc.mcdetector
+    det = D.Detectors(top=D.Radial(D.RadialAxis(0.0, 3e-3, 60)),
+                      bottom=D.Total(), specular=D.Total())
+    sim = mc.Mc(W.fiber_layers(300.0, 2000.0), src, det, rnginit=XINIT)
+    sim.rmax = 5e-3
+    lanes, npk = 8192, 60000
+    cfg, lut, naccum = sim.pack()
+    ctx = N.Context(0, 0, XINIT)
+    gpu, rep = ctx.run_mcml(cfg, lut, npk, naccum, lanes,
+                            N.XO_FLAG_STATIC_SCHEDULE)
+    x, a = N.init_rng(lanes, XINIT)
+    ref, counts, _ = O.mcml_run(cfg, lut, x, a, npk, naccum, O.MATH_KERNEL)
+    assert int(rep.steps) == int(counts[0])
+    np.testing.assert_array_equal(gpu, ref)
+    assert gpu[-1] > 0
+    ctx.close()

@@ -6,11 +6,15 @@ reference passes its run when the mean relative error over the grid is
 below 0.5 % at 1e7 packets per point; at 1e6 packets on eight points the
 mean must be below 2.5 % (about four standard errors)."""
 import numpy as np
+import pytest
 
 from pyxopto_amd import _native as N
 from pyxopto_amd import workloads as W
+from pyxopto_amd.mcml import mc
 from pyxopto_amd.util.convolve import fiber_reflectance
 from oracle import pyoracle as O
+
+from helpers import XINIT
 
 import os
 FIXTURE = os.path.join(os.path.dirname(__file__), 'golden',
@@ -55,3 +59,91 @@ def test_fiber_source_matches_cudamcml_fixture():
     errs = np.array(errs)
     assert abs(errs.mean()) < 0.025, errs.mean()
     assert np.abs(errs).max() < 0.2
+
+
+# ---- UniformFiber: a fiber meeting the surface at an angle
+def _tilted_sim(direction, ncore=1.45, n_sample=1.33, na=0.22):
+    fib = mc.fiberutil.MultimodeFiber(400e-6, 420e-6, ncore, na)
+    src = mc.mcsource.UniformFiber(fib, position=(1e-4, -2e-4, 0.0),
+                                   direction=direction)
+    L = mc.mclayer
+    pf = mc.mcpf.Hg(0.8)
+    layers = L.Layers([L.Layer(d=np.inf, n=ncore, mua=0, mus=0, pf=pf),
+                       L.Layer(d=np.inf, n=n_sample, mua=100.0, mus=1e4,
+                               pf=pf),
+                       L.Layer(d=np.inf, n=1.0, mua=0, mus=0, pf=pf)])
+    D = mc.mcdetector
+    det = D.Detectors(top=D.Radial(D.RadialAxis(0.0, 3e-3, 30)),
+                      specular=D.Total())
+    sim = mc.Mc(layers, src, det, rnginit=XINIT)
+    sim.rmax = 1e-2
+    return sim, src
+
+
+def test_uniform_fiber_at_normal_incidence_equals_uniform_fiber_ni():
+    # matched core and sample indices: no specular loss, the same draws in
+    # the same order (fiber.py:265-330 vs fiberni.py:246-290)
+    sim, src = _tilted_sim((0.0, 0.0, 1.0), ncore=1.33, n_sample=1.33)
+    ni = mc.Mc(sim.layers, mc.mcsource.UniformFiberNI(src.fiber,
+                                                      src.position),
+               sim.detectors, rnginit=XINIT)
+    ni.rmax = sim.rmax
+    c1, lut, naccum = sim.pack()
+    c2, _, _ = ni.pack()
+    assert c1.source_kind == N.XO_SRC_FIBER and c2.source_kind == \
+        N.XO_SRC_FIBER_NI
+    x, a = N.init_rng(256, XINIT)
+    a1, k1, _ = O.mcml_run(c1, lut, x, a, 20000, naccum, O.MATH_LIBM)
+    a2, k2, _ = O.mcml_run(c2, lut, x, a, 20000, naccum, O.MATH_LIBM)
+    np.testing.assert_array_equal(a1, a2)
+    np.testing.assert_array_equal(k1, k2)
+
+
+@pytest.mark.parametrize('tilt_deg', [0.0, 20.0, 45.0])
+def test_uniform_fiber_launch_geometry(tilt_deg):
+    t = np.radians(tilt_deg)
+    axis = np.array([np.sin(t) * np.cos(0.7), np.sin(t) * np.sin(0.7),
+                     np.cos(t)])
+    sim, src = _tilted_sim(tuple(axis))
+    cfg, lut, naccum = sim.pack()
+    tr = N.XoTrace()
+    tr.max_events, tr.options, tr.plon = 1, N.XO_TRACE_START, 1
+    x, a = N.init_rng(256, XINIT)
+    acc, _, _, data, n = O.mcml_run_trace(cfg, lut, x, a, 20000, naccum, tr,
+                                          O.MATH_LIBM)
+    e = data[:, 0].astype(np.float64)
+    T = np.array(cfg.fiber_axis.transformation, dtype=np.float64).reshape(3, 3)
+    # T maps the fiber z axis onto the fiber direction
+    np.testing.assert_allclose(T @ [0, 0, 1], axis, atol=1e-6)
+    assert np.all(e[:, 2] == 0.0)
+    # launch points lie on the tilted cut of the core: back in the fiber
+    # frame, p - k d has no z component and lies within the core radius
+    rel = e[:, 0:3] - np.array([1e-4, -2e-4, 0.0])
+    v = rel @ T                      # T^T rel
+    k = v[:, 2]
+    core = v - np.outer(k, [0, 0, 1])
+    r = np.linalg.norm(core[:, :2], axis=1)
+    assert r.max() <= 200e-6 * (1 + 1e-5)
+    # uniform over the core: <r^2> = R^2 / 2
+    assert np.mean(r ** 2) == pytest.approx(0.5 * 200e-6 ** 2, rel=0.03)
+    # directions inside the acceptance cone in the core, about the axis
+    d = e[:, 3:6]
+    cos_axis = d @ axis
+    cmin = np.sqrt(1.0 - (0.22 / 1.45) ** 2)
+    assert cos_axis.min() >= cmin - 1e-6
+    # cos(theta) in air is uniform on [sqrt(1 - NA^2), 1]
+    sin_in = np.sqrt(np.clip(1.0 - cos_axis ** 2, 0.0, None))
+    cos_air = np.sqrt(1.0 - (1.45 * sin_in) ** 2)
+    lo = np.sqrt(1.0 - 0.22 ** 2)
+    assert cos_air.mean() == pytest.approx(0.5 * (1.0 + lo), abs=1e-3)
+    # the weight is 1 - Fresnel(n_core -> n_sample) at the direction, which
+    # is not refracted (the reference's behaviour)
+    c1 = d[:, 2]
+    s2 = np.clip(1.45 / 1.33 * np.sqrt(1 - c1 ** 2), 0, 1)
+    c2 = np.sqrt(1 - s2 ** 2)
+    rs = ((1.45 * c1 - 1.33 * c2) / (1.45 * c1 + 1.33 * c2)) ** 2
+    rp = ((1.45 * c2 - 1.33 * c1) / (1.45 * c2 + 1.33 * c1)) ** 2
+    np.testing.assert_allclose(e[:, 6], 1.0 - 0.5 * (rs + rp), atol=2e-6)
+    # the specular detector holds the rest
+    spec = acc[-1] / N.XO_ACCU_K
+    assert spec == pytest.approx(np.sum(1.0 - e[:, 6]), rel=1e-4, abs=1e-3)
