@@ -82,7 +82,8 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
                   'detectors': '100-bin radial + total R/T per instance',
                   'launch': {'chunk': opts.chunk, 'block': opts.block,
                              'blocks_per_cu': opts.blocks_per_cu,
-                             'refill_min': opts.refill_min or 4}}
+                             'refill_min': opts.refill_min or 4,
+                             'defer_min': opts.defer_min or 12}}
         batch.bench_plain = lambda: plain_rate(batch, insts)
         batch.bench_insts = insts
         return batch, npk, step, False, config
@@ -376,7 +377,8 @@ def main():
                     help='LUT builder processes (1 under a profiler: no '
                          'fork after the GPU is initialised)')
     ap.add_argument('--launch', default=None,
-                    help='CHUNK:BLOCK:BLOCKS_PER_CU[:REFILL_MIN] instead of the '
+                    help='CHUNK:BLOCK:BLOCKS_PER_CU[:REFILL_MIN[:DEFER_MIN]] '
+                         'instead of the '
                          'autotuner')
     args = ap.parse_args()
     if args.gpus != 1:

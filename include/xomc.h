@@ -355,6 +355,8 @@ typedef struct xo_batch_opts {
     int32_t use_lottery;
     uint32_t refill_min;        /* a wave hands out new packets once this
                                    many of its lanes are idle (0: 4)          */
+    uint32_t defer_min;         /* a wave handles its boundary events once
+                                   this many lanes hold one (0: 12)           */
 } xo_batch_opts;
 
 /* ---------------- library ---------------- */

@@ -144,7 +144,8 @@ class XoBatchOpts(ctypes.Structure):
     _fields_ = [('chunk', c_uint32), ('block', c_uint32),
                 ('blocks_per_cu', c_uint32), ('flags', c_uint32),
                 ('w_min', c_float), ('lottery_chance', c_float),
-                ('use_lottery', c_int32), ('refill_min', c_uint32)]
+                ('use_lottery', c_int32), ('refill_min', c_uint32),
+                ('defer_min', c_uint32)]
 
 
 class XoServeOpts(ctypes.Structure):

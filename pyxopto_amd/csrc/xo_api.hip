@@ -831,6 +831,7 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
     P.chance = opts->lottery_chance;
     P.use_lottery = opts->use_lottery;
     P.refill_min = opts->refill_min ? opts->refill_min : 4u;
+    P.defer_min = opts->defer_min ? opts->defer_min : xo::kDeferMinDefault;
     int bpc = 1;
     XO_HIP(xo::occupancy_batch(block, &bpc));
     if (opts->blocks_per_cu && (int)opts->blocks_per_cu < bpc) bpc = (int)opts->blocks_per_cu;

@@ -72,6 +72,9 @@ struct BatchItem {
     uint64_t first;
 };
 
+/* Default of xo_batch_opts.defer_min. */
+constexpr uint32_t kDeferMinDefault = 12;
+
 struct BatchParams {
     const LayerL *layers;        /* [instance][XO_BATCH_MAX_LAYERS] */
     const BatchHdr *hdr;
@@ -87,6 +90,8 @@ struct BatchParams {
     float chance;
     int32_t use_lottery;
     uint32_t refill_min;         /* idle lanes that trigger a refill (>= 1) */
+    uint32_t defer_min;          /* lanes holding a boundary event that make
+                                    the wave handle them (>= 1)             */
 };
 
 hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,

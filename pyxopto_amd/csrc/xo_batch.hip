@@ -34,6 +34,26 @@ XO_DEV float pf_sample_rt(int kind, const PfL &pf, const float *lut, uint64_t &x
     return pf_sample<XO_PF_LUT>(pf, lut, x, a, xi_fi);
 }
 
+/* Pending events of the split transport step. */
+constexpr uint32_t kEvNone = 0, kEvInteraction = 1, kEvBoundary = 2;
+/* Boundary events are handled once P.defer_min lanes of the wave hold one,
+ * or after kDeferWait iterations (bounded wait, every event is handled). */
+constexpr uint32_t kDeferWait = 8;
+
+/* End of an event (mcml.template.c:756-782): the simulation radius and the
+ * step limit; a finished packet frees its lane. */
+__device__ __forceinline__ void end_event(bool done, bool &alive, float px, float py, float pz,
+                                          float spx, float spy, float spz, float rmax2,
+                                          uint32_t &pkt_steps, uint32_t *st) {
+    const float ex = px - spx, ey = py - spy, ez = pz - spz;
+    if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
+    if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
+        done = true;
+        st[XO_STAT_TRUNCATED]++;
+    }
+    if (done) alive = false;
+}
+
 __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
     __shared__ BatchBlockStats sstats;
     if (threadIdx.x < XO_STAT_COUNT) sstats.v[threadIdx.x] = 0ull;
@@ -60,6 +80,9 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
     uint32_t a = 0;
     uint32_t pkt_steps = 0;
     uint32_t st[XO_STAT_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t pend = kEvNone;  /* event the lane's last step ended on     */
+    float pstep = 0.f;        /* drawn step of a pending boundary event  */
+    uint32_t bwait = 0;       /* iterations boundary events have waited  */
 
     for (;;) {
         /* ---- hand out packets to the lanes that need one, once at least
@@ -147,54 +170,96 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
         }
         if (!alive) continue;
 
-        /* ---- one transport step, mcml.template.c:514-788 ---- */
-        const LayerL L = layers[layer];
-        st[XO_STAT_STEPS]++;
-        float step = -xo_logf(rng_next(x, a)) * L.inv_mut;
-        step = fminf(step, XO_FLT_MAX);
-        int32_t next = layer;
-        if (__builtin_fmaf(step, dz, pz) < L.top) {
-            --next;
-            if (fabsf(dz) != 0.0f) step = (L.top - pz) / dz;
-        }
-        if (__builtin_fmaf(step, dz, pz) >= L.bottom) {
-            ++next;
-            if (fabsf(dz) != 0.0f) step = (L.bottom - pz) / dz;
-        }
-        px = __builtin_fmaf(dx, step, px);
-        py = __builtin_fmaf(dy, step, py);
-        pz = __builtin_fmaf(dz, step, pz);
-        if (layer < next) pz = L.bottom;
-        if (layer > next) pz = L.top;
-
-        bool done = false;
-        if (next != layer) {
-            st[XO_STAT_BOUNDARY]++;
-            mcml_boundary(layers, layer, next, dx, dy, dz, x, a);
-            if (!(layer > 0 && layer < nl - 1)) {
-                const BatchHdr &H = P.hdr[inst];
-                if (layer <= 0)
-                    detector_deposit(H.top, bins, px, py, dx, dy, dz, w);
-                else
-                    detector_deposit(H.bottom, bins, px, py, dx, dy, dz, w);
-                done = true;
+        /* ---- transport steps, mcml.template.c:514-788, in three phases
+         * per iteration. (B) lanes without a pending event draw a step; a
+         * step that stays inside the layer moves the packet and leaves an
+         * interaction event, a step that reaches a layer boundary is kept
+         * pending. (C) every interaction event is handled. (A, at the start
+         * of the next iterations) boundary events, a few percent of the
+         * steps in turbid layers, are handled together once defer_min lanes
+         * of the wave hold one, every alive lane holds one, or they waited
+         * kDeferWait iterations: the clipping of the step to the boundary,
+         * the move and the Fresnel event then run for those lanes only,
+         * instead of in nearly every iteration for two or three lanes.
+         * Each packet's arithmetic and draws are those of the reference's
+         * loop; only when the wave issues them changes. ---- */
+        {
+            const uint32_t nb = (uint32_t)__builtin_popcountll(__ballot(pend == kEvBoundary));
+            const uint32_t nal = (uint32_t)__builtin_popcountll(__ballot(true));
+            /* the wait count of the first active lane (lanes that were idle
+             * hold stale counts), so the decision is wave-uniform */
+            const uint32_t bw = (uint32_t)__builtin_amdgcn_readfirstlane((int)bwait);
+            const bool run_b = nb != 0 && (nb >= P.defer_min || nb == nal || bw >= kDeferWait);
+            bwait = (nb != 0 && !run_b) ? bw + 1u : 0u;
+            if (run_b && pend == kEvBoundary) {
+                /* (A) the step as drawn, clipped to the boundary it reaches */
+                const LayerL L = layers[layer];
+                float step = pstep;
+                int32_t next = layer;
+                if (__builtin_fmaf(step, dz, pz) < L.top) {
+                    --next;
+                    if (fabsf(dz) != 0.0f) step = (L.top - pz) / dz;
+                }
+                if (__builtin_fmaf(step, dz, pz) >= L.bottom) {
+                    ++next;
+                    if (fabsf(dz) != 0.0f) step = (L.bottom - pz) / dz;
+                }
+                px = __builtin_fmaf(dx, step, px);
+                py = __builtin_fmaf(dy, step, py);
+                pz = __builtin_fmaf(dz, step, pz);
+                if (layer < next) pz = L.bottom;
+                if (layer > next) pz = L.top;
+                if (next != layer) {
+                    st[XO_STAT_BOUNDARY]++;
+                    mcml_boundary(layers, layer, next, dx, dy, dz, x, a);
+                    bool done = false;
+                    if (!(layer > 0 && layer < nl - 1)) {
+                        const BatchHdr &H = P.hdr[inst];
+                        if (layer <= 0)
+                            detector_deposit(H.top, bins, px, py, dx, dy, dz, w);
+                        else
+                            detector_deposit(H.bottom, bins, px, py, dx, dy, dz, w);
+                        done = true;
+                    }
+                    pend = kEvNone;
+                    end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, st);
+                } else {
+                    /* crossed both planes of a degenerate layer: an
+                     * interaction at the clipped point, handled below */
+                    pend = kEvInteraction;
+                }
             }
-        } else {
+        }
+        if (alive && pend == kEvNone) {
+            /* (B) */
+            const LayerL L = layers[layer];
+            st[XO_STAT_STEPS]++;
+            float step = -xo_logf(rng_next(x, a)) * L.inv_mut;
+            step = fminf(step, XO_FLT_MAX);
+            const float zend = __builtin_fmaf(step, dz, pz);
+            if (zend < L.top || zend >= L.bottom) {
+                pend = kEvBoundary;
+                pstep = step;
+            } else {
+                px = __builtin_fmaf(dx, step, px);
+                py = __builtin_fmaf(dy, step, py);
+                pz = zend;
+                pend = kEvInteraction;
+            }
+        }
+        if (alive && pend == kEvInteraction) {
+            /* (C) absorption and scattering, mcml.template.c:724-753 */
+            const LayerL L = layers[layer];
             float deposit = w * L.mua_inv_mut;
             w -= deposit;
             PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
             float xi_fi;
             float ct = pf_sample_rt(pf_kind, pf, P.lut, x, a, xi_fi);
             scatter_direction(dx, dy, dz, ct, xi_fi);
-            if (roulette(w, x, a, P.w_min, P.chance, P.use_lottery)) done = true;
+            const bool done = roulette(w, x, a, P.w_min, P.chance, P.use_lottery);
+            pend = kEvNone;
+            end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, st);
         }
-        float ex = px - spx, ey = py - spy, ez = pz - spz;
-        if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
-        if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
-            done = true;
-            st[XO_STAT_TRUNCATED]++;
-        }
-        if (done) alive = false;
     }
 
     for (int i = 0; i < XO_STAT_COUNT; ++i)

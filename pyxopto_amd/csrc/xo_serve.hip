@@ -195,6 +195,7 @@ int build_graph(xo_server *S, int k) {
     P.chance = S->opts.lottery_chance;
     P.use_lottery = S->opts.use_lottery;
     P.refill_min = 4u;
+    P.defer_min = xo::kDeferMinDefault;
     XO_CHECK(hipStreamBeginCapture(S->s_ml, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = xo::launch_zero_u32((uint32_t *)T.ml_counters, 2 * kCounterWords, S->s_ml);
     hipError_t e2 = xo::launch_batch(P, S->ml_blocks, S->opts.block, S->s_ml);
@@ -394,6 +395,7 @@ int dispatch_tick(xo_server *S, int k) {
             P.chance = S->opts.lottery_chance;
             P.use_lottery = S->opts.use_lottery;
             P.refill_min = 4u;
+            P.defer_min = xo::kDeferMinDefault;
             XO_CHECK(xo::launch_batch(P, S->ml_blocks, S->opts.block, S->s_ml));
         }
     } else {

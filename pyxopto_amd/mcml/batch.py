@@ -108,10 +108,12 @@ class Batch(McWorker):
         return arr, self._lut_array(), max(self._alloc_offset, 1)
 
     def opts(self, chunk: int = 2048, block: int = 256,
-             blocks_per_cu: int = 0, refill_min: int = 0) -> N.XoBatchOpts:
+             blocks_per_cu: int = 0, refill_min: int = 0,
+             defer_min: int = 0) -> N.XoBatchOpts:
         o = N.XoBatchOpts()
         o.chunk, o.block, o.blocks_per_cu = chunk, block, blocks_per_cu
         o.refill_min = refill_min
+        o.defer_min = defer_min
         o.w_min = float(self._opts['w_min'])
         o.lottery_chance = float(self._opts['lottery_chance'])
         o.use_lottery = int(bool(self._opts['use_lottery']))
