@@ -38,6 +38,12 @@ XO_DEV void stats_flush(BlockStats *ss, const uint32_t *mine,
         atomicAdd(&gstats[threadIdx.x], ss->v[threadIdx.x]);
 }
 
+/* Pending events of the split transport step of the layered kernel;
+ * boundary events wait until kDeferMin lanes of a wave hold one (at most
+ * kDeferWait iterations). */
+constexpr uint32_t kEvNone = 0, kEvInteraction = 1, kEvBoundary = 2;
+constexpr uint32_t kDeferMin = 12, kDeferWait = 8;
+
 template <int PF, int METHOD, bool TRACE>
 __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -115,6 +121,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     uint32_t pkt_steps = 0;
     TraceState ts{0, 0, 0, 0.0f};
     const xo_trace &T = P.trace;
+    uint32_t pend = kEvNone; /* event the lane's last step ended on      */
+    float pstep = 0.f;       /* drawn step of a pending boundary event   */
+    uint32_t bwait = 0;      /* iterations boundary events have waited   */
 
     for (;;) {
         /* ---- refill lanes whose packet ended ---- */
@@ -193,34 +202,18 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         }
         if (!__any(alive)) break;
         if (!alive) continue;
-        /* ---- one iteration of while(!done), mcml.template.c:514-788 ---- */
-        const LayerL &L = sl[layer];
-        st[XO_STAT_STEPS]++;
-        float step;
-        if (METHOD == XO_METHOD_MBL)
-            step = (-xo_logf(rng_next(x, a))) / L.mus; /* mcml.template.c:529-531 */
-        else
-            step = -xo_logf(rng_next(x, a)) * L.inv_mut;
-        step = fminf(step, XO_FLT_MAX);
-        int32_t next = layer;
-        if (__builtin_fmaf(step, dz, pz) < L.top) {
-            --next;
-            if (fabsf(dz) != 0.0f) step = (L.top - pz) / dz;
-        }
-        if (__builtin_fmaf(step, dz, pz) >= L.bottom) {
-            ++next;
-            if (fabsf(dz) != 0.0f) step = (L.bottom - pz) / dz;
-        }
-        px = __builtin_fmaf(dx, step, px);
-        py = __builtin_fmaf(dy, step, py);
-        pz = __builtin_fmaf(dz, step, pz);
-        /* optical path length, mcml.template.c:570-574 */
-        if (TRACE && T.plon) ts.pl += L.n * step;
-        if (layer < next) pz = L.bottom;
-        if (layer > next) pz = L.top;
-
-        bool done = false;
-        if (METHOD == XO_METHOD_MBL) {
+        /* ---- transport steps, the while(!done) loop of
+         * mcml.template.c:514-788, in three phases per iteration: (B)
+         * lanes without a pending event draw a step; a step inside the layer
+         * moves the packet (and, for microscopic Beer-Lambert, absorbs along
+         * it) and leaves an interaction event, a step reaching a boundary is
+         * kept pending; (C) interaction events are handled; (A, at the top
+         * of later iterations) boundary events are handled together once
+         * kDeferMin lanes of the wave hold one, every live lane holds one,
+         * or they waited kDeferWait iterations. Each packet's arithmetic and
+         * draws follow the reference's loop; only when the wave issues them
+         * changes. ---- */
+        auto mbl_absorb = [&](const LayerL &L, float step) {
             /* absorption along the whole step, mcml.template.c:584-606 */
             const float mua = L.mua;
             const float fraction = 1.0f - xo_expf(-(mua * step));
@@ -235,70 +228,137 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                                     deposit))
                     st[XO_STAT_DEPOSITS]++;
             }
-        }
-        if (next != layer) {
-            st[XO_STAT_BOUNDARY]++;
-            const int32_t before = layer;
-            mcml_boundary(sl, layer, next, dx, dy, dz, x, a);
-            if (TRACE)
-                ts.ev |= XO_EVENT_BOUNDARY_HIT |
-                         (layer != before ? XO_EVENT_REFRACTION : XO_EVENT_REFLECTION);
-            if (!(layer > 0 && layer < (int32_t)nl - 1)) {
-                if (layer <= 0)
-                    detector_deposit(P.cfg.top, bins, px, py, dx, dy, dz, w);
-                else
-                    detector_deposit(P.cfg.bottom, bins, px, py, dx, dy, dz, w);
+        };
+        auto end_step = [&](bool done) {
+            /* microscopic Beer-Lambert: roulette after the boundary or the
+             * scattering, whether or not the packet has left (as the
+             * reference does, mcml.template.c:652-667) */
+            if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
                 done = true;
+            float ex = px - spx, ey = py - spy, ez = pz - spz;
+            if (ex * ex + ey * ey + ez * ez > rmax2) {
+                done = true;
+                if (TRACE) ts.ev |= XO_EVENT_ESCAPED;
             }
-        } else if (METHOD == XO_METHOD_AR &&
-                   rng_next(x, a) < L.mua_inv_mut) {
-            /* albedo rejection: the whole weight is absorbed here,
-             * mcml.template.c:696-708 */
-            const float deposit = w;
-            w -= deposit;
-            if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-            if (use_fluence && fluence_deposit(F, P.accum, px, py, pz, deposit))
-                st[XO_STAT_DEPOSITS]++;
-            done = true;
-        } else {
-            if (METHOD == XO_METHOD_AW) {
-                const float deposit = w * L.mua_inv_mut;
+            if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
+                done = true;
+                st[XO_STAT_TRUNCATED]++;
+            }
+            if (TRACE) {
+                /* mcml.template.c:765-781 */
+                if (done) ts.ev |= XO_EVENT_TERMINATED;
+                if (T.options == XO_TRACE_ALL || ((T.options & XO_TRACE_END) && done))
+                    trace_event(T, P.trace_data, ts, px, py, pz, dx, dy, dz, w);
+                ts.ev = 0u;
+                if (done) P.trace_n[ts.pkt] = (int32_t)ts.count;
+            }
+            pend = kEvNone;
+            if (done) alive = false;
+        };
+        {
+            const uint32_t nb = (uint32_t)__builtin_popcountll(__ballot(pend == kEvBoundary));
+            const uint32_t nal = (uint32_t)__builtin_popcountll(__ballot(true));
+            const uint32_t bw = (uint32_t)__builtin_amdgcn_readfirstlane((int)bwait);
+            const bool run_b = nb != 0 && (nb >= kDeferMin || nb == nal || bw >= kDeferWait);
+            bwait = (nb != 0 && !run_b) ? bw + 1u : 0u;
+            if (run_b && pend == kEvBoundary) {
+                /* (A) the step as drawn, clipped to the boundary it reaches */
+                const LayerL &L = sl[layer];
+                float step = pstep;
+                int32_t next = layer;
+                if (__builtin_fmaf(step, dz, pz) < L.top) {
+                    --next;
+                    if (fabsf(dz) != 0.0f) step = (L.top - pz) / dz;
+                }
+                if (__builtin_fmaf(step, dz, pz) >= L.bottom) {
+                    ++next;
+                    if (fabsf(dz) != 0.0f) step = (L.bottom - pz) / dz;
+                }
+                px = __builtin_fmaf(dx, step, px);
+                py = __builtin_fmaf(dy, step, py);
+                pz = __builtin_fmaf(dz, step, pz);
+                /* optical path length, mcml.template.c:570-574 */
+                if (TRACE && T.plon) ts.pl += L.n * step;
+                if (layer < next) pz = L.bottom;
+                if (layer > next) pz = L.top;
+                if (METHOD == XO_METHOD_MBL) mbl_absorb(L, step);
+                if (next != layer) {
+                    st[XO_STAT_BOUNDARY]++;
+                    const int32_t before = layer;
+                    mcml_boundary(sl, layer, next, dx, dy, dz, x, a);
+                    if (TRACE)
+                        ts.ev |= XO_EVENT_BOUNDARY_HIT |
+                                 (layer != before ? XO_EVENT_REFRACTION : XO_EVENT_REFLECTION);
+                    bool done = false;
+                    if (!(layer > 0 && layer < (int32_t)nl - 1)) {
+                        if (layer <= 0)
+                            detector_deposit(P.cfg.top, bins, px, py, dx, dy, dz, w);
+                        else
+                            detector_deposit(P.cfg.bottom, bins, px, py, dx, dy, dz, w);
+                        done = true;
+                    }
+                    end_step(done);
+                } else {
+                    /* crossed both planes of a degenerate layer: an
+                     * interaction at the clipped point, handled below */
+                    pend = kEvInteraction;
+                }
+            }
+        }
+        if (alive && pend == kEvNone) {
+            /* (B) */
+            const LayerL &L = sl[layer];
+            st[XO_STAT_STEPS]++;
+            float step;
+            if (METHOD == XO_METHOD_MBL)
+                step = (-xo_logf(rng_next(x, a))) / L.mus; /* mcml.template.c:529-531 */
+            else
+                step = -xo_logf(rng_next(x, a)) * L.inv_mut;
+            step = fminf(step, XO_FLT_MAX);
+            const float zend = __builtin_fmaf(step, dz, pz);
+            if (zend < L.top || zend >= L.bottom) {
+                pend = kEvBoundary;
+                pstep = step;
+            } else {
+                px = __builtin_fmaf(dx, step, px);
+                py = __builtin_fmaf(dy, step, py);
+                pz = zend;
+                if (TRACE && T.plon) ts.pl += L.n * step;
+                if (METHOD == XO_METHOD_MBL) mbl_absorb(L, step);
+                pend = kEvInteraction;
+            }
+        }
+        if (alive && pend == kEvInteraction) {
+            /* (C) */
+            const LayerL &L = sl[layer];
+            bool done = false;
+            if (METHOD == XO_METHOD_AR && rng_next(x, a) < L.mua_inv_mut) {
+                /* albedo rejection: the whole weight is absorbed here,
+                 * mcml.template.c:696-708 */
+                const float deposit = w;
                 w -= deposit;
                 if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                 if (use_fluence && fluence_deposit(F, P.accum, px, py, pz, deposit))
                     st[XO_STAT_DEPOSITS]++;
-            }
-            PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
-            float xi_fi;
-            float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
-            scatter_direction(dx, dy, dz, ct, xi_fi);
-            if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
-            if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
                 done = true;
+            } else {
+                if (METHOD == XO_METHOD_AW) {
+                    const float deposit = w * L.mua_inv_mut;
+                    w -= deposit;
+                    if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
+                    if (use_fluence && fluence_deposit(F, P.accum, px, py, pz, deposit))
+                        st[XO_STAT_DEPOSITS]++;
+                }
+                PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
+                float xi_fi;
+                float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
+                scatter_direction(dx, dy, dz, ct, xi_fi);
+                if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
+                if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
+                    done = true;
+            }
+            end_step(done);
         }
-        /* microscopic Beer-Lambert: roulette after the boundary or the
-         * scattering, whether or not the packet has left (as the reference
-         * does, mcml.template.c:652-667) */
-        if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
-            done = true;
-        float ex = px - spx, ey = py - spy, ez = pz - spz;
-        if (ex * ex + ey * ey + ez * ez > rmax2) {
-            done = true;
-            if (TRACE) ts.ev |= XO_EVENT_ESCAPED;
-        }
-        if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
-            done = true;
-            st[XO_STAT_TRUNCATED]++;
-        }
-        if (TRACE) {
-            /* mcml.template.c:765-781 */
-            if (done) ts.ev |= XO_EVENT_TERMINATED;
-            if (T.options == XO_TRACE_ALL || ((T.options & XO_TRACE_END) && done))
-                trace_event(T, P.trace_data, ts, px, py, pz, dx, dy, dz, w);
-            ts.ev = 0u;
-            if (done) P.trace_n[ts.pkt] = (int32_t)ts.count;
-        }
-        if (done) alive = false;
     }
 
     if (lane_ok) {
