@@ -39,6 +39,15 @@ __host__ __device__ inline uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 
+/* Two fp32 values as packed bf16 bits (low half a) with the hardware
+ * conversion, which rounds to nearest even: for every non-NaN input the
+ * same bits as f2bf. */
+__device__ inline uint32_t bf16_pair(float a, float b) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    typedef float floatx2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((floatx2){a, b}, bf16x2));
+}
+
 __host__ __device__ inline float bf2f(uint16_t h) {
     uint32_t u = (uint32_t)h << 16;
     float f;
@@ -163,25 +172,135 @@ __global__ void sum_splits_kernel(const float *P, uint32_t splits, uint64_t n, f
 /* ------------------------------------------------------------------ */
 /* Fused inference: every layer of a row block in one launch            */
 /* ------------------------------------------------------------------ */
-/* A 256-thread block takes 64 rows through the whole network: the
- * activations stay in one LDS tile (64 x 256 bf16), each layer is a 64 x N
- * product of that tile with the layer's weights read from L2 as bf16
- * fragments (transposed copies [N][K], so a lane's 8 consecutive k are one
- * 16-byte load), bias + ReLU applied in registers and the result written
- * back over the tile. Networks: input width <= 32, hidden widths 256,
- * output width <= 128, at most 8 layers. */
-constexpr int FBM = 64, FHID = 256, FSTRIDE = FHID + 8, FMAXL = 8;
+/* A 256-thread block takes 16 MT rows through the whole network: the
+ * activations stay in one LDS tile (16 MT x 256 bf16), each layer is the
+ * product of that tile with the layer's weights, read from L2 as bf16 MFMA
+ * fragments (fragment-major copies, so a wave's fragment is 1 KB of whole
+ * cache lines), bias + ReLU applied in registers and the result written
+ * back over the tile. Wave w owns 64 output columns of a hidden layer
+ * (16-column tiles w and w + 4 of the last layer) for all rows, so every
+ * weight fragment it loads feeds MT MFMAs. The weights are the MFMA's first
+ * operand and the activations its second: the 16x16 result tile is
+ * transposed (columns x rows), so a lane holds four consecutive output
+ * columns of one row and writes them back as one 8-byte LDS store. Layer
+ * shapes are compile-time, the k loop is unrolled and the weight fragments
+ * are loaded FPD k steps ahead of their use (the activation fragments one
+ * step ahead), so the MFMAs of a step issue while the next steps' L2 reads
+ * are in flight. Networks: input width <= 32, hidden widths 256, output
+ * width <= 128, 2 to 8 layers. */
+#ifndef XO_MLP_PD
+#define XO_MLP_PD 2
+#endif
+constexpr int FHID = 256, FSTRIDE = FHID + 8, FMAXL = 8, FPD = XO_MLP_PD;
+#ifndef XO_MLP_MT
+#define XO_MLP_MT 4
+#endif
+constexpr int FMT = XO_MLP_MT, FBM = 16 * FMT;
 
 struct FusedArgs {
     const float *X;
     float *Y;
     int M, nin, nout, nlayers;
-    const uint16_t *wt[FMAXL];  /* [Npad][Kpad]: Kpad 32 (first) or 256,
-                                   Npad 256 (hidden) or 128 (last) */
+    const uint16_t *wt[FMAXL];  /* fragment-major: 1 (first) or 8 k steps,
+                                   16 (hidden) or 8 (last) column tiles */
     const float *bias[FMAXL];
 };
 
-__global__ void __launch_bounds__(256) mlp_fused_kernel(const FusedArgs f) {
+/* One layer of a 16 FMT-row block: KS k steps of 32, NJ 16-column tiles
+ * per wave; LAST writes fp32 outputs to Y, otherwise bf16(ReLU) back to
+ * hb. */
+template <int KS, int NJ, bool LAST>
+__device__ __forceinline__ void fused_layer(const FusedArgs &f, int l, uint16_t *hb, int row0,
+                                            int lane, int wave) {
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    const uint16_t *W = f.wt[l];
+    const uint16_t *wp[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int tile = LAST ? wave + 4 * j : wave * 4 + j;
+        wp[j] = W + ((size_t)tile * KS * 64 + lane) * 8;
+    }
+    floatx4 acc[FMT][NJ];
+#pragma unroll
+    for (int i = 0; i < FMT; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 b[KS][NJ], a[2][FMT];
+#pragma unroll
+    for (int s = 0; s < FPD && s < KS; ++s) {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) b[s][j] = *reinterpret_cast<const bf16x8 *>(wp[j] + s * 512);
+        __builtin_amdgcn_sched_barrier(0); /* issued in the order of use */
+    }
+#pragma unroll
+    for (int i = 0; i < FMT; ++i)
+        a[0][i] = *reinterpret_cast<const bf16x8 *>(&hb[(i * 16 + fr) * FSTRIDE + fk]);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        if (s + FPD < KS) {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+                b[s + FPD][j] = *reinterpret_cast<const bf16x8 *>(wp[j] + (s + FPD) * 512);
+        }
+        if (s + 1 < KS) {
+#pragma unroll
+            for (int i = 0; i < FMT; ++i)
+                a[(s + 1) & 1][i] = *reinterpret_cast<const bf16x8 *>(
+                    &hb[(i * 16 + fr) * FSTRIDE + (s + 1) * 32 + fk]);
+        }
+        /* keep the reads above ahead of this step's MFMAs */
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < FMT; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[s][j], a[s & 1][i],
+                                                                   acc[i][j], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads(); /* every wave has read the tile */
+    const float *bias = f.bias[l];
+    const int nq = (lane >> 4) * 4;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int n0 = (LAST ? wave + 4 * j : wave * 4 + j) * 16 + nq;
+        if (!LAST) {
+            const float4 bv = *reinterpret_cast<const float4 *>(bias + n0);
+#pragma unroll
+            for (int i = 0; i < FMT; ++i) {
+                const float z0 = acc[i][j][0] + bv.x, z1 = acc[i][j][1] + bv.y;
+                const float z2 = acc[i][j][2] + bv.z, z3 = acc[i][j][3] + bv.w;
+                uint2 pk;
+                pk.x = bf16_pair(z0 > 0.f ? z0 : 0.f, z1 > 0.f ? z1 : 0.f);
+                pk.y = bf16_pair(z2 > 0.f ? z2 : 0.f, z3 > 0.f ? z3 : 0.f);
+                *reinterpret_cast<uint2 *>(&hb[(i * 16 + fr) * FSTRIDE + n0]) = pk;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < FMT; ++i) {
+                const int m = row0 + i * 16 + fr;
+                if (m >= f.M || n0 >= f.nout) continue;
+                float *y = f.Y + (size_t)m * f.nout + n0;
+                if ((f.nout & 3) == 0) { /* n0 + 3 < nout, 16-byte aligned */
+                    *reinterpret_cast<float4 *>(y) =
+                        make_float4(acc[i][j][0] + bias[n0], acc[i][j][1] + bias[n0 + 1],
+                                    acc[i][j][2] + bias[n0 + 2], acc[i][j][3] + bias[n0 + 3]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (n0 + r < f.nout) y[r] = acc[i][j][r] + bias[n0 + r];
+                }
+            }
+        }
+    }
+    __syncthreads(); /* the tile holds this layer's output */
+}
+
+#ifndef XO_MLP_WPE
+#define XO_MLP_WPE 2
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XO_MLP_WPE)))
+mlp_fused_kernel(const FusedArgs f) {
     __shared__ __attribute__((aligned(16))) uint16_t hb[FBM * FSTRIDE];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int row0 = blockIdx.x * FBM;
@@ -191,66 +310,23 @@ __global__ void __launch_bounds__(256) mlp_fused_kernel(const FusedArgs f) {
         hb[m * FSTRIDE + c] = f2bf(v);
     }
     __syncthreads();
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
-    for (int l = 0; l < f.nlayers; ++l) {
-        const bool last = l + 1 == f.nlayers;
-        const int K = l == 0 ? 32 : FHID;
-        const uint16_t *W = f.wt[l];
-        floatx4 acc[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-        /* hidden: wave w owns columns 64 w .. 64 w + 63; last: 16-column
-         * tiles w and w + 4 of the 128 padded columns */
-        for (int k0 = 0; k0 < K; k0 += 32) {
-            bf16x8 a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                a[i] = *reinterpret_cast<const bf16x8 *>(&hb[(i * 16 + fr) * FSTRIDE + k0 + fk]);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int n = last ? ((wave + 4 * (j & 1)) * 16 + fr) : (wave * 64 + j * 16 + fr);
-                b[j] = *reinterpret_cast<const bf16x8 *>(&W[(size_t)n * K + k0 + fk]);
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (!last || j < 2)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j],
-                                                                           0, 0, 0);
-        }
-        __syncthreads(); /* every wave has read the tile */
-        const float *bias = f.bias[l];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = i * 16 + (lane >> 4) * 4 + r;
-                    if (!last) {
-                        const int n = wave * 64 + j * 16 + fr;
-                        const float z = acc[i][j][r] + bias[n];
-                        hb[m * FSTRIDE + n] = f2bf(z > 0.f ? z : 0.f);
-                    } else if (j < 2) {
-                        const int n = (wave + 4 * j) * 16 + fr;
-                        if (n < f.nout && row0 + m < f.M)
-                            f.Y[(size_t)(row0 + m) * f.nout + n] = acc[i][j][r] + bias[n];
-                    }
-                }
-        __syncthreads(); /* the tile holds this layer's output */
-    }
+    fused_layer<1, 4, false>(f, 0, hb, row0, lane, wave);
+    for (int l = 1; l + 1 < f.nlayers; ++l) fused_layer<8, 4, false>(f, l, hb, row0, lane, wave);
+    fused_layer<8, 2, true>(f, f.nlayers - 1, hb, row0, lane, wave);
 }
 
-/* dst[n][k] = bf16(w[k][n]) for k < in, n < out; zero elsewhere in
- * [npad][kpad]. */
-__global__ void transpose_pad_kernel(const float *w, int in, int out, int kpad, int npad,
-                                     uint16_t *dst) {
+/* The weights of one layer as bf16 MFMA fragments: for 16-column tile t,
+ * 32-wide k step s and lane L, the 8 values W[k][n] with
+ * n = 16 t + (L & 15), k = 32 s + 8 (L >> 4) + e (e = 0..7) are stored at
+ * ((t * ksteps + s) * 64 + L) * 8 + e, zero for k >= in or n >= out. A
+ * wave's fragment of one tile and step is then 1 KB of whole cache lines. */
+__global__ void fragment_pad_kernel(const float *w, int in, int out, int ksteps, int ntiles,
+                                    uint16_t *dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kpad * npad) return;
-    const int n = i / kpad, k = i % kpad;
+    if (i >= ntiles * ksteps * 512) return;
+    const int e = i & 7, L = (i >> 3) & 63, ts = i >> 9;
+    const int t = ts / ksteps, st = ts % ksteps;
+    const int n = t * 16 + (L & 15), k = st * 32 + (L >> 4) * 8 + e;
     dst[i] = (k < in && n < out) ? f2bf(w[(size_t)k * out + n]) : (uint16_t)0;
 }
 
@@ -525,10 +601,10 @@ int refresh_bf16(xo_mlp *N) {
 
 int refresh_transposed(xo_mlp *N) {
     for (int l = 0; l < N->nlayers; ++l) {
-        const int kpad = l == 0 ? 32 : FHID, npad = l + 1 == N->nlayers ? 128 : FHID;
-        hipLaunchKernelGGL(transpose_pad_kernel, dim3((kpad * npad + 255) / 256), dim3(256), 0,
-                           N->stream, N->w + N->woff[l], N->dims[l], N->dims[l + 1], kpad, npad,
-                           N->wt + N->wt_off[l]);
+        const int ksteps = l == 0 ? 1 : FHID / 32, ntiles = l + 1 == N->nlayers ? 8 : FHID / 16;
+        hipLaunchKernelGGL(fragment_pad_kernel, dim3(ntiles * ksteps * 2), dim3(256), 0,
+                           N->stream, N->w + N->woff[l], N->dims[l], N->dims[l + 1], ksteps,
+                           ntiles, N->wt + N->wt_off[l]);
         MLP_CHECK(hipGetLastError());
     }
     N->wt_dirty = false;
