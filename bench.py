@@ -38,8 +38,8 @@ SERVING_QPS = 250.0                     # requests/s of the serving run
 # 1 GiB device-to-device copy (profiles/batch_kernel_layered_batch_4096.md,
 # profiles/mcvox_kernel_skin_vessel_200.md). Offline figures: measured for
 # the kernels as they are in this tree.
-TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 1.561e11,
-                            'skin_vessel_200': 7.344e11}
+TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 1.448e11,
+                            'skin_vessel_200': 7.358e11}
 
 METRIC = ('photon packets/s (1 GPU) batched vs plain; p50/p99 solve latency '
           'at fixed QPS')
@@ -203,36 +203,47 @@ def serving(insts, qps: float, duration: float, seed: int = 20261203,
     return out
 
 
-def surrogate_rates(ctx, rows: int = 65536, batch: int = 4096,
+def surrogate_rates(ctx, rows=(65536, 1048576), batch: int = 4096,
                     steps: int = 50):
     """MFMA throughput of the surrogate network (5-256-256-256-100, bf16
-    products, fp32 accumulation): inference over `rows` synthetic inputs
-    and `steps` captured Adam steps on minibatches of `batch`."""
+    products, fp32 accumulation): fused inference over each count of
+    synthetic input rows in `rows` (the figures without a suffix are the
+    largest), and `steps` captured Adam steps on minibatches of `batch`."""
     import numpy as np
     from pyxopto_amd import surrogate as S
     dims = S.DIMS
-    net = S.Network(dims, ctx, max_batch=rows)
-    net.set_params(S.init_params(dims, seed=1))
-    rng = np.random.default_rng(2)
-    x = rng.uniform(-1, 1, (rows, dims[0])).astype(np.float32)
-    t = rng.normal(0, 1, (rows, dims[-1])).astype(np.float32)
-    net.forward(x)
-    _, ms = net.forward(x)
     flops_row = 2 * sum(i * o for i, o in zip(dims[:-1], dims[1:]))
+    rng = np.random.default_rng(2)
+    out = {'network': '5-256-256-256-100 relu, bf16 MFMA, fp32 accumulate'}
+    infer = None
+    for n in sorted(rows):
+        net = S.Network(dims, ctx, max_batch=n)
+        net.set_params(S.init_params(dims, seed=1))
+        x = rng.uniform(-1, 1, (n, dims[0])).astype(np.float32)
+        net.forward(x)
+        _, ms = net.forward(x)
+        net.close()
+        infer = flops_row * n / (ms * 1e-3) / 1e12
+        out['inference_rows_%d_per_s' % n] = n / (ms * 1e-3)
+        out['inference_TFLOPs_%d_rows' % n] = infer
+    out['inference_rows'] = max(rows)
+    out['inference_TFLOPs'] = infer
+    n = 65536
+    net = S.Network(dims, ctx, max_batch=batch)
+    net.set_params(S.init_params(dims, seed=1))
+    x = rng.uniform(-1, 1, (n, dims[0])).astype(np.float32)
+    t = rng.normal(0, 1, (n, dims[-1])).astype(np.float32)
     net.set_data(x, t)
     net.train(2, batch, 1e-3, seed=3)
     _, tms = net.train(steps, batch, 1e-3, seed=3)
     net.close()
-    infer = flops_row * rows / (ms * 1e-3) / 1e12
     train = 3 * flops_row * batch * steps / (tms * 1e-3) / 1e12
-    return {'network': '5-256-256-256-100 relu, bf16 MFMA, fp32 accumulate',
-            'inference_rows_per_s': rows / (ms * 1e-3),
-            'inference_TFLOPs': infer,
-            'train_step_ms': tms / steps, 'train_batch': batch,
-            'train_TFLOPs': train,
-            'roofline': {'bound': 'mfma', 'peak': 2500.0, 'unit': 'TFLOP/s',
-                         'frac_inference': infer / 2500.0,
-                         'frac_train': train / 2500.0}}
+    out.update({'train_step_ms': tms / steps, 'train_batch': batch,
+                'train_TFLOPs': train,
+                'roofline': {'bound': 'mfma', 'peak': 2500.0, 'unit': 'TFLOP/s',
+                             'frac_inference': infer / 2500.0,
+                             'frac_train': train / 2500.0}})
+    return out
 
 
 PUBLISHED_FIBER_S = 5.7   # RTX A6000, test/performance.py:134-135 (4e9 packets)

@@ -186,6 +186,35 @@ XO_DEV bool fluence_deposit(const xo_fluence &F, unsigned long long *accum, floa
     return true;
 }
 
+/* The same LDS object behind a pointer the optimizer cannot see through:
+ * loads through it stay where they are used instead of being hoisted out of
+ * the transport loop into scalar registers, which the loop would spill and
+ * reload lane by lane. For configuration read once per event (launch,
+ * escape, deposit). */
+template <class T> XO_DEV const T &lds_fresh(const T &ref) {
+    typedef const T __attribute__((address_space(3))) *LdsPtr;
+    LdsPtr q = (LdsPtr)&ref;
+    asm volatile("" : "+v"(q));
+    return *(const T *)q;
+}
+
+/* Event-time configuration of the single-simulation kernels, copied to
+ * LDS once per block. */
+struct LayerEventCfgL {
+    xo_detector top, bottom, specular;
+    xo_fluence fluence;
+    xo_source_line line;
+    xo_source_fiber fiber;
+    xo_fiber_axis fiber_axis;
+};
+
+struct VoxEventCfgL {
+    xo_detector top, bottom, specular;
+    xo_fluence fluence;
+    xo_source_line line;
+    xo_source_gaussian gaussian;
+};
+
 /* Phase-function parameters of one layer/material, read from LDS. */
 struct PfL {
     float g, beta, la, lb, lc;

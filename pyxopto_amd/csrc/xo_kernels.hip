@@ -48,6 +48,7 @@ template <int PF, int METHOD, bool TRACE>
 __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ BlockStats sstats;
+    __shared__ LayerEventCfgL sev;
     LayerL *sl = reinterpret_cast<LayerL *>(smem);
     unsigned long long *sbins =
         reinterpret_cast<unsigned long long *>(smem + XO_MAX_LAYERS * sizeof(LayerL));
@@ -79,6 +80,15 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK) sbins[i] = 0ull;
     for (uint32_t i = tid; i < P.lds_lut; i += XO_BLOCK) slut[i] = P.lut[i];
     if (tid < XO_STAT_COUNT) sstats.v[tid] = 0ull;
+    if (tid == 0) {
+        sev.top = P.cfg.top;
+        sev.bottom = P.cfg.bottom;
+        sev.specular = P.cfg.specular;
+        sev.fluence = P.cfg.fluence;
+        sev.line = P.cfg.source;
+        sev.fiber = P.cfg.fiber;
+        sev.fiber_axis = P.cfg.fiber_axis;
+    }
     __syncthreads();
 
     const Bins bins{P.lds_bins ? sbins : nullptr, P.accum};
@@ -109,8 +119,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     const float spz = sp[2];
     const float w_min = P.cfg.w_min, chance = P.cfg.lottery_chance;
     const int use_lottery = P.cfg.use_lottery;
-    const xo_fluence &F = P.cfg.fluence;
-    const bool use_fluence = F.enabled != XO_FLUENCE_NONE;
+    const bool use_fluence = P.cfg.fluence.enabled != XO_FLUENCE_NONE;
+    const bool use_specular = P.cfg.specular.kind != XO_DET_NONE;
 
     bool active = lane_ok;
     WaveDispenser wq;
@@ -150,8 +160,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
              * same bin: one atomic per claimed block */
             if (claimed && line_src &&
                 lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
-                const xo_source_line &src = P.cfg.source;
-                detector_deposit_n(P.cfg.specular, bins, src.position[0], src.position[1],
+                const xo_source_line &src = lds_fresh(sev.line);
+                detector_deposit_n(lds_fresh(sev.specular), bins, src.position[0], src.position[1],
                                    src.direction_reflected[0], src.direction_reflected[1],
                                    src.direction_reflected[2], src.reflectance,
                                    (uint32_t)claimed);
@@ -163,20 +173,21 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                                 &a);
             if (skind == XO_SRC_FIBER_NI) {
                 const float n1 = sl[1].n;
-                const float R = fiber_ni_launch(P.cfg.fiber, n1, x, a, px, py, pz, dx, dy,
+                const xo_source_fiber &fib = lds_fresh(sev.fiber);
+                const float R = fiber_ni_launch(fib, n1, x, a, px, py, pz, dx, dy,
                                                 dz, w);
-                if (P.cfg.specular.kind != XO_DET_NONE) {
+                if (use_specular) {
                     float rx, ry, rz;
-                    fiber_specular_direction(dx, dy, dz, n1, P.cfg.fiber.n, rx, ry, rz);
-                    detector_deposit(P.cfg.specular, bins, px, py, rx, ry, rz, R);
+                    fiber_specular_direction(dx, dy, dz, n1, fib.n, rx, ry, rz);
+                    detector_deposit(lds_fresh(sev.specular), bins, px, py, rx, ry, rz, R);
                 }
             } else if (skind == XO_SRC_FIBER) {
-                const float R = fiber_launch(P.cfg.fiber, P.cfg.fiber_axis, sl[1].n, x, a, px,
+                const float R = fiber_launch(lds_fresh(sev.fiber), lds_fresh(sev.fiber_axis), sl[1].n, x, a, px,
                                              py, pz, dx, dy, dz, w);
-                detector_deposit(P.cfg.specular, bins, px, py, dx, dy, dz, R);
+                detector_deposit(lds_fresh(sev.specular), bins, px, py, dx, dy, dz, R);
             } else {
                 /* Line source, xopto/mcml/mcsource/line.py:95-110 */
-                const xo_source_line &src = P.cfg.source;
+                const xo_source_line &src = lds_fresh(sev.line);
                 w = 1.0f - src.reflectance;
                 px = src.position[0];
                 py = src.position[1];
@@ -185,7 +196,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 dy = src.direction_sample[1];
                 dz = src.direction_sample[2];
                 if (static_sched)
-                    detector_deposit(P.cfg.specular, bins, px, py, src.direction_reflected[0],
+                    detector_deposit(lds_fresh(sev.specular), bins, px, py, src.direction_reflected[0],
                                      src.direction_reflected[1], src.direction_reflected[2],
                                      src.reflectance);
             }
@@ -224,7 +235,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 const float back =
                     (mua != 0.0f) ? step - (-xo_logf(1.0f - rng_next(x, a) * fraction)) / mua
                                   : 0.0f;
-                if (fluence_deposit(F, P.accum, px - back * dx, py - back * dy, pz - back * dz,
+                if (fluence_deposit(lds_fresh(sev.fluence), P.accum, px - back * dx, py - back * dy, pz - back * dz,
                                     deposit))
                     st[XO_STAT_DEPOSITS]++;
             }
@@ -292,9 +303,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                     bool done = false;
                     if (!(layer > 0 && layer < (int32_t)nl - 1)) {
                         if (layer <= 0)
-                            detector_deposit(P.cfg.top, bins, px, py, dx, dy, dz, w);
+                            detector_deposit(lds_fresh(sev.top), bins, px, py, dx, dy, dz, w);
                         else
-                            detector_deposit(P.cfg.bottom, bins, px, py, dx, dy, dz, w);
+                            detector_deposit(lds_fresh(sev.bottom), bins, px, py, dx, dy, dz, w);
                         done = true;
                     }
                     end_step(done);
@@ -338,7 +349,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 const float deposit = w;
                 w -= deposit;
                 if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-                if (use_fluence && fluence_deposit(F, P.accum, px, py, pz, deposit))
+                if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit))
                     st[XO_STAT_DEPOSITS]++;
                 done = true;
             } else {
@@ -346,7 +357,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                     const float deposit = w * L.mua_inv_mut;
                     w -= deposit;
                     if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-                    if (use_fluence && fluence_deposit(F, P.accum, px, py, pz, deposit))
+                    if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit))
                         st[XO_STAT_DEPOSITS]++;
                 }
                 PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
@@ -435,6 +446,7 @@ template <int PF, int METHOD, bool TRACE>
 __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ BlockStats sstats;
+    __shared__ VoxEventCfgL sev;
     MaterialL *sm = reinterpret_cast<MaterialL *>(smem);
     unsigned long long *sbins = reinterpret_cast<unsigned long long *>(
         smem + XO_MAX_MATERIALS * sizeof(MaterialL));
@@ -462,6 +474,14 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK) sbins[i] = 0ull;
     for (uint32_t i = tid; i < P.lds_lut; i += XO_BLOCK) slut[i] = P.lut[i];
     if (tid < XO_STAT_COUNT) sstats.v[tid] = 0ull;
+    if (tid == 0) {
+        sev.top = P.cfg.top;
+        sev.bottom = P.cfg.bottom;
+        sev.specular = P.cfg.specular;
+        sev.fluence = P.cfg.fluence;
+        sev.line = P.cfg.line;
+        sev.gaussian = P.cfg.gaussian;
+    }
     __syncthreads();
 
     const Bins bins{P.lds_bins ? sbins : nullptr, P.accum};
@@ -475,8 +495,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         g.size[k] = P.cfg.voxels.size[k];
         g.shape[k] = P.cfg.voxels.shape[k];
     }
-    const xo_fluence &F = P.cfg.fluence;
-    const bool use_fluence = F.enabled != 0;
+    const bool use_fluence = P.cfg.fluence.enabled != 0;
 
     const uint32_t gid = blockIdx.x * XO_BLOCK + tid;
     const bool lane_ok = gid < P.lanes;
@@ -534,8 +553,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                               pkt_index, retire, claimed);
             if (retire) active = false;
             if (claimed && line_src && lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
-                const xo_source_line &src = P.cfg.line;
-                detector_deposit_n(P.cfg.specular, bins, src.position[0], src.position[1],
+                const xo_source_line &src = lds_fresh(sev.line);
+                detector_deposit_n(lds_fresh(sev.specular), bins, src.position[0], src.position[1],
                                    src.direction_reflected[0], src.direction_reflected[1],
                                    src.direction_reflected[2], src.reflectance,
                                    (uint32_t)claimed);
@@ -547,19 +566,19 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                                 &a);
             if (line_src) {
                 /* xopto/mcvox/mcsource/line.py:84-104 */
-                const xo_source_line &src = P.cfg.line;
+                const xo_source_line &src = lds_fresh(sev.line);
                 w = 1.0f - src.reflectance;
                 for (int k = 0; k < 3; ++k) {
                     p[k] = src.position[k];
                     d[k] = src.direction_sample[k];
                 }
                 if (static_sched)
-                    detector_deposit(P.cfg.specular, bins, p[0], p[1],
+                    detector_deposit(lds_fresh(sev.specular), bins, p[0], p[1],
                                      src.direction_reflected[0], src.direction_reflected[1],
                                      src.direction_reflected[2], src.reflectance);
             } else {
                 /* xopto/mcvox/mcsource/gaussianbeam.py:112-189 */
-                const xo_source_gaussian &src = P.cfg.gaussian;
+                const xo_source_gaussian &src = lds_fresh(sev.gaussian);
                 float specular_r = 0.0f;
                 float r = __builtin_sqrtf(-2.0f * xo_logf(1.0f - rng_next(x, a)));
                 r = fminf(r, src.clip);
@@ -612,7 +631,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                     specular_r = reflectance(n_out, n_in,
                                              nrm[0] * dir[0] + nrm[1] * dir[1] + nrm[2] * dir[2],
                                              cos_critical(n_out, n_in));
-                    detector_deposit(P.cfg.specular, bins, p[0], p[1], dir[0], dir[1], dir[2],
+                    detector_deposit(lds_fresh(sev.specular), bins, p[0], p[1], dir[0], dir[1], dir[2],
                                      specular_r);
                 } else {
                     for (int k = 0; k < 3; ++k) p[k] = g.tl[k];
@@ -677,7 +696,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 const float back =
                     (mua != 0.0f) ? d_ok - (-xo_logf(1.0f - rng_next(x, a) * fraction)) / mua
                                   : 0.0f;
-                if (fluence_deposit(F, P.accum, p[0] - back * d[0], p[1] - back * d[1],
+                if (fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0] - back * d[0], p[1] - back * d[1],
                                     p[2] - back * d[2], deposit))
                     st[XO_STAT_DEPOSITS]++;
             }
@@ -743,9 +762,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 st[XO_STAT_CROSSINGS]++;
                 if (escaping) {
                     if (vi[2] < 0)
-                        detector_deposit(P.cfg.top, bins, p[0], p[1], d[0], d[1], d[2], w);
+                        detector_deposit(lds_fresh(sev.top), bins, p[0], p[1], d[0], d[1], d[2], w);
                     else if (vi[2] >= g.shape[2])
-                        detector_deposit(P.cfg.bottom, bins, p[0], p[1], d[0], d[1], d[2], w);
+                        detector_deposit(lds_fresh(sev.bottom), bins, p[0], p[1], d[0], d[1], d[2], w);
                     done = true;
                 }
             }
@@ -755,7 +774,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
             const float deposit = w;
             w -= deposit;
             if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-            if (use_fluence && fluence_deposit(F, P.accum, p[0], p[1], p[2], deposit))
+            if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0], p[1], p[2], deposit))
                 st[XO_STAT_DEPOSITS]++;
             done = true;
         } else {
@@ -764,7 +783,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 w -= deposit;
                 if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                 /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
-                if (use_fluence && fluence_deposit(F, P.accum, p[0], p[1], p[2], deposit))
+                if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0], p[1], p[2], deposit))
                     st[XO_STAT_DEPOSITS]++;
             }
             PfL pf{M.g, M.beta, M.la, M.lb, M.lc, M.loff, M.lsize_m1};

@@ -11,7 +11,10 @@ their last items) depends on the chunk but not on the batch size. The two
 runs give rate and tail, the model predicts the whole batch, and the
 candidate with the shortest predicted time wins. A plain "fastest
 calibration run" would pick chunks far too small: the tail weighs much
-more in a short run. Results are cached per device (CU count) and batch
+more in a short run. The model can still misjudge the tail of a large
+chunk, so the two best predictions of different chunks are then timed on
+the whole batch (when it holds at most VERIFY_PACKETS packets) and the
+faster one is kept. Results are cached per device (CU count) and batch
 signature in pyxopto_amd/data/autotune.json so later runs skip the sweep.
 """
 import hashlib
@@ -26,6 +29,7 @@ CANDIDATES = [(chunk, block, bpc)
               for bpc in (0,)]
 CACHE = os.path.join(N.DATA_DIR, 'autotune.json')
 CALIBRATION_PACKETS = 200_000_000
+VERIFY_PACKETS = 2_000_000_000
 
 
 def _signature(arr) -> str:
@@ -110,7 +114,18 @@ def tuned_opts(batch, packed, force: bool = False):
     else:
         res = sweep(ctx, arr, lut, naccum, base)
         _, chunk, block, bpc = res[0][:4]
-        cache[key] = {'best': [chunk, block, bpc],
-                      'sweep': [list(r) for r in res]}
+        entry = {'sweep': [list(r) for r in res]}
+        if _packets(arr) <= VERIFY_PACKETS:
+            finalists = [res[0]] + [r for r in res[1:] if r[1] != res[0][1]][:1]
+            timed = []
+            for r in finalists:
+                o = batch.opts(*r[1:4])
+                _, rep = ctx.run_batch(arr, lut, o, naccum)
+                timed.append((rep.kernel_ms, r[1], r[2], r[3]))
+            timed.sort()
+            _, chunk, block, bpc = timed[0]
+            entry['verified'] = [list(t) for t in timed]
+        entry['best'] = [chunk, block, bpc]
+        cache[key] = entry
         _store_cache(cache)
     return batch.opts(chunk, block, bpc)
