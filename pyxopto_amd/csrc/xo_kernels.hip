@@ -442,6 +442,10 @@ XO_DEV bool box_intersect(const VoxGeom &g, const float pos[3], const float dir[
     return tmax >= tmin && tmax >= 0.0f;
 }
 
+/* Parked interactions of the voxel kernel are handled once this many lanes
+ * of a wave hold one, or after kVoxDeferWait iterations. */
+constexpr uint32_t kVoxDeferMin = 24, kVoxDeferWait = 8;
+
 template <int PF, int METHOD, bool TRACE>
 __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -531,6 +535,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     uint32_t pkt_steps = 0;
     TraceState ts{0, 0, 0, 0.0f};
     const xo_trace &T = P.trace;
+    bool pend_int = false; /* the lane's step ended on an interaction      */
+    uint32_t iwait = 0;    /* iterations parked interactions have waited   */
 
     for (;;) {
         bool want = active && !alive;
@@ -657,9 +663,87 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
             }
         }
         if (!__any(alive)) break;
+        /* ---- interactions are deferred: a lane whose step ends inside its
+         * voxel parks the interaction (absorption, scattering, roulette),
+         * and the parked interactions of the wave are handled together once
+         * kVoxDeferMin lanes hold one, every live lane holds one, or they
+         * waited kVoxDeferWait iterations. Most steps end on a voxel face
+         * (a few instructions when the material does not change), so
+         * otherwise the long interaction code would run in nearly every
+         * iteration for a handful of lanes. Each packet's arithmetic and
+         * draws are those of the reference's loop; only when the wave
+         * issues them changes. The decision is taken by every lane of the
+         * wave, so the wait count stays wave-uniform. ---- */
+        const uint32_t n_int = (uint32_t)__builtin_popcountll(__ballot(alive && pend_int));
+        const uint32_t n_alive = (uint32_t)__builtin_popcountll(__ballot(alive));
+        const bool run_int = n_int != 0 && (n_int >= kVoxDeferMin || n_int == n_alive ||
+                                            iwait >= kVoxDeferWait);
+        iwait = (n_int != 0 && !run_int) ? iwait + 1u : 0u;
         if (!alive) continue;
 
-        /* ---- one iteration of while(!done), mcvox.template.c:748-1012 ---- */
+        /* end of one iteration of while(!done), mcvox.template.c:865-1005 */
+        auto end_iteration = [&](bool done) {
+            /* microscopic Beer-Lambert: roulette after the boundary or the
+             * scattering, whether or not the packet has left
+             * (mcvox.template.c:865-881) */
+            if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
+                done = true;
+            float ex = p[0] - spx, ey = p[1] - spy, ez = p[2] - spz;
+            if (ex * ex + ey * ey + ez * ez > rmax2 || w <= 0.0f) {
+                done = true;
+                if (TRACE) ts.ev |= XO_EVENT_ESCAPED;
+            }
+            if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
+                done = true;
+                st[XO_STAT_TRUNCATED]++;
+            }
+            if (TRACE) {
+                /* mcvox.template.c:989-1005 */
+                if (done) ts.ev |= XO_EVENT_TERMINATED;
+                if (T.options == XO_TRACE_ALL || ((T.options & XO_TRACE_END) && done))
+                    trace_event(T, P.trace_data, ts, p[0], p[1], p[2], d[0], d[1], d[2], w);
+                ts.ev = 0u;
+                if (done) P.trace_n[ts.pkt] = (int32_t)ts.count;
+            }
+            if (done) alive = false;
+        };
+
+        if (run_int && pend_int) {
+            /* the parked interaction, mcvox.template.c:925-968 */
+            const MaterialL &M = sm[mi];
+            bool done = false;
+            if (METHOD == XO_METHOD_AR && rng_next(x, a) < M.mua_inv_mut) {
+                /* albedo rejection: the whole weight is absorbed here,
+                 * mcvox.template.c:925-944 */
+                const float deposit = w;
+                w -= deposit;
+                if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
+                if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0], p[1], p[2], deposit))
+                    st[XO_STAT_DEPOSITS]++;
+                done = true;
+            } else {
+                if (METHOD == XO_METHOD_AW) {
+                    float deposit = w * M.mua_inv_mut;
+                    w -= deposit;
+                    if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
+                    /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
+                    if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0], p[1], p[2], deposit))
+                        st[XO_STAT_DEPOSITS]++;
+                }
+                PfL pf{M.g, M.beta, M.la, M.lb, M.lc, M.loff, M.lsize_m1};
+                float xi_fi;
+                float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
+                scatter_direction(d[0], d[1], d[2], ct, xi_fi);
+                if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
+                if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
+                    done = true;
+            }
+            pend_int = false;
+            end_iteration(done);
+        }
+        if (!alive || pend_int) continue;
+
+        /* ---- one step of while(!done), mcvox.template.c:748-924 ---- */
         const MaterialL &M = sm[mi];
         st[XO_STAT_STEPS]++;
         float step;
@@ -768,55 +852,10 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                     done = true;
                 }
             }
-        } else if (METHOD == XO_METHOD_AR && rng_next(x, a) < M.mua_inv_mut) {
-            /* albedo rejection: the whole weight is absorbed here,
-             * mcvox.template.c:925-944 */
-            const float deposit = w;
-            w -= deposit;
-            if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-            if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0], p[1], p[2], deposit))
-                st[XO_STAT_DEPOSITS]++;
-            done = true;
+            end_iteration(done);
         } else {
-            if (METHOD == XO_METHOD_AW) {
-                float deposit = w * M.mua_inv_mut;
-                w -= deposit;
-                if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-                /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
-                if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0], p[1], p[2], deposit))
-                    st[XO_STAT_DEPOSITS]++;
-            }
-            PfL pf{M.g, M.beta, M.la, M.lb, M.lc, M.loff, M.lsize_m1};
-            float xi_fi;
-            float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
-            scatter_direction(d[0], d[1], d[2], ct, xi_fi);
-            if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
-            if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
-                done = true;
+            pend_int = true;
         }
-        /* microscopic Beer-Lambert: roulette after the boundary or the
-         * scattering, whether or not the packet has left
-         * (mcvox.template.c:865-881) */
-        if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
-            done = true;
-        float ex = p[0] - spx, ey = p[1] - spy, ez = p[2] - spz;
-        if (ex * ex + ey * ey + ez * ez > rmax2 || w <= 0.0f) {
-            done = true;
-            if (TRACE) ts.ev |= XO_EVENT_ESCAPED;
-        }
-        if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
-            done = true;
-            st[XO_STAT_TRUNCATED]++;
-        }
-        if (TRACE) {
-            /* mcvox.template.c:989-1005 */
-            if (done) ts.ev |= XO_EVENT_TERMINATED;
-            if (T.options == XO_TRACE_ALL || ((T.options & XO_TRACE_END) && done))
-                trace_event(T, P.trace_data, ts, p[0], p[1], p[2], d[0], d[1], d[2], w);
-            ts.ev = 0u;
-            if (done) P.trace_n[ts.pkt] = (int32_t)ts.count;
-        }
-        if (done) alive = false;
     }
 
     if (lane_ok) {
