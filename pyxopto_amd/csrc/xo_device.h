@@ -41,6 +41,42 @@ struct alignas(16) MaterialL {
     float mua, mus; /* microscopic Beer-Lambert steps */
 };
 
+/* Correctly rounded square root for x = +0 or 2^-96 <= x < 2^96, in 9
+ * instructions: the hardware estimate s, then s - 1 ulp or s + 1 ulp chosen
+ * from the signs of the residuals x - s' s (Markstein's correction, the
+ * same operations in the same order as the compiler's general expansion,
+ * whose rescaling of inputs below 2^-96 and special-value pass-through do
+ * nothing in this range). Checked against sqrtf for every float of the
+ * range. Callers: arguments 1 - c*c with |c| <= 1 (0 or >= 2^-24) and
+ * squared norms of unit vectors. */
+XO_DEV float sqrt_in_range(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r_dn = __builtin_fmaf(-s_dn, s, x);
+    const float r_up = __builtin_fmaf(-s_up, s, x);
+    float r = (0.0f >= r_dn) ? s_dn : s;
+    r = (0.0f < r_up) ? s_up : r;
+    return r;
+}
+
+/* Correctly rounded a / b for normal a, b with 2^-60 <= |a|, |b| <= 2^60:
+ * reciprocal estimate, one Newton step on it, quotient, two residual
+ * corrections (the compiler's general division sequence without the
+ * operand scaling and the special-case fix-up, which leave such operands
+ * unchanged). Checked against the division on 2^32 random operand pairs of
+ * that range. Callers: 1 / |d| for a direction d of length about 1. */
+XO_DEV float div_in_range(float a, float b) {
+    float y = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    float q = a * y;
+    float r = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+
 /* fp_random_single (mcbase.template.c:1576-1594): one MWC step. The state
  * update is one v_mad_u64_u32; (float)(uint32)x / (float)0xFFFFFFFF divides
  * by 2^32 exactly, which is the multiply below. */
@@ -264,7 +300,7 @@ XO_DEV float pf_sample(const PfL &pf, const float *lut, uint64_t &x, uint32_t a,
 XO_DEV void scatter_direction(float &dx, float &dy, float &dz, float ct,
                               float xi_fi) {
     float sin_fi, cos_fi;
-    float st = __builtin_sqrtf(1.0f - ct * ct);
+    float st = sqrt_in_range(1.0f - ct * ct);
     xo_sincos_turns(xi_fi, &sin_fi, &cos_fi);
     float stcf = st * cos_fi;
     float stsf = st * sin_fi;
@@ -274,12 +310,12 @@ XO_DEV void scatter_direction(float &dx, float &dy, float &dz, float ct,
         dy = stsf;
         dz = copysignf(ct, pz * ct);
     } else {
-        float k = __builtin_sqrtf(1.0f - pz * pz);
+        float k = sqrt_in_range(1.0f - pz * pz);
         dx = (stcf * px * pz - stsf * py) / k + px * ct;
         dy = (stcf * py * pz + stsf * px) / k + py * ct;
         dz = (-stcf) * k + pz * ct;
     }
-    float inv = 1.0f / __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    float inv = div_in_range(1.0f, sqrt_in_range(dx * dx + dy * dy + dz * dz));
     dx = dx * inv;
     dy = dy * inv;
     dz = dz * inv;
@@ -287,7 +323,7 @@ XO_DEV void scatter_direction(float &dx, float &dy, float &dz, float ct,
 
 /* cos_critical (mcbase.template.c:1229-1231). */
 XO_DEV float cos_critical(float n1, float n2) {
-    return (n1 > n2) ? __builtin_sqrtf(1.0f - (n2 * n2) / (n1 * n1)) : 0.0f;
+    return (n1 > n2) ? sqrt_in_range(1.0f - (n2 * n2) / (n1 * n1)) : 0.0f;
 }
 
 /* reflectance (mcbase.template.c:1244-1290). */
@@ -297,10 +333,10 @@ XO_DEV float reflectance(float n1, float n2, float cos1, float cc) {
     if (n1 == n2) return 0.0f;
     if (cos1 > cc) {
         float n1_d_n2 = n1 / n2;
-        float sin1 = __builtin_sqrtf(1.0f - cos1 * cos1);
+        float sin1 = sqrt_in_range(1.0f - cos1 * cos1);
         if (cos1 >= kCos0) sin1 = 0.0f;
         float sin2 = fminf(1.0f, n1_d_n2 * sin1);
-        float cos2 = __builtin_sqrtf(1.0f - sin2 * sin2);
+        float cos2 = sqrt_in_range(1.0f - sin2 * sin2);
         float n_cos1 = n1_d_n2 * cos1;
         float n_cos2 = n1_d_n2 * cos2;
         float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
@@ -459,10 +495,10 @@ XO_DEV void mcml_boundary(const LayerL *sl, int32_t &layer, int32_t next,
             return;
         }
         float n1_d_n2 = n1 / n2;
-        float sin1 = __builtin_sqrtf(1.0f - cos1 * cos1);
+        float sin1 = sqrt_in_range(1.0f - cos1 * cos1);
         if (cos1 >= kCos0) sin1 = 0.0f;
         float sin2 = fminf(1.0f, n1_d_n2 * sin1);
-        float cos2 = __builtin_sqrtf(1.0f - sin2 * sin2);
+        float cos2 = sqrt_in_range(1.0f - sin2 * sin2);
         float n_cos1 = n1_d_n2 * cos1;
         float n_cos2 = n1_d_n2 * cos2;
         float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
