@@ -603,6 +603,14 @@ int xo_probe_atomic_u64(xo_ctx *ctx, uint64_t array_bytes, uint64_t natomics,
  * counterpart in the reference. */
 int xo_probe_valu(xo_ctx *ctx, uint32_t iters, double *ms, double *lane_ops);
 
+/* Check of the transport kernels' range-restricted correctly rounded square
+ * root (every float +0 or in [2^-96, 2^96)) and division (ndiv random
+ * operand pairs with exponents in [-60, 59]) against the general correctly
+ * rounded operations, on the device: the count of square roots checked and
+ * the mismatches of each. No counterpart in the reference. */
+int xo_probe_cr_math(xo_ctx *ctx, uint64_t ndiv, uint64_t seed, uint64_t *sqrt_checked,
+                     uint64_t *sqrt_mismatches, uint64_t *div_mismatches);
+
 /* ---------------- reflectance surrogate (device) ---------------- */
 
 /* A fully connected ReLU network (widths dims[0..nlayers]) trained and

@@ -99,13 +99,13 @@ def max_threads() -> int:
 
 
 def kernel_math(which: str, values):
-    """The kernels' log ('log'), sin/cos of turns ('sincos', returns a
-    pair), cube root ('cbrt') or exponential ('exp') evaluated on the
-    host."""
+    """The kernels' log ('log'), log of a uniform deviate ('log_unit'),
+    sin/cos of turns ('sincos', returns a pair), cube root ('cbrt') or
+    exponential ('exp') evaluated on the host."""
     v = np.ascontiguousarray(values, dtype=np.float32)
     out = np.empty_like(v)
     out2 = np.empty_like(v)
-    code = {'log': 0, 'sincos': 1, 'cbrt': 2, 'exp': 3}[which]
+    code = {'log': 0, 'sincos': 1, 'cbrt': 2, 'exp': 3, 'log_unit': 4}[which]
     rc = lib().oracle_math(c_int(code), c_uint32(v.size), _p(v, c_float),
                            _p(out, c_float), _p(out2, c_float))
     if rc != 0:

@@ -1232,6 +1232,8 @@ int oracle_math(int which, uint32_t n, const float *in, float *out,
             out[i] = xo_cbrtf(in[i]);
         } else if (which == 3) {
             out[i] = xo_expf(in[i]);
+        } else if (which == 4) {
+            out[i] = xo_logf_unit(in[i]);
         } else {
             return -1;
         }

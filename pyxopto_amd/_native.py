@@ -297,6 +297,9 @@ PROTOTYPES = {
                                     POINTER(c_double), POINTER(c_uint64)]),
     'xo_probe_valu': (c_int, [c_void_p, c_uint32, POINTER(c_double),
                               POINTER(c_double)]),
+    'xo_probe_cr_math': (c_int, [c_void_p, c_uint64, c_uint64,
+                                 POINTER(c_uint64), POINTER(c_uint64),
+                                 POINTER(c_uint64)]),
 }
 
 _lib = None
@@ -559,6 +562,17 @@ class Context:
         check(lib().xo_probe_valu(self._h, c_uint32(iters), ctypes.byref(ms),
                                   ctypes.byref(ops)), 'xo_probe_valu')
         return ms.value, ops.value
+
+    def probe_cr_math(self, ndiv: int = 1 << 30, seed: int = 1):
+        """(square roots checked, their mismatches, division mismatches of
+        ndiv random pairs) of the range-restricted correctly rounded
+        operations against the general ones."""
+        chk, bs, bd = c_uint64(), c_uint64(), c_uint64()
+        check(lib().xo_probe_cr_math(self._h, c_uint64(int(ndiv)),
+                                     c_uint64(int(seed)), ctypes.byref(chk),
+                                     ctypes.byref(bs), ctypes.byref(bd)),
+              'xo_probe_cr_math')
+        return int(chk.value), int(bs.value), int(bd.value)
 
     def probe_atomic_u64(self, array_bytes: int, natomics: int,
                          seed: int = 1):

@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams P) {
             /* (B) */
             const LayerL L = layers[layer];
             st[XO_STAT_STEPS]++;
-            float step = -xo_logf(rng_next(x, a)) * L.inv_mut;
+            float step = -xo_logf_unit(rng_next(x, a)) * L.inv_mut;
             step = fminf(step, XO_FLT_MAX);
             const float zend = __builtin_fmaf(step, dz, pz);
             if (zend < L.top || zend >= L.bottom) {

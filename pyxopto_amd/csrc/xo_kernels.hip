@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
             if (use_fluence) {
                 const float back =
-                    (mua != 0.0f) ? step - (-xo_logf(1.0f - rng_next(x, a) * fraction)) / mua
+                    (mua != 0.0f) ? step - (-xo_logf_unit(1.0f - rng_next(x, a) * fraction)) / mua
                                   : 0.0f;
                 if (fluence_deposit(lds_fresh(sev.fluence), P.accum, px - back * dx, py - back * dy, pz - back * dz,
                                     deposit))
@@ -322,9 +322,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             st[XO_STAT_STEPS]++;
             float step;
             if (METHOD == XO_METHOD_MBL)
-                step = (-xo_logf(rng_next(x, a))) / L.mus; /* mcml.template.c:529-531 */
+                step = (-xo_logf_unit(rng_next(x, a))) / L.mus; /* mcml.template.c:529-531 */
             else
-                step = -xo_logf(rng_next(x, a)) * L.inv_mut;
+                step = -xo_logf_unit(rng_next(x, a)) * L.inv_mut;
             step = fminf(step, XO_FLT_MAX);
             const float zend = __builtin_fmaf(step, dz, pz);
             if (zend < L.top || zend >= L.bottom) {
@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 /* xopto/mcvox/mcsource/gaussianbeam.py:112-189 */
                 const xo_source_gaussian &src = lds_fresh(sev.gaussian);
                 float specular_r = 0.0f;
-                float r = __builtin_sqrtf(-2.0f * xo_logf(1.0f - rng_next(x, a)));
+                float r = __builtin_sqrtf(-2.0f * xo_logf_unit(1.0f - rng_next(x, a)));
                 r = fminf(r, src.clip);
                 float sin_fi, cos_fi;
                 xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
@@ -748,9 +748,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         st[XO_STAT_STEPS]++;
         float step;
         if (METHOD == XO_METHOD_MBL)
-            step = (-xo_logf(rng_next(x, a))) / M.mus; /* mcvox.template.c:763-766 */
+            step = (-xo_logf_unit(rng_next(x, a))) / M.mus; /* mcvox.template.c:763-766 */
         else
-            step = -xo_logf(rng_next(x, a)) * M.inv_mut;
+            step = -xo_logf_unit(rng_next(x, a)) * M.inv_mut;
         step = fminf(step, XO_FLT_MAX);
         /* mcsim_intersect, mcvox.template.c:173-196 */
         float dist[3];
@@ -778,7 +778,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
             if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
             if (use_fluence) {
                 const float back =
-                    (mua != 0.0f) ? d_ok - (-xo_logf(1.0f - rng_next(x, a) * fraction)) / mua
+                    (mua != 0.0f) ? d_ok - (-xo_logf_unit(1.0f - rng_next(x, a) * fraction)) / mua
                                   : 0.0f;
                 if (fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0] - back * d[0], p[1] - back * d[1],
                                     p[2] - back * d[2], deposit))

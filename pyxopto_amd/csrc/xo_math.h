@@ -98,6 +98,39 @@ XO_HD float xo_logf(float x) {
     return XO_FMA(fe, 0.693359375f, r);
 }
 
+/* xo_logf for x = +0 or a normal x <= 1, such as a uniform deviate
+ * u * 2^-32 or 1 minus one: the same operations without the infinity and
+ * subnormal branches, so the same result. */
+XO_HD float xo_logf_unit(float x) {
+    if (!(x > 0.0f)) return (x == 0.0f) ? -XO_INF_F : XO_NAN_F;
+    uint32_t bits = xo_f2u(x);
+    int32_t e = (int32_t)((bits >> 23) & 0xffu) - 126;
+    float m = xo_u2f((bits & 0x007fffffu) | 0x3f000000u); /* [0.5, 1) */
+    float f;
+    if (m < 0.707106781186547524f) {
+        e -= 1;
+        f = (m + m) - 1.0f;
+    } else {
+        f = m - 1.0f;
+    }
+    float z = f * f;
+    float p = 7.0376836292e-2f;
+    p = XO_FMA(p, f, -1.1514610310e-1f);
+    p = XO_FMA(p, f, 1.1676998740e-1f);
+    p = XO_FMA(p, f, -1.2420140846e-1f);
+    p = XO_FMA(p, f, 1.4249322787e-1f);
+    p = XO_FMA(p, f, -1.6668057665e-1f);
+    p = XO_FMA(p, f, 2.0000714765e-1f);
+    p = XO_FMA(p, f, -2.4999993993e-1f);
+    p = XO_FMA(p, f, 3.3333331174e-1f);
+    float y = (p * f) * z;
+    float fe = (float)e;
+    y = XO_FMA(fe, -2.12194440e-4f, y);
+    y = XO_FMA(z, -0.5f, y);
+    float r = f + y;
+    return XO_FMA(fe, 0.693359375f, r);
+}
+
 /* Exponential: n = round(x / ln 2), r = x - n ln 2 in two Cody-Waite
  * parts, Cephes' degree-5 polynomial (S. L. Moshier, Cephes Math Library,
  * expf.c) for e^r, scaled by 2^n through the exponent field (two steps

@@ -4,7 +4,8 @@
  * kernels), a streaming device-to-device copy (HBM bandwidth) and
  * random-address u64 atomic adds over a large array (the HBM/L2 atomic
  * rate the fluence and detector deposits run into). No counterpart in the
- * reference.
+ * reference. Also the check of the range-restricted correctly rounded
+ * square root and division of xo_device.h against the general ones.
  */
 #include <hip/hip_runtime.h>
 
@@ -13,6 +14,7 @@
 
 #include "../../include/xomc.h"
 #include "xo_ctx.h"
+#include "xo_device.h"
 #include "xo_seedhash.h"
 
 namespace {
@@ -50,6 +52,42 @@ __global__ void __launch_bounds__(256) valu_kernel(float *out, uint32_t iters, f
     float s = 0.f;
     for (int k = 0; k < 8; ++k) s += v[k];
     out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+/* Every float x = +0 or 2^-96 <= x < 2^96: sqrt_in_range(x) against the
+ * correctly rounded sqrtf. counts[0] checked, counts[1] mismatches. */
+__global__ void __launch_bounds__(256) cr_sqrt_kernel(unsigned long long *counts) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long checked = 0, bad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 0x7F800000ull;
+         i += stride) {
+        const float x = __uint_as_float((uint32_t)i);
+        if (!(i == 0 || (x >= 0x1p-96f && x < 0x1p96f))) continue;
+        ++checked;
+        if (__float_as_uint(xo::sqrt_in_range(x)) != __float_as_uint(__builtin_sqrtf(x))) ++bad;
+    }
+    atomicAdd(&counts[0], checked);
+    if (bad) atomicAdd(&counts[1], bad);
+}
+
+/* A float of random sign and mantissa with exponent in [-60, 59]. */
+__device__ __forceinline__ float cr_operand(uint64_t h) {
+    const uint32_t e = (uint32_t)((h >> 23) % 120u) + 67u;
+    return __uint_as_float(((uint32_t)(h >> 40) & 1u) << 31 | e << 23 | ((uint32_t)h & 0x7FFFFFu));
+}
+
+/* n random operand pairs: div_in_range(a, b) against a / b; counts[2]
+ * mismatches. */
+__global__ void __launch_bounds__(256) cr_div_kernel(uint64_t n, uint64_t seed,
+                                                     unsigned long long *counts) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float a = cr_operand(xo_mix64(seed + 2 * i + 1));
+        const float b = cr_operand(xo_mix64(seed + 2 * i + 2));
+        if (__float_as_uint(xo::div_in_range(a, b)) != __float_as_uint(a / b)) ++bad;
+    }
+    if (bad) atomicAdd(&counts[2], bad);
 }
 
 } // namespace
@@ -181,6 +219,42 @@ int xo_probe_atomic_u64(xo_ctx *ctx, uint64_t array_bytes, uint64_t natomics, ui
     *checksum = sum;
     (void)hipFree(arr);
     return rc;
+}
+
+int xo_probe_cr_math(xo_ctx *ctx, uint64_t ndiv, uint64_t seed, uint64_t *sqrt_checked,
+                     uint64_t *sqrt_mismatches, uint64_t *div_mismatches) {
+    xo::clear_error();
+    if (!ctx || !sqrt_checked || !sqrt_mismatches || !div_mismatches) {
+        xo::set_error("xo_probe_cr_math: null argument");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    unsigned long long *counts = nullptr;
+    XO_HIP(hipMalloc(&counts, 3 * sizeof(unsigned long long)));
+    unsigned long long host[3] = {0, 0, 0};
+    hipError_t e = hipMemsetAsync(counts, 0, sizeof(host), ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(cr_sqrt_kernel, dim3(ctx->cus * 32), dim3(256), 0, ctx->stream, counts);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && ndiv) {
+        hipLaunchKernelGGL(cr_div_kernel, dim3(ctx->cus * 32), dim3(256), 0, ctx->stream, ndiv,
+                           seed, counts);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host, counts, sizeof(host), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(counts);
+    if (e != hipSuccess) {
+        xo::set_error("xo_probe_cr_math: %s", hipGetErrorString(e));
+        return -1;
+    }
+    *sqrt_checked = host[0];
+    *sqrt_mismatches = host[1];
+    *div_mismatches = host[2];
+    return 0;
 }
 
 } // extern "C"

@@ -78,3 +78,19 @@ def test_expf_accuracy():
     e = O.kernel_math('exp', np.float32([0.0, np.inf, -np.inf, 89.0, -104.0]))
     np.testing.assert_array_equal(e, np.float32([1.0, np.inf, 0.0, np.inf, 0.0]))
     assert np.isnan(O.kernel_math('exp', np.float32([np.nan]))[0])
+
+
+def test_log_unit_equals_log_on_its_whole_domain():
+    """xo_logf_unit (the log of uniform deviates in the transport loops)
+    drops xo_logf's infinity and subnormal branches: on +0 and every normal
+    float <= 1 it returns the same bits."""
+    lo, hi = 0x00800000, 0x3F800000          # 2^-126 .. 1.0
+    step = 1 << 24
+    for start in range(lo, hi + 1, step):
+        u = np.arange(start, min(start + step, hi + 1), dtype=np.uint32)
+        x = u.view(np.float32)
+        a = O.kernel_math('log', x)
+        b = O.kernel_math('log_unit', x)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), start
+    z = np.float32([0.0])
+    assert O.kernel_math('log_unit', z)[0] == O.kernel_math('log', z)[0]
