@@ -118,6 +118,10 @@ int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint
     } else if (pf_kind != XO_PF_HG && pf_kind != XO_PF_MHG) {
         xo::set_error("unknown phase function kind %d", pf_kind);
         return -1;
+    } else if (!(pf.g >= -1.0f && pf.g <= 1.0f)) {
+        /* the Python classes clip g to [-1, 1]; the sampling relies on it */
+        xo::set_error("%s %u: anisotropy g = %g outside [-1, 1]", what, idx, (double)pf.g);
+        return -1;
     }
     return 0;
 }

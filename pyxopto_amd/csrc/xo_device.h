@@ -266,7 +266,10 @@ XO_DEV float pf_sample(const PfL &pf, const float *lut, uint64_t &x, uint32_t a,
     xi_fi = rng_next(x, a);
     if constexpr (PF == XO_PF_HG) {
         float g = pf.g;
-        float k = (1.0f - g * g) / (1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
+        /* |g| <= 1 (the host clips it): the numerator is +0 or >= 2^-24, the
+         * denominator 0 or in [2^-24, 2]; a 0 denominator gives NaN either
+         * way, which the clamp below turns into the same cosine */
+        float k = div_in_range(1.0f - g * g, 1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
         float ct = (1.0f + g * g - k * k) / (2.0f * g);
         if (g == 0.0f) ct = 1.0f - 2.0f * rng_next(x, a);
         return fmaxf(fminf(ct, 1.0f), -1.0f);
@@ -274,8 +277,8 @@ XO_DEV float pf_sample(const PfL &pf, const float *lut, uint64_t &x, uint32_t a,
         float g = pf.g;
         float ct;
         if (rng_next(x, a) <= pf.beta) {
-            float k = (1.0f - g * g) /
-                      (1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
+            /* as for Hg: |g| <= 1 */
+            float k = div_in_range(1.0f - g * g, 1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
             ct = (1.0f + g * g - k * k) / (2.0f * g);
             if (g == 0.0f) ct = 1.0f - 2.0f * rng_next(x, a);
         } else {

@@ -78,6 +78,50 @@ def test_static_schedule_bit_exact(name):
     ctx.close()
 
 
+def _extreme_anisotropy_sim(pf_of):
+    L = mc.mclayer
+    layers = L.Layers([
+        L.Layer(d=np.inf, n=1.0, mua=0.0, mus=0.0, pf=pf_of(0.0)),
+        L.Layer(d=2e-4, n=1.37, mua=100.0, mus=2e4, pf=pf_of(1.0)),
+        L.Layer(d=2e-4, n=1.40, mua=100.0, mus=2e4, pf=pf_of(-1.0)),
+        L.Layer(d=np.inf, n=1.45, mua=0.0, mus=0.0, pf=pf_of(0.0))])
+    return mc.Mc(layers, mc.mcsource.Line(), standard_detectors(),
+                 rnginit=XINIT)
+
+
+@pytest.mark.parametrize('kind', ['hg', 'mhg'])
+def test_static_schedule_bit_exact_at_g_plus_minus_one(kind):
+    """|g| = 1 makes the HG numerator 1 - g^2 zero and, for one extreme
+    deviate, the denominator too: the kernels' range-restricted division
+    must still give the oracle's bits."""
+    pf_of = (lambda g: mc.mcpf.Hg(g)) if kind == 'hg' else \
+        (lambda g: mc.mcpf.MHg(g, 0.9))
+    sim = _extreme_anisotropy_sim(pf_of)
+    print('case', kind, flush=True)
+    lanes, npk = 4096, 20000
+    cfg, lut, naccum = sim.pack()
+    ctx = N.Context(0, 0, XINIT)
+    gpu, rep = ctx.run_mcml(cfg, lut, npk, naccum, lanes,
+                            N.XO_FLAG_STATIC_SCHEDULE)
+    gx, _ = ctx.get_rng(lanes)
+    x, a = N.init_rng(lanes, XINIT)
+    ref, counts, rx = O.mcml_run(cfg, lut, x, a, npk, naccum, O.MATH_KERNEL)
+    assert int(rep.steps) == int(counts[0])
+    np.testing.assert_array_equal(gpu, ref)
+    np.testing.assert_array_equal(gx, rx)
+    ctx.close()
+
+
+def test_anisotropy_outside_unit_interval_is_rejected():
+    sim = _extreme_anisotropy_sim(lambda g: mc.mcpf.Hg(g))
+    cfg, lut, naccum = sim.pack()
+    cfg.layers[1].pf.g = 1.5
+    ctx = N.Context(0, 0, XINIT)
+    with pytest.raises(RuntimeError, match='anisotropy'):
+        ctx.run_mcml(cfg, lut, 1000, naccum, 1024, N.XO_FLAG_STATIC_SCHEDULE)
+    ctx.close()
+
+
 def test_static_schedule_global_bins_match_lds_bins():
     sim = sims()['hg_slab']
     cfg, lut, naccum = sim.pack()
