@@ -77,6 +77,14 @@ XO_DEV float div_in_range(float a, float b) {
     return __builtin_fmaf(r, y, q);
 }
 
+/* a / k for k = sqrt(1 - c^2) with |c| < 1 (so 2^-12 < k <= 1) and |a| <= 2:
+ * the range-restricted division when |a| >= 2^-60, the general one for
+ * smaller |a| (rare; it keeps the sign of a zero). Same bits either way. */
+XO_DEV float div_by_k(float a, float k) {
+    if (__builtin_expect(fabsf(a) >= 0x1p-60f, 1)) return div_in_range(a, k);
+    return a / k;
+}
+
 /* fp_random_single (mcbase.template.c:1576-1594): one MWC step. The state
  * update is one v_mad_u64_u32; (float)(uint32)x / (float)0xFFFFFFFF divides
  * by 2^32 exactly, which is the multiply below. */
@@ -314,8 +322,8 @@ XO_DEV void scatter_direction(float &dx, float &dy, float &dz, float ct,
         dz = copysignf(ct, pz * ct);
     } else {
         float k = sqrt_in_range(1.0f - pz * pz);
-        dx = (stcf * px * pz - stsf * py) / k + px * ct;
-        dy = (stcf * py * pz + stsf * px) / k + py * ct;
+        dx = div_by_k(stcf * px * pz - stsf * py, k) + px * ct;
+        dy = div_by_k(stcf * py * pz + stsf * px, k) + py * ct;
         dz = (-stcf) * k + pz * ct;
     }
     float inv = div_in_range(1.0f, sqrt_in_range(dx * dx + dy * dy + dz * dz));
