@@ -357,6 +357,7 @@ typedef struct xo_batch_opts {
                                    many of its lanes are idle (0: 4)          */
     uint32_t defer_min;         /* a wave handles its boundary events once
                                    this many lanes hold one (0: 12)           */
+    int32_t method;             /* enum xo_method of every instance           */
 } xo_batch_opts;
 
 /* ---------------- library ---------------- */

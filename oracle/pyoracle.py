@@ -70,7 +70,7 @@ def lib():
         l.oracle_batch_run.restype = c_int
         l.oracle_batch_run.argtypes = [
             POINTER(N.XoBatchInstance), c_uint32, POINTER(c_float),
-            POINTER(c_uint32), c_float, c_float, c_int, c_int, c_int,
+            POINTER(c_uint32), c_float, c_float, c_int, c_int, c_int, c_int,
             POINTER(c_uint64), c_uint64, POINTER(c_uint64)]
         l.oracle_pf_samples.restype = c_int
         l.oracle_pf_samples.argtypes = [
@@ -291,9 +291,10 @@ def mcvox_run_trace(cfg, voxels, lut, rng_x, rng_a, npackets: int,
 
 def batch_run(instances, lut, naccum: int, w_min=1e-4, lottery_chance=0.1,
               use_lottery=True, math_mode: int = MATH_KERNEL,
-              nthreads: int = 0):
+              nthreads: int = 0, method: int = 0):
     """A batch of layered instances with per-packet generators
-    (schedule independent). instances: ctypes array of XoBatchInstance."""
+    (schedule independent), every instance with the stepping method
+    `method` (xo_method). instances: ctypes array of XoBatchInstance."""
     table = np.ascontiguousarray(N.safeprime_table(N.XO_NUM_PRIMES)[1:])
     accum = np.zeros(naccum, dtype=np.uint64)
     counts = np.zeros(4, dtype=np.uint64)
@@ -304,7 +305,8 @@ def batch_run(instances, lut, naccum: int, w_min=1e-4, lottery_chance=0.1,
     rc = lib().oracle_batch_run(instances, c_uint32(len(instances)), lut_p,
                                 _p(table, c_uint32), c_float(w_min),
                                 c_float(lottery_chance), c_int(int(use_lottery)),
-                                c_int(math_mode), c_int(nthreads),
+                                c_int(int(method)), c_int(math_mode),
+                                c_int(nthreads),
                                 _p(accum, c_uint64), c_uint64(naccum),
                                 _p(counts, c_uint64))
     if rc != 0:

@@ -1165,9 +1165,9 @@ static int mcvox_run(const xo_mcvox_cfg *cfg, const int32_t *voxels,
  * seed-table rows 1..499999. */
 int oracle_batch_run(const xo_batch_instance *inst, uint32_t ninst,
                      const float *lut, const uint32_t *table, float w_min,
-                     float lottery_chance, int use_lottery, int math_mode,
-                     int nthreads, uint64_t *accum, uint64_t naccum,
-                     uint64_t *counts_out) {
+                     float lottery_chance, int use_lottery, int method,
+                     int math_mode, int nthreads, uint64_t *accum,
+                     uint64_t naccum, uint64_t *counts_out) {
     memset(accum, 0, naccum * sizeof(uint64_t));
     xo_mcml_cfg *cfgs = (xo_mcml_cfg *)calloc(ninst, sizeof(xo_mcml_cfg));
     if (!cfgs) return -1;
@@ -1183,6 +1183,7 @@ int oracle_batch_run(const xo_batch_instance *inst, uint32_t ninst,
         c->w_min = w_min;
         c->lottery_chance = lottery_chance;
         c->use_lottery = use_lottery;
+        c->method = method;
         c->source = inst[i].source;
         c->source_kind = inst[i].source_kind;
         c->fiber = inst[i].fiber;

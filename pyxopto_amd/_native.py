@@ -145,7 +145,7 @@ class XoBatchOpts(ctypes.Structure):
                 ('blocks_per_cu', c_uint32), ('flags', c_uint32),
                 ('w_min', c_float), ('lottery_chance', c_float),
                 ('use_lottery', c_int32), ('refill_min', c_uint32),
-                ('defer_min', c_uint32)]
+                ('defer_min', c_uint32), ('method', c_int32)]
 
 
 class XoServeOpts(ctypes.Structure):

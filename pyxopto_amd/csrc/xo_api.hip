@@ -732,6 +732,11 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
         return -1;
     }
     const uint64_t chunk = opts->chunk ? opts->chunk : 1024u;
+    if (opts->method != XO_METHOD_AW && opts->method != XO_METHOD_AR &&
+        opts->method != XO_METHOD_MBL) {
+        xo::set_error("xo_batch_run: unknown stepping method %d", opts->method);
+        return -1;
+    }
     std::vector<xo::LayerL> layers((size_t)ninst * XO_BATCH_MAX_LAYERS);
     std::vector<xo::BatchHdr> hdr(ninst);
     std::vector<xo::BatchItem> items;
@@ -836,6 +841,7 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
     P.use_lottery = opts->use_lottery;
     P.refill_min = opts->refill_min ? opts->refill_min : 4u;
     P.defer_min = opts->defer_min ? opts->defer_min : xo::kDeferMinDefault;
+    P.method = opts->method;
     int bpc = 1;
     XO_HIP(xo::occupancy_batch(block, &bpc));
     if (opts->blocks_per_cu && (int)opts->blocks_per_cu < bpc) bpc = (int)opts->blocks_per_cu;

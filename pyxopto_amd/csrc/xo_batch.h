@@ -92,6 +92,7 @@ struct BatchParams {
     uint32_t refill_min;         /* idle lanes that trigger a refill (>= 1) */
     uint32_t defer_min;          /* lanes holding a boundary event that make
                                     the wave handle them (>= 1)             */
+    int32_t method;              /* enum xo_method of every instance        */
 };
 
 hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,

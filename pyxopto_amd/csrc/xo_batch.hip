@@ -56,7 +56,9 @@ __device__ __forceinline__ void end_event(bool done, bool &alive, float px, floa
 
 /* At most 64 VGPRs (8 waves per SIMD): the compiler fits the loop without
  * spills, and the extra wave hides more of the layer and LUT reads (+2.5 %
- * packets/s over 72 VGPRs and 7 waves). */
+ * packets/s over 72 VGPRs and 7 waves). METHOD is the stepping method of
+ * every instance (xo_method). */
+template <int METHOD>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 batch_kernel(const BatchParams P) {
     __shared__ BatchBlockStats sstats;
@@ -174,6 +176,23 @@ batch_kernel(const BatchParams P) {
         }
         if (!alive) continue;
 
+        /* microscopic Beer-Lambert: absorption along the whole step
+         * (mcml.template.c:584-606; a batch has no fluence grid, so no
+         * deposition point is drawn) */
+        auto mbl_absorb = [&](const LayerL &L, float step) {
+            const float fraction = 1.0f - xo_expf(-(L.mua * step));
+            const float deposit = fraction * w;
+            w -= deposit;
+        };
+        /* microscopic Beer-Lambert: roulette after the boundary or the
+         * scattering, whether or not the packet has left
+         * (mcml.template.c:652-667) */
+        auto mbl_roulette = [&](bool done) {
+            if (METHOD == XO_METHOD_MBL && roulette(w, x, a, P.w_min, P.chance, P.use_lottery))
+                done = true;
+            return done;
+        };
+
         /* ---- transport steps, mcml.template.c:514-788, in three phases
          * per iteration. (B) lanes without a pending event draw a step; a
          * step that stays inside the layer moves the packet and leaves an
@@ -213,6 +232,7 @@ batch_kernel(const BatchParams P) {
                 pz = __builtin_fmaf(dz, step, pz);
                 if (layer < next) pz = L.bottom;
                 if (layer > next) pz = L.top;
+                if (METHOD == XO_METHOD_MBL) mbl_absorb(L, step);
                 if (next != layer) {
                     st[XO_STAT_BOUNDARY]++;
                     mcml_boundary(layers, layer, next, dx, dy, dz, x, a);
@@ -225,6 +245,7 @@ batch_kernel(const BatchParams P) {
                             detector_deposit(H.bottom, bins, px, py, dx, dy, dz, w);
                         done = true;
                     }
+                    done = mbl_roulette(done);
                     pend = kEvNone;
                     end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, st);
                 } else {
@@ -238,7 +259,11 @@ batch_kernel(const BatchParams P) {
             /* (B) */
             const LayerL L = layers[layer];
             st[XO_STAT_STEPS]++;
-            float step = -xo_logf_unit(rng_next(x, a)) * L.inv_mut;
+            float step;
+            if (METHOD == XO_METHOD_MBL)
+                step = (-xo_logf_unit(rng_next(x, a))) / L.mus; /* mcml.template.c:529-531 */
+            else
+                step = -xo_logf_unit(rng_next(x, a)) * L.inv_mut;
             step = fminf(step, XO_FLT_MAX);
             const float zend = __builtin_fmaf(step, dz, pz);
             if (zend < L.top || zend >= L.bottom) {
@@ -248,19 +273,31 @@ batch_kernel(const BatchParams P) {
                 px = __builtin_fmaf(dx, step, px);
                 py = __builtin_fmaf(dy, step, py);
                 pz = zend;
+                if (METHOD == XO_METHOD_MBL) mbl_absorb(L, step);
                 pend = kEvInteraction;
             }
         }
         if (alive && pend == kEvInteraction) {
-            /* (C) absorption and scattering, mcml.template.c:724-753 */
+            /* (C) absorption and scattering, mcml.template.c:696-753 */
             const LayerL L = layers[layer];
-            float deposit = w * L.mua_inv_mut;
-            w -= deposit;
-            PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
-            float xi_fi;
-            float ct = pf_sample_rt(pf_kind, pf, P.lut, x, a, xi_fi);
-            scatter_direction(dx, dy, dz, ct, xi_fi);
-            const bool done = roulette(w, x, a, P.w_min, P.chance, P.use_lottery);
+            bool done = false;
+            if (METHOD == XO_METHOD_AR && rng_next(x, a) < L.mua_inv_mut) {
+                /* albedo rejection: the whole weight is absorbed here */
+                w = 0.0f;
+                done = true;
+            } else {
+                if (METHOD == XO_METHOD_AW) {
+                    float deposit = w * L.mua_inv_mut;
+                    w -= deposit;
+                }
+                PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
+                float xi_fi;
+                float ct = pf_sample_rt(pf_kind, pf, P.lut, x, a, xi_fi);
+                scatter_direction(dx, dy, dz, ct, xi_fi);
+                if (METHOD == XO_METHOD_AW)
+                    done = roulette(w, x, a, P.w_min, P.chance, P.use_lottery);
+            }
+            done = mbl_roulette(done);
             pend = kEvNone;
             end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, st);
         }
@@ -275,7 +312,20 @@ batch_kernel(const BatchParams P) {
 
 hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,
                         hipStream_t stream) {
-    hipLaunchKernelGGL(batch_kernel, dim3(blocks), dim3(block), 0, stream, P);
+    switch (P.method) {
+    case XO_METHOD_AW:
+        hipLaunchKernelGGL(batch_kernel<XO_METHOD_AW>, dim3(blocks), dim3(block), 0, stream, P);
+        break;
+    case XO_METHOD_AR:
+        hipLaunchKernelGGL(batch_kernel<XO_METHOD_AR>, dim3(blocks), dim3(block), 0, stream, P);
+        break;
+    case XO_METHOD_MBL:
+        hipLaunchKernelGGL(batch_kernel<XO_METHOD_MBL>, dim3(blocks), dim3(block), 0, stream,
+                           P);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -291,8 +341,10 @@ hipError_t launch_zero_u32(uint32_t *ptr, uint32_t n, hipStream_t stream) {
 }
 
 hipError_t occupancy_batch(uint32_t block, int *blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_kernel,
-                                                        (int)block, 0);
+    /* the albedo-weight build stands for all three methods */
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                        batch_kernel<XO_METHOD_AW>, (int)block,
+                                                        0);
 }
 
 } // namespace xo

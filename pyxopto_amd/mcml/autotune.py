@@ -89,6 +89,7 @@ def sweep(ctx, arr, lut, naccum, base_opts, candidates=CANDIDATES):
         o.chunk, o.block, o.blocks_per_cu = chunk, block, bpc
         o.w_min, o.lottery_chance = base_opts.w_min, base_opts.lottery_chance
         o.use_lottery = base_opts.use_lottery
+        o.method = base_opts.method
         _, r1 = ctx.run_batch(small, lut, o, naccum)
         _, r2 = ctx.run_batch(big, lut, o, naccum)
         t1, t2 = r1.kernel_ms, r2.kernel_ms

@@ -80,9 +80,6 @@ class Batch(McWorker):
 
     def pack(self):
         """(XoBatchInstance array, lut, naccum)."""
-        if int(self._opts['method']) != N.XO_METHOD_AW:
-            raise NotImplementedError('Batches run the albedo-weight method '
-                                      'only.')
         self._reset_allocations()
         arr = (N.XoBatchInstance * len(self.instances))()
         self._packers = []
@@ -117,6 +114,7 @@ class Batch(McWorker):
         o.w_min = float(self._opts['w_min'])
         o.lottery_chance = float(self._opts['lottery_chance'])
         o.use_lottery = int(bool(self._opts['use_lottery']))
+        o.method = int(self._opts['method'])
         return o
 
     def run(self, opts: N.XoBatchOpts = None, packed=None):

@@ -30,6 +30,31 @@ def test_batch_bit_exact_with_oracle(mixed):
     np.testing.assert_array_equal(acc, ref)
 
 
+@pytest.mark.parametrize('method', [1, 2])
+def test_batch_methods_bit_exact_with_oracle(method):
+    """Albedo rejection (1) and microscopic Beer-Lambert (2) in the batch
+    kernel against the oracle's layered packet loop with the same method."""
+    from pyxopto_amd.mcml import mc
+    print('method', method, flush=True)
+    insts = small_batch(24)
+    b = Batch(insts, options=[mc.mcoptions.McMethod(method)],
+              rnginit=0x5EED000000000001)
+    arr, lut, naccum = b.pack()
+    opts = b.opts(chunk=512)
+    assert opts.method == method
+    acc, rep = b.context.run_batch(arr, lut, opts, naccum)
+    ref, counts = O.batch_run(arr, lut, naccum, math_mode=O.MATH_KERNEL,
+                              method=method)
+    assert int(rep.steps) == int(counts[0])
+    assert int(rep.boundary_steps) == int(counts[1])
+    np.testing.assert_array_equal(acc, ref)
+    # the method changes the result
+    acc0, _ = b.context.run_batch(arr, lut, b.opts(chunk=512).__class__(
+        **{f: getattr(opts, f) for f, _ in opts._fields_ if f != 'method'}),
+        naccum)
+    assert not np.array_equal(acc0, acc)
+
+
 @pytest.mark.parametrize('chunk,block,bpc', [(64, 64, 1), (333, 128, 0),
                                              (8192, 256, 2)])
 def test_launch_shape_does_not_change_results(mixed, chunk, block, bpc):

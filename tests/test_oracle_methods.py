@@ -107,8 +107,8 @@ def test_method_option_objects():
     from helpers import slab_layers
     b = Batch([Instance(slab_layers(lambda: mc.mcpf.Hg(0.8)))],
               options=[opts.McMethod.ar])
-    with pytest.raises(NotImplementedError):
-        b.pack()
+    b.pack()
+    assert b.opts().method == opts.McMethod.ar.value
 
 
 # ---- voxelized media (mcvox.template.c:763-977) ----
