@@ -335,14 +335,15 @@ typedef struct xo_batch_instance {
     float rmax;
     uint32_t accum_base;        /* this instance's bins: [base, base+count)  */
     uint32_t accum_count;
-    int32_t source_kind;        /* XO_SRC_LINE or XO_SRC_FIBER_NI            */
+    int32_t source_kind;        /* XO_SRC_LINE, _FIBER_NI or _FIBER           */
     uint64_t npackets;
     uint64_t seed;
     xo_source_line source;
     xo_detector top, bottom, specular; /* offsets are absolute              */
     xo_layer layers[XO_BATCH_MAX_LAYERS];
     xo_source_fiber fiber;
-} xo_batch_instance; /* 920 B */
+    xo_fiber_axis fiber_axis;   /* XO_SRC_FIBER: the fiber's axis            */
+} xo_batch_instance; /* 968 B */
 
 /* Launch shape of a batch run (chosen by the autotuner). */
 typedef struct xo_batch_opts {

@@ -1187,6 +1187,7 @@ int oracle_batch_run(const xo_batch_instance *inst, uint32_t ninst,
         c->source = inst[i].source;
         c->source_kind = inst[i].source_kind;
         c->fiber = inst[i].fiber;
+        c->fiber_axis = inst[i].fiber_axis;
         c->top = inst[i].top;
         c->bottom = inst[i].bottom;
         c->specular = inst[i].specular;

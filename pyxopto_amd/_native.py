@@ -137,7 +137,7 @@ class XoBatchInstance(ctypes.Structure):
                 ('source', XoSourceLine), ('top', XoDetector),
                 ('bottom', XoDetector), ('specular', XoDetector),
                 ('layers', XoLayer * XO_BATCH_MAX_LAYERS),
-                ('fiber', XoSourceFiber)]
+                ('fiber', XoSourceFiber), ('fiber_axis', XoFiberAxis)]
 
 
 class XoBatchOpts(ctypes.Structure):

@@ -31,7 +31,7 @@ static_assert(sizeof(xo_fluence) == 48, "xo_fluence layout");
 static_assert(sizeof(xo_material) == 64, "xo_material layout");
 static_assert(sizeof(xo_voxel_cfg) == 48, "xo_voxel_cfg layout");
 static_assert(sizeof(xo_source_fiber) == 32, "xo_source_fiber layout");
-static_assert(sizeof(xo_batch_instance) == 920, "xo_batch_instance layout");
+static_assert(sizeof(xo_batch_instance) == 968, "xo_batch_instance layout");
 static_assert(sizeof(xo_trace) == 32, "xo_trace layout");
 static_assert(sizeof(xo_fiber_axis) == 48, "xo_fiber_axis layout");
 static_assert(sizeof(xo_sampling_volume) == 64, "xo_sampling_volume layout");
@@ -775,8 +775,14 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
         for (uint32_t l = 0; l < XO_BATCH_MAX_LAYERS; ++l)
             layers[(size_t)i * XO_BATCH_MAX_LAYERS + l] =
                 xo::layer_to_device(I.layers[l < I.num_layers ? l : I.num_layers - 1]);
-        if (I.source_kind != XO_SRC_LINE && I.source_kind != XO_SRC_FIBER_NI) {
+        if (I.source_kind != XO_SRC_LINE && I.source_kind != XO_SRC_FIBER_NI &&
+            I.source_kind != XO_SRC_FIBER) {
             xo::set_error("instance %u: unknown source kind %d", i, I.source_kind);
+            return -1;
+        }
+        if (I.source_kind == XO_SRC_FIBER && !(I.fiber_axis.direction[2] > 0.0f)) {
+            xo::set_error("instance %u: the fiber direction must point into the sample (z > 0)",
+                          i);
             return -1;
         }
         const xo_detector det[3] = {I.top, I.bottom, I.specular};

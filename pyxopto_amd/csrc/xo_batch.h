@@ -25,10 +25,11 @@ struct BatchHdr {
     float src_pos[3];           /* Line: surface point; fiber: core centre */
     float dir_sample[3];
     float dir_reflected[3];
-    int32_t src_kind;           /* XO_SRC_LINE or XO_SRC_FIBER_NI */
+    int32_t src_kind;           /* XO_SRC_LINE, _FIBER_NI or _FIBER */
     uint64_t seed;
     xo_detector top, bottom, specular;
     xo_source_fiber fiber;
+    xo_fiber_axis fiber_axis;
 };
 
 /* Host: instances of one class run the same branches of the transport
@@ -48,7 +49,8 @@ inline BatchHdr batch_header(const xo_batch_instance &I, const xo_detector det[3
     H.rmax2 = I.rmax * I.rmax;
     H.src_kind = I.source_kind;
     H.fiber = I.fiber;
-    const bool fiber = I.source_kind == XO_SRC_FIBER_NI;
+    H.fiber_axis = I.fiber_axis;
+    const bool fiber = I.source_kind == XO_SRC_FIBER_NI || I.source_kind == XO_SRC_FIBER;
     H.reflectance = fiber ? 0.0f : I.source.reflectance;
     for (int k = 0; k < 3; ++k) {
         H.src_pos[k] = fiber ? I.fiber.position[k] : I.source.position[k];

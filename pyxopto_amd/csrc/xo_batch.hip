@@ -155,6 +155,11 @@ batch_kernel(const BatchParams P) {
                     fiber_specular_direction(dx, dy, dz, n1, H.fiber.n, rx, ry, rz);
                     detector_deposit(H.specular, bins, px, py, rx, ry, rz, R);
                 }
+            } else if (H.src_kind == XO_SRC_FIBER) {
+                /* UniformFiber, xopto/mcml/mcsource/fiber.py:265-330 */
+                const float R = fiber_launch(H.fiber, H.fiber_axis, layers[1].n, x, a, px, py,
+                                             pz, dx, dy, dz, w);
+                detector_deposit(H.specular, bins, px, py, dx, dy, dz, R);
             } else {
                 /* Line source, xopto/mcml/mcsource/line.py:95-110 */
                 w = 1.0f - H.reflectance;

@@ -604,8 +604,13 @@ int xo_serve_admit_mcml(xo_server *S, const xo_batch_instance *inst, const float
                       XO_BATCH_MAX_LAYERS);
         return -1;
     }
-    if (I.source_kind != XO_SRC_LINE && I.source_kind != XO_SRC_FIBER_NI) {
+    if (I.source_kind != XO_SRC_LINE && I.source_kind != XO_SRC_FIBER_NI &&
+        I.source_kind != XO_SRC_FIBER) {
         xo::set_error("xo_serve_admit_mcml: unknown source kind %d", I.source_kind);
+        return -1;
+    }
+    if (I.source_kind == XO_SRC_FIBER && !(I.fiber_axis.direction[2] > 0.0f)) {
+        xo::set_error("xo_serve_admit_mcml: the fiber direction must point into the sample");
         return -1;
     }
     for (uint32_t l = 0; l < I.num_layers; ++l)

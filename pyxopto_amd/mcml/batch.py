@@ -30,9 +30,6 @@ class Instance:
                              'including the surrounding media.'.format(
                                  N.XO_BATCH_MAX_LAYERS))
         self.source = source if source is not None else mcsource.Line()
-        if isinstance(self.source, mcsource.UniformFiber):
-            raise NotImplementedError('Batches take the Line and '
-                                      'UniformFiberNI sources.')
         self.detectors = detectors
         self.nphotons = int(nphotons)
         self.seed = seed

@@ -137,3 +137,33 @@ def test_uniform_fiber_static_schedule_bit_exact(direction):
     np.testing.assert_array_equal(gpu, ref)
     assert gpu[-1] > 0
     ctx.close()
+
+
+def test_uniform_fiber_batch_bit_exact():
+    """Angled UniformFiber instances, mixed with a UniformFiberNI one, in
+    the batch engine: bit-exact with the oracle's batch run."""
+    fib = mc.fiberutil.MultimodeFiber(400e-6, 420e-6, 1.45, 0.22)
+    D = mc.mcdetector
+    dirs = [(0.0, 0.0, 1.0), (0.3, -0.2, 0.9), (-0.5, 0.1, 0.8)]
+    insts = []
+    for k, d in enumerate(dirs):
+        src = mc.mcsource.UniformFiber(fib, position=(1e-4 * k, 0.0, 0.0),
+                                       direction=d)
+        det = D.Detectors(top=D.Radial(D.RadialAxis(0.0, 3e-3, 60)),
+                          bottom=D.Total(), specular=D.Total())
+        insts.append(Instance(W.fiber_layers(300.0 + 200 * k, 2000.0), src,
+                              det, nphotons=15000 + 313 * k, seed=91 + k,
+                              rmax=5e-3))
+    insts.append(Instance(W.fiber_layers(100.0, 1000.0), W.fiber_source(),
+                          W.fiber_detectors(), nphotons=12000, seed=5,
+                          rmax=5e-3))
+    b = Batch(insts, rnginit=XINIT)
+    arr, lut, naccum = b.pack()
+    acc, rep = b.context.run_batch(arr, lut, b.opts(chunk=512), naccum)
+    ref, counts = O.batch_run(arr, lut, naccum, math_mode=O.MATH_KERNEL)
+    assert int(rep.steps) == int(counts[0])
+    np.testing.assert_array_equal(acc, ref)
+    # the angled fibers' specular totals are non-zero
+    for k in range(len(dirs)):
+        base, n = int(arr[k].accum_base), int(arr[k].accum_count)
+        assert acc[base + n - 1] > 0
