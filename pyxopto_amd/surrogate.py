@@ -1,5 +1,5 @@
 """Learned surrogate of the radial reflectance of a semi-infinite turbid
-medium, with Monte Carlo as its fallback and as the judge of its error.
+medium, with Monte Carlo as its fallback and as the measure of its error.
 
 Inputs: (ln mua, ln mus', g, beta, n) of a medium with a modified
 Henyey-Greenstein phase function (mus' = mus (1 - beta g), the first moment
