@@ -83,7 +83,7 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
                   'launch': {'chunk': opts.chunk, 'block': opts.block,
                              'blocks_per_cu': opts.blocks_per_cu,
                              'refill_min': opts.refill_min or 4,
-                             'defer_min': opts.defer_min or 12}}
+                             'defer_min': opts.defer_min or 16}}
         batch.bench_plain = lambda: plain_rate(batch, insts)
         batch.bench_insts = insts
         return batch, npk, step, False, config

@@ -357,7 +357,7 @@ typedef struct xo_batch_opts {
     uint32_t refill_min;        /* a wave hands out new packets once this
                                    many of its lanes are idle (0: 4)          */
     uint32_t defer_min;         /* a wave handles its boundary events once
-                                   this many lanes hold one (0: 12)           */
+                                   this many lanes hold one (0: 16)           */
     int32_t method;             /* enum xo_method of every instance           */
 } xo_batch_opts;
 

@@ -75,7 +75,7 @@ struct BatchItem {
 };
 
 /* Default of xo_batch_opts.defer_min. */
-constexpr uint32_t kDeferMinDefault = 12;
+constexpr uint32_t kDeferMinDefault = 16;
 
 struct BatchParams {
     const LayerL *layers;        /* [instance][XO_BATCH_MAX_LAYERS] */

@@ -44,12 +44,12 @@ constexpr uint32_t kDeferWait = 8;
  * step limit; a finished packet frees its lane. */
 __device__ __forceinline__ void end_event(bool done, bool &alive, float px, float py, float pz,
                                           float spx, float spy, float spz, float rmax2,
-                                          uint32_t &pkt_steps, uint32_t *st) {
+                                          uint32_t &pkt_steps, uint32_t *truncated) {
     const float ex = px - spx, ey = py - spy, ez = pz - spz;
     if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
     if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
         done = true;
-        st[XO_STAT_TRUNCATED]++;
+        atomicAdd(truncated, 1u);
     }
     if (done) alive = false;
 }
@@ -62,6 +62,11 @@ template <int METHOD>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 batch_kernel(const BatchParams P) {
     __shared__ BatchBlockStats sstats;
+    /* per-thread event counts in LDS (packets, steps, boundary events,
+     * truncated packets), added with LDS atomics: kept in registers they
+     * would take four of the loop's 64 */
+    __shared__ uint32_t scount[4][256];
+    for (int k = 0; k < 4; ++k) scount[k][threadIdx.x] = 0u;
     if (threadIdx.x < XO_STAT_COUNT) sstats.v[threadIdx.x] = 0ull;
     __syncthreads();
     const uint32_t nitems = P.nitems_ptr ? *P.nitems_ptr : P.nitems;
@@ -85,7 +90,7 @@ batch_kernel(const BatchParams P) {
     uint64_t x = 0;
     uint32_t a = 0;
     uint32_t pkt_steps = 0;
-    uint32_t st[XO_STAT_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t *const n_trunc = &scount[3][threadIdx.x];
     uint32_t pend = kEvNone;  /* event the lane's last step ended on     */
     float pstep = 0.f;        /* drawn step of a pending boundary event  */
     uint32_t bwait = 0;       /* iterations boundary events have waited  */
@@ -173,7 +178,7 @@ batch_kernel(const BatchParams P) {
             layer = 1;
             pkt_steps = 0;
             alive = true;
-            st[XO_STAT_PACKETS]++;
+            atomicAdd(&scount[0][threadIdx.x], 1u);
         }
         if (!__any(alive)) {
             if (queue_empty) break;
@@ -239,7 +244,7 @@ batch_kernel(const BatchParams P) {
                 if (layer > next) pz = L.top;
                 if (METHOD == XO_METHOD_MBL) mbl_absorb(L, step);
                 if (next != layer) {
-                    st[XO_STAT_BOUNDARY]++;
+                    atomicAdd(&scount[2][threadIdx.x], 1u);
                     mcml_boundary(layers, layer, next, dx, dy, dz, x, a);
                     bool done = false;
                     if (!(layer > 0 && layer < nl - 1)) {
@@ -252,7 +257,7 @@ batch_kernel(const BatchParams P) {
                     }
                     done = mbl_roulette(done);
                     pend = kEvNone;
-                    end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, st);
+                    end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, n_trunc);
                 } else {
                     /* crossed both planes of a degenerate layer: an
                      * interaction at the clipped point, handled below */
@@ -263,7 +268,7 @@ batch_kernel(const BatchParams P) {
         if (alive && pend == kEvNone) {
             /* (B) */
             const LayerL L = layers[layer];
-            st[XO_STAT_STEPS]++;
+            atomicAdd(&scount[1][threadIdx.x], 1u);
             float step;
             if (METHOD == XO_METHOD_MBL)
                 step = (-xo_logf_unit(rng_next(x, a))) / L.mus; /* mcml.template.c:529-531 */
@@ -304,12 +309,14 @@ batch_kernel(const BatchParams P) {
             }
             done = mbl_roulette(done);
             pend = kEvNone;
-            end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, st);
+            end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, n_trunc);
         }
     }
 
-    for (int i = 0; i < XO_STAT_COUNT; ++i)
-        if (st[i]) atomicAdd(&sstats.v[i], (unsigned long long)st[i]);
+    const int kinds[4] = {XO_STAT_PACKETS, XO_STAT_STEPS, XO_STAT_BOUNDARY, XO_STAT_TRUNCATED};
+    for (int k = 0; k < 4; ++k)
+        if (scount[k][threadIdx.x])
+            atomicAdd(&sstats.v[kinds[k]], (unsigned long long)scount[k][threadIdx.x]);
     __syncthreads();
     if (threadIdx.x < XO_STAT_COUNT && sstats.v[threadIdx.x])
         atomicAdd(&P.stats[threadIdx.x], sstats.v[threadIdx.x]);
