@@ -45,8 +45,12 @@ constexpr uint32_t kDeferWait = 8;
 __device__ __forceinline__ void end_event(bool done, bool &alive, float px, float py, float pz,
                                           float spx, float spy, float spz, float rmax2,
                                           uint32_t &pkt_steps, uint32_t *truncated) {
-    const float ex = px - spx, ey = py - spy, ez = pz - spz;
-    if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
+    /* no distance can exceed an infinite radius (the default): skip the
+     * test, which changes nothing */
+    if (rmax2 < XO_INF_F) {
+        const float ex = px - spx, ey = py - spy, ez = pz - spz;
+        if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
+    }
     if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
         done = true;
         atomicAdd(truncated, 1u);

@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
                 done = true;
             float ex = px - spx, ey = py - spy, ez = pz - spz;
-            if (ex * ex + ey * ey + ez * ez > rmax2) {
+            if (rmax2 < XO_INF_F && ex * ex + ey * ey + ez * ez > rmax2) {
                 done = true;
                 if (TRACE) ts.ev |= XO_EVENT_ESCAPED;
             }
