@@ -27,10 +27,19 @@ struct BatchBlockStats {
     unsigned long long v[XO_STAT_COUNT];
 };
 
-XO_DEV float pf_sample_rt(int kind, const PfL &pf, const float *lut, uint64_t &x,
+/* Phase-function sampling by the instance's kind; each branch reads only
+ * the layer fields its kind uses. */
+XO_DEV float pf_sample_rt(int kind, const LayerL &L, const float *lut, uint64_t &x,
                           uint32_t a, float &xi_fi) {
-    if (kind == XO_PF_HG) return pf_sample<XO_PF_HG>(pf, lut, x, a, xi_fi);
-    if (kind == XO_PF_MHG) return pf_sample<XO_PF_MHG>(pf, lut, x, a, xi_fi);
+    if (kind == XO_PF_HG) {
+        const PfL pf{L.g, 0.0f, 0.0f, 0.0f, 0.0f, 0u, 0u};
+        return pf_sample<XO_PF_HG>(pf, lut, x, a, xi_fi);
+    }
+    if (kind == XO_PF_MHG) {
+        const PfL pf{L.g, L.beta, 0.0f, 0.0f, 0.0f, 0u, 0u};
+        return pf_sample<XO_PF_MHG>(pf, lut, x, a, xi_fi);
+    }
+    const PfL pf{0.0f, 0.0f, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
     return pf_sample<XO_PF_LUT>(pf, lut, x, a, xi_fi);
 }
 
@@ -293,7 +302,7 @@ batch_kernel(const BatchParams P) {
         }
         if (alive && pend == kEvInteraction) {
             /* (C) absorption and scattering, mcml.template.c:696-753 */
-            const LayerL L = layers[layer];
+            const LayerL &L = layers[layer];
             bool done = false;
             if (METHOD == XO_METHOD_AR && rng_next(x, a) < L.mua_inv_mut) {
                 /* albedo rejection: the whole weight is absorbed here */
@@ -304,9 +313,8 @@ batch_kernel(const BatchParams P) {
                     float deposit = w * L.mua_inv_mut;
                     w -= deposit;
                 }
-                PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
                 float xi_fi;
-                float ct = pf_sample_rt(pf_kind, pf, P.lut, x, a, xi_fi);
+                float ct = pf_sample_rt(pf_kind, L, P.lut, x, a, xi_fi);
                 scatter_direction(dx, dy, dz, ct, xi_fi);
                 if (METHOD == XO_METHOD_AW)
                     done = roulette(w, x, a, P.w_min, P.chance, P.use_lottery);

@@ -689,7 +689,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
             if (METHOD == XO_METHOD_MBL && roulette(w, x, a, w_min, chance, use_lottery))
                 done = true;
             float ex = p[0] - spx, ey = p[1] - spy, ez = p[2] - spz;
-            if (ex * ex + ey * ey + ez * ez > rmax2 || w <= 0.0f) {
+            if ((rmax2 < XO_INF_F && ex * ex + ey * ey + ez * ez > rmax2) || w <= 0.0f) {
                 done = true;
                 if (TRACE) ts.ev |= XO_EVENT_ESCAPED;
             }
