@@ -83,3 +83,17 @@ def test_batch_api_autotuned_run(mixed):
     assert len(res) == len(b.instances)
     assert res[0].top.nphotons == b.instances[0].nphotons
     assert rep.truncated == 0
+
+
+def test_batch_rejects_unknown_method_and_backward_fiber(mixed):
+    b, (arr, lut, naccum) = mixed
+    opts = b.opts(chunk=1024)
+    opts.method = 7
+    with pytest.raises(RuntimeError, match='stepping method'):
+        b.context.run_batch(arr, lut, opts, naccum)
+    one = (N.XoBatchInstance * 1)()
+    one[0] = arr[0]
+    one[0].source_kind = N.XO_SRC_FIBER
+    one[0].fiber_axis.direction[2] = -1.0
+    with pytest.raises(RuntimeError, match='fiber direction'):
+        b.context.run_batch(one, lut, b.opts(chunk=1024), naccum)
