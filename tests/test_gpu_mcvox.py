@@ -149,3 +149,30 @@ def test_mcvox_api_fluence_normalization():
     absorbed = fl.data.sum() * fl.dx * fl.dy * fl.dz
     assert abs(absorbed - (1 - np.exp(-300.0 * 4e-3))) < 5e-3
     assert fl.nphotons == 200000
+
+
+def test_full_size_skin_vessel_keyed_runs_independent_of_launch_shape():
+    """The 200^3 skin-vessel deposition at its full size (1e8 packets) with
+    packet-keyed generators: two launches with different lane counts give
+    the same integer accumulators bit for bit (every packet draws from its
+    own generator, and u64 adds commute)."""
+    from pyxopto_amd import workloads as W
+    sim = W.skin_vessel()
+    cfg, lut, naccum = sim.pack()
+    vox = sim.voxels.material
+    cfg.packet_seed, cfg.first_packet = 0x5EED1234, 0
+    npk = 100_000_000
+    print('skin_vessel 200^3, 1e8 packets, two launch shapes', flush=True)
+    ctx = N.Context(0, 0, XINIT)
+    a1, r1 = ctx.run_mcvox(cfg, vox, lut, npk, naccum, 0, N.XO_FLAG_PACKET_KEYED)
+    a2, r2 = ctx.run_mcvox(cfg, vox, lut, npk, naccum, 65536,
+                           N.XO_FLAG_PACKET_KEYED)
+    ctx.close()
+    assert r1.packets == npk and r2.packets == npk
+    assert int(r1.steps) == int(r2.steps)
+    np.testing.assert_array_equal(a1, a2)
+    off = int(cfg.fluence.offset)
+    n = 200 ** 3
+    absorbed = a1[off:off + n].astype(np.float64).sum() / N.XO_ACCU_K / npk
+    print('absorbed fraction', absorbed, flush=True)
+    assert 0.1 < absorbed < 1.0, absorbed
