@@ -9,6 +9,7 @@
 #include <algorithm>
 
 #include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -106,6 +107,30 @@ int check_fluence(const char *fn, const xo_fluence &f, uint64_t det_end, uint64_
         return -1;
     }
     return 0;
+}
+
+/* The LDS fluence box of a voxel run: kFlBox bins of a Cartesian grid
+ * around the point where the source enters it (centred in x and y,
+ * starting at that depth), clamped into the grid; off for other grids, or
+ * when the grid or the LDS budget is too small. */
+void place_fluence_box(const xo_mcvox_cfg &cfg, McvoxParams &P) {
+    P.fl_box_on = 0;
+    const xo_fluence &F = cfg.fluence;
+    if (F.enabled != XO_FLUENCE_XYZ || F.shape[0] < (uint32_t)kFlBoxX ||
+        F.shape[1] < (uint32_t)kFlBoxY || F.shape[2] < (uint32_t)kFlBoxZ ||
+        mcvox_lds_bytes(P) > 32768u)
+        return;
+    const float *sp = cfg.source_kind == XO_SRC_LINE ? cfg.line.position : cfg.gaussian.position;
+    const int32_t ext[3] = {kFlBoxX, kFlBoxY, kFlBoxZ};
+    const int32_t back[3] = {kFlBoxX / 2, kFlBoxY / 2, 0};
+    for (int k = 0; k < 3; ++k) {
+        const double f = ((double)sp[k] - F.top_left[k]) * F.inv_step[k];
+        int64_t c = std::isfinite(f) ? (int64_t)std::floor(f) - back[k] : 0;
+        const int64_t hi = (int64_t)F.shape[k] - ext[k];
+        c = c < 0 ? 0 : (c > hi ? hi : c);
+        P.fl_box[k] = (int32_t)c;
+    }
+    P.fl_box_on = 1;
 }
 
 int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint32_t idx) {
@@ -653,6 +678,7 @@ static int mcvox_run(xo_ctx *ctx, const xo_mcvox_cfg *cfg, const int32_t *voxels
                      ? (uint32_t)det_end
                      : 0u;
     P.lds_lut = (nlut && nlut <= XO_LDS_LUT_MAX) ? nlut : 0u;
+    xo::place_fluence_box(*cfg, P);
     int bpc = 1;
     XO_HIP(xo::occupancy_mcvox(cfg->pf_kind, xo::mcvox_lds_bytes(P), &bpc));
     P.lanes = choose_lanes(ctx, lanes, bpc);

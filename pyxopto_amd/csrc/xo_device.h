@@ -259,6 +259,33 @@ struct VoxEventCfgL {
     xo_source_gaussian gaussian;
 };
 
+/* fluence_deposit for a Cartesian grid whose bins in the box
+ * [box, box + (bx, by, bz)) are held in LDS (sbox, x fastest): the same
+ * index and integer as fluence_deposit, added to the box when the bin lies
+ * in it, to accum otherwise. */
+XO_DEV bool fluence_deposit_boxed(const xo_fluence &F, unsigned long long *accum,
+                                  unsigned long long *sbox, const int32_t box[3], int32_t bx,
+                                  int32_t by, int32_t bz, float px, float py, float pz,
+                                  float w) {
+    const float fx = (px - F.top_left[0]) * F.inv_step[0];
+    const float fy = (py - F.top_left[1]) * F.inv_step[1];
+    const float fz = (pz - F.top_left[2]) * F.inv_step[2];
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)F.shape[0] &&
+          fy < (float)F.shape[1] && fz < (float)F.shape[2]))
+        return false;
+    const uint32_t ix = f2u_sat(fx), iy = f2u_sat(fy), iz = f2u_sat(fz);
+    const uint32_t ui = f2u_sat(w * (float)F.k + 0.5f);
+    if (ui == 0u) return false;
+    const uint32_t rx = ix - (uint32_t)box[0], ry = iy - (uint32_t)box[1],
+                   rz = iz - (uint32_t)box[2];
+    if (rx < (uint32_t)bx && ry < (uint32_t)by && rz < (uint32_t)bz)
+        atomicAdd(&sbox[(rz * (uint32_t)by + ry) * (uint32_t)bx + rx], (unsigned long long)ui);
+    else
+        atomicAdd(&accum[F.offset + ((uint64_t)iz * F.shape[1] + iy) * F.shape[0] + ix],
+                  (unsigned long long)ui);
+    return true;
+}
+
 /* Phase-function parameters of one layer/material, read from LDS. */
 struct PfL {
     float g, beta, la, lb, lc;

@@ -69,7 +69,17 @@ struct McvoxParams {
     uint32_t flags;
     uint32_t lds_bins;
     uint32_t lds_lut;
+    /* Cartesian fluence bins of one kFlBox-sized box of the grid (origin
+     * fl_box, in bins) accumulate in LDS and are added to accum once per
+     * block: the box around the beam's entry takes a large share of the
+     * deposits. fl_box_on 0: every deposit goes to accum. */
+    int32_t fl_box[3];
+    uint32_t fl_box_on;
 };
+
+/* Extent of the LDS fluence box (x, y, z bins). */
+constexpr int32_t kFlBoxX = 16, kFlBoxY = 16, kFlBoxZ = 8;
+constexpr uint32_t kFlBoxBins = kFlBoxX * kFlBoxY * kFlBoxZ;
 
 size_t mcml_lds_bytes(const McmlParams &P);
 size_t mcvox_lds_bytes(const McvoxParams &P);

@@ -452,8 +452,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     __shared__ BlockStats sstats;
     __shared__ VoxEventCfgL sev;
     MaterialL *sm = reinterpret_cast<MaterialL *>(smem);
-    unsigned long long *sbins = reinterpret_cast<unsigned long long *>(
+    unsigned long long *sfl = reinterpret_cast<unsigned long long *>(
         smem + XO_MAX_MATERIALS * sizeof(MaterialL));
+    unsigned long long *sbins = sfl + (P.fl_box_on ? kFlBoxBins : 0u);
     float *slut = reinterpret_cast<float *>(sbins + P.lds_bins);
 
     const uint32_t tid = threadIdx.x;
@@ -476,6 +477,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         sm[tid] = o;
     }
     for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK) sbins[i] = 0ull;
+    if (P.fl_box_on)
+        for (uint32_t i = tid; i < kFlBoxBins; i += XO_BLOCK) sfl[i] = 0ull;
     for (uint32_t i = tid; i < P.lds_lut; i += XO_BLOCK) slut[i] = P.lut[i];
     if (tid < XO_STAT_COUNT) sstats.v[tid] = 0ull;
     if (tid == 0) {
@@ -500,6 +503,13 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         g.shape[k] = P.cfg.voxels.shape[k];
     }
     const bool use_fluence = P.cfg.fluence.enabled != 0;
+    /* a fluence deposit: through the LDS box when there is one */
+    auto deposit_fluence = [&](float qx, float qy, float qz, float wgt) {
+        if (P.fl_box_on)
+            return fluence_deposit_boxed(lds_fresh(sev.fluence), P.accum, sfl, P.fl_box, kFlBoxX,
+                                         kFlBoxY, kFlBoxZ, qx, qy, qz, wgt);
+        return fluence_deposit(lds_fresh(sev.fluence), P.accum, qx, qy, qz, wgt);
+    };
 
     const uint32_t gid = blockIdx.x * XO_BLOCK + tid;
     const bool lane_ok = gid < P.lanes;
@@ -718,7 +728,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 const float deposit = w;
                 w -= deposit;
                 if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-                if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0], p[1], p[2], deposit))
+                if (use_fluence && deposit_fluence(p[0], p[1], p[2], deposit))
                     st[XO_STAT_DEPOSITS]++;
                 done = true;
             } else {
@@ -727,7 +737,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                     w -= deposit;
                     if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                     /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
-                    if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0], p[1], p[2], deposit))
+                    if (use_fluence && deposit_fluence(p[0], p[1], p[2], deposit))
                         st[XO_STAT_DEPOSITS]++;
                 }
                 PfL pf{M.g, M.beta, M.la, M.lb, M.lc, M.loff, M.lsize_m1};
@@ -780,7 +790,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 const float back =
                     (mua != 0.0f) ? d_ok - (-xo_logf_unit(1.0f - rng_next(x, a) * fraction)) / mua
                                   : 0.0f;
-                if (fluence_deposit(lds_fresh(sev.fluence), P.accum, p[0] - back * d[0], p[1] - back * d[1],
+                if (deposit_fluence(p[0] - back * d[0], p[1] - back * d[1],
                                     p[2] - back * d[2], deposit))
                     st[XO_STAT_DEPOSITS]++;
             }
@@ -867,6 +877,16 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK)
             if (sbins[i]) atomicAdd(&P.accum[i], sbins[i]);
     }
+    if (P.fl_box_on) {
+        const xo_fluence &F = P.cfg.fluence;
+        for (uint32_t i = tid; i < kFlBoxBins; i += XO_BLOCK) {
+            if (!sfl[i]) continue;
+            const uint64_t ix = (uint64_t)P.fl_box[0] + i % kFlBoxX;
+            const uint64_t iy = (uint64_t)P.fl_box[1] + (i / kFlBoxX) % kFlBoxY;
+            const uint64_t iz = (uint64_t)P.fl_box[2] + i / (kFlBoxX * kFlBoxY);
+            atomicAdd(&P.accum[F.offset + (iz * F.shape[1] + iy) * F.shape[0] + ix], sfl[i]);
+        }
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -877,7 +897,8 @@ size_t mcml_lds_bytes(const McmlParams &P) {
 }
 
 size_t mcvox_lds_bytes(const McvoxParams &P) {
-    return XO_MAX_MATERIALS * sizeof(MaterialL) + 8u * P.lds_bins + 4u * P.lds_lut;
+    return XO_MAX_MATERIALS * sizeof(MaterialL) + (P.fl_box_on ? 8u * kFlBoxBins : 0u) +
+           8u * P.lds_bins + 4u * P.lds_lut;
 }
 
 template <int PF, bool TRACE>
