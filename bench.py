@@ -38,8 +38,8 @@ SERVING_QPS = 250.0                     # requests/s of the serving run
 # 1 GiB device-to-device copy (profiles/batch_kernel_layered_batch_4096.md,
 # profiles/mcvox_kernel_skin_vessel_200.md). Offline figures: measured for
 # the kernels as they are in this tree.
-TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 1.795e11,
-                            'skin_vessel_200': 7.973e11}
+TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 1.784e11,
+                            'skin_vessel_200': 7.844e11}
 
 METRIC = ('photon packets/s (1 GPU) batched vs plain; p50/p99 solve latency '
           'at fixed QPS')
