@@ -44,6 +44,12 @@ TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 1.784e11,
 METRIC = ('photon packets/s (1 GPU) batched vs plain; p50/p99 solve latency '
           'at fixed QPS')
 
+# The production arithmetic: the hardware approximations the reference's own
+# benchmark asks for with -cl-fast-relaxed-math (test/performance.py:104);
+# --exact-math times the build that matches the oracle bit for bit.
+FAST_MATH = ['-cl-fast-relaxed-math', '-cl-mad-enable']
+BUILD_OPTS = FAST_MATH
+
 
 def algorithmic_slots(rep, voxel: bool) -> float:
     interior = rep.steps - rep.boundary_steps
@@ -61,7 +67,8 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
         insts = workloads.layered_batch(
             budget_scale=scale, processes=lut_processes,
             cache_dir=os.path.join(REPO, 'pyxopto_amd', 'data'))
-        batch = Batch(insts, rnginit=0x5EED000000000001)
+        batch = Batch(insts, rnginit=0x5EED000000000001,
+                      cl_build_options=BUILD_OPTS)
         packed = batch.pack()
         if launch:
             fields = [int(v) for v in launch.split(':')]
@@ -88,7 +95,7 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
         batch.bench_insts = insts
         return batch, npk, step, False, config
     if name == 'skin_vessel_200':
-        sim = workloads.skin_vessel()
+        sim = workloads.skin_vessel(cl_build_options=BUILD_OPTS)
         npk = packets or workloads.SKIN_VESSEL_PACKETS
         cfg, lut, naccum = sim.pack()
         ctx = sim.context
@@ -104,7 +111,7 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
                   'accumulator': 'uint64 deposition per voxel'}
         return sim, npk, step, True, config
     if name == 'slab_1mm':
-        sim = workloads.slab_1mm()
+        sim = workloads.slab_1mm(cl_build_options=BUILD_OPTS)
         npk = packets or workloads.SLAB_PACKETS
         cfg, lut, naccum = sim.pack()
         ctx = sim.context
@@ -137,15 +144,17 @@ def ceilings(ctx):
                      '2^28 u64 atomics, 64 MiB array'}
 
 
-def serving(insts, qps: float, duration: float, seed: int = 20261203,
+def serving(insts, rates, duration: float, seed: int = 20261203,
             voxel_fraction: float = 0.1, saturation_requests: int = 400,
             tick_packets: int = None):
-    """Poisson arrivals at a fixed rate for `duration` seconds: 90 % of the
-    requests are instances of the layered batch (in a seeded random order),
-    10 % are 64^3 two-layer voxel simulations of 1e6 packets. Requests are
-    packed before the run; the clock runs from the first submit to the last
-    result. Latency is submit to result. Before it, a burst of requests
-    submitted at once measures the capacity of the same mix."""
+    """Poisson arrivals at each fixed rate of `rates` for `duration` seconds:
+    90 % of the requests are instances of the layered batch (in a seeded
+    random order), 10 % are 64^3 two-layer voxel simulations of 1e6
+    packets. Requests are packed before the run; the clock runs from the
+    first submit to the last result. Latency is submit to result. Before
+    them, a burst of requests submitted at once measures the capacity of
+    the same mix. The top-level figures are those of the first rate; every
+    rate is listed under 'sweep'."""
     import numpy as np
     from pyxopto_amd import serve, workloads
     from pyxopto_amd.mcml.batch import Batch
@@ -168,7 +177,10 @@ def serving(insts, qps: float, duration: float, seed: int = 20261203,
         inst, lut = packed[k % len(packed)]
         return srv.submit_packed_layered(inst, lut)
 
-    opts = {} if tick_packets is None else {'tick_packets': tick_packets}
+    opts = {'fast_math': 0 if BUILD_OPTS is None else 1}
+    if tick_packets is not None:
+        opts['tick_packets'] = tick_packets
+    sweep = []
     with serve.Server(**opts) as srv:
         # first use of the server (graph, tables) outside the clock
         warm = [make(srv, k) for k in range(4)]
@@ -178,28 +190,38 @@ def serving(insts, qps: float, duration: float, seed: int = 20261203,
         burst = [make(srv, (1 << 19) + k) for k in range(saturation_requests)]
         srv.drain(600)
         t_burst = time.perf_counter() - t0
-        s0 = srv.stats()
-        reqs, span = serve.poisson_load(srv, lambda sv, k: make(sv, k + 4),
-                                        qps, duration, seed=seed)
-        s1 = srv.stats()
+        for j, qps in enumerate(rates):
+            s0 = srv.stats()
+            reqs, span = serve.poisson_load(
+                srv, lambda sv, k, j=j: make(sv, (j << 16) + k + 4), qps,
+                duration, seed=seed + j)
+            s1 = srv.stats()
+            packets = sum(r.npackets for r in reqs)
+            point = {'qps': qps, 'arrivals_s': duration,
+                     'requests': len(reqs),
+                     'voxel_requests': sum(r.kind == 1 for r in reqs),
+                     'packets': packets, 'packets_per_s': packets / span,
+                     'steps_per_s': (s1['steps'] - s0['steps']) / span,
+                     'ticks': s1['ticks'] - s0['ticks']}
+            point.update(serve.latency_summary(reqs))
+            for kind, name in ((0, 'layered'), (1, 'voxel')):
+                lat = serve.latency_summary([r for r in reqs
+                                             if r.kind == kind])
+                point[name + '_p50_ms'] = lat['p50_ms']
+                point[name + '_p99_ms'] = lat['p99_ms']
+            print('bench: serving {} requests/s: p50 {:.1f} ms p99 {:.1f} '
+                  'ms'.format(qps, point['p50_ms'], point['p99_ms']),
+                  file=sys.stderr, flush=True)
+            sweep.append(point)
     del warm
     burst_packets = sum(r.npackets for r in burst)
-    packets = sum(r.npackets for r in reqs)
     out = {'capacity': {'requests': len(burst),
                         'requests_per_s': len(burst) / t_burst,
                         'packets_per_s': burst_packets / t_burst},
-           'qps': qps, 'arrivals_s': duration, 'requests': len(reqs),
-           'voxel_requests': sum(r.kind == 1 for r in reqs),
-           'packets': packets, 'packets_per_s': packets / span,
-           'steps_per_s': (s1['steps'] - s0['steps']) / span,
-           'ticks': s1['ticks'] - s0['ticks'],
            'mix': '90% layered-batch instances, 10% 64^3 two-layer voxel '
                   'requests of 1e6 packets'}
-    out.update(serve.latency_summary(reqs))
-    for kind, name in ((0, 'layered'), (1, 'voxel')):
-        lat = serve.latency_summary([r for r in reqs if r.kind == kind])
-        out[name + '_p50_ms'] = lat['p50_ms']
-        out[name + '_p99_ms'] = lat['p99_ms']
+    out.update(sweep[0])
+    out['sweep'] = sweep
     return out
 
 
@@ -261,7 +283,7 @@ def published_benchmark(packets: int = None):
     mua, musr = W.fiber_grid()
     # plain, as the reference test loops
     t0 = time.perf_counter()
-    sim = W.fiber_sim()
+    sim = W.fiber_sim(cl_build_options=BUILD_OPTS)
     for m, s in zip(mua, musr):
         sim.layers[1].mua = m
         sim.layers[1].mus = s / (1.0 - 0.85)
@@ -269,7 +291,8 @@ def published_benchmark(packets: int = None):
     t_plain = time.perf_counter() - t0
     # batched
     t0 = time.perf_counter()
-    b = Batch(W.fiber_batch(packets=npk), rnginit=0x5EED000000000001)
+    b = Batch(W.fiber_batch(packets=npk), rnginit=0x5EED000000000001,
+              cl_build_options=BUILD_OPTS)
     b._ctx = sim.context
     res, rep = b.run(b.opts(chunk=8192, block=256))
     t_batch = time.perf_counter() - t0
@@ -294,7 +317,8 @@ def plain_rate(batch, insts, every: int = 16):
     lane-owned generators), on every 16th instance: packets/s."""
     from pyxopto_amd.mcml import mc
     sub = insts[::every]
-    sims = [mc.Mc(i.layers, i.source, i.detectors, rnginit=0x5EED000000000001)
+    sims = [mc.Mc(i.layers, i.source, i.detectors, rnginit=0x5EED000000000001,
+                  cl_build_options=BUILD_OPTS)
             for i in sub]
     for s in sims:             # one context, created outside the timing
         s._ctx = batch.context
@@ -309,60 +333,150 @@ def plain_rate(batch, insts, every: int = 16):
             'packets': npk, 'seconds': dt}
 
 
-def cpu_baseline(sim, voxel: bool, budget_s: float = 12.0):
-    """The oracle (C, OpenMP, C-library math) on a bounded sample of the
-    same workload, timed on this host's cores."""
-    import numpy as np
-    from pyxopto_amd import _native as N
+_NATIVE_ORACLE = None
+
+
+def native_oracle():
+    """Load the oracle built for this host's cores (gcc -O3 -march=native
+    -fopenmp, oracle/Makefile `native`) for the CPU baselines; 'port' is the
+    prebuilt portable build if that compile is not possible here."""
+    global _NATIVE_ORACLE
     from oracle import pyoracle as O
+    if _NATIVE_ORACLE is None:
+        import tempfile
+        out = os.path.join(tempfile.mkdtemp(prefix='xo_oracle_'),
+                           'liboracle_native.so')
+        try:
+            O.build_native(out)
+            O.use_library(out)
+            _NATIVE_ORACLE = 'gcc -O3 -march=native -fopenmp'
+        except Exception as e:        # no compiler: the prebuilt oracle
+            print('bench: native oracle build failed ({}); using the '
+                  'prebuilt one'.format(e), file=sys.stderr, flush=True)
+            _NATIVE_ORACLE = 'prebuilt (gcc -O2 -march=x86-64-v3 -fopenmp)'
+    return O, _NATIVE_ORACLE
+
+
+def cpu_baseline(sim, voxel: bool, npackets: int, budget_s: float = 30.0):
+    """The oracle (C, OpenMP on every host core, C-library math) on the
+    same workload: npackets packets, in chunks, stopping early once
+    budget_s has passed. Returns the measured rate."""
+    from pyxopto_amd import _native as N
+    O, build = native_oracle()
     cores = len(os.sched_getaffinity(0))
     if hasattr(sim, 'instances'):
-        return cpu_baseline_batch(sim, cores, budget_s)
+        return cpu_baseline_batch(sim, cores, budget_s, build)
     cfg, lut, naccum = sim.pack()
     lanes = 4 * cores * 64
     x, a = N.init_rng(lanes, 0x0123456789ABCDEF)
-    chunk = 20000 if voxel else 200000
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
+    chunk = min(npackets, 50000 if voxel else 250000)
+    done, steps, t0 = 0, 0, time.perf_counter()
+    while done < npackets and time.perf_counter() - t0 < budget_s:
         if voxel:
-            _, _, x = O.mcvox_run(cfg, sim.voxels.material, lut, x, a, chunk,
-                                  naccum, O.MATH_LIBM, cores)
+            _, counts, x = O.mcvox_run(cfg, sim.voxels.material, lut, x, a,
+                                       chunk, naccum, O.MATH_LIBM, cores)
         else:
-            _, _, x = O.mcml_run(cfg, lut, x, a, chunk, naccum, O.MATH_LIBM,
-                                 cores)
+            _, counts, x = O.mcml_run(cfg, lut, x, a, chunk, naccum,
+                                      O.MATH_LIBM, cores)
         done += chunk
+        steps += int(counts[0])
     dt = time.perf_counter() - t0
     return {'value': done / dt, 'unit': 'photon packets/s', 'cores': cores,
-            'kind': 'port',
+            'kind': 'port', 'build': build,
+            'transport_steps_per_s': steps / dt,
             'sample': '{} packets of the same workload in {:.1f} s'.format(
                 done, dt)}
 
 
-def cpu_baseline_batch(batch, cores: int, budget_s: float):
-    """The oracle on a seeded 256-instance subset of the batch (budgets cut
-    to 2000 packets each), repeated until the time budget is spent."""
+def cpu_baseline_batch(batch, cores: int, budget_s: float, build: str):
+    """The oracle on the seeded 256-instance subset of the batch (every
+    16th instance, full budgets, 5.3e7 packets), 16 instances at a time,
+    stopping early once budget_s has passed."""
     from pyxopto_amd.mcml.batch import Batch
-    from oracle import pyoracle as O
-    import ctypes
+    O, _ = native_oracle()
     sub = Batch(batch.instances[::16], rnginit=0x5EED000000000001)
     arr, lut, naccum = sub.pack()
-    for i in range(len(arr)):
-        arr[i].npackets = min(int(arr[i].npackets), 2000)
-    per = sum(int(arr[i].npackets) for i in range(len(arr)))
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        O.batch_run(arr, lut, naccum, math_mode=O.MATH_LIBM, nthreads=cores)
-        done += per
-        for i in range(len(arr)):
-            arr[i].seed += 1
+    n = len(arr)
+    done, steps, ninst, t0 = 0, 0, 0, time.perf_counter()
+    for first in range(0, n, 16):
+        if time.perf_counter() - t0 >= budget_s:
+            break
+        part = (type(arr[0]) * min(16, n - first))()
+        for j in range(len(part)):
+            part[j] = arr[first + j]
+        _, counts = O.batch_run(part, lut, naccum, math_mode=O.MATH_LIBM,
+                                nthreads=cores)
+        done += sum(int(part[j].npackets) for j in range(len(part)))
+        steps += int(counts[0])
+        ninst += len(part)
     dt = time.perf_counter() - t0
     return {'value': done / dt, 'unit': 'photon packets/s', 'cores': cores,
-            'kind': 'port',
-            'sample': '{} packets of a 256-instance subset (2000 packets '
-                      'each) in {:.1f} s'.format(done, dt)}
+            'kind': 'port', 'build': build,
+            'transport_steps_per_s': steps / dt,
+            'sample': '{} instances of the 256-instance subset (every 16th, '
+                      'full budgets): {} packets in {:.1f} s'.format(
+                          ninst, done, dt)}
+
+
+def timed(step, steps: int, warmup: int):
+    """Warm-up steps, then `steps` timed steps: (reports, wall seconds)."""
+    first = True
+    for _ in range(warmup):
+        step(first)
+        first = False
+    reps = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        reps.append(step(first))
+        first = False
+    return reps, time.perf_counter() - t0
+
+
+def roofline(reps, voxel: bool, workload: str, ceil=None):
+    kernel_s = sum(r.kernel_ms for r in reps) * 1e-3
+    achieved = sum(algorithmic_slots(r, voxel) for r in reps) / kernel_s / 1e12
+    out = {'bound': 'valu', 'achieved': achieved, 'peak': PEAK_SLOTS / 1e12,
+           'unit': 'T lane-op slots/s', 'frac': achieved / (PEAK_SLOTS / 1e12),
+           'traffic': TRAFFIC_BYTES_PER_LAUNCH.get(workload)}
+    if voxel and ceil:
+        dep = sum(r.deposits for r in reps)
+        out['atomic_deposits_per_s'] = dep / kernel_s
+        out['atomic_GBps'] = dep * 8 / kernel_s / 1e9
+        out['atomic_frac_of_probe'] = dep / kernel_s / ceil['atomic_u64_per_s']
+    return out
+
+
+def config_leg(name: str, steps: int, warmup: int, ceil, cpu: bool,
+               cpu_packets: int):
+    """One single-simulation configuration timed on its own: packets/s,
+    steps/s, its roofline and its CPU baseline."""
+    sim, npk, step, voxel, config = make_workload(name, None)
+    print('bench: leg {} packets/step {}'.format(name, npk), file=sys.stderr,
+          flush=True)
+    reps, wall = timed(step, steps, warmup)
+    kernel_s = sum(r.kernel_ms for r in reps) * 1e-3
+    out = {'config': config, 'steps': steps, 'warmup': warmup,
+           'packets_per_s': npk * steps / wall,
+           'kernel_packets_per_s': npk * steps / kernel_s,
+           'ms_per_step': wall / steps * 1e3,
+           'kernel_ms_per_step': kernel_s / steps * 1e3,
+           'transport_steps_per_s': sum(r.steps for r in reps) / wall,
+           'lanes': int(reps[-1].lanes),
+           'roofline': roofline(reps, voxel, name, ceil)}
+    if voxel:
+        out['voxel_crossings_per_s'] = sum(r.voxel_crossings
+                                           for r in reps) / kernel_s
+    if cpu:
+        out['cpu_baseline'] = cpu_baseline(sim, voxel, cpu_packets)
+        if voxel:
+            out['cpu_baseline']['extrapolated_s_per_step'] = \
+                npk / out['cpu_baseline']['value']
+    sim.context.close()
+    return out
 
 
 def main():
+    global BUILD_OPTS
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
@@ -371,7 +485,12 @@ def main():
                     choices=['layered_batch_4096', 'skin_vessel_200',
                              'slab_1mm'])
     ap.add_argument('--packets', type=int, default=None)
+    ap.add_argument('--exact-math', action='store_true',
+                    help='the exact (bit-exact with the oracle) arithmetic '
+                         'instead of the hardware approximations')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-legs', action='store_true',
+                    help='skip the 1 mm slab and 200^3 voxel configurations')
     ap.add_argument('--no-plain', action='store_true',
                     help='skip the one-launch-per-instance comparison')
     ap.add_argument('--no-serving', action='store_true',
@@ -381,12 +500,14 @@ def main():
     ap.add_argument('--no-surrogate', action='store_true',
                     help='skip the surrogate network throughput')
     ap.add_argument('--qps', type=float, default=SERVING_QPS)
+    ap.add_argument('--qps-sweep', default='400,550',
+                    help='further request rates of the serving run '
+                         '(comma-separated, empty for none)')
     ap.add_argument('--tick-packets', type=int, default=None,
                     help='layered packets per serving tick')
     ap.add_argument('--serving-seconds', type=float, default=8.0)
     ap.add_argument('--lut-processes', type=int, default=None,
-                    help='LUT builder processes (1 under a profiler: no '
-                         'fork after the GPU is initialised)')
+                    help='LUT builder processes')
     ap.add_argument('--launch', default=None,
                     help='CHUNK:BLOCK:BLOCKS_PER_CU[:REFILL_MIN[:DEFER_MIN]] '
                          'instead of the '
@@ -398,29 +519,22 @@ def main():
               'integer all-reduce); got --gpus {}'.format(args.gpus),
               file=sys.stderr)
         sys.exit(2)
+    if args.exact_math:
+        BUILD_OPTS = None
 
     sim, npk, step, voxel, config = make_workload(
         args.workload, args.packets, args.launch, args.lut_processes)
+    config['arithmetic'] = 'exact' if args.exact_math else \
+        'hardware approximations (-cl-fast-relaxed-math)'
     print('bench: workload {} packets/step {} warmup {} steps {}'.format(
         config['workload'], npk, args.warmup, args.steps), file=sys.stderr,
         flush=True)
-    first = True
-    for _ in range(args.warmup):
-        step(first)
-        first = False
-    reps = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        reps.append(step(first))
-        first = False
-    wall = time.perf_counter() - t0
+    reps, wall = timed(step, args.steps, args.warmup)
     print('bench: timed {} steps in {:.3f} s'.format(args.steps, wall),
           file=sys.stderr, flush=True)
 
     kernel_s = sum(r.kernel_ms for r in reps) * 1e-3
     steps_total = sum(r.steps for r in reps)
-    slots = sum(algorithmic_slots(r, voxel) for r in reps)
-    achieved = slots / kernel_s / 1e12
     value = npk * args.steps / wall
     out = {
         'metric': METRIC,
@@ -438,43 +552,45 @@ def main():
         'config': config,
         'transport_steps_per_s': steps_total / wall,
         'kernel_ms_per_step': kernel_s / args.steps * 1e3,
-        'roofline': {
-            'bound': 'valu',
-            'achieved': achieved,
-            'peak': PEAK_SLOTS / 1e12,
-            'unit': 'T lane-op slots/s',
-            'frac': achieved / (PEAK_SLOTS / 1e12),
-            'traffic': TRAFFIC_BYTES_PER_LAUNCH.get(config['workload']),
-        },
     }
     ceil = ceilings(sim.context)
-    out['measured_ceilings'] = ceil
+    out['roofline'] = roofline(reps, voxel, config['workload'], ceil)
     out['roofline']['hbm_copy_GBps'] = ceil['copy_GBps']
+    out['measured_ceilings'] = ceil
     if voxel:
-        dep = sum(r.deposits for r in reps)
-        out['atomic_deposits_per_s'] = dep / kernel_s
-        out['atomic_GBps'] = dep * 8 / kernel_s / 1e9
-        out['atomic_frac_of_probe'] = dep / kernel_s / ceil['atomic_u64_per_s']
+        for key in ('atomic_deposits_per_s', 'atomic_GBps',
+                    'atomic_frac_of_probe'):
+            out[key] = out['roofline'][key]
         out['voxel_crossings_per_s'] = sum(r.voxel_crossings
                                            for r in reps) / kernel_s
     if hasattr(sim, 'bench_plain') and not args.no_plain:
         plain = sim.bench_plain()
         out['plain_one_launch_per_instance'] = plain
         out['batched_vs_plain'] = value / plain['packets_per_s']
+    if hasattr(sim, 'bench_insts') and not args.no_legs:
+        out['configs'] = {
+            'slab_1mm': config_leg('slab_1mm', 20, 3, ceil,
+                                   not args.no_cpu_baseline, 1_000_000),
+            'skin_vessel_200': config_leg('skin_vessel_200', 3, 1, ceil,
+                                          not args.no_cpu_baseline,
+                                          1_000_000)}
     if hasattr(sim, 'bench_insts') and not args.no_published:
         print('bench: published fiber benchmark', file=sys.stderr, flush=True)
         out['published_benchmark'] = published_benchmark()
     if not args.no_surrogate:
         out['surrogate'] = surrogate_rates(sim.context)
     if hasattr(sim, 'bench_insts') and not args.no_serving:
-        print('bench: serving at {} requests/s for {} s'.format(
-            args.qps, args.serving_seconds), file=sys.stderr, flush=True)
-        out['serving'] = serving(sim.bench_insts, args.qps,
+        rates = [args.qps] + [float(v) for v in args.qps_sweep.split(',')
+                              if v.strip()]
+        print('bench: serving at {} requests/s for {} s each'.format(
+            rates, args.serving_seconds), file=sys.stderr, flush=True)
+        out['serving'] = serving(sim.bench_insts, rates,
                                  args.serving_seconds,
                                  tick_packets=args.tick_packets)
         print('bench: serving done', file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        out['cpu_baseline'] = cpu_baseline(sim, voxel)
+        print('bench: cpu baseline', file=sys.stderr, flush=True)
+        out['cpu_baseline'] = cpu_baseline(sim, voxel, npk)
     print(json.dumps(out), flush=True)
 
 

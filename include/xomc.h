@@ -70,8 +70,8 @@ enum xo_src_kind {
 };
 
 /* Stepping method (xopto/mcbase/mcoptions.py:269-318; the branches of
- * mcml.template.c:529-755, mcvox.template.c:763-977). Layered and
- * voxelized runs take all three; batches take albedo weight. */
+ * mcml.template.c:529-755, mcvox.template.c:763-977). Layered, voxelized
+ * and batched runs take all three. */
 enum xo_method {
     XO_METHOD_AW = 0,  /* albedo weight                 */
     XO_METHOD_AR = 1,  /* albedo rejection              */
@@ -122,7 +122,15 @@ enum xo_run_flags {
      * keyed on (cfg->packet_seed, p) (xo_packet_seed) instead of the lane's
      * stream; the context's lane streams are left untouched. Makes a run a
      * function of the configuration alone, like a batch instance. */
-    XO_FLAG_PACKET_KEYED = 1u << 2
+    XO_FLAG_PACKET_KEYED = 1u << 2,
+    /* Hardware approximations (v_log_f32, v_exp_f32, v_sin_f32/v_cos_f32,
+     * v_sqrt_f32, v_rsq_f32, v_rcp_f32) for the logarithm, exponential,
+     * sine/cosine, square root and division of the transport step instead
+     * of the fixed-sequence, correctly rounded arithmetic: the build the
+     * reference makes with -cl-fast-relaxed-math (xopto/cl/cloptions.py:
+     * 137-152; test/performance.py:104). Results then agree with the exact
+     * build statistically, not bit for bit. Ignored by traced runs. */
+    XO_FLAG_FAST_MATH = 1u << 3
 };
 
 /* Phase-function parameters; the union of McPf for Hg, MHg and Lut. */
@@ -350,7 +358,7 @@ typedef struct xo_batch_opts {
     uint32_t chunk;             /* packets per work-queue item (>= 1)        */
     uint32_t block;             /* threads per block: 64, 128 or 256          */
     uint32_t blocks_per_cu;     /* 0: as many as fit                          */
-    uint32_t flags;             /* XO_FLAG_NO_LDS_BINS is ignored             */
+    uint32_t flags;             /* XO_FLAG_FAST_MATH; other flags are ignored */
     float w_min;                /* roulette threshold for every instance     */
     float lottery_chance;
     int32_t use_lottery;
@@ -531,7 +539,9 @@ typedef struct xo_serve_opts {
     float lottery_chance;
     int32_t use_lottery;
     int32_t use_graph;          /* replay a captured graph per tick          */
-} xo_serve_opts; /* 80 B */
+    int32_t fast_math;          /* 1: the XO_FLAG_FAST_MATH arithmetic       */
+    int32_t reserved;
+} xo_serve_opts; /* 88 B */
 
 typedef struct xo_serve_stats {
     uint64_t admitted, completed, cancelled;
@@ -612,6 +622,26 @@ int xo_probe_valu(xo_ctx *ctx, uint32_t iters, double *ms, double *lane_ops);
  * the mismatches of each. No counterpart in the reference. */
 int xo_probe_cr_math(xo_ctx *ctx, uint64_t ndiv, uint64_t seed, uint64_t *sqrt_checked,
                      uint64_t *sqrt_mismatches, uint64_t *div_mismatches);
+
+/* Functions of the transport kernels' arithmetic, for xo_probe_math. */
+enum xo_math_fn {
+    XO_MATH_LOG = 0,        /* natural logarithm                          */
+    XO_MATH_SIN_TURNS = 1,  /* sin(2 pi x)                                */
+    XO_MATH_COS_TURNS = 2,  /* cos(2 pi x)                                */
+    XO_MATH_SQRT = 3,
+    XO_MATH_RCP = 4,        /* 1 / x (the division of the kernels)        */
+    XO_MATH_RSQRT = 5,      /* 1 / sqrt(x) (direction renormalization)    */
+    XO_MATH_CBRT = 6,
+    XO_MATH_EXP = 7,
+    XO_MATH_COUNT = 8
+};
+
+/* out[i] = f(in[i]) on the device for the function `which` (enum
+ * xo_math_fn) as the transport kernels compute it: fast = 0 the exact
+ * arithmetic, fast = 1 the XO_FLAG_FAST_MATH arithmetic. The accuracy
+ * check of the fast build. No counterpart in the reference. */
+int xo_probe_math(xo_ctx *ctx, int fast, int which, uint32_t n, const float *in,
+                  float *out);
 
 /* ---------------- reflectance surrogate (device) ---------------- */
 

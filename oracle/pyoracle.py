@@ -33,61 +33,80 @@ def lib():
     if _lib is None:
         if not os.path.isfile(LIB_PATH):
             build()
-        l = ctypes.CDLL(LIB_PATH)
-        l.oracle_mcml_run.restype = c_int
-        l.oracle_mcml_run.argtypes = [
-            POINTER(N.XoMcmlCfg), POINTER(c_float), c_uint32,
-            POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int, c_int,
-            POINTER(c_uint64), c_uint64, POINTER(c_uint64), POINTER(c_uint32)]
-        l.oracle_mcml_run_trace.restype = c_int
-        l.oracle_mcml_run_trace.argtypes = [
-            POINTER(N.XoMcmlCfg), POINTER(c_float), c_uint32,
-            POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int, c_int,
-            POINTER(c_uint64), c_uint64, POINTER(c_uint64), POINTER(c_uint32),
-            POINTER(N.XoTrace), POINTER(c_float), POINTER(c_int32)]
-        l.oracle_mcvox_run_trace.restype = c_int
-        l.oracle_mcvox_run_trace.argtypes = [
-            POINTER(N.XoMcvoxCfg), POINTER(c_int32), POINTER(c_float),
-            c_uint32, POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int,
-            c_int, POINTER(c_uint64), c_uint64, POINTER(c_uint64),
-            POINTER(c_uint32), POINTER(N.XoTrace), POINTER(c_float),
-            POINTER(c_int32)]
-        l.oracle_sampling_volume.restype = c_int
-        l.oracle_sampling_volume.argtypes = [
-            POINTER(N.XoSamplingVolume), c_int32, POINTER(c_float),
-            POINTER(c_int32), c_uint64, POINTER(c_uint64), c_uint64,
-            POINTER(c_uint64), POINTER(c_uint64)]
-        l.oracle_mcvox_run.restype = c_int
-        l.oracle_mcvox_run.argtypes = [
-            POINTER(N.XoMcvoxCfg), POINTER(c_int32), POINTER(c_float),
-            c_uint32, POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int,
-            c_int, POINTER(c_uint64), c_uint64, POINTER(c_uint64),
-            POINTER(c_uint32)]
-        l.oracle_rng.restype = c_int
-        l.oracle_rng.argtypes = [POINTER(c_uint64), c_uint32, c_uint32,
-                                 POINTER(c_float)]
-        l.oracle_max_threads.restype = c_int
-        l.oracle_batch_run.restype = c_int
-        l.oracle_batch_run.argtypes = [
-            POINTER(N.XoBatchInstance), c_uint32, POINTER(c_float),
-            POINTER(c_uint32), c_float, c_float, c_int, c_int, c_int, c_int,
-            POINTER(c_uint64), c_uint64, POINTER(c_uint64)]
-        l.oracle_pf_samples.restype = c_int
-        l.oracle_pf_samples.argtypes = [
-            c_int, POINTER(N.XoPf), POINTER(c_float), c_uint64, c_uint32,
-            c_uint32, c_int, POINTER(c_float), POINTER(c_float)]
-        l.oracle_scatter.restype = c_int
-        l.oracle_scatter.argtypes = [c_uint32, POINTER(c_float),
-                                     POINTER(c_float), POINTER(c_float), c_int,
-                                     POINTER(c_float)]
-        l.oracle_reflectance.restype = c_int
-        l.oracle_reflectance.argtypes = [c_float, c_float, c_uint32,
-                                         POINTER(c_float), POINTER(c_float)]
-        l.oracle_math.restype = c_int
-        l.oracle_math.argtypes = [c_int, c_uint32, POINTER(c_float),
-                                  POINTER(c_float), POINTER(c_float)]
-        _lib = l
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def use_library(path: str):
+    """Load the oracle from another build of xo_oracle.c (the host-tuned
+    CPU baseline of bench.py) for every later call."""
+    global _lib
+    _lib = _load(path)
+
+
+def build_native(out_path: str) -> str:
+    """Build xo_oracle.c for this host (gcc -O3 -march=native -fopenmp,
+    the oracle/Makefile `native` recipe) into out_path."""
+    subprocess.run(['make', '-s', '-C', HERE, 'native', 'OUT=' + out_path],
+                   check=True)
+    return out_path
+
+
+def _load(path):
+    l = ctypes.CDLL(path)
+    l.oracle_mcml_run.restype = c_int
+    l.oracle_mcml_run.argtypes = [
+        POINTER(N.XoMcmlCfg), POINTER(c_float), c_uint32,
+        POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int, c_int,
+        POINTER(c_uint64), c_uint64, POINTER(c_uint64), POINTER(c_uint32)]
+    l.oracle_mcml_run_trace.restype = c_int
+    l.oracle_mcml_run_trace.argtypes = [
+        POINTER(N.XoMcmlCfg), POINTER(c_float), c_uint32,
+        POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int, c_int,
+        POINTER(c_uint64), c_uint64, POINTER(c_uint64), POINTER(c_uint32),
+        POINTER(N.XoTrace), POINTER(c_float), POINTER(c_int32)]
+    l.oracle_mcvox_run_trace.restype = c_int
+    l.oracle_mcvox_run_trace.argtypes = [
+        POINTER(N.XoMcvoxCfg), POINTER(c_int32), POINTER(c_float),
+        c_uint32, POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int,
+        c_int, POINTER(c_uint64), c_uint64, POINTER(c_uint64),
+        POINTER(c_uint32), POINTER(N.XoTrace), POINTER(c_float),
+        POINTER(c_int32)]
+    l.oracle_sampling_volume.restype = c_int
+    l.oracle_sampling_volume.argtypes = [
+        POINTER(N.XoSamplingVolume), c_int32, POINTER(c_float),
+        POINTER(c_int32), c_uint64, POINTER(c_uint64), c_uint64,
+        POINTER(c_uint64), POINTER(c_uint64)]
+    l.oracle_mcvox_run.restype = c_int
+    l.oracle_mcvox_run.argtypes = [
+        POINTER(N.XoMcvoxCfg), POINTER(c_int32), POINTER(c_float),
+        c_uint32, POINTER(c_uint64), POINTER(c_uint32), c_uint64, c_int,
+        c_int, POINTER(c_uint64), c_uint64, POINTER(c_uint64),
+        POINTER(c_uint32)]
+    l.oracle_rng.restype = c_int
+    l.oracle_rng.argtypes = [POINTER(c_uint64), c_uint32, c_uint32,
+                             POINTER(c_float)]
+    l.oracle_max_threads.restype = c_int
+    l.oracle_batch_run.restype = c_int
+    l.oracle_batch_run.argtypes = [
+        POINTER(N.XoBatchInstance), c_uint32, POINTER(c_float),
+        POINTER(c_uint32), c_float, c_float, c_int, c_int, c_int, c_int,
+        POINTER(c_uint64), c_uint64, POINTER(c_uint64)]
+    l.oracle_pf_samples.restype = c_int
+    l.oracle_pf_samples.argtypes = [
+        c_int, POINTER(N.XoPf), POINTER(c_float), c_uint64, c_uint32,
+        c_uint32, c_int, POINTER(c_float), POINTER(c_float)]
+    l.oracle_scatter.restype = c_int
+    l.oracle_scatter.argtypes = [c_uint32, POINTER(c_float),
+                                 POINTER(c_float), POINTER(c_float), c_int,
+                                 POINTER(c_float)]
+    l.oracle_reflectance.restype = c_int
+    l.oracle_reflectance.argtypes = [c_float, c_float, c_uint32,
+                                     POINTER(c_float), POINTER(c_float)]
+    l.oracle_math.restype = c_int
+    l.oracle_math.argtypes = [c_int, c_uint32, POINTER(c_float),
+                              POINTER(c_float), POINTER(c_float)]
+    return l
 
 
 def _p(arr, ctype):

@@ -41,6 +41,9 @@ XO_TRACE_ENTRY_LEN = 8
 XO_FLAG_STATIC_SCHEDULE = 1 << 0
 XO_FLAG_NO_LDS_BINS = 1 << 1
 XO_FLAG_PACKET_KEYED = 1 << 2
+XO_FLAG_FAST_MATH = 1 << 3
+(XO_MATH_LOG, XO_MATH_SIN_TURNS, XO_MATH_COS_TURNS, XO_MATH_SQRT, XO_MATH_RCP,
+ XO_MATH_RSQRT, XO_MATH_CBRT, XO_MATH_EXP) = range(8)
 
 f3 = c_float * 3
 
@@ -156,7 +159,8 @@ class XoServeOpts(ctypes.Structure):
                 ('chunk', c_uint32), ('block', c_uint32),
                 ('blocks_per_cu', c_uint32), ('w_min', c_float),
                 ('lottery_chance', c_float), ('use_lottery', c_int32),
-                ('use_graph', c_int32)]
+                ('use_graph', c_int32), ('fast_math', c_int32),
+                ('reserved', c_int32)]
 
 
 class XoServeStats(ctypes.Structure):
@@ -300,6 +304,8 @@ PROTOTYPES = {
     'xo_probe_cr_math': (c_int, [c_void_p, c_uint64, c_uint64,
                                  POINTER(c_uint64), POINTER(c_uint64),
                                  POINTER(c_uint64)]),
+    'xo_probe_math': (c_int, [c_void_p, c_int, c_int, c_uint32,
+                              POINTER(c_float), POINTER(c_float)]),
 }
 
 _lib = None
@@ -573,6 +579,17 @@ class Context:
                                      ctypes.byref(bs), ctypes.byref(bd)),
               'xo_probe_cr_math')
         return int(chk.value), int(bs.value), int(bd.value)
+
+    def probe_math(self, which: int, values, fast: bool):
+        """f(values) on the device for the function `which` (XO_MATH_*) in
+        the exact or the fast arithmetic of the kernels."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        out = np.zeros_like(v)
+        check(lib().xo_probe_math(self._h, c_int(int(bool(fast))),
+                                  c_int(int(which)), c_uint32(v.size),
+                                  _ptr(v, c_float), _ptr(out, c_float)),
+              'xo_probe_math')
+        return out
 
     def probe_atomic_u64(self, array_bytes: int, natomics: int,
                          seed: int = 1):

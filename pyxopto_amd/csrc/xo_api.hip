@@ -874,6 +874,7 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
     P.refill_min = opts->refill_min ? opts->refill_min : 4u;
     P.defer_min = opts->defer_min ? opts->defer_min : xo::kDeferMinDefault;
     P.method = opts->method;
+    P.fast_math = (opts->flags & XO_FLAG_FAST_MATH) ? 1 : 0;
     int bpc = 1;
     XO_HIP(xo::occupancy_batch(block, &bpc));
     if (opts->blocks_per_cu && (int)opts->blocks_per_cu < bpc) bpc = (int)opts->blocks_per_cu;

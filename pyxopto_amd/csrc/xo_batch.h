@@ -95,6 +95,7 @@ struct BatchParams {
     uint32_t defer_min;          /* lanes holding a boundary event that make
                                     the wave handle them (>= 1)             */
     int32_t method;              /* enum xo_method of every instance        */
+    int32_t fast_math;           /* 1: MathFast arithmetic, 0: MathExact    */
 };
 
 hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,

@@ -29,18 +29,19 @@ struct BatchBlockStats {
 
 /* Phase-function sampling by the instance's kind; each branch reads only
  * the layer fields its kind uses. */
+template <class M>
 XO_DEV float pf_sample_rt(int kind, const LayerL &L, const float *lut, uint64_t &x,
                           uint32_t a, float &xi_fi) {
     if (kind == XO_PF_HG) {
         const PfL pf{L.g, 0.0f, 0.0f, 0.0f, 0.0f, 0u, 0u};
-        return pf_sample<XO_PF_HG>(pf, lut, x, a, xi_fi);
+        return pf_sample<XO_PF_HG, M>(pf, lut, x, a, xi_fi);
     }
     if (kind == XO_PF_MHG) {
         const PfL pf{L.g, L.beta, 0.0f, 0.0f, 0.0f, 0u, 0u};
-        return pf_sample<XO_PF_MHG>(pf, lut, x, a, xi_fi);
+        return pf_sample<XO_PF_MHG, M>(pf, lut, x, a, xi_fi);
     }
     const PfL pf{0.0f, 0.0f, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
-    return pf_sample<XO_PF_LUT>(pf, lut, x, a, xi_fi);
+    return pf_sample<XO_PF_LUT, M>(pf, lut, x, a, xi_fi);
 }
 
 /* Pending events of the split transport step. */
@@ -70,8 +71,8 @@ __device__ __forceinline__ void end_event(bool done, bool &alive, float px, floa
 /* At most 64 VGPRs (8 waves per SIMD): the compiler fits the loop without
  * spills, and the extra wave hides more of the layer and LUT reads (+2.5 %
  * packets/s over 72 VGPRs and 7 waves). METHOD is the stepping method of
- * every instance (xo_method). */
-template <int METHOD>
+ * every instance (xo_method), M the arithmetic (MathExact, MathFast). */
+template <int METHOD, class M>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 batch_kernel(const BatchParams P) {
     __shared__ BatchBlockStats sstats;
@@ -139,7 +140,7 @@ batch_kernel(const BatchParams P) {
                 if (lane == 0) {
                     const BatchHdr &H = P.hdr[cur_inst];
                     if (H.src_kind == XO_SRC_LINE)
-                        detector_deposit_n(H.specular, bins, H.src_pos[0], H.src_pos[1],
+                        detector_deposit_n<M>(H.specular, bins, H.src_pos[0], H.src_pos[1],
                                            H.dir_reflected[0], H.dir_reflected[1],
                                            H.dir_reflected[2], H.reflectance, item.count);
                 }
@@ -167,17 +168,18 @@ batch_kernel(const BatchParams P) {
             xo_packet_state(H.seed, my_p, P.table, &x, &a);
             if (H.src_kind == XO_SRC_FIBER_NI) {
                 const float n1 = layers[1].n;
-                const float R = fiber_ni_launch(H.fiber, n1, x, a, px, py, pz, dx, dy, dz, w);
+                const float R =
+                    fiber_ni_launch<M>(H.fiber, n1, x, a, px, py, pz, dx, dy, dz, w);
                 if (H.specular.kind != XO_DET_NONE) {
                     float rx, ry, rz;
-                    fiber_specular_direction(dx, dy, dz, n1, H.fiber.n, rx, ry, rz);
-                    detector_deposit(H.specular, bins, px, py, rx, ry, rz, R);
+                    fiber_specular_direction<M>(dx, dy, dz, n1, H.fiber.n, rx, ry, rz);
+                    detector_deposit<M>(H.specular, bins, px, py, rx, ry, rz, R);
                 }
             } else if (H.src_kind == XO_SRC_FIBER) {
                 /* UniformFiber, xopto/mcml/mcsource/fiber.py:265-330 */
-                const float R = fiber_launch(H.fiber, H.fiber_axis, layers[1].n, x, a, px, py,
-                                             pz, dx, dy, dz, w);
-                detector_deposit(H.specular, bins, px, py, dx, dy, dz, R);
+                const float R = fiber_launch<M>(H.fiber, H.fiber_axis, layers[1].n, x, a, px,
+                                                py, pz, dx, dy, dz, w);
+                detector_deposit<M>(H.specular, bins, px, py, dx, dy, dz, R);
             } else {
                 /* Line source, xopto/mcml/mcsource/line.py:95-110 */
                 w = 1.0f - H.reflectance;
@@ -203,7 +205,7 @@ batch_kernel(const BatchParams P) {
          * (mcml.template.c:584-606; a batch has no fluence grid, so no
          * deposition point is drawn) */
         auto mbl_absorb = [&](const LayerL &L, float step) {
-            const float fraction = 1.0f - xo_expf(-(L.mua * step));
+            const float fraction = 1.0f - M::exp(-(L.mua * step));
             const float deposit = fraction * w;
             w -= deposit;
         };
@@ -244,11 +246,11 @@ batch_kernel(const BatchParams P) {
                 int32_t next = layer;
                 if (__builtin_fmaf(step, dz, pz) < L.top) {
                     --next;
-                    if (fabsf(dz) != 0.0f) step = (L.top - pz) / dz;
+                    if (fabsf(dz) != 0.0f) step = M::div(L.top - pz, dz);
                 }
                 if (__builtin_fmaf(step, dz, pz) >= L.bottom) {
                     ++next;
-                    if (fabsf(dz) != 0.0f) step = (L.bottom - pz) / dz;
+                    if (fabsf(dz) != 0.0f) step = M::div(L.bottom - pz, dz);
                 }
                 px = __builtin_fmaf(dx, step, px);
                 py = __builtin_fmaf(dy, step, py);
@@ -258,14 +260,14 @@ batch_kernel(const BatchParams P) {
                 if (METHOD == XO_METHOD_MBL) mbl_absorb(L, step);
                 if (next != layer) {
                     atomicAdd(&scount[2][threadIdx.x], 1u);
-                    mcml_boundary(layers, layer, next, dx, dy, dz, x, a);
+                    mcml_boundary<M>(layers, layer, next, dx, dy, dz, x, a);
                     bool done = false;
                     if (!(layer > 0 && layer < nl - 1)) {
                         const BatchHdr &H = P.hdr[inst];
                         if (layer <= 0)
-                            detector_deposit(H.top, bins, px, py, dx, dy, dz, w);
+                            detector_deposit<M>(H.top, bins, px, py, dx, dy, dz, w);
                         else
-                            detector_deposit(H.bottom, bins, px, py, dx, dy, dz, w);
+                            detector_deposit<M>(H.bottom, bins, px, py, dx, dy, dz, w);
                         done = true;
                     }
                     done = mbl_roulette(done);
@@ -284,9 +286,9 @@ batch_kernel(const BatchParams P) {
             atomicAdd(&scount[1][threadIdx.x], 1u);
             float step;
             if (METHOD == XO_METHOD_MBL)
-                step = (-xo_logf_unit(rng_next(x, a))) / L.mus; /* mcml.template.c:529-531 */
+                step = M::div(-M::log_unit(rng_next(x, a)), L.mus); /* mcml.template.c:529-531 */
             else
-                step = -xo_logf_unit(rng_next(x, a)) * L.inv_mut;
+                step = -M::log_unit(rng_next(x, a)) * L.inv_mut;
             step = fminf(step, XO_FLT_MAX);
             const float zend = __builtin_fmaf(step, dz, pz);
             if (zend < L.top || zend >= L.bottom) {
@@ -314,8 +316,8 @@ batch_kernel(const BatchParams P) {
                     w -= deposit;
                 }
                 float xi_fi;
-                float ct = pf_sample_rt(pf_kind, L, P.lut, x, a, xi_fi);
-                scatter_direction(dx, dy, dz, ct, xi_fi);
+                float ct = pf_sample_rt<M>(pf_kind, L, P.lut, x, a, xi_fi);
+                scatter_direction<M>(dx, dy, dz, ct, xi_fi);
                 if (METHOD == XO_METHOD_AW)
                     done = roulette(w, x, a, P.w_min, P.chance, P.use_lottery);
             }
@@ -334,23 +336,32 @@ batch_kernel(const BatchParams P) {
         atomicAdd(&P.stats[threadIdx.x], sstats.v[threadIdx.x]);
 }
 
-hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,
-                        hipStream_t stream) {
+template <class M>
+static hipError_t launch_batch_m(const BatchParams &P, uint32_t blocks, uint32_t block,
+                                 hipStream_t stream) {
     switch (P.method) {
     case XO_METHOD_AW:
-        hipLaunchKernelGGL(batch_kernel<XO_METHOD_AW>, dim3(blocks), dim3(block), 0, stream, P);
+        hipLaunchKernelGGL((batch_kernel<XO_METHOD_AW, M>), dim3(blocks), dim3(block), 0,
+                           stream, P);
         break;
     case XO_METHOD_AR:
-        hipLaunchKernelGGL(batch_kernel<XO_METHOD_AR>, dim3(blocks), dim3(block), 0, stream, P);
+        hipLaunchKernelGGL((batch_kernel<XO_METHOD_AR, M>), dim3(blocks), dim3(block), 0,
+                           stream, P);
         break;
     case XO_METHOD_MBL:
-        hipLaunchKernelGGL(batch_kernel<XO_METHOD_MBL>, dim3(blocks), dim3(block), 0, stream,
-                           P);
+        hipLaunchKernelGGL((batch_kernel<XO_METHOD_MBL, M>), dim3(blocks), dim3(block), 0,
+                           stream, P);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,
+                        hipStream_t stream) {
+    return P.fast_math ? launch_batch_m<MathFast>(P, blocks, block, stream)
+                       : launch_batch_m<MathExact>(P, blocks, block, stream);
 }
 
 __global__ void zero_u32_kernel(uint32_t *ptr, uint32_t n) {
@@ -366,9 +377,8 @@ hipError_t launch_zero_u32(uint32_t *ptr, uint32_t n, hipStream_t stream) {
 
 hipError_t occupancy_batch(uint32_t block, int *blocks_per_cu) {
     /* the albedo-weight build stands for all three methods */
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
-                                                        batch_kernel<XO_METHOD_AW>, (int)block,
-                                                        0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, batch_kernel<XO_METHOD_AW, MathExact>, (int)block, 0);
 }
 
 } // namespace xo

@@ -85,6 +85,65 @@ XO_DEV float div_by_k(float a, float k) {
     return a / k;
 }
 
+/* Arithmetic of the transport step, a template parameter of the kernels.
+ *
+ * MathExact: the fixed-sequence log, sin/cos, exp and cube root of
+ * xo_math.h and correctly rounded division and square root (IEEE 754 fixes
+ * their results), so a host build of the same operations rounds the same
+ * way: the kernels then match the oracle bit for bit.
+ *
+ * MathFast: one hardware instruction each, v_log_f32 (log2), v_exp_f32
+ * (exp2), v_sin_f32 / v_cos_f32 (argument in turns), v_sqrt_f32, v_rsq_f32
+ * and v_rcp_f32 (division = multiplication by the reciprocal). Each is
+ * within a few ulp (checked against float64 by the math probe); the
+ * results are then statistically, not bitwise, those of the oracle. This
+ * is the build the reference makes with -cl-fast-relaxed-math
+ * (xopto/cl/cloptions.py:137-152), the option of its published benchmark
+ * (test/performance.py:104). */
+struct MathExact {
+    static constexpr bool fast = false;
+    /* ln(u) for u = +0 or a normal u <= 1 */
+    XO_DEV static float log_unit(float u) { return xo_logf_unit(u); }
+    XO_DEV static float log(float x) { return xo_logf(x); }
+    XO_DEV static float exp(float x) { return xo_expf(x); }
+    XO_DEV static void sincos_turns(float t, float *s, float *c) { xo_sincos_turns(t, s, c); }
+    XO_DEV static float cbrt(float x) { return xo_cbrtf(x); }
+    XO_DEV static float sqrt(float x) { return __builtin_sqrtf(x); }
+    /* the range-restricted forms (see sqrt_in_range, div_in_range,
+     * div_by_k for their ranges) */
+    XO_DEV static float sqrt_r(float x) { return sqrt_in_range(x); }
+    XO_DEV static float div_r(float a, float b) { return div_in_range(a, b); }
+    XO_DEV static float div_k(float a, float k) { return div_by_k(a, k); }
+    /* 1 / sqrt(x) for the squared norm x of a near-unit vector */
+    XO_DEV static float rsqrt_r(float x) { return div_in_range(1.0f, sqrt_in_range(x)); }
+    XO_DEV static float div(float a, float b) { return a / b; }
+};
+
+struct MathFast {
+    static constexpr bool fast = true;
+    XO_DEV static float log_unit(float u) {
+        return __builtin_amdgcn_logf(u) * 0.693147180559945309f;
+    }
+    XO_DEV static float log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+    XO_DEV static float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+    XO_DEV static void sincos_turns(float t, float *s, float *c) {
+        *s = __builtin_amdgcn_sinf(t);
+        *c = __builtin_amdgcn_cosf(t);
+    }
+    /* sign(x) 2^(log2|x| / 3); +-0 stays +-0 (log2 0 = -inf, 2^-inf = 0) */
+    XO_DEV static float cbrt(float x) {
+        const float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(fabsf(x)) *
+                                               0.333333333333333333f);
+        return copysignf(y, x);
+    }
+    XO_DEV static float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    XO_DEV static float sqrt_r(float x) { return __builtin_amdgcn_sqrtf(x); }
+    XO_DEV static float div_r(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    XO_DEV static float div_k(float a, float k) { return a * __builtin_amdgcn_rcpf(k); }
+    XO_DEV static float rsqrt_r(float x) { return __builtin_amdgcn_rsqf(x); }
+    XO_DEV static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+};
+
 /* fp_random_single (mcbase.template.c:1576-1594): one MWC step. The state
  * update is one v_mad_u64_u32; (float)(uint32)x / (float)0xFFFFFFFF divides
  * by 2^32 exactly, which is the multiply below. */
@@ -124,6 +183,7 @@ XO_DEV void bins_add(const Bins &b, uint32_t index, uint32_t w) {
 /* Radial (radial.py:117-150) / Total (total.py:86-108) detector deposit.
  * accu_64_deposit_32 (mcbase.template.c:58-61) becomes one u64 atomic add:
  * the modular sum is identical. */
+template <class M = MathExact>
 XO_DEV void detector_deposit(const xo_detector &d, const Bins &bins, float px,
                              float py, float dx, float dy, float dz, float w) {
     if (d.kind == XO_DET_NONE) return;
@@ -131,8 +191,8 @@ XO_DEV void detector_deposit(const xo_detector &d, const Bins &bins, float px,
     if (d.kind == XO_DET_RADIAL) {
         float ddx = px - d.position[0];
         float ddy = py - d.position[1];
-        float r = __builtin_sqrtf(ddx * ddx + ddy * ddy);
-        if (d.log_scale) r = xo_logf(fmaxf(r, kRmin));
+        float r = M::sqrt(ddx * ddx + ddy * ddy);
+        if (d.log_scale) r = M::log(fmaxf(r, kRmin));
         int32_t ri = f2i_sat((r - d.r_min) * d.inv_dr);
         ri = ri < 0 ? 0 : (ri > (int32_t)d.n - 1 ? (int32_t)d.n - 1 : ri);
         index = (uint32_t)ri;
@@ -147,6 +207,7 @@ XO_DEV void detector_deposit(const xo_detector &d, const Bins &bins, float px,
 /* The deposit of n packets that all leave at the same point in the same
  * direction with the same weight: n times detector_deposit's integer, in
  * one atomic (the modular sum is the same). */
+template <class M = MathExact>
 XO_DEV void detector_deposit_n(const xo_detector &d, const Bins &bins, float px, float py,
                                float dx, float dy, float dz, float w, uint32_t n) {
     if (d.kind == XO_DET_NONE || n == 0u) return;
@@ -154,8 +215,8 @@ XO_DEV void detector_deposit_n(const xo_detector &d, const Bins &bins, float px,
     if (d.kind == XO_DET_RADIAL) {
         float ddx = px - d.position[0];
         float ddy = py - d.position[1];
-        float r = __builtin_sqrtf(ddx * ddx + ddy * ddy);
-        if (d.log_scale) r = xo_logf(fmaxf(r, kRmin));
+        float r = M::sqrt(ddx * ddx + ddy * ddy);
+        if (d.log_scale) r = M::log(fmaxf(r, kRmin));
         int32_t ri = f2i_sat((r - d.r_min) * d.inv_dr);
         ri = ri < 0 ? 0 : (ri > (int32_t)d.n - 1 ? (int32_t)d.n - 1 : ri);
         index = (uint32_t)ri;
@@ -199,6 +260,7 @@ XO_DEV void trace_event(const xo_trace &T, float *data, TraceState &ts, float px
 /* Fluence deposit at a point (deposition mode): the Cartesian voxel of
  * mcsim_fluence_deposit_at (fluence.py:102-147) or the (r, z) cell of
  * fluencerz.py:112-160. Returns true if a non-zero integer was added. */
+template <class M = MathExact>
 XO_DEV bool fluence_deposit(const xo_fluence &F, unsigned long long *accum, float px,
                             float py, float pz, float w) {
     uint64_t index;
@@ -214,7 +276,7 @@ XO_DEV bool fluence_deposit(const xo_fluence &F, unsigned long long *accum, floa
     } else if (F.enabled == XO_FLUENCE_RZ) {
         const float ddx = px - F.top_left[0];
         const float ddy = py - F.top_left[1];
-        const float r = __builtin_sqrtf(ddx * ddx + ddy * ddy);
+        const float r = M::sqrt(ddx * ddx + ddy * ddy);
         const float ddz = pz - F.top_left[2];
         const float fr = r * F.inv_step[0];
         const float fz = ddz * F.inv_step[2];
@@ -295,7 +357,7 @@ struct PfL {
 /* mcsim_pf_sample_angles: returns cos(theta), the azimuth as a fraction of
  * a turn in xi_fi. HG hg.py:80-96; MHg mhg.py:87-109; Lut lut.py:125-150
  * (index clamped to the table). */
-template <int PF>
+template <int PF, class M = MathExact>
 XO_DEV float pf_sample(const PfL &pf, const float *lut, uint64_t &x, uint32_t a,
                        float &xi_fi) {
     xi_fi = rng_next(x, a);
@@ -304,8 +366,8 @@ XO_DEV float pf_sample(const PfL &pf, const float *lut, uint64_t &x, uint32_t a,
         /* |g| <= 1 (the host clips it): the numerator is +0 or >= 2^-24, the
          * denominator 0 or in [2^-24, 2]; a 0 denominator gives NaN either
          * way, which the clamp below turns into the same cosine */
-        float k = div_in_range(1.0f - g * g, 1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
-        float ct = (1.0f + g * g - k * k) / (2.0f * g);
+        float k = M::div_r(1.0f - g * g, 1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
+        float ct = M::div(1.0f + g * g - k * k, 2.0f * g);
         if (g == 0.0f) ct = 1.0f - 2.0f * rng_next(x, a);
         return fmaxf(fminf(ct, 1.0f), -1.0f);
     } else if constexpr (PF == XO_PF_MHG) {
@@ -313,16 +375,16 @@ XO_DEV float pf_sample(const PfL &pf, const float *lut, uint64_t &x, uint32_t a,
         float ct;
         if (rng_next(x, a) <= pf.beta) {
             /* as for Hg: |g| <= 1 */
-            float k = div_in_range(1.0f - g * g, 1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
-            ct = (1.0f + g * g - k * k) / (2.0f * g);
+            float k = M::div_r(1.0f - g * g, 1.0f + g * (2.0f * rng_next(x, a) - 1.0f));
+            ct = M::div(1.0f + g * g - k * k, 2.0f * g);
             if (g == 0.0f) ct = 1.0f - 2.0f * rng_next(x, a);
         } else {
-            ct = xo_cbrtf(2.0f * rng_next(x, a) - 1.0f);
+            ct = M::cbrt(2.0f * rng_next(x, a) - 1.0f);
         }
         return fminf(fmaxf(ct, -1.0f), 1.0f);
     } else {
         float fsize = (float)pf.lsize_m1;
-        float fi = (pf.la / (rng_next(x, a) - pf.lc) - pf.lb + 1.0f) * fsize *
+        float fi = (M::div(pf.la, rng_next(x, a) - pf.lc) - pf.lb + 1.0f) * fsize *
                    0.5f;
         fi = fminf(fmaxf(fi, 0.0f), fsize);
         float fl = floorf(fi);
@@ -335,11 +397,12 @@ XO_DEV float pf_sample(const PfL &pf, const float *lut, uint64_t &x, uint32_t a,
 
 /* scatter_direction (mcbase.template.c:1522-1556) with the single-precision
  * renormalization (mc_normalize_fpv3, :975-981). */
+template <class M = MathExact>
 XO_DEV void scatter_direction(float &dx, float &dy, float &dz, float ct,
                               float xi_fi) {
     float sin_fi, cos_fi;
-    float st = sqrt_in_range(1.0f - ct * ct);
-    xo_sincos_turns(xi_fi, &sin_fi, &cos_fi);
+    float st = M::sqrt_r(1.0f - ct * ct);
+    M::sincos_turns(xi_fi, &sin_fi, &cos_fi);
     float stcf = st * cos_fi;
     float stsf = st * sin_fi;
     float px = dx, py = dy, pz = dz;
@@ -348,38 +411,39 @@ XO_DEV void scatter_direction(float &dx, float &dy, float &dz, float ct,
         dy = stsf;
         dz = copysignf(ct, pz * ct);
     } else {
-        float k = sqrt_in_range(1.0f - pz * pz);
-        dx = div_by_k(stcf * px * pz - stsf * py, k) + px * ct;
-        dy = div_by_k(stcf * py * pz + stsf * px, k) + py * ct;
+        float k = M::sqrt_r(1.0f - pz * pz);
+        dx = M::div_k(stcf * px * pz - stsf * py, k) + px * ct;
+        dy = M::div_k(stcf * py * pz + stsf * px, k) + py * ct;
         dz = (-stcf) * k + pz * ct;
     }
-    float inv = div_in_range(1.0f, sqrt_in_range(dx * dx + dy * dy + dz * dz));
+    float inv = M::rsqrt_r(dx * dx + dy * dy + dz * dz);
     dx = dx * inv;
     dy = dy * inv;
     dz = dz * inv;
 }
 
 /* cos_critical (mcbase.template.c:1229-1231). */
-XO_DEV float cos_critical(float n1, float n2) {
-    return (n1 > n2) ? sqrt_in_range(1.0f - (n2 * n2) / (n1 * n1)) : 0.0f;
+template <class M = MathExact> XO_DEV float cos_critical(float n1, float n2) {
+    return (n1 > n2) ? M::sqrt_r(1.0f - M::div(n2 * n2, n1 * n1)) : 0.0f;
 }
 
 /* reflectance (mcbase.template.c:1244-1290). */
+template <class M = MathExact>
 XO_DEV float reflectance(float n1, float n2, float cos1, float cc) {
     float R = 1.0f;
     cos1 = fabsf(cos1);
     if (n1 == n2) return 0.0f;
     if (cos1 > cc) {
-        float n1_d_n2 = n1 / n2;
-        float sin1 = sqrt_in_range(1.0f - cos1 * cos1);
+        float n1_d_n2 = M::div(n1, n2);
+        float sin1 = M::sqrt_r(1.0f - cos1 * cos1);
         if (cos1 >= kCos0) sin1 = 0.0f;
         float sin2 = fminf(1.0f, n1_d_n2 * sin1);
-        float cos2 = sqrt_in_range(1.0f - sin2 * sin2);
+        float cos2 = M::sqrt_r(1.0f - sin2 * sin2);
         float n_cos1 = n1_d_n2 * cos1;
         float n_cos2 = n1_d_n2 * cos2;
-        float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
+        float Rs = M::div(n_cos1 - cos2, n_cos1 + cos2);
         Rs *= Rs;
-        float Rp = (n_cos2 - cos1) / (n_cos2 + cos1);
+        float Rp = M::div(n_cos2 - cos1, n_cos2 + cos1);
         Rp *= Rp;
         R = 0.5f * (Rp + Rs);
         if (cos1 <= kCos90 || sin2 == 1.0f) return 1.0f;
@@ -391,21 +455,22 @@ XO_DEV float fsign(float v) { return (v >= 0.0f) ? 1.0f : -1.0f; }
 
 /* reflectance_cos2 (mcbase.template.c:1303-1346): Fresnel reflectance from
  * the cosine of the transmitted angle. */
+template <class M = MathExact>
 XO_DEV float reflectance_cos2(float n1, float n2, float cos2) {
     float R = 1.0f;
     cos2 = fabsf(cos2);
     if (n1 == n2) return 0.0f;
-    float sin2 = __builtin_sqrtf(1.0f - cos2 * cos2);
+    float sin2 = M::sqrt(1.0f - cos2 * cos2);
     if (cos2 >= kCos0) sin2 = 0.0f;
-    float sin1 = (n2 / n1) * sin2;
+    float sin1 = M::div(n2, n1) * sin2;
     if (sin1 < 1.0f) {
-        float cos1 = __builtin_sqrtf(1.0f - sin1 * sin1);
-        float n1_d_n2 = n1 / n2;
+        float cos1 = M::sqrt(1.0f - sin1 * sin1);
+        float n1_d_n2 = M::div(n1, n2);
         float n_cos1 = n1_d_n2 * cos1;
         float n_cos2 = n1_d_n2 * cos2;
-        float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
+        float Rs = M::div(n_cos1 - cos2, n_cos1 + cos2);
         Rs *= Rs;
-        float Rp = (n_cos2 - cos1) / (n_cos2 + cos1);
+        float Rp = M::div(n_cos2 - cos1, n_cos2 + cos1);
         Rp *= Rp;
         R = 0.5f * (Rp + Rs);
         if (cos1 <= kCos90 || sin2 == 1.0f) return 1.0f;
@@ -417,24 +482,25 @@ XO_DEV float reflectance_cos2(float n1, float n2, float cos2) {
  * uniform over the core, a direction uniform in cos(theta) over the
  * acceptance cone, refracted into the sample (n_sample, layer 1), weight
  * 1 - R with R the Fresnel reflectance core -> sample. Returns R. */
+template <class M = MathExact>
 XO_DEV float fiber_ni_launch(const xo_source_fiber &src, float n_sample, uint64_t &x,
                              uint32_t a, float &px, float &py, float &pz, float &dx,
                              float &dy, float &dz, float &w) {
     float sin_fi, cos_fi;
-    const float r = __builtin_sqrtf(rng_next(x, a)) * src.radius;
-    xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+    const float r = M::sqrt(rng_next(x, a)) * src.radius;
+    M::sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
     px = src.position[0] + r * cos_fi;
     py = src.position[1] + r * sin_fi;
     pz = 0.0f;
-    xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+    M::sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
     float cos_theta = 1.0f - rng_next(x, a) * (1.0f - src.cos_min);
-    float sin_theta = __builtin_sqrtf(1.0f - cos_theta * cos_theta);
-    sin_theta = sin_theta / n_sample;
-    cos_theta = __builtin_sqrtf(1.0f - sin_theta * sin_theta);
+    float sin_theta = M::sqrt(1.0f - cos_theta * cos_theta);
+    sin_theta = M::div(sin_theta, n_sample);
+    cos_theta = M::sqrt(1.0f - sin_theta * sin_theta);
     dx = cos_fi * sin_theta;
     dy = sin_fi * sin_theta;
     dz = cos_theta;
-    const float R = reflectance_cos2(src.n, n_sample, dz);
+    const float R = reflectance_cos2<M>(src.n, n_sample, dz);
     w = 1.0f - R;
     return R;
 }
@@ -447,33 +513,34 @@ XO_DEV float fiber_ni_launch(const xo_source_fiber &src, float n_sample, uint64_
  * before refracting, so the direction enters the sample unrefracted; the
  * weight is 1 - the Fresnel reflectance at that direction. Returns that
  * reflectance (the specular part, deposited along the launch direction). */
+template <class M = MathExact>
 XO_DEV float fiber_launch(const xo_source_fiber &src, const xo_fiber_axis &ax, float n_sample,
                           uint64_t &x, uint32_t a, float &px, float &py, float &pz, float &dx,
                           float &dy, float &dz, float &w) {
     const float *T = ax.transformation;
     float sin_fi, cos_fi;
-    const float r = __builtin_sqrtf(rng_next(x, a)) * src.radius;
-    xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+    const float r = M::sqrt(rng_next(x, a)) * src.radius;
+    M::sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
     const float s0 = r * cos_fi, s1 = r * sin_fi, s2 = 0.0f;
     float m0 = T[0] * s0 + T[1] * s1 + T[2] * s2;
     float m1 = T[3] * s0 + T[4] * s1 + T[5] * s2;
     const float m2 = T[6] * s0 + T[7] * s1 + T[8] * s2;
-    const float k = (0.0f - m2) / ax.direction[2];
+    const float k = M::div(0.0f - m2, ax.direction[2]);
     m0 += k * ax.direction[0];
     m1 += k * ax.direction[1];
     px = src.position[0] + m0;
     py = src.position[1] + m1;
     pz = 0.0f;
-    xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+    M::sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
     float cos_theta = 1.0f - rng_next(x, a) * (1.0f - src.cos_min);
-    float sin_theta = __builtin_sqrtf(1.0f - cos_theta * cos_theta);
-    sin_theta = sin_theta / src.n;
-    cos_theta = __builtin_sqrtf(1.0f - sin_theta * sin_theta);
+    float sin_theta = M::sqrt(1.0f - cos_theta * cos_theta);
+    sin_theta = M::div(sin_theta, src.n);
+    cos_theta = M::sqrt(1.0f - sin_theta * sin_theta);
     const float q0 = cos_fi * sin_theta, q1 = sin_fi * sin_theta, q2 = cos_theta;
     dx = T[0] * q0 + T[1] * q1 + T[2] * q2;
     dy = T[3] * q0 + T[4] * q1 + T[5] * q2;
     dz = T[6] * q0 + T[7] * q1 + T[8] * q2;
-    const float R = reflectance(src.n, n_sample, dz, cos_critical(src.n, n_sample));
+    const float R = reflectance<M>(src.n, n_sample, dz, cos_critical<M>(src.n, n_sample));
     w = 1.0f - R;
     return R;
 }
@@ -482,13 +549,14 @@ XO_DEV float fiber_launch(const xo_source_fiber &src, const xo_fiber_axis &ax, f
  * sample direction mirrored at the surface, (dx, dy, -dz), refracted from
  * the sample (n1) into the core (n2) across the normal (0, 0, -1)
  * (refract, mcbase.template.c:1443-1462). */
+template <class M = MathExact>
 XO_DEV void fiber_specular_direction(float dx, float dy, float dz, float n1, float n2,
                                      float &rx, float &ry, float &rz) {
     const float p0 = dx, p1 = dy, p2 = -dz;
     const float cos1 = p0 * 0.0f + p1 * 0.0f + p2 * -1.0f;
-    const float n1_d_n2 = n1 / n2;
+    const float n1_d_n2 = M::div(n1, n2);
     const float sin2_squared = n1_d_n2 * n1_d_n2 * (1.0f - cos1 * cos1);
-    const float k = fsign(cos1) * (n1_d_n2 * fabsf(cos1) - __builtin_sqrtf(1.0f - sin2_squared));
+    const float k = fsign(cos1) * (n1_d_n2 * fabsf(cos1) - M::sqrt(1.0f - sin2_squared));
     rx = n1_d_n2 * p0 - k * 0.0f;
     ry = n1_d_n2 * p1 - k * 0.0f;
     rz = n1_d_n2 * p2 - k * -1.0f;
@@ -509,6 +577,7 @@ XO_DEV bool roulette(float &w, uint64_t &x, uint32_t a, float w_min,
 }
 
 /* mcsim_boundary, mcml.template.c:80-203 (no surface layouts). */
+template <class M = MathExact>
 XO_DEV void mcml_boundary(const LayerL *sl, int32_t &layer, int32_t next,
                           float &dx, float &dy, float &dz, uint64_t &x,
                           uint32_t a) {
@@ -525,23 +594,23 @@ XO_DEV void mcml_boundary(const LayerL *sl, int32_t &layer, int32_t next,
     float cos1 = fabsf(dz);
     if (cos1 > cos_crit) {
         if (cos1 > kCos0) {
-            float R = (n1 - n2) / (n1 + n2);
+            float R = M::div(n1 - n2, n1 + n2);
             if (R * R < rng_next(x, a)) {
                 layer = next;
                 dz = -dz;
             }
             return;
         }
-        float n1_d_n2 = n1 / n2;
-        float sin1 = sqrt_in_range(1.0f - cos1 * cos1);
+        float n1_d_n2 = M::div(n1, n2);
+        float sin1 = M::sqrt_r(1.0f - cos1 * cos1);
         if (cos1 >= kCos0) sin1 = 0.0f;
         float sin2 = fminf(1.0f, n1_d_n2 * sin1);
-        float cos2 = sqrt_in_range(1.0f - sin2 * sin2);
+        float cos2 = M::sqrt_r(1.0f - sin2 * sin2);
         float n_cos1 = n1_d_n2 * cos1;
         float n_cos2 = n1_d_n2 * cos2;
-        float Rs = (n_cos1 - cos2) / (n_cos1 + cos2);
+        float Rs = M::div(n_cos1 - cos2, n_cos1 + cos2);
         Rs *= Rs;
-        float Rp = (n_cos2 - cos1) / (n_cos2 + cos1);
+        float Rp = M::div(n_cos2 - cos1, n_cos2 + cos1);
         Rp *= Rp;
         float R = 0.5f * (Rs + Rp);
         if (cos1 <= kCos90 || sin2 == 1.0f) R = 1.0f;

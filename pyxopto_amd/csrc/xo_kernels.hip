@@ -44,7 +44,7 @@ XO_DEV void stats_flush(BlockStats *ss, const uint32_t *mine,
 constexpr uint32_t kEvNone = 0, kEvInteraction = 1, kEvBoundary = 2;
 constexpr uint32_t kDeferMin = 12, kDeferWait = 8;
 
-template <int PF, int METHOD, bool TRACE>
+template <int PF, int METHOD, bool TRACE, class M>
 __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ BlockStats sstats;
@@ -161,7 +161,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             if (claimed && line_src &&
                 lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
                 const xo_source_line &src = lds_fresh(sev.line);
-                detector_deposit_n(lds_fresh(sev.specular), bins, src.position[0], src.position[1],
+                detector_deposit_n<M>(lds_fresh(sev.specular), bins, src.position[0],
+                                      src.position[1],
                                    src.direction_reflected[0], src.direction_reflected[1],
                                    src.direction_reflected[2], src.reflectance,
                                    (uint32_t)claimed);
@@ -174,17 +175,17 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             if (skind == XO_SRC_FIBER_NI) {
                 const float n1 = sl[1].n;
                 const xo_source_fiber &fib = lds_fresh(sev.fiber);
-                const float R = fiber_ni_launch(fib, n1, x, a, px, py, pz, dx, dy,
-                                                dz, w);
+                const float R = fiber_ni_launch<M>(fib, n1, x, a, px, py, pz, dx, dy,
+                                                   dz, w);
                 if (use_specular) {
                     float rx, ry, rz;
-                    fiber_specular_direction(dx, dy, dz, n1, fib.n, rx, ry, rz);
-                    detector_deposit(lds_fresh(sev.specular), bins, px, py, rx, ry, rz, R);
+                    fiber_specular_direction<M>(dx, dy, dz, n1, fib.n, rx, ry, rz);
+                    detector_deposit<M>(lds_fresh(sev.specular), bins, px, py, rx, ry, rz, R);
                 }
             } else if (skind == XO_SRC_FIBER) {
-                const float R = fiber_launch(lds_fresh(sev.fiber), lds_fresh(sev.fiber_axis), sl[1].n, x, a, px,
-                                             py, pz, dx, dy, dz, w);
-                detector_deposit(lds_fresh(sev.specular), bins, px, py, dx, dy, dz, R);
+                const float R = fiber_launch<M>(lds_fresh(sev.fiber), lds_fresh(sev.fiber_axis),
+                                                sl[1].n, x, a, px, py, pz, dx, dy, dz, w);
+                detector_deposit<M>(lds_fresh(sev.specular), bins, px, py, dx, dy, dz, R);
             } else {
                 /* Line source, xopto/mcml/mcsource/line.py:95-110 */
                 const xo_source_line &src = lds_fresh(sev.line);
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 dy = src.direction_sample[1];
                 dz = src.direction_sample[2];
                 if (static_sched)
-                    detector_deposit(lds_fresh(sev.specular), bins, px, py, src.direction_reflected[0],
+                    detector_deposit<M>(lds_fresh(sev.specular), bins, px, py, src.direction_reflected[0],
                                      src.direction_reflected[1], src.direction_reflected[2],
                                      src.reflectance);
             }
@@ -227,15 +228,16 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         auto mbl_absorb = [&](const LayerL &L, float step) {
             /* absorption along the whole step, mcml.template.c:584-606 */
             const float mua = L.mua;
-            const float fraction = 1.0f - xo_expf(-(mua * step));
+            const float fraction = 1.0f - M::exp(-(mua * step));
             const float deposit = fraction * w;
             w -= deposit;
             if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
             if (use_fluence) {
                 const float back =
-                    (mua != 0.0f) ? step - (-xo_logf_unit(1.0f - rng_next(x, a) * fraction)) / mua
-                                  : 0.0f;
-                if (fluence_deposit(lds_fresh(sev.fluence), P.accum, px - back * dx, py - back * dy, pz - back * dz,
+                    (mua != 0.0f)
+                        ? step - M::div(-M::log_unit(1.0f - rng_next(x, a) * fraction), mua)
+                        : 0.0f;
+                if (fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px - back * dx, py - back * dy, pz - back * dz,
                                     deposit))
                     st[XO_STAT_DEPOSITS]++;
             }
@@ -279,11 +281,11 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 int32_t next = layer;
                 if (__builtin_fmaf(step, dz, pz) < L.top) {
                     --next;
-                    if (fabsf(dz) != 0.0f) step = (L.top - pz) / dz;
+                    if (fabsf(dz) != 0.0f) step = M::div(L.top - pz, dz);
                 }
                 if (__builtin_fmaf(step, dz, pz) >= L.bottom) {
                     ++next;
-                    if (fabsf(dz) != 0.0f) step = (L.bottom - pz) / dz;
+                    if (fabsf(dz) != 0.0f) step = M::div(L.bottom - pz, dz);
                 }
                 px = __builtin_fmaf(dx, step, px);
                 py = __builtin_fmaf(dy, step, py);
@@ -296,16 +298,17 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 if (next != layer) {
                     st[XO_STAT_BOUNDARY]++;
                     const int32_t before = layer;
-                    mcml_boundary(sl, layer, next, dx, dy, dz, x, a);
+                    mcml_boundary<M>(sl, layer, next, dx, dy, dz, x, a);
                     if (TRACE)
                         ts.ev |= XO_EVENT_BOUNDARY_HIT |
                                  (layer != before ? XO_EVENT_REFRACTION : XO_EVENT_REFLECTION);
                     bool done = false;
                     if (!(layer > 0 && layer < (int32_t)nl - 1)) {
                         if (layer <= 0)
-                            detector_deposit(lds_fresh(sev.top), bins, px, py, dx, dy, dz, w);
+                            detector_deposit<M>(lds_fresh(sev.top), bins, px, py, dx, dy, dz, w);
                         else
-                            detector_deposit(lds_fresh(sev.bottom), bins, px, py, dx, dy, dz, w);
+                            detector_deposit<M>(lds_fresh(sev.bottom), bins, px, py, dx, dy, dz,
+                                                w);
                         done = true;
                     }
                     end_step(done);
@@ -322,9 +325,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             st[XO_STAT_STEPS]++;
             float step;
             if (METHOD == XO_METHOD_MBL)
-                step = (-xo_logf_unit(rng_next(x, a))) / L.mus; /* mcml.template.c:529-531 */
+                step = M::div(-M::log_unit(rng_next(x, a)), L.mus); /* mcml.template.c:529-531 */
             else
-                step = -xo_logf_unit(rng_next(x, a)) * L.inv_mut;
+                step = -M::log_unit(rng_next(x, a)) * L.inv_mut;
             step = fminf(step, XO_FLT_MAX);
             const float zend = __builtin_fmaf(step, dz, pz);
             if (zend < L.top || zend >= L.bottom) {
@@ -349,7 +352,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 const float deposit = w;
                 w -= deposit;
                 if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-                if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit))
+                if (use_fluence &&
+                    fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit))
                     st[XO_STAT_DEPOSITS]++;
                 done = true;
             } else {
@@ -357,13 +361,14 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                     const float deposit = w * L.mua_inv_mut;
                     w -= deposit;
                     if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-                    if (use_fluence && fluence_deposit(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit))
+                    if (use_fluence &&
+                        fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit))
                         st[XO_STAT_DEPOSITS]++;
                 }
                 PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
                 float xi_fi;
-                float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
-                scatter_direction(dx, dy, dz, ct, xi_fi);
+                float ct = pf_sample<PF, M>(pf, lut, x, a, xi_fi);
+                scatter_direction<M>(dx, dy, dz, ct, xi_fi);
                 if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
                 if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
                     done = true;
@@ -446,7 +451,7 @@ XO_DEV bool box_intersect(const VoxGeom &g, const float pos[3], const float dir[
  * of a wave hold one, or after kVoxDeferWait iterations. */
 constexpr uint32_t kVoxDeferMin = 24, kVoxDeferWait = 8;
 
-template <int PF, int METHOD, bool TRACE>
+template <int PF, int METHOD, bool TRACE, class M>
 __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ BlockStats sstats;
@@ -460,20 +465,20 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     const uint32_t tid = threadIdx.x;
     const uint32_t nm = P.cfg.num_materials;
     if (tid < nm) {
-        const xo_material &M = P.cfg.materials[tid];
+        const xo_material &Mi = P.cfg.materials[tid];
         MaterialL o;
-        o.inv_mut = M.inv_mut;
-        o.mua_inv_mut = M.mua_inv_mut;
-        o.n = M.n;
-        o.g = M.pf.g;
-        o.beta = M.pf.beta;
-        o.la = M.pf.a;
-        o.lb = M.pf.b;
-        o.lc = M.pf.c;
-        o.loff = M.pf.offset;
-        o.lsize_m1 = M.pf.size > 0u ? M.pf.size - 1u : 0u;
-        o.mua = M.mua;
-        o.mus = M.mus;
+        o.inv_mut = Mi.inv_mut;
+        o.mua_inv_mut = Mi.mua_inv_mut;
+        o.n = Mi.n;
+        o.g = Mi.pf.g;
+        o.beta = Mi.pf.beta;
+        o.la = Mi.pf.a;
+        o.lb = Mi.pf.b;
+        o.lc = Mi.pf.c;
+        o.loff = Mi.pf.offset;
+        o.lsize_m1 = Mi.pf.size > 0u ? Mi.pf.size - 1u : 0u;
+        o.mua = Mi.mua;
+        o.mus = Mi.mus;
         sm[tid] = o;
     }
     for (uint32_t i = tid; i < P.lds_bins; i += XO_BLOCK) sbins[i] = 0ull;
@@ -508,7 +513,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         if (P.fl_box_on)
             return fluence_deposit_boxed(lds_fresh(sev.fluence), P.accum, sfl, P.fl_box, kFlBoxX,
                                          kFlBoxY, kFlBoxZ, qx, qy, qz, wgt);
-        return fluence_deposit(lds_fresh(sev.fluence), P.accum, qx, qy, qz, wgt);
+        return fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, qx, qy, qz, wgt);
     };
 
     const uint32_t gid = blockIdx.x * XO_BLOCK + tid;
@@ -570,7 +575,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
             if (retire) active = false;
             if (claimed && line_src && lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
                 const xo_source_line &src = lds_fresh(sev.line);
-                detector_deposit_n(lds_fresh(sev.specular), bins, src.position[0], src.position[1],
+                detector_deposit_n<M>(lds_fresh(sev.specular), bins, src.position[0],
+                                      src.position[1],
                                    src.direction_reflected[0], src.direction_reflected[1],
                                    src.direction_reflected[2], src.reflectance,
                                    (uint32_t)claimed);
@@ -589,17 +595,17 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                     d[k] = src.direction_sample[k];
                 }
                 if (static_sched)
-                    detector_deposit(lds_fresh(sev.specular), bins, p[0], p[1],
+                    detector_deposit<M>(lds_fresh(sev.specular), bins, p[0], p[1],
                                      src.direction_reflected[0], src.direction_reflected[1],
                                      src.direction_reflected[2], src.reflectance);
             } else {
                 /* xopto/mcvox/mcsource/gaussianbeam.py:112-189 */
                 const xo_source_gaussian &src = lds_fresh(sev.gaussian);
                 float specular_r = 0.0f;
-                float r = __builtin_sqrtf(-2.0f * xo_logf_unit(1.0f - rng_next(x, a)));
+                float r = M::sqrt(-2.0f * M::log_unit(1.0f - rng_next(x, a)));
                 r = fminf(r, src.clip);
                 float sin_fi, cos_fi;
-                xo_sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
+                M::sincos_turns(rng_next(x, a), &sin_fi, &cos_fi);
                 float ps0 = r * cos_fi * src.sigma[0];
                 float ps1 = r * sin_fi * src.sigma[1];
                 float ps2 = 0.0f;
@@ -635,19 +641,19 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                     {
                         /* refract_safe, mcbase.template.c:1487-1508 */
                         float cos1 = dir[0] * nrm[0] + dir[1] * nrm[1] + dir[2] * nrm[2];
-                        float n1_d_n2 = n_out / n_in;
+                        float n1_d_n2 = M::div(n_out, n_in);
                         float s2 = n1_d_n2 * n1_d_n2 * (1.0f - cos1 * cos1);
                         if (!(s2 > 1.0f)) {
                             float k = fsign(cos1) *
-                                      (n1_d_n2 * fabsf(cos1) - __builtin_sqrtf(1.0f - s2));
+                                      (n1_d_n2 * fabsf(cos1) - M::sqrt(1.0f - s2));
                             for (int q = 0; q < 3; ++q) refr[q] = n1_d_n2 * dir[q] - k * nrm[q];
                         }
                     }
                     for (int k = 0; k < 3; ++k) d[k] = refr[k];
-                    specular_r = reflectance(n_out, n_in,
-                                             nrm[0] * dir[0] + nrm[1] * dir[1] + nrm[2] * dir[2],
-                                             cos_critical(n_out, n_in));
-                    detector_deposit(lds_fresh(sev.specular), bins, p[0], p[1], dir[0], dir[1], dir[2],
+                    specular_r = reflectance<M>(n_out, n_in,
+                                                nrm[0] * dir[0] + nrm[1] * dir[1] + nrm[2] * dir[2],
+                                                cos_critical<M>(n_out, n_in));
+                    detector_deposit<M>(lds_fresh(sev.specular), bins, p[0], p[1], dir[0], dir[1], dir[2],
                                      specular_r);
                 } else {
                     for (int k = 0; k < 3; ++k) p[k] = g.tl[k];
@@ -720,9 +726,9 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
 
         if (run_int && pend_int) {
             /* the parked interaction, mcvox.template.c:925-968 */
-            const MaterialL &M = sm[mi];
+            const MaterialL &Mt = sm[mi];
             bool done = false;
-            if (METHOD == XO_METHOD_AR && rng_next(x, a) < M.mua_inv_mut) {
+            if (METHOD == XO_METHOD_AR && rng_next(x, a) < Mt.mua_inv_mut) {
                 /* albedo rejection: the whole weight is absorbed here,
                  * mcvox.template.c:925-944 */
                 const float deposit = w;
@@ -733,17 +739,17 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 done = true;
             } else {
                 if (METHOD == XO_METHOD_AW) {
-                    float deposit = w * M.mua_inv_mut;
+                    float deposit = w * Mt.mua_inv_mut;
                     w -= deposit;
                     if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                     /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
                     if (use_fluence && deposit_fluence(p[0], p[1], p[2], deposit))
                         st[XO_STAT_DEPOSITS]++;
                 }
-                PfL pf{M.g, M.beta, M.la, M.lb, M.lc, M.loff, M.lsize_m1};
+                PfL pf{Mt.g, Mt.beta, Mt.la, Mt.lb, Mt.lc, Mt.loff, Mt.lsize_m1};
                 float xi_fi;
-                float ct = pf_sample<PF>(pf, lut, x, a, xi_fi);
-                scatter_direction(d[0], d[1], d[2], ct, xi_fi);
+                float ct = pf_sample<PF, M>(pf, lut, x, a, xi_fi);
+                scatter_direction<M>(d[0], d[1], d[2], ct, xi_fi);
                 if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
                 if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
                     done = true;
@@ -754,20 +760,20 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         if (!alive || pend_int) continue;
 
         /* ---- one step of while(!done), mcvox.template.c:748-924 ---- */
-        const MaterialL &M = sm[mi];
+        const MaterialL &Mt = sm[mi];
         st[XO_STAT_STEPS]++;
         float step;
         if (METHOD == XO_METHOD_MBL)
-            step = (-xo_logf_unit(rng_next(x, a))) / M.mus; /* mcvox.template.c:763-766 */
+            step = M::div(-M::log_unit(rng_next(x, a)), Mt.mus); /* mcvox.template.c:763-766 */
         else
-            step = -xo_logf_unit(rng_next(x, a)) * M.inv_mut;
+            step = -M::log_unit(rng_next(x, a)) * Mt.inv_mut;
         step = fminf(step, XO_FLT_MAX);
         /* mcsim_intersect, mcvox.template.c:173-196 */
         float dist[3];
         for (int k = 0; k < 3; ++k) {
             float side = (float)((int32_t)(d[k] >= 0.0f) + vi[k]);
             dist[k] = g.tl[k] + side * g.size[k] - p[k];
-            dist[k] = (d[k] != 0.0f) ? dist[k] / d[k] : XO_INF_F;
+            dist[k] = (d[k] != 0.0f) ? M::div(dist[k], d[k]) : XO_INF_F;
         }
         float dd = fminf(dist[0], fminf(dist[1], dist[2]));
         float d_ok = fminf(dd, step);
@@ -775,21 +781,22 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         p[1] = __builtin_fmaf(d_ok, d[1], p[1]);
         p[2] = __builtin_fmaf(d_ok, d[2], p[2]);
         /* optical path length, mcvox.template.c:792-798 */
-        if (TRACE && T.plon) ts.pl += M.n * d_ok;
+        if (TRACE && T.plon) ts.pl += Mt.n * d_ok;
 
         bool done = false;
         if (METHOD == XO_METHOD_MBL) {
             /* absorption along the step inside this voxel,
              * mcvox.template.c:800-818 */
-            const float mua = M.mua;
-            const float fraction = 1.0f - xo_expf(-(mua * d_ok));
+            const float mua = Mt.mua;
+            const float fraction = 1.0f - M::exp(-(mua * d_ok));
             const float deposit = fraction * w;
             w -= deposit;
             if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
             if (use_fluence) {
                 const float back =
-                    (mua != 0.0f) ? d_ok - (-xo_logf_unit(1.0f - rng_next(x, a) * fraction)) / mua
-                                  : 0.0f;
+                    (mua != 0.0f)
+                        ? d_ok - M::div(-M::log_unit(1.0f - rng_next(x, a) * fraction), mua)
+                        : 0.0f;
                 if (deposit_fluence(p[0] - back * d[0], p[1] - back * d[1],
                                     p[2] - back * d[2], deposit))
                     st[XO_STAT_DEPOSITS]++;
@@ -815,20 +822,19 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 if (n1 == n2) {
                     crossed = true;
                 } else {
-                    float cc = cos_critical(n1, n2);
+                    float cc = cos_critical<M>(n1, n2);
                     float cos1 = (float)nrm[0] * d[0] + (float)nrm[1] * d[1] +
                                  (float)nrm[2] * d[2];
                     float fn0 = (float)nrm[0], fn1 = (float)nrm[1], fn2 = (float)nrm[2];
                     bool refracted = false;
                     if (cos1 > cc) {
-                        float R = reflectance(n1, n2, cos1, cc);
+                        float R = reflectance<M>(n1, n2, cos1, cc);
                         if (R < rng_next(x, a)) {
                             /* refract, mcbase.template.c:1393-1410 */
                             float c1 = d[0] * fn0 + d[1] * fn1 + d[2] * fn2;
-                            float n1_d_n2 = n1 / n2;
+                            float n1_d_n2 = M::div(n1, n2);
                             float s2 = n1_d_n2 * n1_d_n2 * (1.0f - c1 * c1);
-                            float k = fsign(c1) *
-                                      (n1_d_n2 * fabsf(c1) - __builtin_sqrtf(1.0f - s2));
+                            float k = fsign(c1) * (n1_d_n2 * fabsf(c1) - M::sqrt(1.0f - s2));
                             d[0] = n1_d_n2 * d[0] - k * fn0;
                             d[1] = n1_d_n2 * d[1] - k * fn1;
                             d[2] = n1_d_n2 * d[2] - k * fn2;
@@ -856,9 +862,11 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 st[XO_STAT_CROSSINGS]++;
                 if (escaping) {
                     if (vi[2] < 0)
-                        detector_deposit(lds_fresh(sev.top), bins, p[0], p[1], d[0], d[1], d[2], w);
+                        detector_deposit<M>(lds_fresh(sev.top), bins, p[0], p[1], d[0], d[1], d[2],
+                                            w);
                     else if (vi[2] >= g.shape[2])
-                        detector_deposit(lds_fresh(sev.bottom), bins, p[0], p[1], d[0], d[1], d[2], w);
+                        detector_deposit<M>(lds_fresh(sev.bottom), bins, p[0], p[1], d[0], d[1],
+                                            d[2], w);
                     done = true;
                 }
             }
@@ -901,20 +909,20 @@ size_t mcvox_lds_bytes(const McvoxParams &P) {
            8u * P.lds_bins + 4u * P.lds_lut;
 }
 
-template <int PF, bool TRACE>
+template <int PF, bool TRACE, class M>
 static hipError_t launch_mcml_pf(const McmlParams &P, uint32_t blocks, size_t lds,
                                  hipStream_t stream) {
     switch (P.cfg.method) {
     case XO_METHOD_AW:
-        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AW, TRACE>), dim3(blocks),
+        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AW, TRACE, M>), dim3(blocks),
                            dim3(XO_BLOCK), lds, stream, P);
         break;
     case XO_METHOD_AR:
-        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AR, TRACE>), dim3(blocks),
+        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_AR, TRACE, M>), dim3(blocks),
                            dim3(XO_BLOCK), lds, stream, P);
         break;
     case XO_METHOD_MBL:
-        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_MBL, TRACE>), dim3(blocks),
+        hipLaunchKernelGGL((mcml_kernel<PF, XO_METHOD_MBL, TRACE, M>), dim3(blocks),
                            dim3(XO_BLOCK), lds, stream, P);
         break;
     default:
@@ -923,11 +931,15 @@ static hipError_t launch_mcml_pf(const McmlParams &P, uint32_t blocks, size_t ld
     return hipGetLastError();
 }
 
+/* Traced runs use the exact arithmetic whatever the flags say. */
 template <int PF>
 static hipError_t launch_mcml_pf(const McmlParams &P, uint32_t blocks, size_t lds,
                                  hipStream_t stream) {
-    return P.trace.max_events > 0 ? launch_mcml_pf<PF, true>(P, blocks, lds, stream)
-                                  : launch_mcml_pf<PF, false>(P, blocks, lds, stream);
+    if (P.trace.max_events > 0)
+        return launch_mcml_pf<PF, true, MathExact>(P, blocks, lds, stream);
+    if (P.flags & XO_FLAG_FAST_MATH)
+        return launch_mcml_pf<PF, false, MathFast>(P, blocks, lds, stream);
+    return launch_mcml_pf<PF, false, MathExact>(P, blocks, lds, stream);
 }
 
 hipError_t launch_mcml(const McmlParams &P, uint32_t blocks, hipStream_t stream) {
@@ -944,20 +956,20 @@ hipError_t launch_mcml(const McmlParams &P, uint32_t blocks, hipStream_t stream)
     }
 }
 
-template <int PF, bool TRACE>
+template <int PF, bool TRACE, class M>
 static hipError_t launch_mcvox_pf(const McvoxParams &P, uint32_t blocks, size_t lds,
-                                  hipStream_t stream) {
+                                 hipStream_t stream) {
     switch (P.cfg.method) {
     case XO_METHOD_AW:
-        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AW, TRACE>), dim3(blocks),
+        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AW, TRACE, M>), dim3(blocks),
                            dim3(XO_BLOCK), lds, stream, P);
         break;
     case XO_METHOD_AR:
-        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AR, TRACE>), dim3(blocks),
+        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_AR, TRACE, M>), dim3(blocks),
                            dim3(XO_BLOCK), lds, stream, P);
         break;
     case XO_METHOD_MBL:
-        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_MBL, TRACE>), dim3(blocks),
+        hipLaunchKernelGGL((mcvox_kernel<PF, XO_METHOD_MBL, TRACE, M>), dim3(blocks),
                            dim3(XO_BLOCK), lds, stream, P);
         break;
     default:
@@ -966,11 +978,15 @@ static hipError_t launch_mcvox_pf(const McvoxParams &P, uint32_t blocks, size_t 
     return hipGetLastError();
 }
 
+/* Traced runs use the exact arithmetic whatever the flags say. */
 template <int PF>
 static hipError_t launch_mcvox_pf(const McvoxParams &P, uint32_t blocks, size_t lds,
-                                  hipStream_t stream) {
-    return P.trace.max_events > 0 ? launch_mcvox_pf<PF, true>(P, blocks, lds, stream)
-                                  : launch_mcvox_pf<PF, false>(P, blocks, lds, stream);
+                                 hipStream_t stream) {
+    if (P.trace.max_events > 0)
+        return launch_mcvox_pf<PF, true, MathExact>(P, blocks, lds, stream);
+    if (P.flags & XO_FLAG_FAST_MATH)
+        return launch_mcvox_pf<PF, false, MathFast>(P, blocks, lds, stream);
+    return launch_mcvox_pf<PF, false, MathExact>(P, blocks, lds, stream);
 }
 
 hipError_t launch_mcvox(const McvoxParams &P, uint32_t blocks, hipStream_t stream) {
@@ -993,13 +1009,13 @@ hipError_t occupancy_mcml(int pf_kind, size_t lds, int *blocks_per_cu) {
     switch (pf_kind) {
     case XO_PF_MHG:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcml_kernel<XO_PF_MHG, XO_METHOD_AW, false>, XO_BLOCK, lds);
+            blocks_per_cu, mcml_kernel<XO_PF_MHG, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
     case XO_PF_LUT:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcml_kernel<XO_PF_LUT, XO_METHOD_AW, false>, XO_BLOCK, lds);
+            blocks_per_cu, mcml_kernel<XO_PF_LUT, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
     default:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcml_kernel<XO_PF_HG, XO_METHOD_AW, false>, XO_BLOCK, lds);
+            blocks_per_cu, mcml_kernel<XO_PF_HG, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
     }
 }
 
@@ -1008,13 +1024,13 @@ hipError_t occupancy_mcvox(int pf_kind, size_t lds, int *blocks_per_cu) {
     switch (pf_kind) {
     case XO_PF_MHG:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcvox_kernel<XO_PF_MHG, XO_METHOD_AW, false>, XO_BLOCK, lds);
+            blocks_per_cu, mcvox_kernel<XO_PF_MHG, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
     case XO_PF_LUT:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcvox_kernel<XO_PF_LUT, XO_METHOD_AW, false>, XO_BLOCK, lds);
+            blocks_per_cu, mcvox_kernel<XO_PF_LUT, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
     default:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcvox_kernel<XO_PF_HG, XO_METHOD_AW, false>, XO_BLOCK, lds);
+            blocks_per_cu, mcvox_kernel<XO_PF_HG, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
     }
 }
 

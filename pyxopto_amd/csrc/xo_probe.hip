@@ -90,6 +90,29 @@ __global__ void __launch_bounds__(256) cr_div_kernel(uint64_t n, uint64_t seed,
     if (bad) atomicAdd(&counts[2], bad);
 }
 
+/* out[i] = f(in[i]) for the function `which` of a math policy (the
+ * arithmetic of the transport kernels). */
+template <class M>
+__global__ void __launch_bounds__(256) math_kernel(int which, const float *in, float *out,
+                                                   uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = in[i];
+    float r = 0.0f, c = 0.0f;
+    switch (which) {
+    case XO_MATH_LOG: r = M::log(v); break;
+    case XO_MATH_SIN_TURNS: M::sincos_turns(v, &r, &c); break;
+    case XO_MATH_COS_TURNS: M::sincos_turns(v, &c, &r); break;
+    case XO_MATH_SQRT: r = M::sqrt(v); break;
+    case XO_MATH_RCP: r = M::div(1.0f, v); break;
+    case XO_MATH_RSQRT: r = M::rsqrt_r(v); break;
+    case XO_MATH_CBRT: r = M::cbrt(v); break;
+    case XO_MATH_EXP: r = M::exp(v); break;
+    default: break;
+    }
+    out[i] = r;
+}
+
 } // namespace
 
 extern "C" {
@@ -254,6 +277,39 @@ int xo_probe_cr_math(xo_ctx *ctx, uint64_t ndiv, uint64_t seed, uint64_t *sqrt_c
     *sqrt_checked = host[0];
     *sqrt_mismatches = host[1];
     *div_mismatches = host[2];
+    return 0;
+}
+
+int xo_probe_math(xo_ctx *ctx, int fast, int which, uint32_t n, const float *in, float *out) {
+    xo::clear_error();
+    if (!ctx || !in || !out || which < 0 || which >= XO_MATH_COUNT) {
+        xo::set_error("xo_probe_math: invalid argument");
+        return -1;
+    }
+    if (n == 0) return 0;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XO_HIP(hipSetDevice(ctx->device));
+    float *buf = nullptr;
+    XO_HIP(hipMalloc(&buf, 2 * (size_t)n * sizeof(float)));
+    hipError_t e = hipMemcpyAsync(buf, in, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        const dim3 grid((n + 255u) / 256u);
+        if (fast)
+            hipLaunchKernelGGL(math_kernel<xo::MathFast>, grid, dim3(256), 0, ctx->stream, which,
+                               buf, buf + n, n);
+        else
+            hipLaunchKernelGGL(math_kernel<xo::MathExact>, grid, dim3(256), 0, ctx->stream,
+                               which, buf, buf + n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, buf + n, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) {
+        xo::set_error("xo_probe_math: %s", hipGetErrorString(e));
+        return -1;
+    }
     return 0;
 }
 

@@ -196,6 +196,7 @@ int build_graph(xo_server *S, int k) {
     P.use_lottery = S->opts.use_lottery;
     P.refill_min = 4u;
     P.defer_min = xo::kDeferMinDefault;
+    P.fast_math = S->opts.fast_math ? 1 : 0;
     XO_CHECK(hipStreamBeginCapture(S->s_ml, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = xo::launch_zero_u32((uint32_t *)T.ml_counters, 2 * kCounterWords, S->s_ml);
     hipError_t e2 = xo::launch_batch(P, S->ml_blocks, S->opts.block, S->s_ml);
@@ -310,7 +311,7 @@ int launch_vox(xo_server *S, Request &R, uint64_t first, uint64_t count,
     P.stats = counters + 1;
     P.accum = S->accum + R.accum_off;
     P.npackets = count;
-    P.flags = XO_FLAG_PACKET_KEYED;
+    P.flags = XO_FLAG_PACKET_KEYED | (S->opts.fast_math ? XO_FLAG_FAST_MATH : 0u);
     P.lds_bins = R.lds_bins;
     P.lds_lut = (R.lut_n && R.lut_n <= XO_LDS_LUT_MAX) ? (uint32_t)R.lut_n : 0u;
     int bpc = 1;
@@ -396,6 +397,7 @@ int dispatch_tick(xo_server *S, int k) {
             P.use_lottery = S->opts.use_lottery;
             P.refill_min = 4u;
             P.defer_min = xo::kDeferMinDefault;
+            P.fast_math = S->opts.fast_math ? 1 : 0;
             XO_CHECK(xo::launch_batch(P, S->ml_blocks, S->opts.block, S->s_ml));
         }
     } else {

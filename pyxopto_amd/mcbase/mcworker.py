@@ -18,6 +18,22 @@ def random_initializer(rng=None) -> int:
             return xinit
 
 
+# OpenCL build options that allow approximate math (xopto/cl/cloptions.py:
+# 23-30, 137-152): with any of them the kernels use the hardware
+# approximations of log, exp, sin/cos, sqrt and division (XO_FLAG_FAST_MATH).
+FAST_MATH_OPTIONS = ('-cl-fast-relaxed-math', '-cl-unsafe-math-optimizations')
+
+
+def fast_math_requested(cl_build_options) -> bool:
+    """True if a build option (a string, or an object whose str() is the
+    option, as cloptions.ClBuildOption) asks for approximate math."""
+    for item in cl_build_options or ():
+        text = str(getattr(item, 'value', item))
+        if any(opt in text for opt in FAST_MATH_OPTIONS):
+            return True
+    return False
+
+
 def validate_initializer(xinit: int) -> int:
     """init_RNG's validity test (xopto/src/rng/rng.cpp:74-80)."""
     xinit = int(xinit)
@@ -42,6 +58,9 @@ class McWorker:
             rnginit = random_initializer()
         self._rnginit = validate_initializer(rnginit)
         self._device = cl_devices if isinstance(cl_devices, int) else 0
+        self._cl_build_options = [str(getattr(item, 'value', item))
+                                  for item in (cl_build_options or [])]
+        self._fast_math = fast_math_requested(cl_build_options)
         self._ctx = None
         self._alloc_offset = 0
         self._allocations = {}
@@ -60,6 +79,11 @@ class McWorker:
     context = property(_context, None, None, 'Device context (created lazily).')
     rnginit = property(lambda self: self._rnginit, None, None,
                        'RNG initializer.')
+    cl_build_options = property(lambda self: tuple(self._cl_build_options),
+                                None, None, 'Build options as given.')
+    fast_math = property(lambda self: self._fast_math, None, None,
+                         'True if the build options allow approximate '
+                         'math (hardware log, exp, sin/cos, sqrt, division).')
     run_report = property(lambda self: dict(self._run_report), None, None,
                           'Timing and thread report of the last run.')
 
@@ -117,7 +141,10 @@ class McWorker:
         return nphotons
 
     def _run_flags(self, static_schedule: bool) -> int:
-        return N.XO_FLAG_STATIC_SCHEDULE if static_schedule else 0
+        flags = N.XO_FLAG_STATIC_SCHEDULE if static_schedule else 0
+        if self._fast_math:
+            flags |= N.XO_FLAG_FAST_MATH
+        return flags
 
     def _record_report(self, rep: N.XoRunReport, nphotons: int):
         self._run_report.update(

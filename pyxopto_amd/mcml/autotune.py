@@ -90,6 +90,7 @@ def sweep(ctx, arr, lut, naccum, base_opts, candidates=CANDIDATES):
         o.w_min, o.lottery_chance = base_opts.w_min, base_opts.lottery_chance
         o.use_lottery = base_opts.use_lottery
         o.method = base_opts.method
+        o.flags = base_opts.flags
         _, r1 = ctx.run_batch(small, lut, o, naccum)
         _, r2 = ctx.run_batch(big, lut, o, naccum)
         t1, t2 = r1.kernel_ms, r2.kernel_ms
@@ -108,7 +109,8 @@ def tuned_opts(batch, packed, force: bool = False):
     arr, lut, naccum = packed
     ctx = batch.context
     base = batch.opts()
-    key = '{}cu-{}'.format(ctx.compute_units, _signature(arr))
+    key = '{}cu-{}{}'.format(ctx.compute_units, _signature(arr),
+                             '-fast' if base.flags & N.XO_FLAG_FAST_MATH else '')
     cache = _load_cache()
     if not force and key in cache:
         chunk, block, bpc = cache[key]['best']

@@ -61,9 +61,10 @@ class _InstancePacker:
 
 class Batch(McWorker):
     def __init__(self, instances: List[Instance], options=None, rnginit=None,
-                 cl_devices=None):
+                 cl_devices=None, cl_build_options=None):
         super().__init__(options=options, rnginit=rnginit,
-                         cl_devices=cl_devices)
+                         cl_devices=cl_devices,
+                         cl_build_options=cl_build_options)
         self.instances = list(instances)
         if not self.instances:
             raise ValueError('A batch needs at least one instance.')
@@ -112,6 +113,7 @@ class Batch(McWorker):
         o.lottery_chance = float(self._opts['lottery_chance'])
         o.use_lottery = int(bool(self._opts['use_lottery']))
         o.method = int(self._opts['method'])
+        o.flags = N.XO_FLAG_FAST_MATH if self._fast_math else 0
         return o
 
     def run(self, opts: N.XoBatchOpts = None, packed=None):

@@ -31,7 +31,7 @@ SLAB_PACKETS = 1_000_000
 SKIN_VESSEL_PACKETS = 100_000_000
 
 
-def slab_1mm(rnginit: int = 0x5EED000000000001):
+def slab_1mm(rnginit: int = 0x5EED000000000001, cl_build_options=None):
     L = mcml.mclayer
     layers = L.Layers([
         L.Layer(d=np.inf, n=1.0, mua=0.0, mus=0.0, pf=mcml.mcpf.Hg(0.8)),
@@ -40,12 +40,14 @@ def slab_1mm(rnginit: int = 0x5EED000000000001):
     D = mcml.mcdetector
     det = D.Detectors(top=D.Radial(D.RadialAxis(0.0, 5e-3, 100)),
                       bottom=D.Total(), specular=D.Total())
-    return mcml.Mc(layers, mcml.mcsource.Line(), det, rnginit=rnginit)
+    return mcml.Mc(layers, mcml.mcsource.Line(), det, rnginit=rnginit,
+                   cl_build_options=cl_build_options)
 
 
 def skin_vessel(n: int = 200, voxel: float = 10e-6,
                 rnginit: int = 0x5EED000000000001, epidermis: float = 100e-6,
-                vessel_depth: float = 400e-6, vessel_radius: float = 100e-6):
+                vessel_depth: float = 400e-6, vessel_radius: float = 100e-6,
+                cl_build_options=None):
     half = 0.5 * n * voxel
     M = mcvox.mcmaterial
     materials = M.Materials([
@@ -64,7 +66,8 @@ def skin_vessel(n: int = 200, voxel: float = 10e-6,
     det = D.Detectors(top=D.Radial(D.RadialAxis(0.0, half, 100)),
                       bottom=D.Total(), specular=D.Total())
     sim = mcvox.Mc(voxels, materials, mcvox.mcsource.GaussianBeam(50e-6), det,
-                   fluence=fluence, rnginit=rnginit)
+                   fluence=fluence, rnginit=rnginit,
+                   cl_build_options=cl_build_options)
     z, _, x = sim.voxels.meshgrid()
     mat = sim.voxels.material
     mat[...] = 2
@@ -76,7 +79,7 @@ def skin_vessel(n: int = 200, voxel: float = 10e-6,
 
 def skin_two_layer(n: int = 64, size: float = 2e-3,
                    rnginit: int = 0x5EED000000000001,
-                   epidermis: float = 100e-6):
+                   epidermis: float = 100e-6, cl_build_options=None):
     half = 0.5 * size
     M = mcvox.mcmaterial
     materials = M.Materials([
@@ -93,7 +96,8 @@ def skin_two_layer(n: int = 64, size: float = 2e-3,
     det = D.Detectors(top=D.Radial(D.RadialAxis(0.0, half, 100)),
                       bottom=D.Total(), specular=D.Total())
     sim = mcvox.Mc(voxels, materials, mcvox.mcsource.GaussianBeam(50e-6), det,
-                   fluence=fluence, rnginit=rnginit)
+                   fluence=fluence, rnginit=rnginit,
+                   cl_build_options=cl_build_options)
     z, _, _ = sim.voxels.meshgrid()
     mat = sim.voxels.material
     mat[...] = 2
@@ -142,11 +146,11 @@ def fiber_grid():
     return mua.ravel(), musr.ravel()
 
 
-def fiber_sim(rnginit: int = 0x5EED000000000001):
+def fiber_sim(rnginit: int = 0x5EED000000000001, cl_build_options=None):
     """The benchmark's simulator (mua, mus' set per run, as the reference
     test does)."""
     sim = mcml.Mc(fiber_layers(), fiber_source(), fiber_detectors(),
-                  rnginit=rnginit)
+                  rnginit=rnginit, cl_build_options=cl_build_options)
     sim.rmax = 5e-3
     return sim
 
@@ -187,7 +191,7 @@ def _luts_cached(lut_jobs, cache_file, processes):
     else:
         import multiprocessing as mp
         procs = processes or min(16, len(os.sched_getaffinity(0)))
-        with mp.get_context('fork').Pool(procs) as pool:
+        with mp.get_context('spawn').Pool(procs) as pool:
             results = pool.map(_lut_params, lut_jobs, chunksize=16)
     if cache_file and results:
         os.makedirs(os.path.dirname(cache_file), exist_ok=True)
@@ -199,9 +203,47 @@ def _luts_cached(lut_jobs, cache_file, processes):
     return results
 
 
+def layered_batch_extremes(n: int = BATCH_INSTANCES, seed: int = 20261202,
+                           per_kind: int = 8):
+    """Indices of instances of layered_batch(n, seed) at the corners of
+    its parameter box: the largest total optical thickness (sum of mus d),
+    the lowest absorption, the highest scattering, the thickest layer,
+    five-layer lookup-table stacks, plus evenly spaced ones; per_kind of
+    each, without repeats, in increasing order."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for _ in range(n):
+        nl = int(rng.integers(1, 6))
+        d = np.exp(rng.uniform(np.log(0.05e-3), np.log(5e-3), nl))
+        mua = np.exp(rng.uniform(np.log(1.0), np.log(5000.0), nl))
+        mus = np.exp(rng.uniform(np.log(1e3), np.log(5e4), nl))
+        rng.uniform(1.33, 1.50, nl)
+        use_lut = bool(rng.integers(0, 2))
+        rng.uniform(0.6, 0.95, nl)
+        rng.uniform(0.0, 1.0, nl)
+        rng.uniform(np.log(1e4), np.log(1e6))
+        rng.integers(1, 2 ** 63)
+        rows.append((float((mus * d).sum()), float(mua.min()),
+                     float(mus.max()), float(d.max()), nl == 5 and use_lut))
+    t = np.array([r[:4] for r in rows])
+    picks = []
+    for order in (np.argsort(-t[:, 0]), np.argsort(t[:, 1]),
+                  np.argsort(-t[:, 2]), np.argsort(-t[:, 3]),
+                  np.array([i for i, r in enumerate(rows) if r[4]]),
+                  np.linspace(0, n - 1, per_kind).astype(int)):
+        k = 0
+        for i in order:
+            if k == per_kind:
+                break
+            if int(i) not in picks:
+                picks.append(int(i))
+                k += 1
+    return sorted(picks)
+
+
 def layered_batch(n: int = BATCH_INSTANCES, seed: int = 20261202,
                   budget_scale: float = 1.0, processes: int = None,
-                  cache_dir: str = None):
+                  cache_dir: str = None, indices=None):
     """``n`` layered instances drawn from a seeded generator: 1-5 turbid
     layers each (thickness log-uniform 0.05-5 mm, mua log-uniform
     0.01-50 /cm, mus log-uniform 10-500 /cm, n uniform 1.33-1.50) between
@@ -210,9 +252,10 @@ def layered_batch(n: int = BATCH_INSTANCES, seed: int = 20261202,
     uniform 0.6-0.95, beta uniform 0-1 per layer); packet budget
     log-uniform 1e4-1e6 (times budget_scale); 100-bin radial reflectance
     0-5 mm plus total transmittance and specular reflectance each.
-    The lookup tables are built by a pool of `processes` forked workers
+    The lookup tables are built by a pool of `processes` spawned workers
     (1: in this process) or read back from an .npz in cache_dir written by
-    an earlier call. Returns a list of batch.Instance."""
+    an earlier call. Returns a list of batch.Instance; with `indices`, only
+    those instances of the same draw (and only their tables are built)."""
     from .mcml import batch as mbatch
     rng = np.random.default_rng(seed)
     specs = []
@@ -231,12 +274,16 @@ def layered_batch(n: int = BATCH_INSTANCES, seed: int = 20261202,
         inst_seed = int(rng.integers(1, 2 ** 63))
         specs.append((nl, d, mua, mus, nn, use_lut, g, beta, max(budget, 1),
                       inst_seed))
+    if indices is not None:
+        specs = [specs[int(i)] for i in indices]
+    for nl, _, _, _, _, use_lut, g, beta, _, _ in specs:
         if use_lut:
             lut_jobs.extend((float(g[k]), float(beta[k])) for k in range(nl))
     luts = {}
     if lut_jobs:
-        cache_file = None if cache_dir is None else os.path.join(
-            cache_dir, 'layered_batch_luts_{}_{}.npz'.format(n, seed))
+        cache_file = None if cache_dir is None or indices is not None else \
+            os.path.join(cache_dir,
+                         'layered_batch_luts_{}_{}.npz'.format(n, seed))
         luts = dict(zip(lut_jobs, _luts_cached(lut_jobs, cache_file,
                                                processes)))
     L = mcml.mclayer
