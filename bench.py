@@ -3,9 +3,12 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-One step is one simulation run (the C-ABI call behind Mc.run: kernel launch,
-accumulator download) over the workload's packet budget, with the voxel grid
-already resident on the device. Prints one JSON line.
+One step is one simulation run over the workload's packet budget with its
+inputs already resident on the device: for the layered batch a replay of
+the captured batch launch (xo_batch_plan_run) and the accumulator download,
+for the single simulations the C-ABI call behind Mc.run (kernel launch,
+accumulator download) with the voxel grid kept on the device. Prints one
+JSON line.
 
 Roofline: the transport kernels do scalar binary32 work with no matrix
 contraction, so the bound is the VALU issue rate: 256 CUs x 64 lanes/clock
@@ -79,9 +82,19 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
         arr, lut, naccum = packed
         npk = sum(int(i.nphotons) for i in insts)
         ctx = batch.context
+        # inputs resident on the device, the launch a captured graph: each
+        # step replays it and downloads the accumulators
+        plan = ctx.batch_plan(arr, lut, opts, naccum)
 
         def step(first):
-            return ctx.run_batch(arr, lut, opts, naccum)[1]
+            return plan.run()[1]
+
+        def upload_inclusive():
+            """One xo_batch_run: upload of the packed instances and LUTs,
+            launch, download."""
+            t0 = time.perf_counter()
+            ctx.run_batch(arr, lut, opts, naccum)
+            return npk / (time.perf_counter() - t0)
 
         config = {'workload': 'layered_batch_4096', 'instances': len(insts),
                   'layers': '1-5 turbid + 2 ambient', 'packets': npk,
@@ -93,6 +106,7 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
                              'defer_min': opts.defer_min or 16}}
         batch.bench_plain = lambda: plain_rate(batch, insts)
         batch.bench_insts = insts
+        batch.bench_upload_inclusive = upload_inclusive
         return batch, npk, step, False, config
     if name == 'skin_vessel_200':
         sim = workloads.skin_vessel(cl_build_options=BUILD_OPTS)
@@ -563,6 +577,8 @@ def main():
             out[key] = out['roofline'][key]
         out['voxel_crossings_per_s'] = sum(r.voxel_crossings
                                            for r in reps) / kernel_s
+    if hasattr(sim, 'bench_upload_inclusive'):
+        out['upload_inclusive_packets_per_s'] = sim.bench_upload_inclusive()
     if hasattr(sim, 'bench_plain') and not args.no_plain:
         plain = sim.bench_plain()
         out['plain_one_launch_per_instance'] = plain

@@ -205,10 +205,13 @@ typedef struct xo_detector {
     int32_t log_scale;
 } xo_detector; /* 48 B */
 
-/* Deposition accumulator; McFluence of xopto/mcbase/mcfluence/fluence.py:
+/* Fluence accumulator; McFluence of xopto/mcbase/mcfluence/fluence.py:
  * 35-65 (XO_FLUENCE_XYZ: top_left, inv_step and shape per x, y, z) or of
  * fluencerz.py (XO_FLUENCE_RZ: top_left = centre, inv_step[0] = 1/dr,
- * inv_step[2] = 1/dz, shape = {n_r, 1, n_z}). Deposition mode only. */
+ * inv_step[2] = 1/dz, shape = {n_r, 1, n_z}). rate: 0 deposits the absorbed
+ * weight (mode 'deposition'), 1 the absorbed weight / mua of the medium the
+ * packet is in, 0 where mua is 0 (mode 'fluence', MC_FLUENCE_MODE_RATE of
+ * fluence.py:102-147 and fluencerz.py:104-160). */
 typedef struct xo_fluence {
     int32_t enabled;            /* enum xo_fluence_kind                       */
     float inv_step[3];
@@ -216,7 +219,8 @@ typedef struct xo_fluence {
     uint32_t shape[3];
     uint32_t offset;
     int32_t k;
-} xo_fluence; /* 48 B */
+    int32_t rate;
+} xo_fluence; /* 52 B */
 
 /* Axis of a fiber source that meets the surface at an angle (UniformFiber,
  * xopto/mcml/mcsource/fiber.py:204-227): the rotation of the fiber frame
@@ -504,6 +508,25 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *instances,
                  uint32_t ninstances, const float *lut, uint32_t nlut,
                  const xo_batch_opts *opts, uint64_t *accum, uint64_t naccum,
                  xo_run_report *report);
+
+/* A batch uploaded once and launched many times, for a fixed-shape batch
+ * run repeatedly (benchmarks, parameter sweeps that only change what the
+ * host does with the results). xo_batch_plan_create validates and packs
+ * the instances as xo_batch_run does, uploads the tables and the LUT into
+ * device buffers the plan owns and captures the launch (accumulator and
+ * counter clears by fill kernels, then the batch kernel) as a hipGraph;
+ * xo_batch_plan_run replays it and copies the naccum accumulators to the
+ * host: the integers of xo_batch_run on the same arguments. The reference
+ * re-uploads every instance on every Mc.run of its batch loop
+ * (xopto/mcml/mcrun/helper.py:640-686); no counterpart entry point.
+ * ctx must outlive the plan; report->kernel_ms covers the replayed graph.
+ * Returns NULL on error. */
+typedef struct xo_batch_plan xo_batch_plan;
+xo_batch_plan *xo_batch_plan_create(xo_ctx *ctx, const xo_batch_instance *instances,
+                                    uint32_t ninstances, const float *lut, uint32_t nlut,
+                                    const xo_batch_opts *opts, uint64_t naccum);
+int xo_batch_plan_run(xo_batch_plan *plan, uint64_t *accum, xo_run_report *report);
+void xo_batch_plan_destroy(xo_batch_plan *plan);
 
 /* The MWC state (x, a) of packet p of a batch instance with key seed:
  * a = table multiplier (rows 1..XO_MAX_SEEDS) chosen by a hash of

@@ -356,7 +356,7 @@ static void o_trace_step(const o_trace *t, o_tstate *ts, const o_pkt *s,
  * mcsim_fluence_deposit_at (xopto/mcbase/mcfluence/fluence.py:102-147) or
  * the (r, z) cell of fluencerz.py:112-160. */
 static void o_fluence_deposit(const xo_fluence *f, uint64_t *acc,
-                              const float p[3], float w, uint64_t *deposits) {
+                              const float p[3], float w, float mua, uint64_t *deposits) {
     uint64_t index;
     if (f->enabled == XO_FLUENCE_XYZ) {
         float fx = (p[0] - f->top_left[0]) * f->inv_step[0];
@@ -380,6 +380,9 @@ static void o_fluence_deposit(const xo_fluence *f, uint64_t *acc,
     } else {
         return;
     }
+    /* fluence-rate mode: weight / mua of the current medium, 0 if mua is 0
+     * (xopto/mcbase/mcfluence/fluence.py:139-141) */
+    if (f->rate) w *= (mua != 0.0f) ? 1.0f / mua : 0.0f;
     uint32_t ui = o_uint(w * (float)f->k + 0.5f);
     if (ui > 0u) {
         o_accu(acc, f->offset + index, ui);
@@ -621,7 +624,7 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
                                  : 0.0f;
                 float q[3] = {s->p[0] - back * s->d[0], s->p[1] - back * s->d[1],
                               s->p[2] - back * s->d[2]};
-                o_fluence_deposit(&cfg->fluence, acc, q, deposit, &cnt->deposits);
+                o_fluence_deposit(&cfg->fluence, acc, q, deposit, mua, &cnt->deposits);
             }
         }
         if (next != s->layer) {
@@ -646,7 +649,7 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
             s->w -= deposit;
             ts.ev |= XO_EVENT_ABSORPTION;
             if (cfg->fluence.enabled)
-                o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, &cnt->deposits);
+                o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, L->mua, &cnt->deposits);
             done = 1;
         } else {
             if (method == XO_METHOD_AW) {
@@ -654,7 +657,7 @@ static void o_mcml_packet(const xo_mcml_cfg *cfg, const float *lut, o_pkt *s,
                 s->w -= deposit;
                 ts.ev |= XO_EVENT_ABSORPTION;
                 if (cfg->fluence.enabled)
-                    o_fluence_deposit(&cfg->fluence, acc, s->p, deposit,
+                    o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, L->mua,
                                       &cnt->deposits);
             }
             float xi_fi;
@@ -1014,7 +1017,7 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
                                  : 0.0f;
                 float q[3] = {s->p[0] - back * s->d[0], s->p[1] - back * s->d[1],
                               s->p[2] - back * s->d[2]};
-                o_fluence_deposit(&cfg->fluence, acc, q, deposit, &cnt->deposits);
+                o_fluence_deposit(&cfg->fluence, acc, q, deposit, mua, &cnt->deposits);
             }
         }
         if (d < step) {
@@ -1040,7 +1043,7 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
             s->w -= deposit;
             ts.ev |= XO_EVENT_ABSORPTION;
             if (cfg->fluence.enabled)
-                o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, &cnt->deposits);
+                o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, M->mua, &cnt->deposits);
             done = 1;
         } else {
             if (method == XO_METHOD_AW) {
@@ -1048,7 +1051,7 @@ static void o_mcvox_packet(const o_vox *V, o_pkt *s, uint64_t *acc,
                 s->w -= deposit;
                 ts.ev |= XO_EVENT_ABSORPTION;
                 if (cfg->fluence.enabled)
-                    o_fluence_deposit(&cfg->fluence, acc, s->p, deposit,
+                    o_fluence_deposit(&cfg->fluence, acc, s->p, deposit, M->mua,
                                       &cnt->deposits);
             }
             float xi_fi;

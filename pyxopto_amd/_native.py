@@ -9,6 +9,7 @@ fallback implementation.
 import ctypes
 import os
 import threading
+import weakref
 from ctypes import (POINTER, c_char_p, c_double, c_float, c_int, c_int32,
                     c_uint32, c_uint64, c_void_p)
 
@@ -83,7 +84,8 @@ class XoDetector(ctypes.Structure):
 
 class XoFluence(ctypes.Structure):
     _fields_ = [('enabled', c_int32), ('inv_step', f3), ('top_left', f3),
-                ('shape', c_uint32 * 3), ('offset', c_uint32), ('k', c_int32)]
+                ('shape', c_uint32 * 3), ('offset', c_uint32), ('k', c_int32),
+                ('rate', c_int32)]
 
 
 class XoSourceFiber(ctypes.Structure):
@@ -255,6 +257,12 @@ PROTOTYPES = {
                              POINTER(XoRunReport)]),
     'xo_packet_seed': (c_int, [c_uint64, c_uint64, POINTER(c_uint64),
                                POINTER(c_uint32)]),
+    'xo_batch_plan_create': (c_void_p, [c_void_p, POINTER(XoBatchInstance),
+                                        c_uint32, POINTER(c_float), c_uint32,
+                                        POINTER(XoBatchOpts), c_uint64]),
+    'xo_batch_plan_run': (c_int, [c_void_p, POINTER(c_uint64),
+                                  POINTER(XoRunReport)]),
+    'xo_batch_plan_destroy': (None, [c_void_p]),
     'xo_mcvox_run': (c_int, [c_void_p, POINTER(XoMcvoxCfg), POINTER(c_int32),
                              c_uint64, POINTER(c_float), c_uint32, c_uint64,
                              c_uint32, c_uint32, POINTER(c_uint64), c_uint64,
@@ -435,6 +443,46 @@ def device_count() -> int:
     return int(lib().xo_device_count())
 
 
+class BatchPlan:
+    """A batch whose tables live on the device and whose launch is a
+    captured graph (xo_batch_plan_*): run() returns (accum, report) with the
+    integers of Context.run_batch on the same arguments."""
+
+    def __init__(self, ctx, instances, lut, opts, naccum: int):
+        self._ctx = ctx            # the context must outlive the plan
+        self.naccum = int(naccum)
+        lut_p, nlut = None, 0
+        if lut is not None and len(lut):
+            lut = np.ascontiguousarray(lut, dtype=np.float32)
+            lut_p, nlut = _ptr(lut, c_float), lut.size
+        h = lib().xo_batch_plan_create(ctx._h, instances,
+                                       c_uint32(len(instances)), lut_p,
+                                       c_uint32(nlut), ctypes.byref(opts),
+                                       c_uint64(self.naccum))
+        if not h:
+            raise NativeError('xo_batch_plan_create: {}'.format(last_error()))
+        self._h = h
+        ctx._plans.add(self)
+
+    def run(self):
+        accum = np.zeros(self.naccum, dtype=np.uint64)
+        rep = XoRunReport()
+        check(lib().xo_batch_plan_run(self._h, _ptr(accum, c_uint64),
+                                      ctypes.byref(rep)), 'xo_batch_plan_run')
+        return accum, rep
+
+    def close(self):
+        if getattr(self, '_h', None):
+            lib().xo_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """Owner of one xo_ctx: RNG streams and buffers on one device."""
 
@@ -449,6 +497,7 @@ class Context:
                 raise ValueError(msg)
             raise NativeError('xo_ctx_create: ' + msg)
         self._h = h
+        self._plans = weakref.WeakSet()
         self.device = device
         p, m, c = c_uint32(), c_uint32(), c_uint32()
         check(l.xo_ctx_info(h, ctypes.byref(p), ctypes.byref(m),
@@ -459,6 +508,9 @@ class Context:
 
     def close(self):
         if getattr(self, '_h', None):
+            # plans hold the context's device and stream: release them first
+            for plan in list(getattr(self, '_plans', ())):
+                plan.close()
             lib().xo_ctx_destroy(self._h)
             self._h = None
 
@@ -553,6 +605,11 @@ class Context:
                                  _ptr(accum, c_uint64), c_uint64(naccum),
                                  ctypes.byref(rep)), 'xo_batch_run')
         return accum, rep
+
+    def batch_plan(self, instances, lut, opts: XoBatchOpts, naccum: int):
+        """xo_batch_plan_create: the batch uploaded and its launch captured
+        once; BatchPlan.run replays it."""
+        return BatchPlan(self, instances, lut, opts, naccum)
 
     def probe_copy(self, nbytes: int, reps: int = 10) -> float:
         """Average ms of one device-to-device copy of nbytes."""

@@ -28,7 +28,7 @@ static_assert(sizeof(xo_layer) == 80, "xo_layer layout");
 static_assert(sizeof(xo_source_line) == 64, "xo_source_line layout");
 static_assert(sizeof(xo_source_gaussian) == 72, "xo_source_gaussian layout");
 static_assert(sizeof(xo_detector) == 48, "xo_detector layout");
-static_assert(sizeof(xo_fluence) == 48, "xo_fluence layout");
+static_assert(sizeof(xo_fluence) == 52, "xo_fluence layout");
 static_assert(sizeof(xo_material) == 64, "xo_material layout");
 static_assert(sizeof(xo_voxel_cfg) == 48, "xo_voxel_cfg layout");
 static_assert(sizeof(xo_source_fiber) == 32, "xo_source_fiber layout");
@@ -211,6 +211,128 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
         .count();
 }
 
+void fill_report(xo_run_report *r, const unsigned long long *stats, uint32_t lanes,
+                 float kernel_ms, double total_ms) {
+    if (!r) return;
+    r->packets = stats[XO_STAT_PACKETS];
+    r->steps = stats[XO_STAT_STEPS];
+    r->boundary_steps = stats[XO_STAT_BOUNDARY];
+    r->voxel_crossings = stats[XO_STAT_CROSSINGS];
+    r->deposits = stats[XO_STAT_DEPOSITS];
+    r->truncated = stats[XO_STAT_TRUNCATED];
+    r->lanes = lanes;
+    r->threads_used = (uint32_t)stats[XO_STAT_THREADS];
+    r->kernel_ms = kernel_ms;
+    r->total_ms = total_ms;
+}
+
+int pack_batch(const char *fn, const xo_batch_instance *inst, uint32_t ninst, bool have_lut,
+               uint32_t nlut, const xo_batch_opts *opts, uint64_t naccum,
+               std::vector<LayerL> &layers, std::vector<BatchHdr> &hdr,
+               std::vector<BatchItem> &items, uint64_t *total, uint32_t *block_out) {
+    const uint32_t block = opts->block ? opts->block : 256u;
+    if (block != 64 && block != 128 && block != 256) {
+        set_error("%s: block must be 64, 128 or 256 (got %u)", fn, block);
+        return -1;
+    }
+    const uint64_t chunk = opts->chunk ? opts->chunk : 1024u;
+    if (opts->method != XO_METHOD_AW && opts->method != XO_METHOD_AR &&
+        opts->method != XO_METHOD_MBL) {
+        set_error("%s: unknown stepping method %d", fn, opts->method);
+        return -1;
+    }
+    layers.assign((size_t)ninst * XO_BATCH_MAX_LAYERS, LayerL{});
+    hdr.assign(ninst, BatchHdr{});
+    items.clear();
+    *total = 0;
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const xo_batch_instance &I = inst[i];
+        if (I.num_layers < 3 || I.num_layers > XO_BATCH_MAX_LAYERS) {
+            set_error("instance %u: %u layers (3..%d supported)", i, I.num_layers,
+                      XO_BATCH_MAX_LAYERS);
+            return -1;
+        }
+        for (uint32_t l = 0; l < I.num_layers; ++l)
+            if (check_pf(I.pf_kind, I.layers[l].pf, nlut, "layer", l)) return -1;
+        if (I.pf_kind == XO_PF_LUT && !have_lut) {
+            set_error("instance %u: a lookup-table phase function needs the LUT", i);
+            return -1;
+        }
+        if ((uint64_t)I.accum_base + I.accum_count > naccum) {
+            set_error("instance %u: bins [%u, +%u) outside the accumulator (%llu)", i,
+                      I.accum_base, I.accum_count, (unsigned long long)naccum);
+            return -1;
+        }
+        uint64_t end = 0;
+        const xo_detector *dets[3] = {&I.top, &I.bottom, &I.specular};
+        for (int k = 0; k < 3; ++k) {
+            if (check_detector("instance", *dets[k], naccum, &end)) return -1;
+            if (dets[k]->kind != XO_DET_NONE &&
+                (dets[k]->offset < I.accum_base ||
+                 (uint64_t)dets[k]->offset +
+                         (dets[k]->kind == XO_DET_TOTAL ? 1u : dets[k]->n) >
+                     (uint64_t)I.accum_base + I.accum_count)) {
+                set_error("instance %u: detector bins outside its range", i);
+                return -1;
+            }
+        }
+        for (uint32_t l = 0; l < XO_BATCH_MAX_LAYERS; ++l)
+            layers[(size_t)i * XO_BATCH_MAX_LAYERS + l] =
+                layer_to_device(I.layers[l < I.num_layers ? l : I.num_layers - 1]);
+        if (I.source_kind != XO_SRC_LINE && I.source_kind != XO_SRC_FIBER_NI &&
+            I.source_kind != XO_SRC_FIBER) {
+            set_error("instance %u: unknown source kind %d", i, I.source_kind);
+            return -1;
+        }
+        if (I.source_kind == XO_SRC_FIBER && !(I.fiber_axis.direction[2] > 0.0f)) {
+            set_error("instance %u: the fiber direction must point into the sample (z > 0)",
+                      i);
+            return -1;
+        }
+        const xo_detector det[3] = {I.top, I.bottom, I.specular};
+        hdr[i] = batch_header(I, det, 0u);
+        for (uint64_t first = 0; first < I.npackets; first += chunk) {
+            uint64_t cnt = I.npackets - first < chunk ? I.npackets - first : chunk;
+            items.push_back(BatchItem{i, (uint32_t)cnt, first});
+        }
+        *total += I.npackets;
+        if (items.size() >= 0xFFFFFFF0ull) {
+            set_error("%s: too many work items; raise the chunk size", fn);
+            return -1;
+        }
+    }
+
+    /* Waves claim consecutive items: grouping the items of instances that
+     * run the same code (phase-function kind, source, layer count) keeps the
+     * lanes of a wave on one path when it moves from item to item. The
+     * order does not change any result. */
+    std::stable_sort(items.begin(), items.end(), [inst](const BatchItem &p, const BatchItem &q) {
+        return item_class(inst[p.inst]) < item_class(inst[q.inst]);
+    });
+    *block_out = block;
+    return 0;
+}
+
+void batch_params(const xo_batch_opts &opts, BatchParams &P) {
+    std::memset(&P, 0, sizeof(P));
+    P.w_min = opts.w_min;
+    P.chance = opts.lottery_chance;
+    P.use_lottery = opts.use_lottery;
+    P.refill_min = opts.refill_min ? opts.refill_min : 4u;
+    P.defer_min = opts.defer_min ? opts.defer_min : kDeferMinDefault;
+    P.method = opts.method;
+    P.fast_math = (opts.flags & XO_FLAG_FAST_MATH) ? 1 : 0;
+}
+
+int batch_grid(xo_ctx *ctx, uint32_t block, const xo_batch_opts &opts, uint32_t *blocks) {
+    int bpc = 1;
+    XO_HIP(occupancy_batch(block, &bpc));
+    if (opts.blocks_per_cu && (int)opts.blocks_per_cu < bpc) bpc = (int)opts.blocks_per_cu;
+    if (bpc < 1) bpc = 1;
+    *blocks = (uint32_t)bpc * ctx->cus;
+    return 0;
+}
+
 } // namespace xo
 
 using xo::check_detector;
@@ -220,6 +342,7 @@ using xo::choose_lanes;
 using xo::ensure_buffer;
 using xo::ensure_table;
 using xo::ms_since;
+using xo::fill_report;
 
 extern "C" {
 
@@ -379,20 +502,6 @@ int xo_ctx_set_rng(xo_ctx *ctx, uint32_t n, const uint64_t *x, const uint32_t *a
     return 0;
 }
 
-static void fill_report(xo_run_report *r, const unsigned long long *stats, uint32_t lanes,
-                        float kernel_ms, double total_ms) {
-    if (!r) return;
-    r->packets = stats[XO_STAT_PACKETS];
-    r->steps = stats[XO_STAT_STEPS];
-    r->boundary_steps = stats[XO_STAT_BOUNDARY];
-    r->voxel_crossings = stats[XO_STAT_CROSSINGS];
-    r->deposits = stats[XO_STAT_DEPOSITS];
-    r->truncated = stats[XO_STAT_TRUNCATED];
-    r->lanes = lanes;
-    r->threads_used = (uint32_t)stats[XO_STAT_THREADS];
-    r->kernel_ms = kernel_ms;
-    r->total_ms = total_ms;
-}
 
 /* Trace arguments of xo_mcml_run_trace / xo_mcvox_run_trace; *floats: the
  * size of the trace data. */
@@ -752,86 +861,14 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
         xo::set_error("xo_batch_run: null or empty argument");
         return -1;
     }
-    const uint32_t block = opts->block ? opts->block : 256u;
-    if (block != 64 && block != 128 && block != 256) {
-        xo::set_error("xo_batch_run: block must be 64, 128 or 256 (got %u)", block);
-        return -1;
-    }
-    const uint64_t chunk = opts->chunk ? opts->chunk : 1024u;
-    if (opts->method != XO_METHOD_AW && opts->method != XO_METHOD_AR &&
-        opts->method != XO_METHOD_MBL) {
-        xo::set_error("xo_batch_run: unknown stepping method %d", opts->method);
-        return -1;
-    }
-    std::vector<xo::LayerL> layers((size_t)ninst * XO_BATCH_MAX_LAYERS);
-    std::vector<xo::BatchHdr> hdr(ninst);
+    std::vector<xo::LayerL> layers;
+    std::vector<xo::BatchHdr> hdr;
     std::vector<xo::BatchItem> items;
     uint64_t total = 0;
-    for (uint32_t i = 0; i < ninst; ++i) {
-        const xo_batch_instance &I = inst[i];
-        if (I.num_layers < 3 || I.num_layers > XO_BATCH_MAX_LAYERS) {
-            xo::set_error("instance %u: %u layers (3..%d supported)", i, I.num_layers,
-                          XO_BATCH_MAX_LAYERS);
-            return -1;
-        }
-        for (uint32_t l = 0; l < I.num_layers; ++l)
-            if (check_pf(I.pf_kind, I.layers[l].pf, nlut, "layer", l)) return -1;
-        if (I.pf_kind == XO_PF_LUT && !lut) {
-            xo::set_error("instance %u: a lookup-table phase function needs the LUT", i);
-            return -1;
-        }
-        if ((uint64_t)I.accum_base + I.accum_count > naccum) {
-            xo::set_error("instance %u: bins [%u, +%u) outside the accumulator (%llu)", i,
-                          I.accum_base, I.accum_count, (unsigned long long)naccum);
-            return -1;
-        }
-        uint64_t end = 0;
-        const xo_detector *dets[3] = {&I.top, &I.bottom, &I.specular};
-        for (int k = 0; k < 3; ++k) {
-            if (check_detector("instance", *dets[k], naccum, &end)) return -1;
-            if (dets[k]->kind != XO_DET_NONE &&
-                (dets[k]->offset < I.accum_base ||
-                 (uint64_t)dets[k]->offset +
-                         (dets[k]->kind == XO_DET_TOTAL ? 1u : dets[k]->n) >
-                     (uint64_t)I.accum_base + I.accum_count)) {
-                xo::set_error("instance %u: detector bins outside its range", i);
-                return -1;
-            }
-        }
-        for (uint32_t l = 0; l < XO_BATCH_MAX_LAYERS; ++l)
-            layers[(size_t)i * XO_BATCH_MAX_LAYERS + l] =
-                xo::layer_to_device(I.layers[l < I.num_layers ? l : I.num_layers - 1]);
-        if (I.source_kind != XO_SRC_LINE && I.source_kind != XO_SRC_FIBER_NI &&
-            I.source_kind != XO_SRC_FIBER) {
-            xo::set_error("instance %u: unknown source kind %d", i, I.source_kind);
-            return -1;
-        }
-        if (I.source_kind == XO_SRC_FIBER && !(I.fiber_axis.direction[2] > 0.0f)) {
-            xo::set_error("instance %u: the fiber direction must point into the sample (z > 0)",
-                          i);
-            return -1;
-        }
-        const xo_detector det[3] = {I.top, I.bottom, I.specular};
-        hdr[i] = xo::batch_header(I, det, 0u);
-        for (uint64_t first = 0; first < I.npackets; first += chunk) {
-            uint64_t cnt = I.npackets - first < chunk ? I.npackets - first : chunk;
-            items.push_back(xo::BatchItem{i, (uint32_t)cnt, first});
-        }
-        total += I.npackets;
-        if (items.size() >= 0xFFFFFFF0ull) {
-            xo::set_error("xo_batch_run: too many work items; raise the chunk size");
-            return -1;
-        }
-    }
-
-    /* Waves claim consecutive items: grouping the items of instances that
-     * run the same code (phase-function kind, source, layer count) keeps the
-     * lanes of a wave on one path when it moves from item to item. The
-     * order does not change any result. */
-    std::stable_sort(items.begin(), items.end(),
-                     [inst](const xo::BatchItem &p, const xo::BatchItem &q) {
-                         return xo::item_class(inst[p.inst]) < xo::item_class(inst[q.inst]);
-                     });
+    uint32_t block = 0;
+    if (xo::pack_batch("xo_batch_run", inst, ninst, lut != nullptr, nlut, opts, naccum,
+                       layers, hdr, items, &total, &block))
+        return -1;
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     XO_HIP(hipSetDevice(ctx->device));
@@ -858,7 +895,7 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
                               ctx->stream));
 
     xo::BatchParams P;
-    std::memset(&P, 0, sizeof(P));
+    xo::batch_params(*opts, P);
     P.layers = (const xo::LayerL *)ctx->batch_layers;
     P.hdr = (const xo::BatchHdr *)ctx->batch_hdr;
     P.items = (const xo::BatchItem *)ctx->batch_items;
@@ -868,18 +905,8 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
     P.accum = ctx->accum;
     P.stats = ctx->counters + 1;
     P.nitems = (uint32_t)items.size();
-    P.w_min = opts->w_min;
-    P.chance = opts->lottery_chance;
-    P.use_lottery = opts->use_lottery;
-    P.refill_min = opts->refill_min ? opts->refill_min : 4u;
-    P.defer_min = opts->defer_min ? opts->defer_min : xo::kDeferMinDefault;
-    P.method = opts->method;
-    P.fast_math = (opts->flags & XO_FLAG_FAST_MATH) ? 1 : 0;
-    int bpc = 1;
-    XO_HIP(xo::occupancy_batch(block, &bpc));
-    if (opts->blocks_per_cu && (int)opts->blocks_per_cu < bpc) bpc = (int)opts->blocks_per_cu;
-    if (bpc < 1) bpc = 1;
-    uint32_t blocks = (uint32_t)bpc * ctx->cus;
+    uint32_t blocks = 0;
+    if (xo::batch_grid(ctx, block, *opts, &blocks)) return -1;
 
     XO_HIP(hipMemsetAsync(ctx->accum, 0, naccum * 8, ctx->stream));
     XO_HIP(hipMemsetAsync(ctx->counters, 0, sizeof(unsigned long long) * (1 + XO_STAT_COUNT),

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/xomc.h"
+#include "xo_batch.h"
 #include "xo_internal.h"
 
 /* Return -1 from the enclosing function, with the message set, when a HIP
@@ -69,6 +70,22 @@ int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint
 int check_fluence(const char *fn, const xo_fluence &f, uint64_t det_end, uint64_t naccum);
 uint32_t choose_lanes(xo_ctx *ctx, uint32_t requested, int blocks_per_cu);
 double ms_since(std::chrono::steady_clock::time_point t0);
+void fill_report(xo_run_report *r, const unsigned long long *stats, uint32_t lanes,
+                 float kernel_ms, double total_ms);
+
+/* Validate a batch (fn names the entry point in messages) and build its
+ * device tables: layer records [instance][XO_BATCH_MAX_LAYERS], per-instance
+ * headers and the work items (chunks of opts->chunk packets, grouped by
+ * instance class). *total: packets of all instances; *block: threads per
+ * block. Returns -1 (message set) on error. */
+int pack_batch(const char *fn, const xo_batch_instance *inst, uint32_t ninst, bool have_lut,
+               uint32_t nlut, const xo_batch_opts *opts, uint64_t naccum,
+               std::vector<LayerL> &layers, std::vector<BatchHdr> &hdr,
+               std::vector<BatchItem> &items, uint64_t *total, uint32_t *block);
+/* The launch options of a batch as kernel parameters (buffers left null). */
+void batch_params(const xo_batch_opts &opts, BatchParams &P);
+/* Blocks of the persistent batch grid for this block size and options. */
+int batch_grid(xo_ctx *ctx, uint32_t block, const xo_batch_opts &opts, uint32_t *blocks);
 
 /* Device layout of a voxel grid: uint8 material indices in 4x4x4 bricks of
  * 64 bytes, bricks in [z][y][x] order, voxel (x, y, z) at byte

@@ -257,12 +257,21 @@ XO_DEV void trace_event(const xo_trace &T, float *data, TraceState &ts, float px
     ts.count++;
 }
 
-/* Fluence deposit at a point (deposition mode): the Cartesian voxel of
+/* The weight a fluence accumulator adds for an absorbed weight w: w in
+ * deposition mode, w / mua in fluence-rate mode (0 where mua is 0),
+ * fluence.py:139-141. */
+template <class M = MathExact> XO_DEV float fluence_weight(const xo_fluence &F, float w, float mua) {
+    if (F.rate) w *= (mua != 0.0f) ? M::div(1.0f, mua) : 0.0f;
+    return w;
+}
+
+/* Fluence deposit at a point: the Cartesian voxel of
  * mcsim_fluence_deposit_at (fluence.py:102-147) or the (r, z) cell of
- * fluencerz.py:112-160. Returns true if a non-zero integer was added. */
+ * fluencerz.py:112-160; mua is that of the medium the packet is in (used
+ * in fluence-rate mode). Returns true if a non-zero integer was added. */
 template <class M = MathExact>
 XO_DEV bool fluence_deposit(const xo_fluence &F, unsigned long long *accum, float px,
-                            float py, float pz, float w) {
+                            float py, float pz, float w, float mua) {
     uint64_t index;
     if (F.enabled == XO_FLUENCE_XYZ) {
         const float fx = (px - F.top_left[0]) * F.inv_step[0];
@@ -286,7 +295,7 @@ XO_DEV bool fluence_deposit(const xo_fluence &F, unsigned long long *accum, floa
     } else {
         return false;
     }
-    const uint32_t ui = f2u_sat(w * (float)F.k + 0.5f);
+    const uint32_t ui = f2u_sat(fluence_weight<M>(F, w, mua) * (float)F.k + 0.5f);
     if (ui == 0u) return false;
     atomicAdd(&accum[F.offset + index], (unsigned long long)ui);
     return true;
@@ -325,10 +334,11 @@ struct VoxEventCfgL {
  * [box, box + (bx, by, bz)) are held in LDS (sbox, x fastest): the same
  * index and integer as fluence_deposit, added to the box when the bin lies
  * in it, to accum otherwise. */
+template <class M = MathExact>
 XO_DEV bool fluence_deposit_boxed(const xo_fluence &F, unsigned long long *accum,
                                   unsigned long long *sbox, const int32_t box[3], int32_t bx,
                                   int32_t by, int32_t bz, float px, float py, float pz,
-                                  float w) {
+                                  float w, float mua) {
     const float fx = (px - F.top_left[0]) * F.inv_step[0];
     const float fy = (py - F.top_left[1]) * F.inv_step[1];
     const float fz = (pz - F.top_left[2]) * F.inv_step[2];
@@ -336,7 +346,7 @@ XO_DEV bool fluence_deposit_boxed(const xo_fluence &F, unsigned long long *accum
           fy < (float)F.shape[1] && fz < (float)F.shape[2]))
         return false;
     const uint32_t ix = f2u_sat(fx), iy = f2u_sat(fy), iz = f2u_sat(fz);
-    const uint32_t ui = f2u_sat(w * (float)F.k + 0.5f);
+    const uint32_t ui = f2u_sat(fluence_weight<M>(F, w, mua) * (float)F.k + 0.5f);
     if (ui == 0u) return false;
     const uint32_t rx = ix - (uint32_t)box[0], ry = iy - (uint32_t)box[1],
                    rz = iz - (uint32_t)box[2];

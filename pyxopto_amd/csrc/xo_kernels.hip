@@ -237,8 +237,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                     (mua != 0.0f)
                         ? step - M::div(-M::log_unit(1.0f - rng_next(x, a) * fraction), mua)
                         : 0.0f;
-                if (fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px - back * dx, py - back * dy, pz - back * dz,
-                                    deposit))
+                if (fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px - back * dx,
+                                       py - back * dy, pz - back * dz, deposit, mua))
                     st[XO_STAT_DEPOSITS]++;
             }
         };
@@ -353,7 +353,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 w -= deposit;
                 if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                 if (use_fluence &&
-                    fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit))
+                    fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit,
+                                       L.mua))
                     st[XO_STAT_DEPOSITS]++;
                 done = true;
             } else {
@@ -362,7 +363,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                     w -= deposit;
                     if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                     if (use_fluence &&
-                        fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit))
+                        fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit,
+                                           L.mua))
                         st[XO_STAT_DEPOSITS]++;
                 }
                 PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
@@ -509,11 +511,11 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     }
     const bool use_fluence = P.cfg.fluence.enabled != 0;
     /* a fluence deposit: through the LDS box when there is one */
-    auto deposit_fluence = [&](float qx, float qy, float qz, float wgt) {
+    auto deposit_fluence = [&](float qx, float qy, float qz, float wgt, float mua) {
         if (P.fl_box_on)
-            return fluence_deposit_boxed(lds_fresh(sev.fluence), P.accum, sfl, P.fl_box, kFlBoxX,
-                                         kFlBoxY, kFlBoxZ, qx, qy, qz, wgt);
-        return fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, qx, qy, qz, wgt);
+            return fluence_deposit_boxed<M>(lds_fresh(sev.fluence), P.accum, sfl, P.fl_box,
+                                            kFlBoxX, kFlBoxY, kFlBoxZ, qx, qy, qz, wgt, mua);
+        return fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, qx, qy, qz, wgt, mua);
     };
 
     const uint32_t gid = blockIdx.x * XO_BLOCK + tid;
@@ -734,7 +736,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                 const float deposit = w;
                 w -= deposit;
                 if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-                if (use_fluence && deposit_fluence(p[0], p[1], p[2], deposit))
+                if (use_fluence && deposit_fluence(p[0], p[1], p[2], deposit, Mt.mua))
                     st[XO_STAT_DEPOSITS]++;
                 done = true;
             } else {
@@ -742,8 +744,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                     float deposit = w * Mt.mua_inv_mut;
                     w -= deposit;
                     if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-                    /* mcsim_fluence_deposit_at (deposition), fluence.py:102-147 */
-                    if (use_fluence && deposit_fluence(p[0], p[1], p[2], deposit))
+                    /* mcsim_fluence_deposit_at, fluence.py:102-147 */
+                    if (use_fluence && deposit_fluence(p[0], p[1], p[2], deposit, Mt.mua))
                         st[XO_STAT_DEPOSITS]++;
                 }
                 PfL pf{Mt.g, Mt.beta, Mt.la, Mt.lb, Mt.lc, Mt.loff, Mt.lsize_m1};
@@ -798,7 +800,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
                         ? d_ok - M::div(-M::log_unit(1.0f - rng_next(x, a) * fraction), mua)
                         : 0.0f;
                 if (deposit_fluence(p[0] - back * d[0], p[1] - back * d[1],
-                                    p[2] - back * d[2], deposit))
+                                    p[2] - back * d[2], deposit, mua))
                     st[XO_STAT_DEPOSITS]++;
             }
         }

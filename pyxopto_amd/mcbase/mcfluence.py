@@ -1,10 +1,13 @@
-"""Deposition accumulators: Cartesian (xopto/mcbase/mcfluence/fluence.py)
+"""Fluence accumulators: Cartesian (xopto/mcbase/mcfluence/fluence.py)
 and cylindrical r-z (xopto/mcbase/mcfluence/fluencerz.py).
 
-Deposition mode: each absorbed weight is added to the voxel (or ring cell)
-that contains the packet; ``data`` is raw/(nphotons*volume), volume
-dx*dy*dz (fluence.py:343-347) or pi (r1^2 - r0^2) dz (fluencerz.py:343-
-348)."""
+Mode 'deposition': each absorbed weight is added to the voxel (or ring
+cell) that contains the packet. Mode 'fluence' (fluence rate): the absorbed
+weight divided by the absorption coefficient of the medium the packet is in
+(nothing where that is 0), fluence.py:102-147 with MC_FLUENCE_MODE_RATE.
+``data`` is raw/(nphotons*volume) in both modes, volume dx*dy*dz
+(fluence.py:343-347) or pi (r1^2 - r0^2) dz (fluencerz.py:343-348): the
+absorbed energy density (1/m^3) or the fluence rate (1/m^2)."""
 import numpy as np
 
 from .. import _native as N
@@ -21,6 +24,14 @@ def _pop_type(cls, data: dict) -> dict:
     return data
 
 
+def _check_mode(mode):
+    """fluence.py:207-210"""
+    if mode not in ('fluence', 'deposition'):
+        raise ValueError(
+            'The value of mode parameter must be '
+            '"fluence" or "deposition" but got {}!'.format(mode))
+
+
 class Fluence:
     def __init__(self, xaxis, yaxis: Axis = None, zaxis: Axis = None,
                  mode: str = 'deposition'):
@@ -33,9 +44,7 @@ class Fluence:
             mode, k, nphotons = other.mode, other.k, other.nphotons
             if other.raw is not None:
                 data = np.copy(other.raw)
-        if mode not in ('deposition',):
-            raise NotImplementedError(
-                'Only the "deposition" fluence mode is implemented!')
+        _check_mode(mode)
         self._x_axis, self._y_axis, self._z_axis = xaxis, yaxis, zaxis
         self._mode = mode
         self._k = int(k)
@@ -58,7 +67,8 @@ class Fluence:
     def _get_data(self):
         k = 1.0 / (max(self._nphotons, 1) * self.dx * self.dy * self.dz)
         return self._data * k
-    data = property(_get_data, None, None, 'Deposition (1/m^3).')
+    data = property(_get_data, None, None,
+                    'Deposition (1/m^3) or fluence rate (1/m^2).')
 
     def update_data(self, mc, accumulators, nphotons: int):
         new = np.asarray(accumulators[0]).reshape(self.shape) * (1.0 / self._k)
@@ -71,6 +81,7 @@ class Fluence:
 
     def pack(self, mc, target: N.XoFluence):
         n = self._x_axis.n * self._y_axis.n * self._z_axis.n
+        target.rate = int(self._mode == 'fluence')
         target.enabled = N.XO_FLUENCE_XYZ
         target.offset = mc.allocate_accumulator(self, n)
         for i, ax in enumerate((self._x_axis, self._y_axis, self._z_axis)):
@@ -114,9 +125,7 @@ class FluenceRz:
                                          other.nphotons)
             if other.raw is not None:
                 data = np.copy(other.raw)
-        if mode not in ('deposition',):
-            raise NotImplementedError(
-                'Only the "deposition" fluence mode is implemented!')
+        _check_mode(mode)
         self._r_axis, self._z_axis = raxis, zaxis
         self._center = np.array([float(center[0]), float(center[1])])
         self._mode = mode
@@ -141,7 +150,8 @@ class FluenceRz:
         area = np.pi * (e[1:] ** 2 - e[:-1] ** 2)
         k = 1.0 / (max(self._nphotons, 1) * area * self.dz)
         return self._data * k[np.newaxis, :]
-    data = property(_get_data, None, None, 'Deposition (1/m^3).')
+    data = property(_get_data, None, None,
+                    'Deposition (1/m^3) or fluence rate (1/m^2).')
 
     def update_data(self, mc, accumulators, nphotons: int):
         new = np.asarray(accumulators[0]).reshape(self.shape) * (1.0 / self._k)
@@ -154,6 +164,7 @@ class FluenceRz:
 
     def pack(self, mc, target: N.XoFluence):
         """cl_pack of fluencerz.py:425-450."""
+        target.rate = int(self._mode == 'fluence')
         target.enabled = N.XO_FLUENCE_RZ
         target.offset = mc.allocate_accumulator(
             self, self._r_axis.n * self._z_axis.n)

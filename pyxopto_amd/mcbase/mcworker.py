@@ -62,6 +62,8 @@ class McWorker:
                                   for item in (cl_build_options or [])]
         self._fast_math = fast_math_requested(cl_build_options)
         self._ctx = None
+        self._rng_seeds_x = None
+        self._rng_seeds_a = None
         self._alloc_offset = 0
         self._allocations = {}
         self._luts = []
@@ -93,6 +95,30 @@ class McWorker:
 
     rng_seeds = property(_get_rng_seeds, None, None,
                          'Current (x, a) state of all RNG streams.')
+
+    def _host_seeds(self):
+        if self._rng_seeds_x is None:
+            self._rng_seeds_x, self._rng_seeds_a = N.init_rng(
+                N.XO_MAX_SEEDS, self._rnginit)
+        return self._rng_seeds_x, self._rng_seeds_a
+
+    def _copy_seeds(self):
+        """run(copyseeds=True): the mutable generator states of every lane
+        downloaded into rng_seeds_x (xopto/mcml/mc.py:962-966)."""
+        ctx = self._context()
+        x, a = ctx.get_rng(ctx.max_lanes)
+        self._host_seeds()
+        self._rng_seeds_x[:len(x)] = x
+        self._rng_seeds_a[:len(a)] = a
+
+    rng_seeds_x = property(lambda self: self._host_seeds()[0], None, None,
+                           'Mutable random number generator seeds X: the '
+                           'initial states, or the device states after a '
+                           'run with copyseeds=True '
+                           '(xopto/mcbase/mcworker.py:1229-1232).')
+    rng_seeds_a = property(lambda self: self._host_seeds()[1], None, None,
+                           'Immutable random number generator seeds A '
+                           '(xopto/mcbase/mcworker.py:1234-1237).')
 
     # ---------------- packing helpers ----------------
     def _reset_allocations(self):

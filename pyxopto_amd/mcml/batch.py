@@ -126,6 +126,17 @@ class Batch(McWorker):
         self._record_report(rep, int(sum(i.nphotons for i in self.instances)))
         return self.results(accum), rep
 
+    def plan(self, opts: N.XoBatchOpts = None, packed=None) -> 'BatchRunPlan':
+        """The batch uploaded once with its launch captured as a graph
+        (xo_batch_plan_create); BatchRunPlan.run() replays it and returns
+        what run() returns, the same integers."""
+        arr, lut, naccum = packed if packed is not None else self.pack()
+        if opts is None:
+            from .autotune import tuned_opts
+            opts = tuned_opts(self, (arr, lut, naccum))
+        return BatchRunPlan(self, self._context().batch_plan(arr, lut, opts,
+                                                             naccum))
+
     def results(self, accum) -> List[mcdetector.Detectors]:
         out = []
         for inst, packer in zip(self.instances, self._packers):
@@ -141,3 +152,25 @@ class Batch(McWorker):
                                                inst.nphotons)
             out.append(res)
         return out
+
+
+class BatchRunPlan:
+    """A Batch whose device tables and captured launch are kept for repeated
+    runs (Batch.plan)."""
+
+    def __init__(self, batch: Batch, plan: N.BatchPlan):
+        self._batch, self._plan = batch, plan
+        self._npk = int(sum(i.nphotons for i in batch.instances))
+
+    def run(self):
+        """(list of Detectors, run report), as Batch.run."""
+        accum, rep = self._plan.run()
+        self._batch._record_report(rep, self._npk)
+        return self._batch.results(accum), rep
+
+    def run_raw(self):
+        """(accumulator array, run report) without building result objects."""
+        return self._plan.run()
+
+    def close(self):
+        self._plan.close()

@@ -156,6 +156,8 @@ class Mc(McWorker):
             accum, rep = ctx.run_mcml(cfg, lut, nphotons, naccum, lanes,
                                       flags)
         self._record_report(rep, nphotons)
+        if copyseeds:
+            self._copy_seeds()
         self._last_accum = accum
         detectors_res = out[2]
         if self._detectors is not None:

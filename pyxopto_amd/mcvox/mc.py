@@ -144,6 +144,8 @@ class Mc(McWorker):
                                        lanes, flags)
         self._uploaded_shape = self._voxels.shape
         self._record_report(rep, nphotons)
+        if copyseeds:
+            self._copy_seeds()
         _, fluence_res, detectors_res = self.unpack(accum, nphotons, out)
         if verbose:
             print('McKernel processed {:,d} packets: {:.3f} ms'.format(

@@ -18,6 +18,9 @@ The reference keeps no such model: its inverse-problem helpers fit
 regressors to simulated datasets (xopto/util/regression/ann.py, TensorFlow).
 """
 import ctypes
+import json
+import os
+import sys
 import time
 from ctypes import POINTER, c_double, c_float, c_uint32, c_uint64
 from typing import Dict, Optional, Tuple
@@ -34,6 +37,19 @@ DEFAULT_BOX = {'mua': (10.0, 1000.0),      # 1/m
                'g': (0.6, 0.95),
                'beta': (0.0, 1.0),
                'n': (1.33, 1.50)}
+# The box of the shipped network: mua 0.01-50 /cm, g 0.6-0.95, beta 0-1 and
+# n 1.33-1.50 as the 4096-instance layered batch draws its layers; mus'
+# 3-60 /cm, which holds the reference's validation grid (mus' 5-60 /cm,
+# xopto/mcml/test/validate.py:348-366).
+TISSUE_BOX = {'mua': (1.0, 5000.0),
+              'musr': (300.0, 6000.0),
+              'g': (0.6, 0.95),
+              'beta': (0.0, 1.0),
+              'n': (1.33, 1.50)}
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                           'weights')
+TISSUE_WEIGHTS = os.path.join(WEIGHTS_DIR, 'reflectance_semiinfinite_mhg.npz')
+TISSUE_RECORD = os.path.join(WEIGHTS_DIR, 'reflectance_semiinfinite_mhg.json')
 XINIT = 0x5EED000000000001
 
 
@@ -172,27 +188,43 @@ def _instance(row, packets: int, seed: int):
                            seed=seed)
 
 
+def _row_seed(seed: int, k: int) -> int:
+    return (seed * 0x9E3779B97F4A7C15 + k + 1) & 0xFFFFFFFFFFFFFFFF
+
+
 def simulate(rows, packets: int, seed: int = 1,
-             context: N.Context = None) -> Tuple[np.ndarray, float]:
+             context: N.Context = None, fast_math: bool = True,
+             first_row: int = 0) -> Tuple[np.ndarray, float]:
     """Monte Carlo radial reflectance [len(rows)][100] of each feature row
-    (one batch launch, packet generators keyed on (seed, row)); returns
-    (R, seconds)."""
+    (one batch launch, packet generators keyed on (seed, first_row + k) for
+    row k); returns (R, seconds of the run call). fast_math: the
+    production arithmetic (the reference's -cl-fast-relaxed-math build)."""
+    R, stats = _simulate(rows, packets, seed, context, fast_math, first_row)
+    return R, stats['run_s']
+
+
+def _simulate(rows, packets, seed, context, fast_math, first_row=0):
     from .mcml.batch import Batch
     rows = np.atleast_2d(rows)
-    insts = [_instance(r, packets, (seed * 0x9E3779B97F4A7C15 + k + 1) &
-                       0xFFFFFFFFFFFFFFFF) for k, r in enumerate(rows)]
-    b = Batch(insts, rnginit=XINIT)
+    insts = [_instance(r, packets, _row_seed(seed, first_row + k))
+             for k, r in enumerate(rows)]
+    b = Batch(insts, rnginit=XINIT,
+              cl_build_options=['-cl-fast-relaxed-math'] if fast_math
+              else None)
     if context is not None:
         b._ctx = context
     packed = b.pack()
     t0 = time.perf_counter()
-    res, _ = b.run(b.opts(chunk=2048, block=256), packed=packed)
-    dt = time.perf_counter() - t0
-    return np.array([r.top.reflectance for r in res]), dt
+    res, rep = b.run(b.opts(chunk=8192 if packets >= 65536 else 2048,
+                            block=256), packed=packed)
+    run_s = time.perf_counter() - t0
+    R = np.array([r.top.reflectance for r in res])
+    return R, {'run_s': run_s, 'kernel_s': rep.kernel_ms * 1e-3,
+               'steps': int(rep.steps), 'packets': int(rep.packets)}
 
 
 def write_dataset(folder: str, rows, packets: int, seed: int = 1,
-                  context: N.Context = None) -> list:
+                  context: N.Context = None, fast_math: bool = True) -> list:
     """Simulate the feature rows (one batch launch) and write one file per
     row in the layered dataset layout (analytical_pf.template.py:226-246;
     ``dataset.mcml_record``), named by mua and mus' like the reference's
@@ -201,9 +233,11 @@ def write_dataset(folder: str, rows, packets: int, seed: int = 1,
     from . import dataset
     from .mcml.batch import Batch
     rows = np.atleast_2d(rows)
-    insts = [_instance(r, packets, (seed * 0x9E3779B97F4A7C15 + k + 1) &
-                       0xFFFFFFFFFFFFFFFF) for k, r in enumerate(rows)]
-    b = Batch(insts, rnginit=XINIT)
+    insts = [_instance(r, packets, _row_seed(seed, k))
+             for k, r in enumerate(rows)]
+    b = Batch(insts, rnginit=XINIT,
+              cl_build_options=['-cl-fast-relaxed-math'] if fast_math
+              else None)
     if context is not None:
         b._ctx = context
     res, _ = b.run(b.opts(chunk=2048, block=256))
@@ -254,6 +288,26 @@ class ReflectanceSurrogate:
         self.net.set_data(self._x(rows), self._y(R).astype(np.float32))
         return self.net.train(steps, min(batch, self.net.max_batch), lr, seed)
 
+    def fit_schedule(self, rows, R, schedule, batch: int = 4096,
+                     seed: int = 1, log=None):
+        """Train on (rows, R) through phases of (steps, learning rate) with
+        Adam state carried across them; returns (losses, kernel ms)."""
+        logR = np.log(np.maximum(R, self.floor))
+        self.y_mean = logR.mean(axis=0)
+        self.y_std = np.maximum(logR.std(axis=0), 1e-6)
+        self.net.set_data(self._x(rows), self._y(R).astype(np.float32))
+        losses, ms = [], 0.0
+        for k, (steps, lr) in enumerate(schedule):
+            part, t = self.net.train(int(steps), min(batch, self.net.max_batch),
+                                     float(lr), seed + k)
+            losses.append(part)
+            ms += t
+            if log:
+                log('train: phase {} ({} steps, lr {:g}): loss {:.3e} -> '
+                    '{:.3e}, {:.1f} s'.format(k, steps, lr, part[:20].mean(),
+                                              part[-100:].mean(), t * 1e-3))
+        return np.concatenate(losses), ms
+
     def predict(self, rows):
         """(R [rows][100], kernel ms)."""
         y, ms = self.net.forward(self._x(rows))
@@ -270,6 +324,11 @@ class ReflectanceSurrogate:
             self.y_mean, self.y_std = z['y_mean'], z['y_std']
             self._lo, self._hi = z['lo'], z['hi']
             self.floor = float(z['floor'])
+        self.box = {'mua': (float(np.exp(self._lo[0])), float(np.exp(self._hi[0]))),
+                    'musr': (float(np.exp(self._lo[1])), float(np.exp(self._hi[1]))),
+                    'g': (float(self._lo[2]), float(self._hi[2])),
+                    'beta': (float(self._lo[3]), float(self._hi[3])),
+                    'n': (float(self._lo[4]), float(self._hi[4]))}
 
 
 class Service:
@@ -321,3 +380,141 @@ class Service:
                 out[i] = (R_mc[k], 'mc_outside', None)
                 self.counts['mc_outside'] += 1
         return out
+
+
+def generate_training_set(n: int, packets: int, box=TISSUE_BOX,
+                          seed: int = 20261203, shard: int = 2048,
+                          context: N.Context = None, log=None):
+    """n feature rows sampled in the box and their Monte Carlo radial
+    reflectance at `packets` packets each, simulated by the batch engine
+    `shard` rows per launch (row k's packets are keyed on (seed, k), so a
+    row's result does not depend on the shard it runs in). Returns
+    (rows, R float32 [n][100], totals)."""
+    rows = sample_box(n, box, seed=seed)
+    R = np.empty((n, NBINS), dtype=np.float32)
+    tot = {'packets': 0, 'steps': 0, 'kernel_s': 0.0}
+    t0 = time.perf_counter()
+    for first in range(0, n, shard):
+        part = rows[first:first + shard]
+        Rs, st = _simulate(part, packets, seed, context, True, first)
+        R[first:first + len(part)] = Rs
+        for k in tot:
+            tot[k] += st[k]
+        if log:
+            log('generate: rows {}-{} of {}: {:.3g} packets/s, {:.3g} steps/s, '
+                '{:.0f} s so far'.format(first, first + len(part), n,
+                                         st['packets'] / st['kernel_s'],
+                                         st['steps'] / st['kernel_s'],
+                                         time.perf_counter() - t0))
+    tot['wall_s'] = time.perf_counter() - t0
+    return rows, R, tot
+
+
+DEFAULT_SCHEDULE = ((20000, 1e-3), (20000, 3e-4), (15000, 1e-4),
+                    (10000, 3e-5))
+
+
+def build_tissue_surrogate(samples: int = 200000, packets: int = 1000000,
+                           out_dir: str = WEIGHTS_DIR, seed: int = 20261203,
+                           schedule=DEFAULT_SCHEDULE, batch: int = 4096,
+                           validation: int = 2048, log=print):
+    """Generate the training set over TISSUE_BOX, train the network, and
+    write its weights (TISSUE_WEIGHTS layout) and a JSON record of the
+    generation, the training and the validation errors into out_dir. The
+    last `validation` rows are held out; their Monte Carlo noise floor is
+    measured with a second, independent simulation of the same rows."""
+    ctx = N.Context(0, 0, XINIT)
+    rows, R, gen = generate_training_set(samples, packets, TISSUE_BOX, seed,
+                                         context=ctx, log=log)
+    ntr = samples - validation
+    sur = ReflectanceSurrogate(TISSUE_BOX, context=ctx, seed=seed & 0xFFFF,
+                               max_batch=batch)
+    losses, train_ms = sur.fit_schedule(rows[:ntr], R[:ntr], schedule,
+                                        batch=batch, seed=seed, log=log)
+    vrows, vR = rows[ntr:], R[ntr:]
+    R_sur, _ = sur.predict(vrows)
+    R_mc2, _ = simulate(vrows, packets, seed + 1, ctx)
+    err_sur = profile_error(R_sur, vR)
+    err_mc = profile_error(R_mc2, vR)
+
+    def q(e):
+        return {'median': float(np.median(e)), 'mean': float(e.mean()),
+                'p90': float(np.quantile(e, 0.9)),
+                'p99': float(np.quantile(e, 0.99)), 'max': float(e.max()),
+                'frac_above_0.02': float(np.mean(e > 0.02))}
+
+    os.makedirs(out_dir, exist_ok=True)
+    wpath = os.path.join(out_dir, os.path.basename(TISSUE_WEIGHTS))
+    sur.save(wpath)
+    record = {
+        'network': '5-256-256-256-100 relu, bf16 MFMA products, fp32 '
+                   'accumulation, Adam on fp32 master weights',
+        'inputs': 'ln mua [1/m], ln musr [1/m], g, beta, n (scaled to '
+                  '[-1, 1] over the box)',
+        'outputs': 'ln R over 100 radial bins 0-5 mm [1/m^2], standardized '
+                   'per bin',
+        'medium': 'semi-infinite, modified Henyey-Greenstein phase '
+                  'function, air above, normal pencil beam (Line source)',
+        'box': TISSUE_BOX,
+        'samples': samples, 'packets_per_sample': packets, 'seed': seed,
+        'engine': 'batch kernel, hardware arithmetic '
+                  '(-cl-fast-relaxed-math), packets keyed on (seed, row)',
+        'generation': gen,
+        'training': {'rows': ntr, 'batch': batch,
+                     'schedule': [list(p) for p in schedule],
+                     'steps': int(len(losses)), 'kernel_s': train_ms * 1e-3,
+                     'loss_first': float(losses[:100].mean()),
+                     'loss_last': float(losses[-500:].mean())},
+        'validation': {'rows': int(validation),
+                       'error': 'area-weighted mean absolute relative error '
+                                'of the profile against the Monte Carlo run '
+                                'of the same row',
+                       'surrogate_vs_mc': q(err_sur),
+                       'mc_vs_mc_same_budget': q(err_mc)},
+    }
+    with open(os.path.join(out_dir, os.path.basename(TISSUE_RECORD)), 'w') as f:
+        json.dump(record, f, indent=1)
+    log('validation: surrogate vs MC median {:.4f} p90 {:.4f}; MC vs MC '
+        'median {:.4f} p90 {:.4f}'.format(
+            record['validation']['surrogate_vs_mc']['median'],
+            record['validation']['surrogate_vs_mc']['p90'],
+            record['validation']['mc_vs_mc_same_budget']['median'],
+            record['validation']['mc_vs_mc_same_budget']['p90']))
+    ctx.close()
+    return record
+
+
+def load_tissue_surrogate(context: N.Context = None,
+                          path: str = TISSUE_WEIGHTS) -> ReflectanceSurrogate:
+    """The shipped network of TISSUE_BOX (weights written by
+    build_tissue_surrogate)."""
+    sur = ReflectanceSurrogate(TISSUE_BOX, context=context)
+    sur.load(path)
+    return sur
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(
+        description='Generate the Monte Carlo training set of the '
+                    'reflectance surrogate over its box, train the network '
+                    'and write its weights and record.')
+    ap.add_argument('--samples', type=int, default=200000)
+    ap.add_argument('--packets', type=int, default=1000000)
+    ap.add_argument('--out', default=WEIGHTS_DIR)
+    ap.add_argument('--seed', type=int, default=20261203)
+    ap.add_argument('--steps-scale', type=float, default=1.0,
+                    help='scale of every phase of the training schedule')
+    ap.add_argument('--validation', type=int, default=2048)
+    a = ap.parse_args(argv)
+    sched = tuple((max(1, int(s * a.steps_scale)), lr)
+                  for s, lr in DEFAULT_SCHEDULE)
+
+    def log(msg):
+        print(msg, flush=True)
+    build_tissue_surrogate(a.samples, a.packets, a.out, a.seed, sched,
+                           validation=a.validation, log=log)
+
+
+if __name__ == '__main__':
+    main(sys.argv[1:])

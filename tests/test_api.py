@@ -168,3 +168,13 @@ def test_no_gpu_raises_loudly_without_device():
         pytest.skip('a device is present')
     with pytest.raises(N.NativeError, match='no HIP device'):
         N.Context(0, 0, XINIT)
+
+
+def test_rng_seeds_before_any_run_are_the_initial_states():
+    """rng_seeds_x / rng_seeds_a hold init_RNG's states for every seed row
+    (xopto/mcbase/mcworker.py:1219-1237) without touching a device."""
+    sim = config1_sim()
+    x, a = N.init_rng(64, sim.rnginit)
+    assert sim.rng_seeds_x.shape == (N.XO_MAX_SEEDS,)
+    np.testing.assert_array_equal(sim.rng_seeds_x[:64], x)
+    np.testing.assert_array_equal(sim.rng_seeds_a[:64], a)

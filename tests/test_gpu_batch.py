@@ -97,3 +97,28 @@ def test_batch_rejects_unknown_method_and_backward_fiber(mixed):
     one[0].fiber_axis.direction[2] = -1.0
     with pytest.raises(RuntimeError, match='fiber direction'):
         b.context.run_batch(one, lut, b.opts(chunk=1024), naccum)
+
+
+@pytest.mark.parametrize('fast', [False, True])
+def test_batch_plan_replays_equal_batch_run(fast):
+    """A captured batch launch (xo_batch_plan_*): every replay returns the
+    integers of xo_batch_run, with eager batch runs on the same stream
+    between the replays."""
+    b = Batch(small_batch(40, seed=11), rnginit=0x5EED000000000001,
+              cl_build_options=['-cl-fast-relaxed-math'] if fast else None)
+    arr, lut, naccum = b.pack()
+    opts = b.opts(chunk=1024)
+    ctx = b.context
+    ref, rep_ref = ctx.run_batch(arr, lut, opts, naccum)
+    plan = ctx.batch_plan(arr, lut, opts, naccum)
+    part = (type(arr[0]) * 5)(*arr[:5])
+    for k in range(3):
+        print('case plan replay', k, 'fast', fast, flush=True)
+        acc, rep = plan.run()
+        np.testing.assert_array_equal(acc, ref)
+        assert rep.steps == rep_ref.steps and rep.packets == rep_ref.packets
+        other, _ = ctx.run_batch(part, lut, b.opts(chunk=512), naccum)
+        assert other.sum() > 0
+    res, rep = b.plan(opts).run()
+    assert rep.steps == rep_ref.steps
+    ctx.close()        # closes the plans first

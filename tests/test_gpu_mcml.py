@@ -192,3 +192,24 @@ def test_mc_run_api_matches_oracle_statistics():
     m1, s1 = batch_mean_se(np.asarray(res))
     m2, s2 = batch_mean_se(o)
     assert abs(m1 - m2) < 3.0 * np.sqrt(s1 ** 2 + s2 ** 2)
+
+
+def test_run_copyseeds_downloads_the_generator_states():
+    """Mc.run(copyseeds=True) copies the device's mutable generator states
+    into rng_seeds_x (xopto/mcml/mc.py:962-966); without it rng_seeds_x
+    keeps the initial states. Under the static schedule the states of the
+    used lanes are the oracle's final states."""
+    sim = config1_sim()
+    x0 = sim.rng_seeds_x.copy()
+    lanes, npk = 4096, 10000
+    sim.run(npk, maxthreads=lanes, static_schedule=True)
+    np.testing.assert_array_equal(sim.rng_seeds_x, x0)
+    sim.run(npk, maxthreads=lanes, static_schedule=True, copyseeds=True)
+    cfg, lut, naccum = sim.pack()
+    x, a = N.init_rng(lanes, XINIT)
+    _, _, x1 = O.mcml_run(cfg, lut, x, a, npk, naccum, O.MATH_KERNEL)
+    _, _, x2 = O.mcml_run(cfg, lut, x1, a, npk, naccum, O.MATH_KERNEL)
+    np.testing.assert_array_equal(sim.rng_seeds_x[:lanes], x2)
+    np.testing.assert_array_equal(sim.rng_seeds_x[lanes:], x0[lanes:])
+    np.testing.assert_array_equal(sim.rng_seeds_a[:lanes], a)
+    sim.context.close()

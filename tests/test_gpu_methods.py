@@ -126,3 +126,38 @@ def test_voxel_mbl_dynamic_schedule_statistics():
     z = z_scores(mg[ok], sg[ok], mo[ok], so[ok])
     assert np.abs(z).max() < 4.5, np.abs(z).max()
     ctx.close()
+
+
+@pytest.mark.parametrize('kernel,method,geometry', [
+    ('mcml', N.XO_METHOD_AW, 'xyz'), ('mcml', N.XO_METHOD_MBL, 'rz'),
+    ('mcml', N.XO_METHOD_AR, 'rz'), ('mcvox', N.XO_METHOD_AW, 'xyz'),
+    ('mcvox', N.XO_METHOD_MBL, 'xyz'), ('mcvox', N.XO_METHOD_AR, 'rz')])
+def test_fluence_rate_static_schedule_bit_exact(kernel, method, geometry):
+    """Fluence-rate mode (weight / mua of the current layer or material,
+    fluence.py:139-141) in both kernels, the voxel one also through its LDS
+    box of Cartesian bins: bit-exact with the oracle."""
+    print('case fluence rate', kernel, method, geometry, flush=True)
+    lanes, npk = 8192, 40000
+    ctx = N.Context(0, 0, XINIT)
+    x, a = N.init_rng(lanes, XINIT)
+    if kernel == 'mcml':
+        sim = _sim(method, geometry)
+        cfg, lut, naccum = sim.pack()
+        cfg.fluence.rate = 1
+        gpu, rep = ctx.run_mcml(cfg, lut, npk, naccum, lanes,
+                                N.XO_FLAG_STATIC_SCHEDULE)
+        ref, counts, _ = O.mcml_run(cfg, lut, x, a, npk, naccum,
+                                    O.MATH_KERNEL)
+    else:
+        sim = skin_vessel_sim(method=method, rz=geometry == 'rz')
+        cfg, lut, naccum = sim.pack()
+        cfg.fluence.rate = 1
+        vox = sim.voxels.material
+        gpu, rep = ctx.run_mcvox(cfg, vox, lut, npk, naccum, lanes,
+                                 N.XO_FLAG_STATIC_SCHEDULE)
+        ref, counts, _ = O.mcvox_run(cfg, vox, lut, x, a, npk, naccum,
+                                     O.MATH_KERNEL)
+    assert int(rep.steps) == int(counts[0])
+    np.testing.assert_array_equal(gpu, ref)
+    assert gpu[cfg.fluence.offset:].sum() > 0
+    ctx.close()

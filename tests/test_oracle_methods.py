@@ -26,7 +26,7 @@ METHODS = {'aw': N.XO_METHOD_AW, 'ar': N.XO_METHOD_AR,
 
 
 def _sim(method, mua, mus, n_slab=1.0, n_out=(1.0, 1.0), g=0.8, d=1e-3,
-         fluence='xyz', nz=20):
+         fluence='xyz', nz=20, mode='deposition'):
     L, D = mc.mclayer, mc.mcdetector
     pf = mc.mcpf.Hg(g)
     layers = L.Layers([
@@ -37,9 +37,9 @@ def _sim(method, mua, mus, n_slab=1.0, n_out=(1.0, 1.0), g=0.8, d=1e-3,
     A = mc.mcfluence.Axis
     if fluence == 'xyz':
         flu = mc.mcfluence.Fluence(A(-1.0, 1.0, 1), A(-1.0, 1.0, 1),
-                                   A(0.0, d, nz))
+                                   A(0.0, d, nz), mode=mode)
     else:
-        flu = mc.mcfluence.FluenceRz(A(0.0, 1.0, 1), A(0.0, d, nz))
+        flu = mc.mcfluence.FluenceRz(A(0.0, 1.0, 1), A(0.0, d, nz), mode=mode)
     return mc.Mc(layers, mc.mcsource.Line(), det, fluence=flu,
                  options=[mc.mcoptions.McMethod(method)],
                  rnginit=0x5EED000000000001)
@@ -73,6 +73,88 @@ def test_nonscattering_slab_beer_lambert(method, geometry):
     assert top == 0.0 and spec == 0.0
     # every packet's weight ends up deposited or transmitted
     assert abs(dep.sum() + bottom - 1.0) < 1e-4
+
+
+@pytest.mark.parametrize('method', ['aw', 'ar', 'mbl'])
+@pytest.mark.parametrize('geometry', ['xyz', 'rz'])
+def test_fluence_rate_nonscattering_slab_closed_form(method, geometry):
+    """Fluence-rate mode (fluence.py:139-141, weight / mua): the collimated
+    beam's fluence averaged over each depth bin, (exp(-mua z0) -
+    exp(-mua z1)) / (mua dz), times dz: a multiplication by mua instead of
+    a division, or no scaling, is off by a factor of mua^2 or mua."""
+    mua, d, nz = 1000.0, 1e-3, 20
+    sim = _sim(METHODS[method], mua, 0.0, d=d, fluence=geometry, nz=nz,
+               mode='fluence')
+    cfg, _, _ = sim.pack()
+    assert cfg.fluence.rate == 1
+    b = _batches(sim, 4000)
+    m, se = batch_mean_se(b)
+    rate = m[3:3 + nz]
+    z = np.linspace(0.0, d, nz + 1)
+    expect = (np.exp(-mua * z[:-1]) - np.exp(-mua * z[1:])) / mua
+    zs = (rate - expect) / np.maximum(se[3:3 + nz], 1e-15)
+    assert np.abs(zs).max() < 5.0, (method, zs)
+    # the path length of the unscattered beam inside the slab, 1/mua (1 -
+    # exp(-mua d)), is the integral of the fluence over depth
+    ms, ss = batch_mean_se(b[:, 3:3 + nz].sum(axis=1))
+    expect_sum = (1.0 - np.exp(-mua * d)) / mua
+    # microscopic Beer-Lambert has no variance here: fp32 rounding only
+    assert abs(ms - expect_sum) < 5.0 * ss + 2e-4 * expect_sum
+
+
+@pytest.mark.parametrize('method', ['aw', 'mbl'])
+def test_fluence_rate_is_deposition_over_the_local_mua(method):
+    """Same seeds, two layers of different mua: the packets walk the same
+    paths in both modes (the mode changes no draw), so each bin of the rate
+    run times the mua of the layer holding that bin equals the deposition
+    run's bin up to the per-deposit rounding; the mua of the wrong layer is
+    off by a factor of 10."""
+    L, D = mc.mclayer, mc.mcdetector
+    pf = mc.mcpf.Hg(0.85)
+    mua = (500.0, 50.0)
+    d = (5e-4, 1e-3)
+
+    def make(mode):
+        layers = L.Layers([
+            L.Layer(d=np.inf, n=1.0, mua=0.0, mus=0.0, pf=pf),
+            L.Layer(d=d[0], n=1.37, mua=mua[0], mus=1.5e4, pf=pf),
+            L.Layer(d=d[1], n=1.40, mua=mua[1], mus=2e4, pf=pf),
+            L.Layer(d=np.inf, n=1.45, mua=0.0, mus=0.0, pf=pf)])
+        det = D.Detectors(top=D.Total(), bottom=D.Total(), specular=D.Total())
+        A = mc.mcfluence.Axis
+        flu = mc.mcfluence.FluenceRz(A(0.0, 2e-3, 10), A(0.0, 1.5e-3, 15),
+                                     mode=mode)
+        return mc.Mc(layers, mc.mcsource.Line(), det, fluence=flu,
+                     options=[mc.mcoptions.McMethod(METHODS[method])],
+                     rnginit=0x5EED000000000001)
+    out = {}
+    for mode in ('deposition', 'fluence'):
+        cfg, lut, naccum = make(mode).pack()
+        x, a = N.init_rng(512, 0x5EED000000000001)
+        acc, counts, _ = O.mcml_run(cfg, lut, x, a, 20000, naccum,
+                                    O.MATH_KERNEL)
+        out[mode] = (acc[cfg.fluence.offset:].astype(np.float64)
+                     .reshape(15, 10), counts)
+    dep, rate = out['deposition'][0], out['fluence'][0]
+    assert out['deposition'][1][0] == out['fluence'][1][0]   # same steps
+    zc = (np.arange(15) + 0.5) * 1e-4
+    layer_mua = np.where(zc < d[0], mua[0], mua[1])[:, None]
+    ok = dep > 20.0 * K        # bins holding 20 packets' weight or more
+    assert ok.sum() > 50
+    np.testing.assert_allclose((rate * layer_mua)[ok], dep[ok], rtol=2e-3)
+
+
+def test_fluence_mode_values():
+    A = mc.mcfluence.Axis
+    f = mc.mcfluence.Fluence(A(0, 1, 2), A(0, 1, 2), A(0, 1, 2),
+                             mode='fluence')
+    assert f.mode == 'fluence'
+    assert mc.mcfluence.Fluence(f).mode == 'fluence'
+    assert mc.mcfluence.fromdict(f.todict()).mode == 'fluence'
+    with pytest.raises(ValueError):
+        mc.mcfluence.Fluence(A(0, 1, 2), A(0, 1, 2), A(0, 1, 2), mode='rate')
+    with pytest.raises(ValueError):
+        mc.mcfluence.FluenceRz(A(0, 1, 2), A(0, 1, 2), mode='energy')
 
 
 def test_methods_agree_on_a_scattering_slab():
