@@ -114,10 +114,11 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
         cfg, lut, naccum = sim.pack()
         ctx = sim.context
         vox = sim.voxels.material
+        flags = sim._run_flags(False)     # the simulator's arithmetic
 
         def step(first):
             return ctx.run_mcvox(cfg, vox if first else None, lut, npk,
-                                 naccum)[1]
+                                 naccum, 0, flags)[1]
 
         config = {'workload': 'skin_vessel_200',
                   'voxels': '200x200x200 x 10 um', 'packets': npk,
@@ -129,9 +130,10 @@ def make_workload(name: str, packets, launch=None, lut_processes=None):
         npk = packets or workloads.SLAB_PACKETS
         cfg, lut, naccum = sim.pack()
         ctx = sim.context
+        flags = sim._run_flags(False)     # the simulator's arithmetic
 
         def step(first):
-            return ctx.run_mcml(cfg, lut, npk, naccum)[1]
+            return ctx.run_mcml(cfg, lut, npk, naccum, 0, flags)[1]
 
         config = {'workload': 'slab_1mm', 'layers': 3, 'packets': npk,
                   'detectors': 'radial 100 bins + total R/T'}
@@ -340,8 +342,9 @@ def plain_rate(batch, insts, every: int = 16):
     ctx = batch.context
     npk = sum(int(i.nphotons) for i in sub)
     t0 = time.perf_counter()
+    flags = sims[0]._run_flags(False)   # the batch's arithmetic
     for s, (cfg, lut, naccum), i in zip(sims, packed, sub):
-        ctx.run_mcml(cfg, lut, i.nphotons, naccum)
+        ctx.run_mcml(cfg, lut, i.nphotons, naccum, 0, flags)
     dt = time.perf_counter() - t0
     return {'packets_per_s': npk / dt, 'instances': len(sub),
             'packets': npk, 'seconds': dt}

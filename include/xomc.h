@@ -355,7 +355,9 @@ typedef struct xo_batch_instance {
     xo_layer layers[XO_BATCH_MAX_LAYERS];
     xo_source_fiber fiber;
     xo_fiber_axis fiber_axis;   /* XO_SRC_FIBER: the fiber's axis            */
-} xo_batch_instance; /* 968 B */
+    xo_fluence fluence;         /* optional grid; its bins lie inside the
+                                   instance's range, after its detector bins */
+} xo_batch_instance; /* 1024 B */
 
 /* Launch shape of a batch run (chosen by the autotuner). */
 typedef struct xo_batch_opts {

@@ -1194,13 +1194,14 @@ int oracle_batch_run(const xo_batch_instance *inst, uint32_t ninst,
         c->top = inst[i].top;
         c->bottom = inst[i].bottom;
         c->specular = inst[i].specular;
+        c->fluence = inst[i].fluence;
         memcpy(c->layers, inst[i].layers, sizeof(xo_layer) * inst[i].num_layers);
     }
-    uint64_t st = 0, sb = 0;
+    uint64_t st = 0, sb = 0, sd = 0;
     if (nthreads > 0) omp_set_num_threads(nthreads);
     for (uint32_t i = 0; i < ninst; ++i) {
         const int64_t np = (int64_t)inst[i].npackets;
-#pragma omp parallel for schedule(dynamic, 256) reduction(+ : st, sb)
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : st, sb, sd)
         for (int64_t p = 0; p < np; ++p) {
             o_pkt s;
             o_counts c = {0, 0, 0, 0, 0};
@@ -1208,6 +1209,7 @@ int oracle_batch_run(const xo_batch_instance *inst, uint32_t ninst,
             o_mcml_packet(&cfgs[i], lut, &s, accum, &c, math_mode, NULL, 0);
             st += c.steps;
             sb += c.boundary;
+            sd += c.deposits;
         }
     }
     free(cfgs);
@@ -1215,7 +1217,7 @@ int oracle_batch_run(const xo_batch_instance *inst, uint32_t ninst,
         counts_out[0] = st;
         counts_out[1] = sb;
         counts_out[2] = 0;
-        counts_out[3] = 0;
+        counts_out[3] = sd;
     }
     return 0;
 }

@@ -142,7 +142,8 @@ class XoBatchInstance(ctypes.Structure):
                 ('source', XoSourceLine), ('top', XoDetector),
                 ('bottom', XoDetector), ('specular', XoDetector),
                 ('layers', XoLayer * XO_BATCH_MAX_LAYERS),
-                ('fiber', XoSourceFiber), ('fiber_axis', XoFiberAxis)]
+                ('fiber', XoSourceFiber), ('fiber_axis', XoFiberAxis),
+                ('fluence', XoFluence)]
 
 
 class XoBatchOpts(ctypes.Structure):

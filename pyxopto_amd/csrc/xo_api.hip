@@ -32,7 +32,7 @@ static_assert(sizeof(xo_fluence) == 52, "xo_fluence layout");
 static_assert(sizeof(xo_material) == 64, "xo_material layout");
 static_assert(sizeof(xo_voxel_cfg) == 48, "xo_voxel_cfg layout");
 static_assert(sizeof(xo_source_fiber) == 32, "xo_source_fiber layout");
-static_assert(sizeof(xo_batch_instance) == 968, "xo_batch_instance layout");
+static_assert(sizeof(xo_batch_instance) == 1024, "xo_batch_instance layout");
 static_assert(sizeof(xo_trace) == 32, "xo_trace layout");
 static_assert(sizeof(xo_fiber_axis) == 48, "xo_fiber_axis layout");
 static_assert(sizeof(xo_sampling_volume) == 64, "xo_sampling_volume layout");
@@ -288,6 +288,15 @@ int pack_batch(const char *fn, const xo_batch_instance *inst, uint32_t ninst, bo
             set_error("instance %u: the fiber direction must point into the sample (z > 0)",
                       i);
             return -1;
+        }
+        if (I.fluence.enabled != XO_FLUENCE_NONE) {
+            if (check_fluence("instance", I.fluence, end, naccum)) return -1;
+            const xo_fluence &F = I.fluence;
+            const uint64_t nf = (uint64_t)F.shape[0] * F.shape[1] * F.shape[2];
+            if ((uint64_t)F.offset + nf > (uint64_t)I.accum_base + I.accum_count) {
+                set_error("instance %u: fluence bins outside its range", i);
+                return -1;
+            }
         }
         const xo_detector det[3] = {I.top, I.bottom, I.specular};
         hdr[i] = batch_header(I, det, 0u);
@@ -905,6 +914,7 @@ int xo_batch_run(xo_ctx *ctx, const xo_batch_instance *inst, uint32_t ninst,
     P.accum = ctx->accum;
     P.stats = ctx->counters + 1;
     P.nitems = (uint32_t)items.size();
+    P.any_fluence = xo::any_fluence(hdr.data(), hdr.size()) ? 1 : 0;
     uint32_t blocks = 0;
     if (xo::batch_grid(ctx, block, *opts, &blocks)) return -1;
 

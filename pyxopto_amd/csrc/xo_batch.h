@@ -30,6 +30,7 @@ struct BatchHdr {
     xo_detector top, bottom, specular;
     xo_source_fiber fiber;
     xo_fiber_axis fiber_axis;
+    xo_fluence fluence;         /* enabled 0: none */
 };
 
 /* Host: instances of one class run the same branches of the transport
@@ -64,7 +65,16 @@ inline BatchHdr batch_header(const xo_batch_instance &I, const xo_detector det[3
     H.top.offset += accum_shift;
     H.bottom.offset += accum_shift;
     H.specular.offset += accum_shift;
+    H.fluence = I.fluence;
+    if (H.fluence.enabled != XO_FLUENCE_NONE) H.fluence.offset += accum_shift;
     return H;
+}
+
+/* Host: whether any instance of a batch has a fluence grid. */
+inline bool any_fluence(const BatchHdr *hdr, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (hdr[i].fluence.enabled != XO_FLUENCE_NONE) return true;
+    return false;
 }
 
 /* One work-queue item: packets [first, first + count) of one instance. */
@@ -96,6 +106,7 @@ struct BatchParams {
                                     the wave handle them (>= 1)             */
     int32_t method;              /* enum xo_method of every instance        */
     int32_t fast_math;           /* 1: MathFast arithmetic, 0: MathExact    */
+    int32_t any_fluence;         /* 1 if some instance has a fluence grid   */
 };
 
 hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,

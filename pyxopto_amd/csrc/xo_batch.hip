@@ -77,10 +77,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 batch_kernel(const BatchParams P) {
     __shared__ BatchBlockStats sstats;
     /* per-thread event counts in LDS (packets, steps, boundary events,
-     * truncated packets), added with LDS atomics: kept in registers they
-     * would take four of the loop's 64 */
-    __shared__ uint32_t scount[4][256];
-    for (int k = 0; k < 4; ++k) scount[k][threadIdx.x] = 0u;
+     * truncated packets, fluence deposits), added with LDS atomics: kept in
+     * registers they would take five of the loop's 64 */
+    __shared__ uint32_t scount[5][256];
+    for (int k = 0; k < 5; ++k) scount[k][threadIdx.x] = 0u;
     if (threadIdx.x < XO_STAT_COUNT) sstats.v[threadIdx.x] = 0ull;
     __syncthreads();
     const uint32_t nitems = P.nitems_ptr ? *P.nitems_ptr : P.nitems;
@@ -201,13 +201,29 @@ batch_kernel(const BatchParams P) {
         }
         if (!alive) continue;
 
+        /* an instance's fluence deposit (fluence.py:102-147); the grid is
+         * read from the instance header only when the batch has one */
+        auto fluence_at = [&](float qx, float qy, float qz, float deposit, float mua) {
+            const xo_fluence &F = P.hdr[inst].fluence;
+            if (F.enabled != XO_FLUENCE_NONE &&
+                fluence_deposit<M>(F, P.accum, qx, qy, qz, deposit, mua))
+                atomicAdd(&scount[4][threadIdx.x], 1u);
+        };
         /* microscopic Beer-Lambert: absorption along the whole step
-         * (mcml.template.c:584-606; a batch has no fluence grid, so no
-         * deposition point is drawn) */
+         * (mcml.template.c:584-606); with a fluence grid the deposit goes
+         * to a point drawn along the step, as in the layered kernel */
         auto mbl_absorb = [&](const LayerL &L, float step) {
-            const float fraction = 1.0f - M::exp(-(L.mua * step));
+            const float mua = L.mua;
+            const float fraction = 1.0f - M::exp(-(mua * step));
             const float deposit = fraction * w;
             w -= deposit;
+            if (P.any_fluence && P.hdr[inst].fluence.enabled != XO_FLUENCE_NONE) {
+                const float back =
+                    (mua != 0.0f)
+                        ? step - M::div(-M::log_unit(1.0f - rng_next(x, a) * fraction), mua)
+                        : 0.0f;
+                fluence_at(px - back * dx, py - back * dy, pz - back * dz, deposit, mua);
+            }
         };
         /* microscopic Beer-Lambert: roulette after the boundary or the
          * scattering, whether or not the packet has left
@@ -308,12 +324,14 @@ batch_kernel(const BatchParams P) {
             bool done = false;
             if (METHOD == XO_METHOD_AR && rng_next(x, a) < L.mua_inv_mut) {
                 /* albedo rejection: the whole weight is absorbed here */
+                if (P.any_fluence) fluence_at(px, py, pz, w, L.mua);
                 w = 0.0f;
                 done = true;
             } else {
                 if (METHOD == XO_METHOD_AW) {
                     float deposit = w * L.mua_inv_mut;
                     w -= deposit;
+                    if (P.any_fluence) fluence_at(px, py, pz, deposit, L.mua);
                 }
                 float xi_fi;
                 float ct = pf_sample_rt<M>(pf_kind, L, P.lut, x, a, xi_fi);
@@ -327,8 +345,9 @@ batch_kernel(const BatchParams P) {
         }
     }
 
-    const int kinds[4] = {XO_STAT_PACKETS, XO_STAT_STEPS, XO_STAT_BOUNDARY, XO_STAT_TRUNCATED};
-    for (int k = 0; k < 4; ++k)
+    const int kinds[5] = {XO_STAT_PACKETS, XO_STAT_STEPS, XO_STAT_BOUNDARY, XO_STAT_TRUNCATED,
+                          XO_STAT_DEPOSITS};
+    for (int k = 0; k < 5; ++k)
         if (scount[k][threadIdx.x])
             atomicAdd(&sstats.v[kinds[k]], (unsigned long long)scount[k][threadIdx.x]);
     __syncthreads();

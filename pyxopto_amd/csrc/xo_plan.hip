@@ -79,6 +79,7 @@ int plan_build(xo_batch_plan *p, const std::vector<xo::LayerL> &layers,
     P.accum = p->accum;
     P.stats = p->counters + 1;
     P.nitems = (uint32_t)items.size();
+    P.any_fluence = xo::any_fluence(hdr.data(), hdr.size()) ? 1 : 0;
     if (xo::batch_grid(ctx, p->block, opts, &p->blocks)) return -1;
 
     XO_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

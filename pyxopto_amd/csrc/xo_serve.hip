@@ -617,6 +617,11 @@ int xo_serve_admit_mcml(xo_server *S, const xo_batch_instance *inst, const float
     }
     for (uint32_t l = 0; l < I.num_layers; ++l)
         if (xo::check_pf(I.pf_kind, I.layers[l].pf, nlut, "layer", l)) return -1;
+    if (I.fluence.enabled != XO_FLUENCE_NONE) {
+        xo::set_error("xo_serve_admit_mcml: layered requests with a fluence grid are not "
+                      "served (run them with xo_batch_run)");
+        return -1;
+    }
     if (I.pf_kind == XO_PF_LUT && !lut) {
         xo::set_error("xo_serve_admit_mcml: a lookup-table phase function needs the LUT");
         return -1;

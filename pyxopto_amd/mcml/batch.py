@@ -19,11 +19,12 @@ from . import mclayer, mcsource
 
 
 class Instance:
-    """One layered simulation: layers, Line source, detectors, budget."""
+    """One layered simulation: layers, source, detectors, optional fluence
+    accumulator (Fluence or FluenceRz), packet budget."""
 
     def __init__(self, layers, source=None, detectors=None,
                  nphotons: int = 100000, seed: Optional[int] = None,
-                 rmax: float = float('inf')):
+                 rmax: float = float('inf'), fluence=None):
         self.layers = mclayer.Layers(layers)
         if len(self.layers) > N.XO_BATCH_MAX_LAYERS:
             raise ValueError('A batch instance supports at most {} layers '
@@ -34,6 +35,7 @@ class Instance:
         self.nphotons = int(nphotons)
         self.seed = seed
         self.rmax = float(rmax)
+        self.fluence = fluence
 
     def layer(self, index: int):
         return self.layers.layer(index)
@@ -96,6 +98,8 @@ class Batch(McWorker):
             base = self._alloc_offset
             if inst.detectors is not None:
                 inst.detectors.pack(packer, t)
+            if inst.fluence is not None:
+                inst.fluence.pack(packer, t.fluence)
             t.accum_base = base
             t.accum_count = self._alloc_offset - base
             t.npackets = inst.nphotons
@@ -116,14 +120,21 @@ class Batch(McWorker):
         o.flags = N.XO_FLAG_FAST_MATH if self._fast_math else 0
         return o
 
-    def run(self, opts: N.XoBatchOpts = None, packed=None):
-        """Run every instance; returns (list of Detectors, run report)."""
+    def run(self, opts: N.XoBatchOpts = None, packed=None,
+            fluence: bool = False):
+        """Run every instance; returns (list of Detectors, run report), or
+        with fluence=True (list of (fluence, detectors), run report) — the
+        last two items of Mc.run's result per instance, fluence None for an
+        instance without a fluence accumulator."""
         arr, lut, naccum = packed if packed is not None else self.pack()
         if opts is None:
             from .autotune import tuned_opts
             opts = tuned_opts(self, (arr, lut, naccum))
         accum, rep = self._context().run_batch(arr, lut, opts, naccum)
         self._record_report(rep, int(sum(i.nphotons for i in self.instances)))
+        if fluence:
+            return list(zip(self.fluence_results(accum),
+                            self.results(accum))), rep
         return self.results(accum), rep
 
     def plan(self, opts: N.XoBatchOpts = None, packed=None) -> 'BatchRunPlan':
@@ -136,6 +147,20 @@ class Batch(McWorker):
             opts = tuned_opts(self, (arr, lut, naccum))
         return BatchRunPlan(self, self._context().batch_plan(arr, lut, opts,
                                                              naccum))
+
+    def fluence_results(self, accum) -> list:
+        """Each instance's fluence object filled from the accumulators (None
+        where the instance has none)."""
+        out = []
+        for inst, packer in zip(self.instances, self._packers):
+            if inst.fluence is None:
+                out.append(None)
+                continue
+            off, n = packer.allocations[id(inst.fluence)]
+            res = type(inst.fluence)(inst.fluence)
+            res.update_data(self, [accum[off:off + n]], inst.nphotons)
+            out.append(res)
+        return out
 
     def results(self, accum) -> List[mcdetector.Detectors]:
         out = []

@@ -289,12 +289,27 @@ class ReflectanceSurrogate:
         return self.net.train(steps, min(batch, self.net.max_batch), lr, seed)
 
     def fit_schedule(self, rows, R, schedule, batch: int = 4096,
-                     seed: int = 1, log=None):
+                     seed: int = 1, log=None, power_weighted: bool = True):
         """Train on (rows, R) through phases of (steps, learning rate) with
-        Adam state carried across them; returns (losses, kernel ms)."""
+        Adam state carried across them; returns (losses, kernel ms).
+
+        power_weighted: each bin's squared log error counts in proportion
+        to the mean share of the reflected power that bin holds (R A / sum
+        R A over the training rows, floored at 2 % of the largest share),
+        the weight with which a bin's relative error enters profile_error:
+        the targets of bin j are scaled by sqrt(w_j) through y_std, so the
+        network and its mean squared error are unchanged."""
         logR = np.log(np.maximum(R, self.floor))
         self.y_mean = logR.mean(axis=0)
         self.y_std = np.maximum(logR.std(axis=0), 1e-6)
+        if power_weighted:
+            A = ring_areas()
+            share = (R * A) / np.maximum((R * A).sum(axis=1, keepdims=True),
+                                         1e-300)
+            wgt = share.mean(axis=0)
+            wgt = np.maximum(wgt, 0.02 * wgt.max())
+            wgt = wgt / wgt.mean()
+            self.y_std = self.y_std / np.sqrt(wgt)
         self.net.set_data(self._x(rows), self._y(R).astype(np.float32))
         losses, ms = [], 0.0
         for k, (steps, lr) in enumerate(schedule):
@@ -410,7 +425,7 @@ def generate_training_set(n: int, packets: int, box=TISSUE_BOX,
     return rows, R, tot
 
 
-DEFAULT_SCHEDULE = ((20000, 1e-3), (20000, 3e-4), (15000, 1e-4),
+DEFAULT_SCHEDULE = ((30000, 1e-3), (30000, 3e-4), (20000, 1e-4),
                     (10000, 3e-5))
 
 

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from pyxopto_amd import _native as N
+from pyxopto_amd.mcml import mc
 from pyxopto_amd.mcml.batch import Batch
 from oracle import pyoracle as O
 
@@ -122,3 +123,49 @@ def test_batch_plan_replays_equal_batch_run(fast):
     res, rep = b.plan(opts).run()
     assert rep.steps == rep_ref.steps
     ctx.close()        # closes the plans first
+
+
+def _with_fluence(insts):
+    """Give every third instance a Cartesian grid, every third an r-z grid
+    (in fluence-rate mode), leave the rest without one."""
+    A = mc.mcfluence.Axis
+    for k, inst in enumerate(insts):
+        if k % 3 == 0:
+            inst.fluence = mc.mcfluence.Fluence(
+                A(-1e-3, 1e-3, 8), A(-1e-3, 1e-3, 8), A(0.0, 2e-3, 10))
+        elif k % 3 == 1:
+            inst.fluence = mc.mcfluence.FluenceRz(A(0.0, 2e-3, 12),
+                                                  A(0.0, 2e-3, 10),
+                                                  mode='fluence')
+    return insts
+
+
+@pytest.mark.parametrize('method', [0, 1, 2])
+def test_batch_fluence_bit_exact_with_oracle(method):
+    """Per-instance fluence grids in the batch engine (the layered kernel's
+    deposits, fluence.py:102-147 / fluencerz.py:104-160, for each stepping
+    method): bit-exact with the oracle under the production schedule, and
+    each instance's integers are those it gets alone."""
+    print('case batch fluence method', method, flush=True)
+    insts = _with_fluence(small_batch(24, seed=21))
+    b = Batch(insts, options=[mc.mcoptions.McMethod(method)],
+              rnginit=0x5EED000000000001)
+    arr, lut, naccum = b.pack()
+    assert sum(int(arr[i].fluence.enabled != 0) for i in range(len(arr))) == 16
+    acc, rep = b.context.run_batch(arr, lut, b.opts(chunk=1024), naccum)
+    ref, counts = O.batch_run(arr, lut, naccum, method=method,
+                              math_mode=O.MATH_KERNEL)
+    assert int(rep.steps) == int(counts[0])
+    assert int(rep.deposits) == int(counts[3]) > 0
+    np.testing.assert_array_equal(acc, ref)
+    res = b.fluence_results(acc)
+    assert res[2] is None and res[0].raw.sum() > 0 and res[1].raw.sum() > 0
+    # instance 4 alone
+    one = Batch([insts[4]], options=[mc.mcoptions.McMethod(method)],
+                rnginit=0x5EED000000000001)
+    a1, l1, n1 = one.pack()
+    alone, _ = one.context.run_batch(a1, l1, one.opts(chunk=256), n1)
+    base, cnt = int(arr[4].accum_base), int(arr[4].accum_count)
+    np.testing.assert_array_equal(alone[:cnt], acc[base:base + cnt])
+    one.context.close()
+    b.context.close()
