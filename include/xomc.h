@@ -45,6 +45,8 @@ extern "C" {
 #define XO_MAX_LAYERS 16
 #define XO_MAX_MATERIALS 32
 #define XO_BATCH_MAX_LAYERS 8
+/* Bytes of the static device array behind xo_batch_run_noalloc (per GPU). */
+#define XO_NOALLOC_SCRATCH_BYTES (16u << 20)
 
 /* Scattering phase function kinds (one kind per simulation, as
  * xopto/mcml/mclayer/layer.py:862-872 requires). */
@@ -535,6 +537,23 @@ void xo_batch_plan_destroy(xo_batch_plan *plan);
  * (seed, p), x a valid state (0 <= c < a-1, x < 2^32-1) from a second hash.
  * Host copy of the device function, for checks. */
 int xo_packet_seed(uint64_t seed, uint64_t p, uint64_t *x, uint32_t *a);
+
+/* xo_batch_run that allocates no device memory. It has no counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * lut_dev: device array of nlut floats (NULL if nlut is 0); accum_dev:
+ * device array of naccum u64, zeroed and filled by the call (the same
+ * integers xo_batch_run returns). The seed table, the instance tables, the
+ * work items and the counters live in a static device array of
+ * XO_NOALLOC_SCRATCH_BYTES per GPU (about 2 MiB of it is the seed table,
+ * written at the first call on a device); a batch whose tables do not fit
+ * is rejected with an error (about 4096 instances at 8192-packet chunks
+ * fit). Synchronous on ctx's stream. Calls that overlap on one GPU, from any
+ * context, race on that array. */
+int xo_batch_run_noalloc(xo_ctx *ctx, const xo_batch_instance *instances,
+                         uint32_t ninstances, const float *lut_dev, uint32_t nlut,
+                         const xo_batch_opts *opts, uint64_t *accum_dev, uint64_t naccum,
+                         xo_run_report *report);
 
 /* ---------------- serving (device) ---------------- */
 

@@ -28,6 +28,7 @@ XO_MAX_PACKET_STEPS = 1 << 26
 XO_MAX_LAYERS = 16
 XO_MAX_MATERIALS = 32
 XO_BATCH_MAX_LAYERS = 8
+XO_NOALLOC_SCRATCH_BYTES = 16 << 20
 
 XO_PF_HG, XO_PF_MHG, XO_PF_LUT = 0, 1, 2
 XO_DET_NONE, XO_DET_TOTAL, XO_DET_RADIAL = 0, 1, 2
@@ -264,6 +265,10 @@ PROTOTYPES = {
     'xo_batch_plan_run': (c_int, [c_void_p, POINTER(c_uint64),
                                   POINTER(XoRunReport)]),
     'xo_batch_plan_destroy': (None, [c_void_p]),
+    'xo_batch_run_noalloc': (c_int, [c_void_p, POINTER(XoBatchInstance),
+                                     c_uint32, c_void_p, c_uint32,
+                                     POINTER(XoBatchOpts), c_void_p, c_uint64,
+                                     POINTER(XoRunReport)]),
     'xo_mcvox_run': (c_int, [c_void_p, POINTER(XoMcvoxCfg), POINTER(c_int32),
                              c_uint64, POINTER(c_float), c_uint32, c_uint64,
                              c_uint32, c_uint32, POINTER(c_uint64), c_uint64,
@@ -611,6 +616,19 @@ class Context:
         """xo_batch_plan_create: the batch uploaded and its launch captured
         once; BatchPlan.run replays it."""
         return BatchPlan(self, instances, lut, opts, naccum)
+
+    def run_batch_noalloc(self, instances, lut_dev: int, nlut: int,
+                          opts: XoBatchOpts, accum_dev: int, naccum: int):
+        """xo_batch_run_noalloc: lut_dev and accum_dev are device addresses
+        (integers) the caller allocated; the accumulator is left on the
+        device. Returns the run report."""
+        rep = XoRunReport()
+        check(lib().xo_batch_run_noalloc(
+            self._h, instances, c_uint32(len(instances)),
+            c_void_p(lut_dev or None), c_uint32(nlut), ctypes.byref(opts),
+            c_void_p(accum_dev), c_uint64(naccum), ctypes.byref(rep)),
+            'xo_batch_run_noalloc')
+        return rep
 
     def probe_copy(self, nbytes: int, reps: int = 10) -> float:
         """Average ms of one device-to-device copy of nbytes."""

@@ -30,7 +30,8 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ['xo_kernels.hip', 'xo_batch.hip', 'xo_api.hip', 'xo_serve.hip',
-               'xo_probe.hip', 'xo_mlp.hip', 'xo_sv.hip', 'xo_plan.hip']
+               'xo_probe.hip', 'xo_mlp.hip', 'xo_sv.hip', 'xo_noalloc.hip',
+               'xo_plan.hip']
 CXX_SOURCES = ['xo_seeds.cpp']
 
 
