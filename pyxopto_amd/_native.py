@@ -468,7 +468,7 @@ class BatchPlan:
         if not h:
             raise NativeError('xo_batch_plan_create: {}'.format(last_error()))
         self._h = h
-        ctx._plans.add(self)
+        ctx._dependents.add(self)
 
     def run(self):
         accum = np.zeros(self.naccum, dtype=np.uint64)
@@ -503,7 +503,9 @@ class Context:
                 raise ValueError(msg)
             raise NativeError('xo_ctx_create: ' + msg)
         self._h = h
-        self._plans = weakref.WeakSet()
+        # objects holding this context's device, stream or buffers (batch
+        # plans, networks): closed before the context is
+        self._dependents = weakref.WeakSet()
         self.device = device
         p, m, c = c_uint32(), c_uint32(), c_uint32()
         check(l.xo_ctx_info(h, ctypes.byref(p), ctypes.byref(m),
@@ -514,9 +516,8 @@ class Context:
 
     def close(self):
         if getattr(self, '_h', None):
-            # plans hold the context's device and stream: release them first
-            for plan in list(getattr(self, '_plans', ())):
-                plan.close()
+            for dep in list(getattr(self, '_dependents', ())):
+                dep.close()
             lib().xo_ctx_destroy(self._h)
             self._h = None
 

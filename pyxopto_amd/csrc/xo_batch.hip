@@ -71,16 +71,19 @@ __device__ __forceinline__ void end_event(bool done, bool &alive, float px, floa
 /* At most 64 VGPRs (8 waves per SIMD): the compiler fits the loop without
  * spills, and the extra wave hides more of the layer and LUT reads (+2.5 %
  * packets/s over 72 VGPRs and 7 waves). METHOD is the stepping method of
- * every instance (xo_method), M the arithmetic (MathExact, MathFast). */
-template <int METHOD, class M>
+ * every instance (xo_method), M the arithmetic (MathExact, MathFast), FLU
+ * whether any instance has a fluence grid. */
+template <int METHOD, class M, bool FLU>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 batch_kernel(const BatchParams P) {
     __shared__ BatchBlockStats sstats;
     /* per-thread event counts in LDS (packets, steps, boundary events,
-     * truncated packets, fluence deposits), added with LDS atomics: kept in
-     * registers they would take five of the loop's 64 */
-    __shared__ uint32_t scount[5][256];
-    for (int k = 0; k < 5; ++k) scount[k][threadIdx.x] = 0u;
+     * truncated packets and, with fluence grids, deposits), added with LDS
+     * atomics: kept in registers they would take four or five of the
+     * loop's 64 */
+    constexpr int kCounts = FLU ? 5 : 4;
+    __shared__ uint32_t scount[kCounts][256];
+    for (int k = 0; k < kCounts; ++k) scount[k][threadIdx.x] = 0u;
     if (threadIdx.x < XO_STAT_COUNT) sstats.v[threadIdx.x] = 0ull;
     __syncthreads();
     const uint32_t nitems = P.nitems_ptr ? *P.nitems_ptr : P.nitems;
@@ -207,7 +210,7 @@ batch_kernel(const BatchParams P) {
             const xo_fluence &F = P.hdr[inst].fluence;
             if (F.enabled != XO_FLUENCE_NONE &&
                 fluence_deposit<M>(F, P.accum, qx, qy, qz, deposit, mua))
-                atomicAdd(&scount[4][threadIdx.x], 1u);
+                atomicAdd(&scount[kCounts - 1][threadIdx.x], 1u);
         };
         /* microscopic Beer-Lambert: absorption along the whole step
          * (mcml.template.c:584-606); with a fluence grid the deposit goes
@@ -217,7 +220,7 @@ batch_kernel(const BatchParams P) {
             const float fraction = 1.0f - M::exp(-(mua * step));
             const float deposit = fraction * w;
             w -= deposit;
-            if (P.any_fluence && P.hdr[inst].fluence.enabled != XO_FLUENCE_NONE) {
+            if (FLU && P.hdr[inst].fluence.enabled != XO_FLUENCE_NONE) {
                 const float back =
                     (mua != 0.0f)
                         ? step - M::div(-M::log_unit(1.0f - rng_next(x, a) * fraction), mua)
@@ -324,14 +327,14 @@ batch_kernel(const BatchParams P) {
             bool done = false;
             if (METHOD == XO_METHOD_AR && rng_next(x, a) < L.mua_inv_mut) {
                 /* albedo rejection: the whole weight is absorbed here */
-                if (P.any_fluence) fluence_at(px, py, pz, w, L.mua);
+                if (FLU) fluence_at(px, py, pz, w, L.mua);
                 w = 0.0f;
                 done = true;
             } else {
                 if (METHOD == XO_METHOD_AW) {
                     float deposit = w * L.mua_inv_mut;
                     w -= deposit;
-                    if (P.any_fluence) fluence_at(px, py, pz, deposit, L.mua);
+                    if (FLU) fluence_at(px, py, pz, deposit, L.mua);
                 }
                 float xi_fi;
                 float ct = pf_sample_rt<M>(pf_kind, L, P.lut, x, a, xi_fi);
@@ -347,7 +350,7 @@ batch_kernel(const BatchParams P) {
 
     const int kinds[5] = {XO_STAT_PACKETS, XO_STAT_STEPS, XO_STAT_BOUNDARY, XO_STAT_TRUNCATED,
                           XO_STAT_DEPOSITS};
-    for (int k = 0; k < 5; ++k)
+    for (int k = 0; k < kCounts; ++k)
         if (scount[k][threadIdx.x])
             atomicAdd(&sstats.v[kinds[k]], (unsigned long long)scount[k][threadIdx.x]);
     __syncthreads();
@@ -355,20 +358,20 @@ batch_kernel(const BatchParams P) {
         atomicAdd(&P.stats[threadIdx.x], sstats.v[threadIdx.x]);
 }
 
-template <class M>
+template <class M, bool FLU>
 static hipError_t launch_batch_m(const BatchParams &P, uint32_t blocks, uint32_t block,
                                  hipStream_t stream) {
     switch (P.method) {
     case XO_METHOD_AW:
-        hipLaunchKernelGGL((batch_kernel<XO_METHOD_AW, M>), dim3(blocks), dim3(block), 0,
+        hipLaunchKernelGGL((batch_kernel<XO_METHOD_AW, M, FLU>), dim3(blocks), dim3(block), 0,
                            stream, P);
         break;
     case XO_METHOD_AR:
-        hipLaunchKernelGGL((batch_kernel<XO_METHOD_AR, M>), dim3(blocks), dim3(block), 0,
+        hipLaunchKernelGGL((batch_kernel<XO_METHOD_AR, M, FLU>), dim3(blocks), dim3(block), 0,
                            stream, P);
         break;
     case XO_METHOD_MBL:
-        hipLaunchKernelGGL((batch_kernel<XO_METHOD_MBL, M>), dim3(blocks), dim3(block), 0,
+        hipLaunchKernelGGL((batch_kernel<XO_METHOD_MBL, M, FLU>), dim3(blocks), dim3(block), 0,
                            stream, P);
         break;
     default:
@@ -377,10 +380,15 @@ static hipError_t launch_batch_m(const BatchParams &P, uint32_t blocks, uint32_t
     return hipGetLastError();
 }
 
+/* The kernel is specialized on whether any instance has a fluence grid, so
+ * batches without one run the transport loop with no trace of it. */
 hipError_t launch_batch(const BatchParams &P, uint32_t blocks, uint32_t block,
                         hipStream_t stream) {
-    return P.fast_math ? launch_batch_m<MathFast>(P, blocks, block, stream)
-                       : launch_batch_m<MathExact>(P, blocks, block, stream);
+    if (P.any_fluence)
+        return P.fast_math ? launch_batch_m<MathFast, true>(P, blocks, block, stream)
+                           : launch_batch_m<MathExact, true>(P, blocks, block, stream);
+    return P.fast_math ? launch_batch_m<MathFast, false>(P, blocks, block, stream)
+                       : launch_batch_m<MathExact, false>(P, blocks, block, stream);
 }
 
 __global__ void zero_u32_kernel(uint32_t *ptr, uint32_t n) {
@@ -397,7 +405,7 @@ hipError_t launch_zero_u32(uint32_t *ptr, uint32_t n, hipStream_t stream) {
 hipError_t occupancy_batch(uint32_t block, int *blocks_per_cu) {
     /* the albedo-weight build stands for all three methods */
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, batch_kernel<XO_METHOD_AW, MathExact>, (int)block, 0);
+        blocks_per_cu, batch_kernel<XO_METHOD_AW, MathExact, false>, (int)block, 0);
 }
 
 } // namespace xo

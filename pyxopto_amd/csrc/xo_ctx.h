@@ -68,7 +68,10 @@ int ensure_table(xo_ctx *ctx);
 int check_detector(const char *name, const xo_detector &d, uint64_t naccum, uint64_t *end);
 int check_pf(int pf_kind, const xo_pf &pf, uint32_t nlut, const char *what, uint32_t idx);
 int check_fluence(const char *fn, const xo_fluence &f, uint64_t det_end, uint64_t naccum);
-uint32_t choose_lanes(xo_ctx *ctx, uint32_t requested, int blocks_per_cu);
+/* Lanes of a single-simulation launch: `requested` if not 0 (capped by the
+ * context's streams), else the persistent grid, fewer for a run of fewer
+ * packets than that. */
+uint32_t choose_lanes(xo_ctx *ctx, uint32_t requested, int blocks_per_cu, uint64_t npackets);
 double ms_since(std::chrono::steady_clock::time_point t0);
 void fill_report(xo_run_report *r, const unsigned long long *stats, uint32_t lanes,
                  float kernel_ms, double total_ms);

@@ -257,6 +257,38 @@ XO_DEV void trace_event(const xo_trace &T, float *data, TraceState &ts, float px
     ts.count++;
 }
 
+/* A lane's one-entry cache of its last global fluence deposit (the
+ * accumulator cache of mcbase.template.h:2780-2872, McUseFluenceCache):
+ * consecutive deposits to the same bin add up in a register and reach
+ * memory as one atomic. The cached integer is flushed before it could pass
+ * 32 bits, so every bin receives the same integers in the same modular u64
+ * sum as without the cache. */
+struct FluenceCache {
+    uint32_t index = 0xFFFFFFFFu;   /* accumulator index held, or none    */
+    uint32_t w = 0u;                /* integer weight not yet added       */
+};
+
+XO_DEV void cache_add(FluenceCache &c, unsigned long long *accum, uint64_t index,
+                      uint32_t ui) {
+    if (index >= 0xFFFFFFFFull) {
+        atomicAdd(&accum[index], (unsigned long long)ui);
+        return;
+    }
+    if ((uint32_t)index == c.index && c.w <= 0xFFFFFFFFu - ui) {
+        c.w += ui;
+        return;
+    }
+    if (c.w) atomicAdd(&accum[c.index], (unsigned long long)c.w);
+    c.index = (uint32_t)index;
+    c.w = ui;
+}
+
+XO_DEV void cache_flush(FluenceCache &c, unsigned long long *accum) {
+    if (c.w) atomicAdd(&accum[c.index], (unsigned long long)c.w);
+    c.w = 0u;
+    c.index = 0xFFFFFFFFu;
+}
+
 /* The weight a fluence accumulator adds for an absorbed weight w: w in
  * deposition mode, w / mua in fluence-rate mode (0 where mua is 0),
  * fluence.py:139-141. */
@@ -271,7 +303,8 @@ template <class M = MathExact> XO_DEV float fluence_weight(const xo_fluence &F, 
  * in fluence-rate mode). Returns true if a non-zero integer was added. */
 template <class M = MathExact>
 XO_DEV bool fluence_deposit(const xo_fluence &F, unsigned long long *accum, float px,
-                            float py, float pz, float w, float mua) {
+                            float py, float pz, float w, float mua,
+                            FluenceCache *cache = nullptr) {
     uint64_t index;
     if (F.enabled == XO_FLUENCE_XYZ) {
         const float fx = (px - F.top_left[0]) * F.inv_step[0];
@@ -297,7 +330,10 @@ XO_DEV bool fluence_deposit(const xo_fluence &F, unsigned long long *accum, floa
     }
     const uint32_t ui = f2u_sat(fluence_weight<M>(F, w, mua) * (float)F.k + 0.5f);
     if (ui == 0u) return false;
-    atomicAdd(&accum[F.offset + index], (unsigned long long)ui);
+    if (cache)
+        cache_add(*cache, accum, F.offset + index, ui);
+    else
+        atomicAdd(&accum[F.offset + index], (unsigned long long)ui);
     return true;
 }
 
@@ -333,12 +369,12 @@ struct VoxEventCfgL {
 /* fluence_deposit for a Cartesian grid whose bins in the box
  * [box, box + (bx, by, bz)) are held in LDS (sbox, x fastest): the same
  * index and integer as fluence_deposit, added to the box when the bin lies
- * in it, to accum otherwise. */
+ * in it, to accum through the lane's cache otherwise. */
 template <class M = MathExact>
 XO_DEV bool fluence_deposit_boxed(const xo_fluence &F, unsigned long long *accum,
                                   unsigned long long *sbox, const int32_t box[3], int32_t bx,
                                   int32_t by, int32_t bz, float px, float py, float pz,
-                                  float w, float mua) {
+                                  float w, float mua, FluenceCache &cache) {
     const float fx = (px - F.top_left[0]) * F.inv_step[0];
     const float fy = (py - F.top_left[1]) * F.inv_step[1];
     const float fz = (pz - F.top_left[2]) * F.inv_step[2];
@@ -353,8 +389,8 @@ XO_DEV bool fluence_deposit_boxed(const xo_fluence &F, unsigned long long *accum
     if (rx < (uint32_t)bx && ry < (uint32_t)by && rz < (uint32_t)bz)
         atomicAdd(&sbox[(rz * (uint32_t)by + ry) * (uint32_t)bx + rx], (unsigned long long)ui);
     else
-        atomicAdd(&accum[F.offset + ((uint64_t)iz * F.shape[1] + iy) * F.shape[0] + ix],
-                  (unsigned long long)ui);
+        cache_add(cache, accum, F.offset + ((uint64_t)iz * F.shape[1] + iy) * F.shape[0] + ix,
+                  ui);
     return true;
 }
 
@@ -706,6 +742,14 @@ XO_DEV bool dispense(WaveDispenser &q, bool want, bool alive, unsigned long long
 
 constexpr uint32_t kClaim = 128;    /* packets per counter atomic */
 constexpr uint32_t kRefillMin = 4;  /* idle lanes that trigger a refill */
+
+/* Packets a wave claims per counter atomic: kClaim when the run has at
+ * least two packets per lane, one per lane otherwise, so that a run with
+ * fewer packets than twice the lanes spreads them over every wave instead
+ * of stacking two on each lane of the first waves. */
+template <class Params> XO_DEV uint32_t claim_size(const Params &P) {
+    return P.npackets >= 2ull * P.lanes ? kClaim : 64u;
+}
 
 } // namespace xo
 

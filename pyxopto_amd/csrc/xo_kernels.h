@@ -85,8 +85,8 @@ size_t mcml_lds_bytes(const McmlParams &P);
 size_t mcvox_lds_bytes(const McvoxParams &P);
 hipError_t launch_mcml(const McmlParams &P, uint32_t blocks, hipStream_t stream);
 hipError_t launch_mcvox(const McvoxParams &P, uint32_t blocks, hipStream_t stream);
-hipError_t occupancy_mcml(int pf_kind, size_t lds, int *blocks_per_cu);
-hipError_t occupancy_mcvox(int pf_kind, size_t lds, int *blocks_per_cu);
+hipError_t occupancy_mcml(int pf_kind, size_t lds, bool fast, int *blocks_per_cu);
+hipError_t occupancy_mcvox(int pf_kind, size_t lds, bool fast, int *blocks_per_cu);
 
 } // namespace xo
 

@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
         } else {
             bool retire = false;
             uint64_t claimed = 0;
-            launch = dispense(wq, want, alive, P.counter, P.npackets, kClaim, kRefillMin,
+            launch = dispense(wq, want, alive, P.counter, P.npackets, claim_size(P), kRefillMin,
                               pkt_index, retire, claimed);
             if (retire) active = false;
             /* every Line packet deposits the same specular integer at the
@@ -511,11 +511,14 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
     }
     const bool use_fluence = P.cfg.fluence.enabled != 0;
     /* a fluence deposit: through the LDS box when there is one */
+    FluenceCache fcache;
     auto deposit_fluence = [&](float qx, float qy, float qz, float wgt, float mua) {
         if (P.fl_box_on)
             return fluence_deposit_boxed<M>(lds_fresh(sev.fluence), P.accum, sfl, P.fl_box,
-                                            kFlBoxX, kFlBoxY, kFlBoxZ, qx, qy, qz, wgt, mua);
-        return fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, qx, qy, qz, wgt, mua);
+                                            kFlBoxX, kFlBoxY, kFlBoxZ, qx, qy, qz, wgt, mua,
+                                            fcache);
+        return fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, qx, qy, qz, wgt, mua,
+                                  &fcache);
     };
 
     const uint32_t gid = blockIdx.x * XO_BLOCK + tid;
@@ -572,7 +575,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         } else {
             bool retire = false;
             uint64_t claimed = 0;
-            launch = dispense(wq, want, alive, P.counter, P.npackets, kClaim, kRefillMin,
+            launch = dispense(wq, want, alive, P.counter, P.npackets, claim_size(P), kRefillMin,
                               pkt_index, retire, claimed);
             if (retire) active = false;
             if (claimed && line_src && lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
@@ -878,6 +881,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         }
     }
 
+    cache_flush(fcache, P.accum);
     if (lane_ok) {
         if (!keyed) P.rng_x[gid] = x;
         if (st[XO_STAT_PACKETS]) st[XO_STAT_THREADS] = 1;
@@ -1005,35 +1009,46 @@ hipError_t launch_mcvox(const McvoxParams &P, uint32_t blocks, hipStream_t strea
     }
 }
 
-hipError_t occupancy_mcml(int pf_kind, size_t lds, int *blocks_per_cu) {
-    /* the albedo-weight build stands for all three methods (same
-     * registers within a few) */
+/* Blocks per CU of the albedo-weight build of the arithmetic that will
+ * run (it stands for all three methods: same registers within a few). */
+template <class M>
+static hipError_t occupancy_mcml_m(int pf_kind, size_t lds, int *blocks_per_cu) {
     switch (pf_kind) {
     case XO_PF_MHG:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcml_kernel<XO_PF_MHG, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
+            blocks_per_cu, mcml_kernel<XO_PF_MHG, XO_METHOD_AW, false, M>, XO_BLOCK, lds);
     case XO_PF_LUT:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcml_kernel<XO_PF_LUT, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
+            blocks_per_cu, mcml_kernel<XO_PF_LUT, XO_METHOD_AW, false, M>, XO_BLOCK, lds);
     default:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcml_kernel<XO_PF_HG, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
+            blocks_per_cu, mcml_kernel<XO_PF_HG, XO_METHOD_AW, false, M>, XO_BLOCK, lds);
     }
 }
 
-hipError_t occupancy_mcvox(int pf_kind, size_t lds, int *blocks_per_cu) {
-    /* the albedo-weight build stands for all three methods */
+template <class M>
+static hipError_t occupancy_mcvox_m(int pf_kind, size_t lds, int *blocks_per_cu) {
     switch (pf_kind) {
     case XO_PF_MHG:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcvox_kernel<XO_PF_MHG, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
+            blocks_per_cu, mcvox_kernel<XO_PF_MHG, XO_METHOD_AW, false, M>, XO_BLOCK, lds);
     case XO_PF_LUT:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcvox_kernel<XO_PF_LUT, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
+            blocks_per_cu, mcvox_kernel<XO_PF_LUT, XO_METHOD_AW, false, M>, XO_BLOCK, lds);
     default:
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, mcvox_kernel<XO_PF_HG, XO_METHOD_AW, false, MathExact>, XO_BLOCK, lds);
+            blocks_per_cu, mcvox_kernel<XO_PF_HG, XO_METHOD_AW, false, M>, XO_BLOCK, lds);
     }
+}
+
+hipError_t occupancy_mcml(int pf_kind, size_t lds, bool fast, int *blocks_per_cu) {
+    return fast ? occupancy_mcml_m<MathFast>(pf_kind, lds, blocks_per_cu)
+                : occupancy_mcml_m<MathExact>(pf_kind, lds, blocks_per_cu);
+}
+
+hipError_t occupancy_mcvox(int pf_kind, size_t lds, bool fast, int *blocks_per_cu) {
+    return fast ? occupancy_mcvox_m<MathFast>(pf_kind, lds, blocks_per_cu)
+                : occupancy_mcvox_m<MathExact>(pf_kind, lds, blocks_per_cu);
 }
 
 } // namespace xo

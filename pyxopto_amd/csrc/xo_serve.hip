@@ -315,7 +315,8 @@ int launch_vox(xo_server *S, Request &R, uint64_t first, uint64_t count,
     P.lds_bins = R.lds_bins;
     P.lds_lut = (R.lut_n && R.lut_n <= XO_LDS_LUT_MAX) ? (uint32_t)R.lut_n : 0u;
     int bpc = 1;
-    XO_CHECK(xo::occupancy_mcvox(R.vcfg.pf_kind, xo::mcvox_lds_bytes(P), &bpc));
+    XO_CHECK(xo::occupancy_mcvox(R.vcfg.pf_kind, xo::mcvox_lds_bytes(P),
+                                 S->opts.fast_math != 0, &bpc));
     uint64_t resident = (uint64_t)std::max(bpc, 1) * S->ctx->cus * XO_BLOCK;
     uint64_t lanes = std::min<uint64_t>(std::min<uint64_t>(count, resident), S->opts.vox_lanes);
     lanes = std::max<uint64_t>(lanes, 64u);
