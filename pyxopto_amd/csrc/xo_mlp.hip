@@ -4,21 +4,23 @@
  * Adam on fp32 master weights. No counterpart in the reference (its
  * regression utilities, xopto/util/regression/ann.py, call TensorFlow).
  *
- * Storage: weights W_l row-major [in_l][out_l] (so a layer is H W), packed
- * with their biases as W_1, b_1, W_2, b_2, ... in fp32; the products read
- * bf16 copies. Activations between layers are bf16, rounded to nearest
+ * Storage: weights W_l row-major [in][out], packed with their biases as
+ * W_1, b_1, W_2, b_2, ... in fp32; the products read bf16 copies of W_l
+ * ([in][out]) and W_l^T ([out][in]) whose rows are padded to multiples of
+ * 8 elements. Activations between layers are bf16, rounded to nearest
  * even; the network output and the loss are fp32.
  *
- * Products: C[M][N] = op(A)[M][K] op(B)[K][N] with op = identity or
- * transpose, one 64x64 tile of C per 256-thread block (four waves, each a
- * 2x2 grid of 16x16 v_mfma_f32_16x16x32_bf16 tiles), K in steps of 32
- * staged through LDS, out-of-range elements read as zero.
+ * Training step (captured once as a hipGraph): minibatch rows, one product
+ * per layer forward, the loss, one product per layer for the data
+ * gradients, one launch for every layer's weight and bias gradients, Adam.
+ * Inference: one fused kernel with the activations resident in LDS.
  */
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "../../include/xomc.h"
@@ -55,31 +57,22 @@ __host__ __device__ inline float bf2f(uint16_t h) {
     return f;
 }
 
-enum Epilogue : int {
-    EPI_STORE = 0,      /* C fp32                                  */
-    EPI_BIAS_RELU = 1,  /* Z = C + b; H = bf16(max(Z, 0))          */
-    EPI_BIAS = 2,       /* Y = C + b, fp32                         */
-    EPI_MASK = 3,       /* D = bf16(C * (Hprev > 0))               */
-};
-
+/* The bf16 product of xo_gemm_bf16: C[M][N] = op(A) op(B), fp32, with op
+ * identity or transpose, one 64x64 tile of C per 256-thread block (four
+ * waves, each a 2x2 grid of 16x16 v_mfma_f32_16x16x32_bf16 tiles), K in
+ * steps of 32 staged through LDS, out-of-range elements read as zero. */
 struct GemmArgs {
     const uint16_t *A;
     const uint16_t *B;
     int lda, ldb;
     int M, N, K;
-    float *C;           /* EPI_STORE, EPI_BIAS: fp32 out           */
+    float *C;
     int ldc;
-    const float *bias;  /* EPI_BIAS_RELU, EPI_BIAS                 */
-    uint16_t *H;        /* EPI_BIAS_RELU, EPI_MASK: bf16 out       */
-    int ldh;
-    const uint16_t *mask; /* EPI_MASK: activations whose sign gates */
-    int ldm;
 };
 
 constexpr int TM = 64, TN = 64, TK = 32, KPAD = 8;
-constexpr uint32_t kMaxSplits = 32;
 
-template <bool TA, bool TB, int EPI>
+template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
     __shared__ __attribute__((aligned(16))) uint16_t sA[TM][TK + KPAD];
     __shared__ __attribute__((aligned(16))) uint16_t sB[TN][TK + KPAD];
@@ -91,13 +84,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
     floatx4 acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-    /* split K: block z takes K range [z kc, (z + 1) kc) and, for
-     * EPI_STORE, writes its partial product to C + z M ldc */
-    const int kc = (g.K + (int)gridDim.z - 1) / (int)gridDim.z;
-    const int kbeg = (int)blockIdx.z * kc;
-    const int kend = kbeg + kc < g.K ? kbeg + kc : g.K;
-    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+    for (int k0 = 0; k0 < g.K; k0 += TK) {
         /* stage op(A)[m0:+64][k0:+32] as sA[m][k], op(B)[k0:+32][n0:+64]
          * as sB[n][k]; each thread moves 8 elements of each */
         for (int e = tid; e < TM * TK; e += 256) {
@@ -105,7 +92,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
             if (TA) { c = e / TM; r = e % TM; } else { r = e / TK; c = e % TK; }
             const int m = m0 + r, k = k0 + c;
             uint16_t v = 0;
-            if (m < g.M && k < kend) v = TA ? g.A[(size_t)k * g.lda + m] : g.A[(size_t)m * g.lda + k];
+            if (m < g.M && k < g.K) v = TA ? g.A[(size_t)k * g.lda + m] : g.A[(size_t)m * g.lda + k];
             sA[r][c] = v;
         }
         for (int e = tid; e < TN * TK; e += 256) {
@@ -113,7 +100,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
             if (TB) { r = e / TK; c = e % TK; } else { c = e / TN; r = e % TN; }
             const int n = n0 + r, k = k0 + c;
             uint16_t v = 0;
-            if (n < g.N && k < kend) v = TB ? g.B[(size_t)n * g.ldb + k] : g.B[(size_t)k * g.ldb + n];
+            if (n < g.N && k < g.K) v = TB ? g.B[(size_t)n * g.ldb + k] : g.B[(size_t)k * g.ldb + n];
             sB[r][c] = v;
         }
         __syncthreads();
@@ -136,37 +123,14 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
             for (int r = 0; r < 4; ++r) {
                 const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
                 const int n = n0 + wn + j * 16 + (lane & 15);
-                if (m >= g.M || n >= g.N) continue;
-                float v = acc[i][j][r];
-                if (EPI == EPI_STORE) {
-                    g.C[((size_t)blockIdx.z * g.M + m) * g.ldc + n] = v;
-                } else if (EPI == EPI_BIAS) {
-                    g.C[(size_t)m * g.ldc + n] = v + g.bias[n];
-                } else if (EPI == EPI_BIAS_RELU) {
-                    float z = v + g.bias[n];
-                    g.H[(size_t)m * g.ldh + n] = f2bf(z > 0.f ? z : 0.f);
-                } else {
-                    const bool on = bf2f(g.mask[(size_t)m * g.ldm + n]) > 0.f;
-                    g.H[(size_t)m * g.ldh + n] = f2bf(on ? v : 0.f);
-                }
+                if (m < g.M && n < g.N) g.C[(size_t)m * g.ldc + n] = acc[i][j][r];
             }
 }
 
-template <bool TA, bool TB, int EPI>
-hipError_t gemm(const GemmArgs &g, hipStream_t s, int splits = 1) {
-    dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, splits);
-    hipLaunchKernelGGL((gemm_kernel<TA, TB, EPI>), grid, dim3(256), 0, s, g);
+template <bool TA, bool TB> hipError_t gemm(const GemmArgs &g, hipStream_t s) {
+    dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, 1);
+    hipLaunchKernelGGL((gemm_kernel<TA, TB>), grid, dim3(256), 0, s, g);
     return hipGetLastError();
-}
-
-/* C[i] = sum over z of P[z][i], z in order (deterministic). */
-__global__ void sum_splits_kernel(const float *P, uint32_t splits, uint64_t n, float *C) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (uint32_t z = 0; z < splits; ++z) s += P[(size_t)z * n + i];
-        C[i] = s;
-    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -330,37 +294,241 @@ __global__ void fragment_pad_kernel(const float *w, int in, int out, int ksteps,
     dst[i] = (k < in && n < out) ? f2bf(w[(size_t)k * out + n]) : (uint16_t)0;
 }
 
-__global__ void to_bf16_kernel(const float *x, uint16_t *y, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        y[i] = f2bf(x[i]);
+
+__global__ void fill_f32_kernel(float *p, uint32_t n, float v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
-/* Minibatch of rows drawn with replacement: row i of step t is
- * hash(seed, t, i) mod n. */
-__global__ void gather_kernel(const float *X, const float *T, uint32_t n, uint32_t nin,
-                              uint32_t nout, uint32_t B, uint64_t seed, const uint32_t *step,
-                              uint16_t *Xb, float *Tb) {
+__global__ void step_kernel(uint32_t *step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1u;
+}
+
+/* ------------------------------------------------------------------ */
+/* Training step: products with both operands stored k-contiguous       */
+/* ------------------------------------------------------------------ */
+/* C[M][N] = A[M][K] B[K][N] with B given as its transpose BT[N][K], so both
+ * operands are read 8 elements (16 bytes) per thread per load. Every product
+ * of a training step has this form:
+ *   forward  H_l = relu(H_{l-1} W_l + b_l)    A = H_{l-1},    BT = W_l^T
+ *   data     dZ_{l-1} = (dZ_l W_l^T) . [H_{l-1} > 0]
+ *                                             A = dZ_l,       BT = W_l
+ *   weights  [dW_l; db_l] = [H_{l-1}^T; 1] dZ_l
+ *                                             A = H_{l-1}^T,  BT = dZ_l^T
+ * The weight gradient reads a row of ones below H_{l-1}^T: its product row
+ * is the bias gradient, which the packed parameters hold right after W_l.
+ * Forward and data products store their bf16 result row-major and
+ * transposed, the two layouts the later products read. A 64x64 tile per
+ * 256-thread block, K in steps of 32 through LDS, fragments and MFMA order
+ * as gemm_kernel (the forward products equal it and the fused inference
+ * kernel bit for bit). The weight gradients of all layers run in one launch,
+ * each split over the batch into fp32 partial sums that the Adam kernel
+ * adds in a fixed order. Leading dimensions are multiples of 8. */
+enum NtEpi : int { NT_RELU = 0, NT_OUT = 1, NT_MASK = 2, NT_PARTIAL = 3 };
+
+struct NtProblem {
+    const uint16_t *A;      /* [M][lda]                                   */
+    const uint16_t *BT;     /* [N][ldb]                                   */
+    const float *bias;      /* NT_RELU, NT_OUT                            */
+    uint16_t *H;            /* NT_RELU, NT_MASK: [M][ldh]                 */
+    uint16_t *HT;           /* NT_RELU, NT_MASK: [N][ldht]                */
+    float *C;               /* NT_OUT: [M][ldc]; NT_PARTIAL: split 0 base */
+    const uint16_t *mask;   /* NT_MASK: [M][ldm]                          */
+    uint64_t part_stride;   /* NT_PARTIAL: floats between splits          */
+    int lda, ldb, M, N, K;
+    int ones_row;           /* row of A read as 1.0, or -1                */
+    int tiles_n, splits, block0;
+    int ldh, ldht, ldc, ldm;
+    int reserved;
+};
+
+/* Elements [k, k + 8) of a k-contiguous row, zero at and past kend. */
+__device__ inline uint4 nt_load(const uint16_t *row, int k, int kend) {
+    if (k + 8 <= kend) return *reinterpret_cast<const uint4 *>(row + k);
+    uint16_t v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (k + e < kend) ? row[k + e] : (uint16_t)0;
+    uint4 r;
+    std::memcpy(&r, v, 16);
+    return r;
+}
+
+/* The row of ones (bf16 1.0 = 0x3F80) over [k, k + 8), zero past kend. */
+__device__ inline uint4 nt_ones(int k, int kend) {
+    uint16_t v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (k + e < kend) ? (uint16_t)0x3F80 : (uint16_t)0;
+    uint4 r;
+    std::memcpy(&r, v, 16);
+    return r;
+}
+
+template <int EPI>
+__device__ __forceinline__ void nt_tile(const NtProblem &g, int tile,
+                                        uint16_t (*sA)[TK + KPAD], uint16_t (*sB)[TK + KPAD]) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int split = tile % g.splits, t = tile / g.splits;
+    const int m0 = (t / g.tiles_n) * TM, n0 = (t % g.tiles_n) * TN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int ksteps = (g.K + TK - 1) / TK;
+    const int kc = (ksteps + g.splits - 1) / g.splits * TK;
+    const int kbeg = split * kc;
+    const int kend = kbeg + kc < g.K ? kbeg + kc : g.K;
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    /* loader: 64 rows x 32 k per operand, 8 k per thread */
+    const int lr = tid >> 2, lc = (tid & 3) * 8;
+    const int am = m0 + lr, bn = n0 + lr;
+    const bool ones = am == g.ones_row;
+    const uint16_t *arow = g.A + (size_t)am * g.lda;
+    const uint16_t *brow = g.BT + (size_t)bn * g.ldb;
+    const bool live[2][2] = {{m0 + wm < g.M && n0 + wn < g.N, m0 + wm < g.M && n0 + wn + 16 < g.N},
+                             {m0 + wm + 16 < g.M && n0 + wn < g.N,
+                              m0 + wm + 16 < g.M && n0 + wn + 16 < g.N}};
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        const int k = k0 + lc;
+        uint4 va = {0u, 0u, 0u, 0u}, vb = {0u, 0u, 0u, 0u};
+        if (am < g.M) va = ones ? nt_ones(k, kend) : nt_load(arow, k, kend);
+        if (bn < g.N) vb = nt_load(brow, k, kend);
+        *reinterpret_cast<uint4 *>(&sA[lr][lc]) = va;
+        *reinterpret_cast<uint4 *>(&sB[lr][lc]) = vb;
+        __syncthreads();
+        const int fr = lane & 15, fk = (lane >> 4) * 8;
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            a[i] = *reinterpret_cast<const bf16x8 *>(&sA[wm + i * 16 + fr][fk]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            b[j] = *reinterpret_cast<const bf16x8 *>(&sB[wn + j * 16 + fr][fk]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                if (live[i][j])
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j],
+                                                                       0, 0, 0);
+        __syncthreads();
+    }
+
+    /* lane l holds column (l % 16), rows 4 (l / 16) + r of each 16x16 */
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int mb = m0 + wm + i * 16 + (lane >> 4) * 4;
+            const int n = n0 + wn + j * 16 + (lane & 15);
+            if (n >= g.N || mb >= g.M) continue;
+            if (EPI == NT_PARTIAL) {
+                float *c = g.C + (size_t)split * g.part_stride + (size_t)mb * g.ldc + n;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (mb + r < g.M) c[(size_t)r * g.ldc] = acc[i][j][r];
+            } else if (EPI == NT_OUT) {
+                const float bv = g.bias[n];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (mb + r < g.M) g.C[(size_t)(mb + r) * g.ldc + n] = acc[i][j][r] + bv;
+            } else {
+                uint16_t hv[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float v = acc[i][j][r];
+                    if (EPI == NT_RELU) {
+                        const float z = v + g.bias[n];
+                        v = z > 0.f ? z : 0.f;
+                    } else {
+                        const bool on = mb + r < g.M &&
+                                        bf2f(g.mask[(size_t)(mb + r) * g.ldm + n]) > 0.f;
+                        v = on ? v : 0.f;
+                    }
+                    hv[r] = f2bf(v);
+                    if (mb + r < g.M) g.H[(size_t)(mb + r) * g.ldh + n] = hv[r];
+                }
+                uint16_t *ht = g.HT + (size_t)n * g.ldht + mb;
+                if (mb + 4 <= g.M) {
+                    uint2 pk;
+                    pk.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
+                    pk.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
+                    *reinterpret_cast<uint2 *>(ht) = pk;
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (mb + r < g.M) ht[r] = hv[r];
+                }
+            }
+        }
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(256) gemm_nt_kernel(const NtProblem g) {
+    __shared__ __attribute__((aligned(16))) uint16_t sA[TM][TK + KPAD];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[TN][TK + KPAD];
+    nt_tile<EPI>(g, (int)blockIdx.x, sA, sB);
+}
+
+/* Weight gradients of every layer: problem p takes blocks [block0, next
+ * block0). */
+__global__ void __launch_bounds__(256) gemm_nt_partial_kernel(const NtProblem *table, int nprob) {
+    __shared__ __attribute__((aligned(16))) uint16_t sA[TM][TK + KPAD];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[TN][TK + KPAD];
+    int pi = 0;
+    while (pi + 1 < nprob && (int)blockIdx.x >= table[pi + 1].block0) ++pi;
+    const NtProblem g = table[pi];
+    nt_tile<NT_PARTIAL>(g, (int)blockIdx.x - g.block0, sA, sB);
+}
+
+/* Data row of minibatch row i at a training step. */
+__device__ inline uint32_t sample_row(uint64_t seed, uint32_t step, uint32_t i, uint32_t n) {
+    const uint64_t h = xo_mix64(seed + 0x9E3779B97F4A7C15ull * (((uint64_t)step << 32) + i + 1));
+    return (uint32_t)(h % n);
+}
+
+/* Input rows as bf16, row-major [B][ldx] and transposed [nin][ldxt]: the
+ * minibatch of the step (sampled) or the first B rows. */
+__global__ void rows_kernel(const float *X, uint32_t n, uint32_t nin, uint32_t B, int sampled,
+                            uint64_t seed, const uint32_t *step, uint16_t *Xr, int ldx,
+                            uint16_t *XT, int ldxt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B) return;
-    const uint64_t h = xo_mix64(seed + 0x9E3779B97F4A7C15ull * (((uint64_t)*step << 32) + i + 1));
-    const uint32_t j = (uint32_t)(h % n);
-    for (uint32_t c = 0; c < nin; ++c) Xb[(size_t)i * nin + c] = f2bf(X[(size_t)j * nin + c]);
-    for (uint32_t c = 0; c < nout; ++c) Tb[(size_t)i * nout + c] = T[(size_t)j * nout + c];
+    const uint32_t j = sampled ? sample_row(seed, *step, i, n) : i;
+    for (uint32_t c = 0; c < nin; ++c) {
+        const uint16_t v = f2bf(X[(size_t)j * nin + c]);
+        Xr[(size_t)i * ldx + c] = v;
+        XT[(size_t)c * ldxt + i] = v;
+    }
 }
 
-/* dY = bf16(2 (Y - T) / (B nout)); the mean squared error is added to
- * loss[0], or to loss[*step % ring] when ring > 0 (one slot per training
- * step, so a captured step needs no per-step pointer). */
-__global__ void loss_kernel(const float *Y, const float *T, uint32_t count, float scale,
-                            uint16_t *dY, float *loss, const uint32_t *step, uint32_t ring) {
+/* dY = bf16(2 (Y - T) scale), row-major [B][ldy] and transposed
+ * [nout][ldyt]; sum (Y - T)^2 scale added to loss[0] or to the step's ring
+ * slot. T rows are those of the minibatch (sampled) or the first B. A block
+ * takes 64 rows. */
+__global__ void __launch_bounds__(256) loss_nt_kernel(const float *Y, const float *T, uint32_t nt,
+                                                      int sampled, uint64_t seed,
+                                                      const uint32_t *step, uint32_t B,
+                                                      uint32_t nout, float scale, uint16_t *dY,
+                                                      int ldy, uint16_t *dYT, int ldyt,
+                                                      float *loss, uint32_t ring) {
+    __shared__ uint32_t rows[64];
     __shared__ float part[256];
+    const uint32_t r0 = blockIdx.x * 64;
+    if (threadIdx.x < 64) {
+        const uint32_t i = r0 + threadIdx.x;
+        rows[threadIdx.x] = i < B ? (sampled ? sample_row(seed, *step, i, nt) : i) : 0u;
+    }
+    __syncthreads();
     float s = 0.f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += gridDim.x * blockDim.x) {
-        const float d = Y[i] - T[i];
+    for (uint32_t e = threadIdx.x; e < 64u * nout; e += 256) {
+        const uint32_t rr = e & 63u, c = e >> 6, i = r0 + rr;
+        if (i >= B) continue;
+        const float d = Y[(size_t)i * nout + c] - T[(size_t)rows[rr] * nout + c];
         s += d * d;
-        dY[i] = f2bf(2.f * d * scale);
+        const uint16_t gv = f2bf(2.f * d * scale);
+        dY[(size_t)i * ldy + c] = gv;
+        dYT[(size_t)c * ldyt + i] = gv;
     }
     part[threadIdx.x] = s;
     __syncthreads();
@@ -371,53 +539,54 @@ __global__ void loss_kernel(const float *Y, const float *T, uint32_t count, floa
     if (threadIdx.x == 0) atomicAdd(ring ? loss + (*step % ring) : loss, part[0] * scale);
 }
 
-/* db[n] = sum_m bf16 D[m][n]: 64 columns per block, four row groups with
- * eight partial sums each, combined in a fixed order. */
-__global__ void __launch_bounds__(256) colsum_kernel(const uint16_t *D, uint32_t M, uint32_t N,
-                                                     float *db) {
-    __shared__ float part[4][64];
-    const uint32_t c = threadIdx.x & 63, grp = threadIdx.x >> 6;
-    const uint32_t n = blockIdx.x * 64 + c;
-    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (n < N) {
-        uint32_t m = grp;
-        for (; m + 28 < M; m += 32)
-            for (int u = 0; u < 8; ++u) acc[u] += bf2f(D[(size_t)(m + 4 * u) * N + n]);
-        for (; m < M; m += 4) acc[0] += bf2f(D[(size_t)m * N + n]);
-    }
-    part[grp][c] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-    __syncthreads();
-    if (grp == 0 && n < N) db[n] = (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
-}
+constexpr uint64_t kNotMatrix = ~0ull;
 
-__global__ void fill_f32_kernel(float *p, uint32_t n, float v) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-/* Adam (t = *step + 1), fp32 master weights; refreshes the bf16 copies of
- * the weight matrices. */
-__global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint64_t n,
-                            const uint32_t *step, float lr, float b1, float b2, float eps,
-                            const uint8_t *is_matrix, const uint64_t *bf_index,
-                            uint16_t *wbf) {
+/* Adam (t = *step + 1) on the sum of the gradient's split partials, added
+ * in split order; refreshes both bf16 copies of the weight matrices. */
+__global__ void adam_nt_kernel(float *w, float *m, float *v, const float *part, uint32_t splits,
+                               uint64_t n, const uint32_t *step, float lr, float b1, float b2,
+                               float eps, const uint64_t *bf_index, const uint64_t *bt_index,
+                               uint16_t *wbf, uint16_t *wtb) {
     const float t = (float)(*step + 1u);
     const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const float gi = g[i];
+        float gi = part[i];
+        for (uint32_t s = 1; s < splits; ++s) gi += part[(size_t)s * n + i];
         const float mi = b1 * m[i] + (1.f - b1) * gi;
         const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
         m[i] = mi;
         v[i] = vi;
         const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
         w[i] = wi;
-        if (is_matrix[i]) wbf[bf_index[i]] = f2bf(wi);
+        if (bf_index[i] != kNotMatrix) {
+            const uint16_t h = f2bf(wi);
+            wbf[bf_index[i]] = h;
+            wtb[bt_index[i]] = h;
+        }
     }
 }
 
-__global__ void step_kernel(uint32_t *step) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1u;
+/* grad[i] = sum of the split partials, in split order. */
+__global__ void sum_partials_kernel(const float *part, uint32_t splits, uint64_t n, float *grad) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        float gi = part[i];
+        for (uint32_t s = 1; s < splits; ++s) gi += part[(size_t)s * n + i];
+        grad[i] = gi;
+    }
+}
+
+/* Both bf16 copies of the weight matrices from the fp32 master weights. */
+__global__ void refresh_kernel(const float *w, uint64_t n, const uint64_t *bf_index,
+                               const uint64_t *bt_index, uint16_t *wbf, uint16_t *wtb) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (bf_index[i] == kNotMatrix) continue;
+        const uint16_t h = f2bf(w[i]);
+        wbf[bf_index[i]] = h;
+        wtb[bt_index[i]] = h;
+    }
 }
 
 } // namespace
@@ -427,25 +596,31 @@ struct xo_mlp {
     hipStream_t stream = nullptr;
     int nlayers = 0;
     std::vector<int> dims;       /* nlayers + 1 widths */
+    std::vector<int> ld;         /* widths rounded up to multiples of 8 */
+    int ldb = 0;                 /* max_batch rounded up to a multiple of 8 */
     std::vector<uint64_t> woff;  /* offset of W_l in the packed parameters */
     std::vector<uint64_t> boff;  /* offset of b_l */
-    std::vector<uint64_t> wbf_off; /* offset of W_l in the bf16 copy */
-    uint64_t nparams = 0, nwbf = 0;
+    std::vector<uint64_t> wbf_off; /* offset of W_l [in][ld out] in wbf */
+    std::vector<uint64_t> wtb_off; /* offset of W_l^T [out][ld in] in wtb */
+    uint64_t nparams = 0, nwbf = 0, nwtb = 0;
     uint32_t max_batch = 0;
+    uint32_t max_splits = 0;
     float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
-    uint16_t *wbf = nullptr;
-    uint8_t *is_matrix = nullptr;
-    uint64_t *bf_index = nullptr;
-    uint16_t *xb = nullptr;           /* [B][in] */
+    uint16_t *wbf = nullptr, *wtb = nullptr;
+    uint64_t *bf_index = nullptr, *bt_index = nullptr;
     float *xf = nullptr;              /* [B][in] host-provided rows */
-    float *tb = nullptr;              /* [B][out] */
-    std::vector<uint16_t *> h;        /* activations of hidden layers */
-    uint16_t *d0 = nullptr, *d1 = nullptr; /* [B][max width] gradients */
+    float *tb = nullptr;              /* [B][out] host-provided targets */
+    uint16_t *xr = nullptr;           /* [B][ld in] bf16 inputs */
+    uint16_t *xt = nullptr;           /* [in][ldb] */
+    std::vector<uint16_t *> hr, ht;   /* hidden activations [B][ld], [w][ldb] */
+    std::vector<uint16_t *> dz, dzt;  /* hidden gradients, same layouts */
+    uint16_t *dy = nullptr, *dyt = nullptr; /* output gradient */
     float *y = nullptr;               /* [B][out] */
-    uint16_t *wt = nullptr;           /* padded transposed bf16 weights */
+    float *part = nullptr;            /* [max_splits][nparams] partial sums */
+    std::map<uint32_t, NtProblem *> dw_tables; /* weight-gradient launches per batch */
+    uint16_t *wt = nullptr;           /* fragment-major bf16 weights (inference) */
     std::vector<uint64_t> wt_off;
     bool fusable = false, wt_dirty = true;
-    float *work = nullptr;            /* split-K partial products */
     float *loss = nullptr;            /* per training step */
     uint32_t loss_cap = 0;
     uint32_t *step = nullptr;
@@ -456,7 +631,6 @@ struct xo_mlp {
     uint32_t graph_batch = 0;
     uint64_t graph_seed = 0;
     float graph_lr = 0.f;
-    float *graph_loss = nullptr;
     std::mutex mu;
 };
 
@@ -471,107 +645,139 @@ namespace {
         }                                                                             \
     } while (0)
 
-int widest(const xo_mlp *N) {
-    int w = 0;
-    for (int d : N->dims) w = d > w ? d : w;
-    return w;
+int pad8(int x) { return (x + 7) & ~7; }
+
+/* Splits of the batch in the weight-gradient products. */
+uint32_t batch_splits(const xo_mlp *N, uint32_t B) {
+    uint32_t s = B / 512u;
+    if (s < 1u) s = 1u;
+    return s < N->max_splits ? s : N->max_splits;
 }
 
-/* Forward pass of B rows already in N->xb; output fp32 in N->y. */
+template <int EPI> int nt_launch(xo_mlp *N, NtProblem g) {
+    g.tiles_n = (g.N + TN - 1) / TN;
+    g.splits = 1;
+    g.block0 = 0;
+    const int blocks = ((g.M + TM - 1) / TM) * g.tiles_n;
+    hipLaunchKernelGGL(gemm_nt_kernel<EPI>, dim3(blocks), dim3(256), 0, N->stream, g);
+    MLP_CHECK(hipGetLastError());
+    return 0;
+}
+
+/* Forward pass of B rows already in xr/xt: hidden activations in hr/ht,
+ * output fp32 in y. */
 int forward(xo_mlp *N, uint32_t B) {
-    for (int l = 0; l < N->nlayers; ++l) {
-        GemmArgs g;
+    const int L = N->nlayers;
+    for (int l = 0; l < L; ++l) {
+        NtProblem g;
         std::memset(&g, 0, sizeof(g));
-        g.A = l == 0 ? N->xb : N->h[l - 1];
-        g.lda = N->dims[l];
-        g.B = N->wbf + N->wbf_off[l];
-        g.ldb = N->dims[l + 1];
+        g.A = l == 0 ? N->xr : N->hr[l - 1];
+        g.lda = N->ld[l];
+        g.BT = N->wtb + N->wtb_off[l];
+        g.ldb = N->ld[l];
         g.M = (int)B;
         g.N = N->dims[l + 1];
         g.K = N->dims[l];
+        g.ones_row = -1;
         g.bias = N->w + N->boff[l];
-        if (l + 1 < N->nlayers) {
-            g.H = N->h[l];
-            g.ldh = N->dims[l + 1];
-            MLP_CHECK((gemm<false, false, EPI_BIAS_RELU>(g, N->stream)));
+        if (l + 1 < L) {
+            g.H = N->hr[l];
+            g.ldh = N->ld[l + 1];
+            g.HT = N->ht[l];
+            g.ldht = N->ldb;
+            if (nt_launch<NT_RELU>(N, g)) return -1;
         } else {
             g.C = N->y;
-            g.ldc = N->dims[l + 1];
-            MLP_CHECK((gemm<false, false, EPI_BIAS>(g, N->stream)));
+            g.ldc = N->dims[L];
+            if (nt_launch<NT_OUT>(N, g)) return -1;
         }
     }
     return 0;
 }
 
-/* Loss and gradient of B rows (xb, tb): loss added to loss[0] (ring 0) or
- * to the step's ring slot, gradient in N->grad. */
-int backward(xo_mlp *N, uint32_t B, float *loss, uint32_t ring) {
+/* The weight-gradient problems of a batch size, uploaded once. */
+int dw_table(xo_mlp *N, uint32_t B, const NtProblem **table, int *blocks) {
+    const int L = N->nlayers;
+    std::vector<NtProblem> t(L);
+    const uint32_t S = batch_splits(N, B);
+    int total = 0;
+    for (int l = 0; l < L; ++l) {
+        NtProblem &g = t[l];
+        std::memset(&g, 0, sizeof(g));
+        g.A = l == 0 ? N->xt : N->ht[l - 1];
+        g.lda = N->ldb;
+        g.BT = l + 1 == L ? N->dyt : N->dzt[l];
+        g.ldb = N->ldb;
+        g.M = N->dims[l] + 1;
+        g.N = N->dims[l + 1];
+        g.K = (int)B;
+        g.ones_row = N->dims[l];
+        g.tiles_n = (g.N + TN - 1) / TN;
+        g.splits = (int)S;
+        g.block0 = total;
+        g.C = N->part + N->woff[l];
+        g.ldc = N->dims[l + 1];
+        g.part_stride = N->nparams;
+        total += ((g.M + TM - 1) / TM) * g.tiles_n * g.splits;
+    }
+    *blocks = total;
+    auto it = N->dw_tables.find(B);
+    if (it == N->dw_tables.end()) {
+        NtProblem *d = nullptr;
+        MLP_CHECK(hipMalloc(&d, sizeof(NtProblem) * L));
+        MLP_CHECK(hipMemcpy(d, t.data(), sizeof(NtProblem) * L, hipMemcpyHostToDevice));
+        it = N->dw_tables.emplace(B, d).first;
+    }
+    *table = it->second;
+    return 0;
+}
+
+/* Loss and gradient of the B rows of the last forward pass against T
+ * (the step's minibatch rows when sampled, else the first B rows): loss
+ * added to loss[0] (ring 0) or to the step's ring slot, gradient as split
+ * partial sums in part. */
+int backward(xo_mlp *N, uint32_t B, const float *T, uint32_t nt, int sampled, uint64_t seed,
+             float *loss, uint32_t ring) {
     const int L = N->nlayers;
     const uint32_t nout = (uint32_t)N->dims[L];
-    const uint32_t count = B * nout;
-    hipLaunchKernelGGL(loss_kernel, dim3((count + 255) / 256 < 1024 ? (count + 255) / 256 : 1024),
-                       dim3(256), 0, N->stream, N->y, N->tb, count,
-                       1.0f / (float)count, N->d0, loss, N->step, ring);
+    hipLaunchKernelGGL(loss_nt_kernel, dim3((B + 63) / 64), dim3(256), 0, N->stream, N->y, T,
+                       nt, sampled, seed, N->step, B, nout, 1.0f / (float)(B * nout), N->dy,
+                       N->ld[L], N->dyt, N->ldb, loss, ring);
     MLP_CHECK(hipGetLastError());
-    uint16_t *dcur = N->d0, *dnext = N->d1;
-    for (int l = L - 1; l >= 0; --l) {
-        const int in = N->dims[l], out = N->dims[l + 1];
-        const uint16_t *Hin = l == 0 ? N->xb : N->h[l - 1];
-        /* dW_l = H_in^T dZ_l */
-        GemmArgs g;
-        std::memset(&g, 0, sizeof(g));
-        g.A = Hin;
-        g.lda = in;
-        g.B = dcur;
-        g.ldb = out;
-        g.M = in;
-        g.N = out;
-        g.K = (int)B;
-        /* few output tiles and a long K (the batch): split K over blocks
-         * into the workspace, then add the partial products in order */
-        const int tiles = ((in + TM - 1) / TM) * ((out + TN - 1) / TN);
-        int splits = (int)std::min<uint32_t>(kMaxSplits, std::max<uint32_t>(1u, 256u / tiles));
-        splits = std::min(splits, std::max(1, (int)B / 128));
-        g.C = splits > 1 ? N->work : N->grad + N->woff[l];
-        g.ldc = out;
-        MLP_CHECK((gemm<true, false, EPI_STORE>(g, N->stream, splits)));
-        if (splits > 1) {
-            const uint64_t n = (uint64_t)in * out;
-            hipLaunchKernelGGL(sum_splits_kernel, dim3(256), dim3(256), 0, N->stream, N->work,
-                               (uint32_t)splits, n, N->grad + N->woff[l]);
-            MLP_CHECK(hipGetLastError());
-        }
-        hipLaunchKernelGGL(colsum_kernel, dim3((out + 63) / 64), dim3(256), 0, N->stream, dcur,
-                           B, (uint32_t)out, N->grad + N->boff[l]);
-        MLP_CHECK(hipGetLastError());
-        if (l == 0) break;
+    const uint16_t *dcur = N->dy;
+    for (int l = L - 1; l >= 1; --l) {
         /* dZ_{l-1} = (dZ_l W_l^T) * (H_{l-1} > 0) */
+        NtProblem g;
         std::memset(&g, 0, sizeof(g));
         g.A = dcur;
-        g.lda = out;
-        g.B = N->wbf + N->wbf_off[l];
-        g.ldb = out;
+        g.lda = N->ld[l + 1];
+        g.BT = N->wbf + N->wbf_off[l];
+        g.ldb = N->ld[l + 1];
         g.M = (int)B;
-        g.N = in;
-        g.K = out;
-        g.H = dnext;
-        g.ldh = in;
-        g.mask = N->h[l - 1];
-        g.ldm = in;
-        MLP_CHECK((gemm<false, true, EPI_MASK>(g, N->stream)));
-        uint16_t *t = dcur;
-        dcur = dnext;
-        dnext = t;
+        g.N = N->dims[l];
+        g.K = N->dims[l + 1];
+        g.ones_row = -1;
+        g.H = N->dz[l - 1];
+        g.ldh = N->ld[l];
+        g.HT = N->dzt[l - 1];
+        g.ldht = N->ldb;
+        g.mask = N->hr[l - 1];
+        g.ldm = N->ld[l];
+        if (nt_launch<NT_MASK>(N, g)) return -1;
+        dcur = N->dz[l - 1];
     }
+    const NtProblem *table = nullptr;
+    int blocks = 0;
+    if (dw_table(N, B, &table, &blocks)) return -1;
+    hipLaunchKernelGGL(gemm_nt_partial_kernel, dim3(blocks), dim3(256), 0, N->stream, table,
+                       L);
+    MLP_CHECK(hipGetLastError());
     return 0;
 }
 
-int adam(xo_mlp *N, float lr) {
-    hipLaunchKernelGGL(adam_kernel, dim3(256), dim3(256), 0, N->stream, N->w, N->m, N->v,
-                       N->grad, N->nparams, N->step, lr, 0.9f, 0.999f, 1e-8f, N->is_matrix,
-                       N->bf_index, N->wbf);
-    MLP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(step_kernel, dim3(1), dim3(64), 0, N->stream, N->step);
+int refresh_bf16(xo_mlp *N) {
+    hipLaunchKernelGGL(refresh_kernel, dim3(256), dim3(256), 0, N->stream, N->w, N->nparams,
+                       N->bf_index, N->bt_index, N->wbf, N->wtb);
     MLP_CHECK(hipGetLastError());
     return 0;
 }
@@ -579,23 +785,18 @@ int adam(xo_mlp *N, float lr) {
 /* One training step on a device-sampled minibatch (capturable: every
  * argument is fixed, the step counter lives on the device). */
 int train_step(xo_mlp *N, uint32_t B, uint64_t seed, float lr) {
-    hipLaunchKernelGGL(gather_kernel, dim3((B + 255) / 256), dim3(256), 0, N->stream,
-                       N->data_x, N->data_t, N->ndata, (uint32_t)N->dims[0],
-                       (uint32_t)N->dims[N->nlayers], B, seed, N->step, N->xb, N->tb);
+    hipLaunchKernelGGL(rows_kernel, dim3((B + 255) / 256), dim3(256), 0, N->stream, N->data_x,
+                       N->ndata, (uint32_t)N->dims[0], B, 1, seed, N->step, N->xr, N->ld[0],
+                       N->xt, N->ldb);
     MLP_CHECK(hipGetLastError());
     if (forward(N, B)) return -1;
-    if (backward(N, B, N->loss, N->loss_cap)) return -1;
-    return adam(N, lr);
-}
-
-int refresh_bf16(xo_mlp *N) {
-    /* W copies from the master weights */
-    for (int l = 0; l < N->nlayers; ++l) {
-        const uint64_t n = (uint64_t)N->dims[l] * N->dims[l + 1];
-        hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, N->stream,
-                           N->w + N->woff[l], N->wbf + N->wbf_off[l], n);
-        MLP_CHECK(hipGetLastError());
-    }
+    if (backward(N, B, N->data_t, N->ndata, 1, seed, N->loss, N->loss_cap)) return -1;
+    hipLaunchKernelGGL(adam_nt_kernel, dim3(256), dim3(256), 0, N->stream, N->w, N->m, N->v,
+                       N->part, batch_splits(N, B), N->nparams, N->step, lr, 0.9f, 0.999f,
+                       1e-8f, N->bf_index, N->bt_index, N->wbf, N->wtb);
+    MLP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(step_kernel, dim3(1), dim3(64), 0, N->stream, N->step);
+    MLP_CHECK(hipGetLastError());
     return 0;
 }
 
@@ -630,11 +831,13 @@ int forward_fused(xo_mlp *N, uint32_t B) {
     return 0;
 }
 
+/* B host rows into xf, then as bf16 into xr and xt. */
 int upload_rows(xo_mlp *N, const float *x, uint32_t B) {
     MLP_CHECK(hipMemcpyAsync(N->xf, x, sizeof(float) * B * N->dims[0], hipMemcpyHostToDevice,
                              N->stream));
-    hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, N->stream, N->xf, N->xb,
-                       (uint64_t)B * N->dims[0]);
+    hipLaunchKernelGGL(rows_kernel, dim3((B + 255) / 256), dim3(256), 0, N->stream, N->xf, B,
+                       (uint32_t)N->dims[0], B, 0, 0ull, N->step, N->xr, N->ld[0], N->xt,
+                       N->ldb);
     MLP_CHECK(hipGetLastError());
     return 0;
 }
@@ -649,13 +852,16 @@ void xo_mlp_destroy(xo_mlp *N) {
     if (N->stream) (void)hipStreamSynchronize(N->stream);
     if (N->exec) (void)hipGraphExecDestroy(N->exec);
     if (N->graph) (void)hipGraphDestroy(N->graph);
-    void *bufs[] = {N->w, N->m, N->v, N->grad, N->wbf, N->is_matrix, N->bf_index, N->xb,
-                    N->xf, N->tb, N->d0, N->d1, N->y, N->work, N->loss, N->step,
-                    N->data_x, N->data_t, N->wt};
+    void *bufs[] = {N->w,   N->m,    N->v,    N->grad,   N->wbf,    N->wtb,  N->bf_index,
+                    N->bt_index, N->xf, N->tb, N->xr,    N->xt,     N->dy,   N->dyt,
+                    N->y,   N->part, N->loss, N->step,   N->data_x, N->data_t, N->wt};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
-    for (auto p : N->h)
-        if (p) (void)hipFree(p);
+    for (auto *list : {&N->hr, &N->ht, &N->dz, &N->dzt})
+        for (auto p : *list)
+            if (p) (void)hipFree(p);
+    for (auto &kv : N->dw_tables)
+        if (kv.second) (void)hipFree(kv.second);
     if (N->stream) (void)hipStreamDestroy(N->stream);
     delete N;
 }
@@ -675,14 +881,19 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
     N->ctx = ctx;
     N->nlayers = (int)nlayers;
     N->dims.assign(dims, dims + nlayers + 1);
+    for (uint32_t l = 0; l <= nlayers; ++l) N->ld.push_back(pad8((int)dims[l]));
     N->max_batch = max_batch;
+    N->ldb = pad8((int)max_batch);
+    N->max_splits = max_batch / 512u < 8u ? (max_batch / 512u > 0u ? max_batch / 512u : 1u) : 8u;
     for (uint32_t l = 0; l < nlayers; ++l) {
         N->woff.push_back(N->nparams);
         N->nparams += (uint64_t)dims[l] * dims[l + 1];
         N->boff.push_back(N->nparams);
         N->nparams += dims[l + 1];
         N->wbf_off.push_back(N->nwbf);
-        N->nwbf += (uint64_t)dims[l] * dims[l + 1];
+        N->nwbf += (uint64_t)dims[l] * N->ld[l + 1];
+        N->wtb_off.push_back(N->nwtb);
+        N->nwtb += (uint64_t)dims[l + 1] * N->ld[l];
     }
     auto bad = [&](const char *what, hipError_t e) -> xo_mlp * {
         xo::set_error("xo_mlp_create: %s: %s", what, hipGetErrorString(e));
@@ -693,31 +904,46 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
     if ((e = hipSetDevice(ctx->device)) != hipSuccess) return bad("device", e);
     if ((e = hipStreamCreateWithFlags(&N->stream, hipStreamNonBlocking)) != hipSuccess)
         return bad("stream", e);
-    const int W = widest(N);
-    const uint64_t P = N->nparams;
-    uint64_t max_matrix = 0;
-    for (uint32_t l = 0; l < nlayers; ++l)
-        max_matrix = std::max<uint64_t>(max_matrix, (uint64_t)dims[l] * dims[l + 1]);
+    const uint64_t P = N->nparams, B = max_batch, LB = (uint64_t)N->ldb;
+    const int L = (int)nlayers;
+    /* bf16 buffers are cleared after allocation: padding columns stay 0 */
     struct {
         void **p;
         size_t bytes;
     } allocs[] = {
-        {(void **)&N->w, 4 * P}, {(void **)&N->m, 4 * P}, {(void **)&N->v, 4 * P},
-        {(void **)&N->grad, 4 * P}, {(void **)&N->wbf, 2 * N->nwbf},
-        {(void **)&N->is_matrix, P}, {(void **)&N->bf_index, 8 * P},
-        {(void **)&N->xb, 2ull * max_batch * dims[0]}, {(void **)&N->xf, 4ull * max_batch * dims[0]},
-        {(void **)&N->tb, 4ull * max_batch * dims[nlayers]},
-        {(void **)&N->d0, 2ull * max_batch * W}, {(void **)&N->d1, 2ull * max_batch * W},
-        {(void **)&N->y, 4ull * max_batch * dims[nlayers]}, {(void **)&N->step, 4},
-        {(void **)&N->work, 4ull * kMaxSplits * max_matrix},
+        {(void **)&N->w, 4 * P},
+        {(void **)&N->m, 4 * P},
+        {(void **)&N->v, 4 * P},
+        {(void **)&N->grad, 4 * P},
+        {(void **)&N->wbf, 2 * N->nwbf},
+        {(void **)&N->wtb, 2 * N->nwtb},
+        {(void **)&N->bf_index, 8 * P},
+        {(void **)&N->bt_index, 8 * P},
+        {(void **)&N->xf, 4 * B * dims[0]},
+        {(void **)&N->tb, 4 * B * dims[nlayers]},
+        {(void **)&N->xr, 2 * B * N->ld[0]},
+        {(void **)&N->xt, 2 * (uint64_t)dims[0] * LB},
+        {(void **)&N->dy, 2 * B * N->ld[L]},
+        {(void **)&N->dyt, 2 * (uint64_t)dims[nlayers] * LB},
+        {(void **)&N->y, 4 * B * dims[nlayers]},
+        {(void **)&N->part, 4ull * N->max_splits * P},
+        {(void **)&N->step, 4},
     };
-    for (auto &a : allocs)
+    for (auto &a : allocs) {
         if ((e = hipMalloc(a.p, a.bytes)) != hipSuccess) return bad("allocation", e);
-    for (uint32_t l = 0; l + 1 < nlayers; ++l) {
-        uint16_t *p = nullptr;
-        if ((e = hipMalloc(&p, 2ull * max_batch * dims[l + 1])) != hipSuccess)
-            return bad("allocation", e);
-        N->h.push_back(p);
+        if ((e = hipMemset(*a.p, 0, a.bytes)) != hipSuccess) return bad("clear", e);
+    }
+    for (int l = 0; l + 1 < L; ++l) {
+        const uint64_t rows_bytes = 2 * B * N->ld[l + 1];
+        const uint64_t cols_bytes = 2 * (uint64_t)dims[l + 1] * LB;
+        std::vector<uint16_t *> *lists[4] = {&N->hr, &N->ht, &N->dz, &N->dzt};
+        const uint64_t sizes[4] = {rows_bytes, cols_bytes, rows_bytes, cols_bytes};
+        for (int k = 0; k < 4; ++k) {
+            uint16_t *p = nullptr;
+            if ((e = hipMalloc(&p, sizes[k])) != hipSuccess) return bad("allocation", e);
+            lists[k]->push_back(p);
+            if ((e = hipMemset(p, 0, sizes[k])) != hipSuccess) return bad("clear", e);
+        }
     }
     /* the fused inference kernel covers input <= 32, hidden 256, output
      * <= 128, at most 8 layers */
@@ -733,25 +959,20 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
         }
         if ((e = hipMalloc(&N->wt, 2 * total)) != hipSuccess) return bad("allocation", e);
     }
-    /* which parameters are matrix entries, and where their bf16 copy is */
-    std::vector<uint8_t> ism(P, 0);
-    std::vector<uint64_t> idx(P, 0);
-    for (uint32_t l = 0; l < nlayers; ++l) {
-        const uint64_t n = (uint64_t)dims[l] * dims[l + 1];
-        for (uint64_t i = 0; i < n; ++i) {
-            ism[N->woff[l] + i] = 1;
-            idx[N->woff[l] + i] = N->wbf_off[l] + i;
-        }
-    }
-    if ((e = hipMemcpy(N->is_matrix, ism.data(), P, hipMemcpyHostToDevice)) != hipSuccess)
+    /* where each matrix entry's bf16 copies are: W_l [in][ld out] and
+     * W_l^T [out][ld in]; biases have none */
+    std::vector<uint64_t> bi(P, kNotMatrix), ti(P, kNotMatrix);
+    for (uint32_t l = 0; l < nlayers; ++l)
+        for (uint64_t k = 0; k < dims[l]; ++k)
+            for (uint64_t n = 0; n < dims[l + 1]; ++n) {
+                const uint64_t i = N->woff[l] + k * dims[l + 1] + n;
+                bi[i] = N->wbf_off[l] + k * N->ld[l + 1] + n;
+                ti[i] = N->wtb_off[l] + n * N->ld[l] + k;
+            }
+    if ((e = hipMemcpy(N->bf_index, bi.data(), 8 * P, hipMemcpyHostToDevice)) != hipSuccess)
         return bad("upload", e);
-    if ((e = hipMemcpy(N->bf_index, idx.data(), 8 * P, hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpy(N->bt_index, ti.data(), 8 * P, hipMemcpyHostToDevice)) != hipSuccess)
         return bad("upload", e);
-    if ((e = hipMemset(N->w, 0, 4 * P)) != hipSuccess) return bad("clear", e);
-    if ((e = hipMemset(N->m, 0, 4 * P)) != hipSuccess) return bad("clear", e);
-    if ((e = hipMemset(N->v, 0, 4 * P)) != hipSuccess) return bad("clear", e);
-    if ((e = hipMemset(N->wbf, 0, 2 * N->nwbf)) != hipSuccess) return bad("clear", e);
-    if ((e = hipMemset(N->step, 0, 4)) != hipSuccess) return bad("clear", e);
     return N;
 }
 
@@ -847,8 +1068,13 @@ int xo_mlp_gradient(xo_mlp *N, const float *x, const float *t, uint32_t n, float
     MLP_CHECK(hipMalloc(&dloss, 4));
     int rc = 0;
     if (hipMemsetAsync(dloss, 0, 4, N->stream) != hipSuccess || forward(N, n) ||
-        backward(N, n, dloss, 0))
+        backward(N, n, N->tb, n, 0, 0ull, dloss, 0))
         rc = -1;
+    if (rc == 0) {
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(256), dim3(256), 0, N->stream, N->part,
+                           batch_splits(N, n), N->nparams, N->grad);
+        if (hipGetLastError() != hipSuccess) rc = -1;
+    }
     if (rc == 0 &&
         (hipMemcpyAsync(grad, N->grad, 4 * N->nparams, hipMemcpyDeviceToHost, N->stream) !=
              hipSuccess ||
@@ -915,6 +1141,12 @@ int xo_mlp_train(xo_mlp *N, uint32_t steps, uint32_t batch, float lr, uint64_t s
         (void)hipGraphDestroy(N->graph);
         N->exec = nullptr;
         N->graph = nullptr;
+    }
+    {
+        /* the weight-gradient launch table is uploaded outside any capture */
+        const NtProblem *table = nullptr;
+        int blocks = 0;
+        if (dw_table(N, batch, &table, &blocks)) return -1;
     }
     if (use_graph && !N->exec) {
         MLP_CHECK(hipStreamBeginCapture(N->stream, hipStreamCaptureModeThreadLocal));
@@ -1014,10 +1246,10 @@ int xo_gemm_bf16(xo_ctx *ctx, int trans_a, int trans_b, uint32_t M, uint32_t N, 
         rc = -1;
     if (rc == 0) {
         hipError_t e;
-        if (!trans_a && !trans_b) e = gemm<false, false, EPI_STORE>(g, s);
-        else if (trans_a && !trans_b) e = gemm<true, false, EPI_STORE>(g, s);
-        else if (!trans_a && trans_b) e = gemm<false, true, EPI_STORE>(g, s);
-        else e = gemm<true, true, EPI_STORE>(g, s);
+        if (!trans_a && !trans_b) e = gemm<false, false>(g, s);
+        else if (trans_a && !trans_b) e = gemm<true, false>(g, s);
+        else if (!trans_a && trans_b) e = gemm<false, true>(g, s);
+        else e = gemm<true, true>(g, s);
         if (e != hipSuccess || hipStreamSynchronize(s) != hipSuccess) rc = -1;
     }
     if (rc == 0 && hipMemcpy(C, dc, 4 * nc, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;

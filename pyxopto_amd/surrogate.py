@@ -70,6 +70,7 @@ class Network:
         if not h:
             raise N.NativeError('xo_mlp_create: ' + N.last_error())
         self._h = h
+        self._ctx._dependents.add(self)
         self.max_batch = int(max_batch)
         self.nparams = int(N.lib().xo_mlp_param_count(h))
 

@@ -172,3 +172,18 @@ def test_surrogate_gains_over_monte_carlo(ctx):
     strict = S.Service(sur, mc_packets=20000, tolerance=0.0, context=ctx)
     res = strict.query(test_rows[:2], spot_check=True)
     assert all(r[1] == 'mc_fallback' for r in res)
+
+
+def test_context_close_releases_its_networks():
+    """A network holds its context's device: closing the context closes the
+    network first, and later runs on a new context are unaffected."""
+    c = N.Context(0, 1, S.XINIT)
+    net = S.Network((5, 16, 4), c, max_batch=64)
+    c.close()
+    assert net._h is None
+    c2 = N.Context(0, 1, S.XINIT)
+    net2 = S.Network((5, 16, 4), c2, max_batch=64)
+    net2.set_params(M.init_params((5, 16, 4), seed=1))
+    y, _ = net2.forward(np.zeros((3, 5), np.float32))
+    assert np.all(np.isfinite(y))
+    c2.close()
