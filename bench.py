@@ -39,10 +39,12 @@ SERVING_QPS = 250.0                     # requests/s of the serving run
 # HBM bytes per launch of the dominant kernel: rocprofv3 FETCH_SIZE and
 # WRITE_SIZE of one launch, each scaled by what the counter reads for a
 # 1 GiB device-to-device copy (profiles/batch_kernel_layered_batch_4096.md,
-# profiles/mcvox_kernel_skin_vessel_200.md). Offline figures: measured for
-# the kernels as they are in this tree.
-TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 1.784e11,
-                            'skin_vessel_200': 7.844e11}
+# profiles/mcvox_kernel_skin_vessel_200.md,
+# profiles/mcml_kernel_slab_1mm_and_surrogate.md). Offline figures: measured
+# for the kernels as they are in this tree, hardware arithmetic.
+TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 1.662e11,
+                            'skin_vessel_200': 7.52e11,
+                            'slab_1mm': 1.09e7}
 
 METRIC = ('photon packets/s (1 GPU) batched vs plain; p50/p99 solve latency '
           'at fixed QPS')
