@@ -154,12 +154,23 @@ def ceilings(ctx):
         raise RuntimeError('atomic probe lost updates: {} of {}'.format(
             total, natom))
     vms, vops = ctx.probe_valu(20000)
+    # the same atomics into 256 KiB, and into one 64 MiB copy per XCD with
+    # workgroup-scope atomics: the rate does not depend on either
+    hms, htot = ctx.probe_atomic_u64_xcd(64 << 20, 256 << 10, natom, False,
+                                         seed=7)
+    xms, xtot = ctx.probe_atomic_u64_xcd(64 << 20, 64 << 20, natom, True,
+                                         seed=7)
+    if htot != natom or xtot != natom:
+        raise RuntimeError('atomic probes lost updates')
     return {'valu_fma_lane_ops_per_s': vops / (vms * 1e-3),
             'copy_GBps': 2 * nbytes / (ms * 1e-3) / 1e9,
             'atomic_u64_per_s': natom / (ams * 1e-3),
             'atomic_u64_GBps': 8 * natom / (ams * 1e-3) / 1e9,
+            'atomic_u64_per_s_256KiB': natom / (hms * 1e-3),
+            'atomic_u64_per_s_per_xcd_copies': natom / (xms * 1e-3),
             'probe': 'fp32 FMA chains 8 x 20000 per lane; copy 1 GiB x10; '
-                     '2^28 u64 atomics, 64 MiB array'}
+                     '2^28 u64 atomics, 64 MiB array (also: targets in '
+                     '256 KiB; one 64 MiB array per XCD, workgroup scope)'}
 
 
 def serving(insts, rates, duration: float, seed: int = 20261203,

@@ -654,6 +654,16 @@ int xo_probe_copy(xo_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms_per_cop
 int xo_probe_atomic_u64(xo_ctx *ctx, uint64_t array_bytes, uint64_t natomics,
                         uint64_t seed, double *ms, uint64_t *checksum);
 
+/* xo_probe_atomic_u64 with the targets among the first hot_bytes of the
+ * array and, when per_xcd, eight arrays (one per XCD): each wave adds into
+ * the array of the XCD it runs on with workgroup-scope atomics. checksum is
+ * the sum over every array. Whether the atomic rate depends on the
+ * footprint or on cross-XCD coherence, which decides how the fluence
+ * accumulators are laid out. No counterpart in the reference. */
+int xo_probe_atomic_u64_xcd(xo_ctx *ctx, uint64_t array_bytes, uint64_t hot_bytes,
+                            uint64_t natomics, uint64_t seed, int per_xcd, double *ms,
+                            uint64_t *checksum);
+
 /* Eight independent fp32 fused multiply-add chains per lane, iters rounds,
  * 32 blocks of 256 threads per CU: kernel ms and the lane-operations done.
  * The VALU issue ceiling the transport kernels are compared against. No

@@ -313,6 +313,10 @@ PROTOTYPES = {
                               POINTER(c_double)]),
     'xo_probe_atomic_u64': (c_int, [c_void_p, c_uint64, c_uint64, c_uint64,
                                     POINTER(c_double), POINTER(c_uint64)]),
+    'xo_probe_atomic_u64_xcd': (c_int, [c_void_p, c_uint64, c_uint64,
+                                        c_uint64, c_uint64, c_int,
+                                        POINTER(c_double),
+                                        POINTER(c_uint64)]),
     'xo_probe_valu': (c_int, [c_void_p, c_uint32, POINTER(c_double),
                               POINTER(c_double)]),
     'xo_probe_cr_math': (c_int, [c_void_p, c_uint64, c_uint64,
@@ -677,6 +681,17 @@ class Context:
                                         c_uint64(int(seed)), ctypes.byref(ms),
                                         ctypes.byref(total)),
               'xo_probe_atomic_u64')
+        return ms.value, int(total.value)
+
+    def probe_atomic_u64_xcd(self, array_bytes: int, hot_bytes: int,
+                             natomics: int, per_xcd: bool, seed: int = 1):
+        """(kernel ms, sum) of natomics random u64 atomic adds among the
+        first hot_bytes, into one array per XCD when per_xcd."""
+        ms, total = c_double(), c_uint64()
+        check(lib().xo_probe_atomic_u64_xcd(
+            self._h, c_uint64(int(array_bytes)), c_uint64(int(hot_bytes)),
+            c_uint64(int(natomics)), c_uint64(int(seed)), c_int(int(per_xcd)),
+            ctypes.byref(ms), ctypes.byref(total)), 'xo_probe_atomic_u64_xcd')
         return ms.value, int(total.value)
 
     def run_mcvox(self, cfg: XoMcvoxCfg, voxels, lut, npackets: int,

@@ -51,21 +51,27 @@ constexpr uint32_t kEvNone = 0, kEvInteraction = 1, kEvBoundary = 2;
 constexpr uint32_t kDeferWait = 8;
 
 /* End of an event (mcml.template.c:756-782): the simulation radius and the
- * step limit; a finished packet frees its lane. */
+ * step limit; a finished packet frees its lane and adds its steps to the
+ * lane's count (every step ends in exactly one event). */
 __device__ __forceinline__ void end_event(bool done, bool &alive, float px, float py, float pz,
                                           float spx, float spy, float spz, float rmax2,
-                                          uint32_t &pkt_steps, uint32_t *truncated) {
+                                          uint32_t &pkt_steps, uint32_t *truncated,
+                                          uint32_t *steps) {
     /* no distance can exceed an infinite radius (the default): skip the
      * test, which changes nothing */
     if (rmax2 < XO_INF_F) {
         const float ex = px - spx, ey = py - spy, ez = pz - spz;
         if (ex * ex + ey * ey + ez * ez > rmax2) done = true;
     }
-    if (!done && ++pkt_steps >= XO_MAX_PACKET_STEPS) {
+    ++pkt_steps;
+    if (!done && pkt_steps >= XO_MAX_PACKET_STEPS) {
         done = true;
         atomicAdd(truncated, 1u);
     }
-    if (done) alive = false;
+    if (done) {
+        alive = false;
+        atomicAdd(steps, pkt_steps);
+    }
 }
 
 /* At most 64 VGPRs (8 waves per SIMD): the compiler fits the loop without
@@ -108,6 +114,7 @@ batch_kernel(const BatchParams P) {
     uint32_t a = 0;
     uint32_t pkt_steps = 0;
     uint32_t *const n_trunc = &scount[3][threadIdx.x];
+    uint32_t *const n_steps = &scount[1][threadIdx.x];
     uint32_t pend = kEvNone;  /* event the lane's last step ended on     */
     float pstep = 0.f;        /* drawn step of a pending boundary event  */
     uint32_t bwait = 0;       /* iterations boundary events have waited  */
@@ -291,7 +298,8 @@ batch_kernel(const BatchParams P) {
                     }
                     done = mbl_roulette(done);
                     pend = kEvNone;
-                    end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, n_trunc);
+                    end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, n_trunc,
+                              n_steps);
                 } else {
                     /* crossed both planes of a degenerate layer: an
                      * interaction at the clipped point, handled below */
@@ -302,7 +310,6 @@ batch_kernel(const BatchParams P) {
         if (alive && pend == kEvNone) {
             /* (B) */
             const LayerL L = layers[layer];
-            atomicAdd(&scount[1][threadIdx.x], 1u);
             float step;
             if (METHOD == XO_METHOD_MBL)
                 step = M::div(-M::log_unit(rng_next(x, a)), L.mus); /* mcml.template.c:529-531 */
@@ -344,7 +351,8 @@ batch_kernel(const BatchParams P) {
             }
             done = mbl_roulette(done);
             pend = kEvNone;
-            end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, n_trunc);
+            end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, n_trunc,
+                      n_steps);
         }
     }
 

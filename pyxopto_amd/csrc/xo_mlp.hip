@@ -300,9 +300,6 @@ __global__ void fill_f32_kernel(float *p, uint32_t n, float v) {
     if (i < n) p[i] = v;
 }
 
-__global__ void step_kernel(uint32_t *step) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1u;
-}
 
 /* ------------------------------------------------------------------ */
 /* Training step: products with both operands stored k-contiguous       */
@@ -324,7 +321,7 @@ __global__ void step_kernel(uint32_t *step) {
  * kernel bit for bit). The weight gradients of all layers run in one launch,
  * each split over the batch into fp32 partial sums that the Adam kernel
  * adds in a fixed order. Leading dimensions are multiples of 8. */
-enum NtEpi : int { NT_RELU = 0, NT_OUT = 1, NT_MASK = 2, NT_PARTIAL = 3 };
+enum NtEpi : int { NT_RELU = 0, NT_OUT = 1, NT_MASK = 2, NT_PARTIAL = 3, NT_LOSS = 4 };
 
 struct NtProblem {
     const uint16_t *A;      /* [M][lda]                                   */
@@ -333,14 +330,29 @@ struct NtProblem {
     uint16_t *H;            /* NT_RELU, NT_MASK: [M][ldh]                 */
     uint16_t *HT;           /* NT_RELU, NT_MASK: [N][ldht]                */
     float *C;               /* NT_OUT: [M][ldc]; NT_PARTIAL: split 0 base */
-    const uint16_t *mask;   /* NT_MASK: [M][ldm]                          */
+    const uint16_t *mask;   /* NT_MASK: transposed activations [N][ldm]   */
     uint64_t part_stride;   /* NT_PARTIAL: floats between splits          */
+    /* NT_LOSS: d = output - T[row]; H, HT receive bf16(2 d scale); the
+     * squared error times scale is added to loss[0] or loss[*step % ring] */
+    const float *T;
+    const uint32_t *step;
+    float *loss;
+    uint64_t seed;
+    float scale;
+    uint32_t nt, ring;
+    int sampled;            /* T rows of the step's minibatch, else row m  */
     int lda, ldb, M, N, K;
     int ones_row;           /* row of A read as 1.0, or -1                */
     int tiles_n, splits, block0;
     int ldh, ldht, ldc, ldm;
     int reserved;
 };
+
+/* Data row of minibatch row i at a training step. */
+__device__ inline uint32_t sample_row(uint64_t seed, uint32_t step, uint32_t i, uint32_t n) {
+    const uint64_t h = xo_mix64(seed + 0x9E3779B97F4A7C15ull * (((uint64_t)step << 32) + i + 1));
+    return (uint32_t)(h % n);
+}
 
 /* Elements [k, k + 8) of a k-contiguous row, zero at and past kend. */
 __device__ inline uint4 nt_load(const uint16_t *row, int k, int kend) {
@@ -363,9 +375,13 @@ __device__ inline uint4 nt_ones(int k, int kend) {
     return r;
 }
 
+/* K is staged through LDS NTK at a time: every thread issues NTK / 32
+ * 16-byte loads per operand before the block synchronizes once. */
+constexpr int NTK = 128, NTPAD = 8;
+typedef uint16_t NtRow[NTK + NTPAD];
+
 template <int EPI>
-__device__ __forceinline__ void nt_tile(const NtProblem &g, int tile,
-                                        uint16_t (*sA)[TK + KPAD], uint16_t (*sB)[TK + KPAD]) {
+__device__ __forceinline__ void nt_tile(const NtProblem &g, int tile, NtRow *sA, NtRow *sB) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int split = tile % g.splits, t = tile / g.splits;
     const int m0 = (t / g.tiles_n) * TM, n0 = (t % g.tiles_n) * TN;
@@ -379,7 +395,7 @@ __device__ __forceinline__ void nt_tile(const NtProblem &g, int tile,
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    /* loader: 64 rows x 32 k per operand, 8 k per thread */
+    /* loader: row tid / 4 of each operand, k offsets 8 (tid % 4) + 32 s */
     const int lr = tid >> 2, lc = (tid & 3) * 8;
     const int am = m0 + lr, bn = n0 + lr;
     const bool ones = am == g.ones_row;
@@ -388,33 +404,45 @@ __device__ __forceinline__ void nt_tile(const NtProblem &g, int tile,
     const bool live[2][2] = {{m0 + wm < g.M && n0 + wn < g.N, m0 + wm < g.M && n0 + wn + 16 < g.N},
                              {m0 + wm + 16 < g.M && n0 + wn < g.N,
                               m0 + wm + 16 < g.M && n0 + wn + 16 < g.N}};
-    for (int k0 = kbeg; k0 < kend; k0 += TK) {
-        const int k = k0 + lc;
-        uint4 va = {0u, 0u, 0u, 0u}, vb = {0u, 0u, 0u, 0u};
-        if (am < g.M) va = ones ? nt_ones(k, kend) : nt_load(arow, k, kend);
-        if (bn < g.N) vb = nt_load(brow, k, kend);
-        *reinterpret_cast<uint4 *>(&sA[lr][lc]) = va;
-        *reinterpret_cast<uint4 *>(&sB[lr][lc]) = vb;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    for (int c0 = kbeg; c0 < kend; c0 += NTK) {
+        uint4 va[NTK / TK], vb[NTK / TK];
+#pragma unroll
+        for (int s8 = 0; s8 < NTK / TK; ++s8) {
+            const int k = c0 + lc + s8 * TK;
+            va[s8] = uint4{0u, 0u, 0u, 0u};
+            vb[s8] = uint4{0u, 0u, 0u, 0u};
+            if (am < g.M) va[s8] = ones ? nt_ones(k, kend) : nt_load(arow, k, kend);
+            if (bn < g.N) vb[s8] = nt_load(brow, k, kend);
+        }
+#pragma unroll
+        for (int s8 = 0; s8 < NTK / TK; ++s8) {
+            *reinterpret_cast<uint4 *>(&sA[lr][lc + s8 * TK]) = va[s8];
+            *reinterpret_cast<uint4 *>(&sB[lr][lc + s8 * TK]) = vb[s8];
+        }
         __syncthreads();
-        const int fr = lane & 15, fk = (lane >> 4) * 8;
-        bf16x8 a[2], b[2];
+        const int nks = (kend - c0 + TK - 1) / TK < NTK / TK ? (kend - c0 + TK - 1) / TK : NTK / TK;
+        for (int ks = 0; ks < nks; ++ks) {
+            bf16x8 a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-            a[i] = *reinterpret_cast<const bf16x8 *>(&sA[wm + i * 16 + fr][fk]);
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-            b[j] = *reinterpret_cast<const bf16x8 *>(&sB[wn + j * 16 + fr][fk]);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8 *>(&sA[wm + i * 16 + fr][ks * TK + fk]);
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                if (live[i][j])
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j],
-                                                                       0, 0, 0);
+                b[j] = *reinterpret_cast<const bf16x8 *>(&sB[wn + j * 16 + fr][ks * TK + fk]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (live[i][j])
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j],
+                                                                           acc[i][j], 0, 0, 0);
+        }
         __syncthreads();
     }
 
     /* lane l holds column (l % 16), rows 4 (l / 16) + r of each 16x16 */
+    float sq = 0.f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -433,20 +461,41 @@ __device__ __forceinline__ void nt_tile(const NtProblem &g, int tile,
                 for (int r = 0; r < 4; ++r)
                     if (mb + r < g.M) g.C[(size_t)(mb + r) * g.ldc + n] = acc[i][j][r] + bv;
             } else {
-                uint16_t hv[4];
+                uint16_t hv[4] = {0, 0, 0, 0};
+                uint16_t mk[4] = {0, 0, 0, 0};
+                if (EPI == NT_MASK) {
+                    const uint16_t *mrow = g.mask + (size_t)n * g.ldm + mb;
+                    if (mb + 4 <= g.M) {
+                        const uint2 pk = *reinterpret_cast<const uint2 *>(mrow);
+                        mk[0] = (uint16_t)pk.x;
+                        mk[1] = (uint16_t)(pk.x >> 16);
+                        mk[2] = (uint16_t)pk.y;
+                        mk[3] = (uint16_t)(pk.y >> 16);
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (mb + r < g.M) mk[r] = mrow[r];
+                    }
+                }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
+                    if (mb + r >= g.M) continue;
                     float v = acc[i][j][r];
                     if (EPI == NT_RELU) {
                         const float z = v + g.bias[n];
                         v = z > 0.f ? z : 0.f;
+                    } else if (EPI == NT_MASK) {
+                        v = bf2f(mk[r]) > 0.f ? v : 0.f;
                     } else {
-                        const bool on = mb + r < g.M &&
-                                        bf2f(g.mask[(size_t)(mb + r) * g.ldm + n]) > 0.f;
-                        v = on ? v : 0.f;
+                        const uint32_t row = g.sampled
+                                                 ? sample_row(g.seed, *g.step, (uint32_t)(mb + r), g.nt)
+                                                 : (uint32_t)(mb + r);
+                        const float d = (v + g.bias[n]) - g.T[(size_t)row * g.N + n];
+                        sq += d * d;
+                        v = 2.f * d * g.scale;
                     }
                     hv[r] = f2bf(v);
-                    if (mb + r < g.M) g.H[(size_t)(mb + r) * g.ldh + n] = hv[r];
+                    g.H[(size_t)(mb + r) * g.ldh + n] = hv[r];
                 }
                 uint16_t *ht = g.HT + (size_t)n * g.ldht + mb;
                 if (mb + 4 <= g.M) {
@@ -461,30 +510,31 @@ __device__ __forceinline__ void nt_tile(const NtProblem &g, int tile,
                 }
             }
         }
+    if (EPI == NT_LOSS) {
+        /* one atomic per wave */
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+        if (lane == 0 && sq != 0.f)
+            atomicAdd(g.ring ? g.loss + (*g.step % g.ring) : g.loss, sq * g.scale);
+    }
 }
 
 template <int EPI>
 __global__ void __launch_bounds__(256) gemm_nt_kernel(const NtProblem g) {
-    __shared__ __attribute__((aligned(16))) uint16_t sA[TM][TK + KPAD];
-    __shared__ __attribute__((aligned(16))) uint16_t sB[TN][TK + KPAD];
+    __shared__ __attribute__((aligned(16))) NtRow sA[TM];
+    __shared__ __attribute__((aligned(16))) NtRow sB[TN];
     nt_tile<EPI>(g, (int)blockIdx.x, sA, sB);
 }
 
 /* Weight gradients of every layer: problem p takes blocks [block0, next
  * block0). */
 __global__ void __launch_bounds__(256) gemm_nt_partial_kernel(const NtProblem *table, int nprob) {
-    __shared__ __attribute__((aligned(16))) uint16_t sA[TM][TK + KPAD];
-    __shared__ __attribute__((aligned(16))) uint16_t sB[TN][TK + KPAD];
+    __shared__ __attribute__((aligned(16))) NtRow sA[TM];
+    __shared__ __attribute__((aligned(16))) NtRow sB[TN];
     int pi = 0;
     while (pi + 1 < nprob && (int)blockIdx.x >= table[pi + 1].block0) ++pi;
     const NtProblem g = table[pi];
     nt_tile<NT_PARTIAL>(g, (int)blockIdx.x - g.block0, sA, sB);
-}
-
-/* Data row of minibatch row i at a training step. */
-__device__ inline uint32_t sample_row(uint64_t seed, uint32_t step, uint32_t i, uint32_t n) {
-    const uint64_t h = xo_mix64(seed + 0x9E3779B97F4A7C15ull * (((uint64_t)step << 32) + i + 1));
-    return (uint32_t)(h % n);
 }
 
 /* Input rows as bf16, row-major [B][ldx] and transposed [nin][ldxt]: the
@@ -502,51 +552,18 @@ __global__ void rows_kernel(const float *X, uint32_t n, uint32_t nin, uint32_t B
     }
 }
 
-/* dY = bf16(2 (Y - T) scale), row-major [B][ldy] and transposed
- * [nout][ldyt]; sum (Y - T)^2 scale added to loss[0] or to the step's ring
- * slot. T rows are those of the minibatch (sampled) or the first B. A block
- * takes 64 rows. */
-__global__ void __launch_bounds__(256) loss_nt_kernel(const float *Y, const float *T, uint32_t nt,
-                                                      int sampled, uint64_t seed,
-                                                      const uint32_t *step, uint32_t B,
-                                                      uint32_t nout, float scale, uint16_t *dY,
-                                                      int ldy, uint16_t *dYT, int ldyt,
-                                                      float *loss, uint32_t ring) {
-    __shared__ uint32_t rows[64];
-    __shared__ float part[256];
-    const uint32_t r0 = blockIdx.x * 64;
-    if (threadIdx.x < 64) {
-        const uint32_t i = r0 + threadIdx.x;
-        rows[threadIdx.x] = i < B ? (sampled ? sample_row(seed, *step, i, nt) : i) : 0u;
-    }
-    __syncthreads();
-    float s = 0.f;
-    for (uint32_t e = threadIdx.x; e < 64u * nout; e += 256) {
-        const uint32_t rr = e & 63u, c = e >> 6, i = r0 + rr;
-        if (i >= B) continue;
-        const float d = Y[(size_t)i * nout + c] - T[(size_t)rows[rr] * nout + c];
-        s += d * d;
-        const uint16_t gv = f2bf(2.f * d * scale);
-        dY[(size_t)i * ldy + c] = gv;
-        dYT[(size_t)c * ldyt + i] = gv;
-    }
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(ring ? loss + (*step % ring) : loss, part[0] * scale);
-}
-
 constexpr uint64_t kNotMatrix = ~0ull;
 
+constexpr int kAdamBlocks = 256;
+
 /* Adam (t = *step + 1) on the sum of the gradient's split partials, added
- * in split order; refreshes both bf16 copies of the weight matrices. */
+ * in split order; refreshes both bf16 copies of the weight matrices. The
+ * last block to finish (counted in *done, which it returns to 0) advances
+ * the step counter, after every block has read it. */
 __global__ void adam_nt_kernel(float *w, float *m, float *v, const float *part, uint32_t splits,
-                               uint64_t n, const uint32_t *step, float lr, float b1, float b2,
-                               float eps, const uint64_t *bf_index, const uint64_t *bt_index,
-                               uint16_t *wbf, uint16_t *wtb) {
+                               uint64_t n, uint32_t *step, uint32_t *done, float lr, float b1,
+                               float b2, float eps, const uint64_t *bf_index,
+                               const uint64_t *bt_index, uint16_t *wbf, uint16_t *wtb) {
     const float t = (float)(*step + 1u);
     const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -564,6 +581,13 @@ __global__ void adam_nt_kernel(float *w, float *m, float *v, const float *part, 
             wbf[bf_index[i]] = h;
             wtb[bt_index[i]] = h;
         }
+    }
+    /* every thread of the block read *step above; the next launch sees the
+     * increment (kernel boundary), so no fence is needed */
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(done, 1u) == gridDim.x - 1u) {
+        *step += 1u;
+        *done = 0u;
     }
 }
 
@@ -624,6 +648,7 @@ struct xo_mlp {
     float *loss = nullptr;            /* per training step */
     uint32_t loss_cap = 0;
     uint32_t *step = nullptr;
+    uint32_t *done = nullptr;         /* Adam blocks finished this step */
     float *data_x = nullptr, *data_t = nullptr;
     uint32_t ndata = 0;
     hipGraph_t graph = nullptr;
@@ -664,9 +689,23 @@ template <int EPI> int nt_launch(xo_mlp *N, NtProblem g) {
     return 0;
 }
 
-/* Forward pass of B rows already in xr/xt: hidden activations in hr/ht,
- * output fp32 in y. */
-int forward(xo_mlp *N, uint32_t B) {
+/* The loss the last layer of a training forward pass is scored against:
+ * target rows T (those of the step's minibatch when sampled, else the
+ * first B of nt), the loss written to loss[0] (ring 0) or to the step's
+ * ring slot. */
+struct LossSpec {
+    const float *T;
+    uint32_t nt;
+    int sampled;
+    uint64_t seed;
+    float *loss;
+    uint32_t ring;
+};
+
+/* Forward pass of B rows already in xr/xt: hidden activations in hr/ht;
+ * the output fp32 in y, or with a loss the output gradient in dy/dyt and
+ * the loss. */
+int forward(xo_mlp *N, uint32_t B, const LossSpec *loss = nullptr) {
     const int L = N->nlayers;
     for (int l = 0; l < L; ++l) {
         NtProblem g;
@@ -686,6 +725,20 @@ int forward(xo_mlp *N, uint32_t B) {
             g.HT = N->ht[l];
             g.ldht = N->ldb;
             if (nt_launch<NT_RELU>(N, g)) return -1;
+        } else if (loss) {
+            g.H = N->dy;
+            g.ldh = N->ld[L];
+            g.HT = N->dyt;
+            g.ldht = N->ldb;
+            g.T = loss->T;
+            g.nt = loss->nt;
+            g.sampled = loss->sampled;
+            g.seed = loss->seed;
+            g.step = N->step;
+            g.loss = loss->loss;
+            g.ring = loss->ring;
+            g.scale = 1.0f / (float)(B * (uint32_t)N->dims[L]);
+            if (nt_launch<NT_LOSS>(N, g)) return -1;
         } else {
             g.C = N->y;
             g.ldc = N->dims[L];
@@ -732,18 +785,11 @@ int dw_table(xo_mlp *N, uint32_t B, const NtProblem **table, int *blocks) {
     return 0;
 }
 
-/* Loss and gradient of the B rows of the last forward pass against T
- * (the step's minibatch rows when sampled, else the first B rows): loss
- * added to loss[0] (ring 0) or to the step's ring slot, gradient as split
- * partial sums in part. */
-int backward(xo_mlp *N, uint32_t B, const float *T, uint32_t nt, int sampled, uint64_t seed,
-             float *loss, uint32_t ring) {
+/* Gradient of the loss of the last forward pass (run with a LossSpec,
+ * which left the output gradient in dy/dyt), as split partial sums in
+ * part. */
+int backward(xo_mlp *N, uint32_t B) {
     const int L = N->nlayers;
-    const uint32_t nout = (uint32_t)N->dims[L];
-    hipLaunchKernelGGL(loss_nt_kernel, dim3((B + 63) / 64), dim3(256), 0, N->stream, N->y, T,
-                       nt, sampled, seed, N->step, B, nout, 1.0f / (float)(B * nout), N->dy,
-                       N->ld[L], N->dyt, N->ldb, loss, ring);
-    MLP_CHECK(hipGetLastError());
     const uint16_t *dcur = N->dy;
     for (int l = L - 1; l >= 1; --l) {
         /* dZ_{l-1} = (dZ_l W_l^T) * (H_{l-1} > 0) */
@@ -761,8 +807,8 @@ int backward(xo_mlp *N, uint32_t B, const float *T, uint32_t nt, int sampled, ui
         g.ldh = N->ld[l];
         g.HT = N->dzt[l - 1];
         g.ldht = N->ldb;
-        g.mask = N->hr[l - 1];
-        g.ldm = N->ld[l];
+        g.mask = N->ht[l - 1];
+        g.ldm = N->ldb;
         if (nt_launch<NT_MASK>(N, g)) return -1;
         dcur = N->dz[l - 1];
     }
@@ -789,13 +835,12 @@ int train_step(xo_mlp *N, uint32_t B, uint64_t seed, float lr) {
                        N->ndata, (uint32_t)N->dims[0], B, 1, seed, N->step, N->xr, N->ld[0],
                        N->xt, N->ldb);
     MLP_CHECK(hipGetLastError());
-    if (forward(N, B)) return -1;
-    if (backward(N, B, N->data_t, N->ndata, 1, seed, N->loss, N->loss_cap)) return -1;
-    hipLaunchKernelGGL(adam_nt_kernel, dim3(256), dim3(256), 0, N->stream, N->w, N->m, N->v,
-                       N->part, batch_splits(N, B), N->nparams, N->step, lr, 0.9f, 0.999f,
-                       1e-8f, N->bf_index, N->bt_index, N->wbf, N->wtb);
-    MLP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(step_kernel, dim3(1), dim3(64), 0, N->stream, N->step);
+    const LossSpec loss{N->data_t, N->ndata, 1, seed, N->loss, N->loss_cap};
+    if (forward(N, B, &loss)) return -1;
+    if (backward(N, B)) return -1;
+    hipLaunchKernelGGL(adam_nt_kernel, dim3(kAdamBlocks), dim3(256), 0, N->stream, N->w, N->m,
+                       N->v, N->part, batch_splits(N, B), N->nparams, N->step, N->done, lr, 0.9f,
+                       0.999f, 1e-8f, N->bf_index, N->bt_index, N->wbf, N->wtb);
     MLP_CHECK(hipGetLastError());
     return 0;
 }
@@ -854,7 +899,8 @@ void xo_mlp_destroy(xo_mlp *N) {
     if (N->graph) (void)hipGraphDestroy(N->graph);
     void *bufs[] = {N->w,   N->m,    N->v,    N->grad,   N->wbf,    N->wtb,  N->bf_index,
                     N->bt_index, N->xf, N->tb, N->xr,    N->xt,     N->dy,   N->dyt,
-                    N->y,   N->part, N->loss, N->step,   N->data_x, N->data_t, N->wt};
+                    N->y,   N->part, N->loss, N->step,   N->done, N->data_x, N->data_t,
+                    N->wt};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto *list : {&N->hr, &N->ht, &N->dz, &N->dzt})
@@ -928,6 +974,7 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
         {(void **)&N->y, 4 * B * dims[nlayers]},
         {(void **)&N->part, 4ull * N->max_splits * P},
         {(void **)&N->step, 4},
+        {(void **)&N->done, 4},
     };
     for (auto &a : allocs) {
         if ((e = hipMalloc(a.p, a.bytes)) != hipSuccess) return bad("allocation", e);
@@ -1067,8 +1114,9 @@ int xo_mlp_gradient(xo_mlp *N, const float *x, const float *t, uint32_t n, float
     float *dloss = nullptr;
     MLP_CHECK(hipMalloc(&dloss, 4));
     int rc = 0;
-    if (hipMemsetAsync(dloss, 0, 4, N->stream) != hipSuccess || forward(N, n) ||
-        backward(N, n, N->tb, n, 0, 0ull, dloss, 0))
+    const LossSpec spec{N->tb, n, 0, 0ull, dloss, 0u};
+    if (hipMemsetAsync(dloss, 0, 4, N->stream) != hipSuccess || forward(N, n, &spec) ||
+        backward(N, n))
         rc = -1;
     if (rc == 0) {
         hipLaunchKernelGGL(sum_partials_kernel, dim3(256), dim3(256), 0, N->stream, N->part,

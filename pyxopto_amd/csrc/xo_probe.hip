@@ -41,6 +41,28 @@ __global__ void __launch_bounds__(256) atomic_kernel(unsigned long long *__restr
     }
 }
 
+/* The atomic_kernel with the targets drawn from the first `hot` elements,
+ * each wave adding into the copy of the array of the XCD it runs on (the
+ * wave's HW_REG_XCC_ID, bits 3:0; eight copies of nelem) with
+ * workgroup-scope atomics, which the XCD's L2 performs for all of its
+ * CUs. */
+__global__ void __launch_bounds__(256) atomic_xcd_kernel(unsigned long long *__restrict__ arr,
+                                                         uint64_t nelem, uint64_t hot,
+                                                         uint64_t total, uint32_t per_thread,
+                                                         uint64_t seed) {
+    const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u;
+    unsigned long long *mine = arr + (uint64_t)xcc * nelem;
+    for (uint32_t k = 0; k < per_thread; ++k) {
+        const uint64_t j = (uint64_t)k * nthreads + t;
+        if (j >= total) break;
+        const uint64_t h = xo_mix64(seed + j * 0x9E3779B97F4A7C15ull);
+        __hip_atomic_fetch_add(&mine[h % hot], 1ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
 /* Eight independent fused multiply-add chains per lane, `iters` rounds:
  * 8 iters lane-ops per thread, no memory traffic but the final store. */
 __global__ void __launch_bounds__(256) valu_kernel(float *out, uint32_t iters, float a) {
@@ -195,53 +217,77 @@ int xo_probe_copy(xo_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms_per_cop
     return rc;
 }
 
-int xo_probe_atomic_u64(xo_ctx *ctx, uint64_t array_bytes, uint64_t natomics, uint64_t seed,
-                        double *ms, uint64_t *checksum) {
+/* natomics random u64 adds into a zeroed array of array_bytes (eight of
+ * them, one per XCD, when per_xcd), the targets among the first hot_bytes:
+ * kernel ms and the sum of everything allocated. */
+static int atomic_probe(xo_ctx *ctx, const char *fn, uint64_t array_bytes, uint64_t hot_bytes,
+                        uint64_t natomics, uint64_t seed, int per_xcd, double *ms,
+                        uint64_t *checksum) {
     xo::clear_error();
-    if (!ctx || !ms || !checksum || array_bytes < 8 || natomics == 0) {
-        xo::set_error("xo_probe_atomic_u64: invalid argument");
+    if (!ctx || !ms || !checksum || array_bytes < 8 || hot_bytes < 8 ||
+        hot_bytes > array_bytes || natomics == 0) {
+        xo::set_error("%s: invalid argument", fn);
         return -1;
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
     XO_HIP(hipSetDevice(ctx->device));
-    const uint64_t nelem = array_bytes / 8;
+    const uint64_t nelem = array_bytes / 8, hot = hot_bytes / 8;
+    const uint64_t ncopies = per_xcd ? 8u : 1u;
+    const uint64_t nall = nelem * ncopies;
     unsigned long long *arr = nullptr;
-    XO_HIP(hipMalloc(&arr, nelem * 8));
+    XO_HIP(hipMalloc(&arr, nall * 8));
     const uint32_t blocks = ctx->cus * 8;
     const uint64_t nthreads = (uint64_t)blocks * 256;
     const uint64_t per = (natomics + nthreads - 1) / nthreads;
     int rc = 0;
-    hipError_t e = hipMemsetAsync(arr, 0, nelem * 8, ctx->stream);
+    hipError_t e = hipMemsetAsync(arr, 0, nall * 8, ctx->stream);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(atomic_kernel, dim3(blocks), dim3(256), 0, ctx->stream, arr, nelem,
-                           natomics, (uint32_t)per, seed);
+        if (per_xcd)
+            hipLaunchKernelGGL(atomic_xcd_kernel, dim3(blocks), dim3(256), 0, ctx->stream, arr,
+                               nelem, hot, natomics, (uint32_t)per, seed);
+        else
+            hipLaunchKernelGGL(atomic_kernel, dim3(blocks), dim3(256), 0, ctx->stream, arr,
+                               hot, natomics, (uint32_t)per, seed);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     float kms = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1);
-    /* every atomic landed: the array sums to natomics */
+    /* every atomic landed: everything sums to natomics */
     uint64_t sum = 0;
     if (e == hipSuccess) {
         const uint64_t chunk = 1u << 20;
         uint64_t *host = new uint64_t[chunk];
-        for (uint64_t off = 0; off < nelem && e == hipSuccess; off += chunk) {
-            uint64_t m = nelem - off < chunk ? nelem - off : chunk;
+        for (uint64_t off = 0; off < nall && e == hipSuccess; off += chunk) {
+            uint64_t m = nall - off < chunk ? nall - off : chunk;
             e = hipMemcpy(host, arr + off, m * 8, hipMemcpyDeviceToHost);
             for (uint64_t i = 0; i < m && e == hipSuccess; ++i) sum += host[i];
         }
         delete[] host;
     }
     if (e != hipSuccess) {
-        xo::set_error("xo_probe_atomic_u64: %s", hipGetErrorString(e));
+        xo::set_error("%s: %s", fn, hipGetErrorString(e));
         rc = -1;
     }
     *ms = kms;
     *checksum = sum;
     (void)hipFree(arr);
     return rc;
+}
+
+int xo_probe_atomic_u64(xo_ctx *ctx, uint64_t array_bytes, uint64_t natomics, uint64_t seed,
+                        double *ms, uint64_t *checksum) {
+    return atomic_probe(ctx, "xo_probe_atomic_u64", array_bytes, array_bytes, natomics, seed,
+                        0, ms, checksum);
+}
+
+int xo_probe_atomic_u64_xcd(xo_ctx *ctx, uint64_t array_bytes, uint64_t hot_bytes,
+                            uint64_t natomics, uint64_t seed, int per_xcd, double *ms,
+                            uint64_t *checksum) {
+    return atomic_probe(ctx, "xo_probe_atomic_u64_xcd", array_bytes, hot_bytes, natomics, seed,
+                        per_xcd, ms, checksum);
 }
 
 int xo_probe_cr_math(xo_ctx *ctx, uint64_t ndiv, uint64_t seed, uint64_t *sqrt_checked,
