@@ -8,7 +8,9 @@
  * of one wave may be on different instances and no lane waits for a slow
  * packet of another. Each lane carries its instance index; the instance's
  * layers (64 B each, a few instances per wave at a time) are read through
- * the L1/L2 caches, detector deposits are u64 atomics to HBM.
+ * the L1/L2 caches (the per-step fields of the current layer stay in
+ * registers between boundary events), detector deposits are u64 atomics
+ * to HBM.
  *
  * Each packet's MWC generator is keyed on (instance seed, packet index)
  * (xo_seedhash.h), so the integer accumulators of an instance are a pure
@@ -27,21 +29,21 @@ struct BatchBlockStats {
     unsigned long long v[XO_STAT_COUNT];
 };
 
-/* Phase-function sampling by the instance's kind; each branch reads only
- * the layer fields its kind uses. */
+/* Phase-function sampling by the instance's kind. */
 template <class M>
-XO_DEV float pf_sample_rt(int kind, const LayerL &L, const float *lut, uint64_t &x,
-                          uint32_t a, float &xi_fi) {
-    if (kind == XO_PF_HG) {
-        const PfL pf{L.g, 0.0f, 0.0f, 0.0f, 0.0f, 0u, 0u};
-        return pf_sample<XO_PF_HG, M>(pf, lut, x, a, xi_fi);
-    }
-    if (kind == XO_PF_MHG) {
-        const PfL pf{L.g, L.beta, 0.0f, 0.0f, 0.0f, 0u, 0u};
-        return pf_sample<XO_PF_MHG, M>(pf, lut, x, a, xi_fi);
-    }
-    const PfL pf{0.0f, 0.0f, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
+XO_DEV float pf_sample_rt(int kind, const PfL &pf, const float *lut, uint64_t &x, uint32_t a,
+                          float &xi_fi) {
+    if (kind == XO_PF_HG) return pf_sample<XO_PF_HG, M>(pf, lut, x, a, xi_fi);
+    if (kind == XO_PF_MHG) return pf_sample<XO_PF_MHG, M>(pf, lut, x, a, xi_fi);
     return pf_sample<XO_PF_LUT, M>(pf, lut, x, a, xi_fi);
+}
+
+/* The per-step fields of a packet's current layer, kept in registers: top,
+ * bottom, 1/mut and mua/mut (the first 16 bytes of the record) and the
+ * phase-function parameters. */
+XO_DEV void layer_state(const LayerL &Q, float4 &lt, PfL &pf) {
+    lt = *reinterpret_cast<const float4 *>(&Q);
+    pf = PfL{Q.g, Q.beta, Q.la, Q.lb, Q.lc, Q.loff, Q.lsize_m1};
 }
 
 /* Pending events of the split transport step. */
@@ -113,6 +115,12 @@ batch_kernel(const BatchParams P) {
     uint64_t x = 0;
     uint32_t a = 0;
     uint32_t pkt_steps = 0;
+    /* the per-step fields of the packet's current layer (layer_state):
+     * read again only when the layer may change (a new packet, a boundary
+     * event), not in every step (+14 % packets/s; the compiler spills a
+     * few rarely used registers to stay at 64) */
+    float4 lt = make_float4(0.f, 0.f, 0.f, 0.f);
+    PfL lpf{0.f, 0.f, 0.f, 0.f, 0.f, 0u, 0u};
     uint32_t *const n_trunc = &scount[3][threadIdx.x];
     uint32_t *const n_steps = &scount[1][threadIdx.x];
     uint32_t pend = kEvNone;  /* event the lane's last step ended on     */
@@ -201,6 +209,7 @@ batch_kernel(const BatchParams P) {
                 dz = H.dir_sample[2];
             }
             layer = 1;
+            layer_state(layers[1], lt, lpf);
             pkt_steps = 0;
             alive = true;
             atomicAdd(&scount[0][threadIdx.x], 1u);
@@ -298,6 +307,7 @@ batch_kernel(const BatchParams P) {
                     }
                     done = mbl_roulette(done);
                     pend = kEvNone;
+                    layer_state(layers[layer], lt, lpf);
                     end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, n_trunc,
                               n_steps);
                 } else {
@@ -309,22 +319,22 @@ batch_kernel(const BatchParams P) {
         }
         if (alive && pend == kEvNone) {
             /* (B) */
-            const LayerL L = layers[layer];
             float step;
             if (METHOD == XO_METHOD_MBL)
-                step = M::div(-M::log_unit(rng_next(x, a)), L.mus); /* mcml.template.c:529-531 */
+                step = M::div(-M::log_unit(rng_next(x, a)),
+                              layers[layer].mus); /* mcml.template.c:529-531 */
             else
-                step = -M::log_unit(rng_next(x, a)) * L.inv_mut;
+                step = -M::log_unit(rng_next(x, a)) * lt.z;
             step = fminf(step, XO_FLT_MAX);
             const float zend = __builtin_fmaf(step, dz, pz);
-            if (zend < L.top || zend >= L.bottom) {
+            if (zend < lt.x || zend >= lt.y) {
                 pend = kEvBoundary;
                 pstep = step;
             } else {
                 px = __builtin_fmaf(dx, step, px);
                 py = __builtin_fmaf(dy, step, py);
                 pz = zend;
-                if (METHOD == XO_METHOD_MBL) mbl_absorb(L, step);
+                if (METHOD == XO_METHOD_MBL) mbl_absorb(layers[layer], step);
                 pend = kEvInteraction;
             }
         }
@@ -332,19 +342,19 @@ batch_kernel(const BatchParams P) {
             /* (C) absorption and scattering, mcml.template.c:696-753 */
             const LayerL &L = layers[layer];
             bool done = false;
-            if (METHOD == XO_METHOD_AR && rng_next(x, a) < L.mua_inv_mut) {
+            if (METHOD == XO_METHOD_AR && rng_next(x, a) < lt.w) {
                 /* albedo rejection: the whole weight is absorbed here */
                 if (FLU) fluence_at(px, py, pz, w, L.mua);
                 w = 0.0f;
                 done = true;
             } else {
                 if (METHOD == XO_METHOD_AW) {
-                    float deposit = w * L.mua_inv_mut;
+                    float deposit = w * lt.w;
                     w -= deposit;
                     if (FLU) fluence_at(px, py, pz, deposit, L.mua);
                 }
                 float xi_fi;
-                float ct = pf_sample_rt<M>(pf_kind, L, P.lut, x, a, xi_fi);
+                float ct = pf_sample_rt<M>(pf_kind, lpf, P.lut, x, a, xi_fi);
                 scatter_direction<M>(dx, dy, dz, ct, xi_fi);
                 if (METHOD == XO_METHOD_AW)
                     done = roulette(w, x, a, P.w_min, P.chance, P.use_lottery);

@@ -10,9 +10,14 @@
  * 8 elements. Activations between layers are bf16, rounded to nearest
  * even; the network output and the loss are fp32.
  *
- * Training step (captured once as a hipGraph): minibatch rows, one product
- * per layer forward, the loss, one product per layer for the data
- * gradients, one launch for every layer's weight and bias gradients, Adam.
+ * Training step (captured once as a hipGraph). Networks of input width
+ * <= 32, hidden widths 256, output width <= 128 and at most six layers:
+ * two fused launches, the forward and backward passes of 32-row blocks
+ * with the activations resident in LDS, then every layer's weight and bias
+ * gradients over the whole batch with Adam applied in place. Other shapes:
+ * minibatch rows, one product per layer forward (the loss fused into the
+ * last), one product per layer for the data gradients, one launch for
+ * every layer's weight and bias gradients, Adam.
  * Inference: one fused kernel with the activations resident in LDS.
  */
 #include <hip/hip_runtime.h>
@@ -279,19 +284,28 @@ mlp_fused_kernel(const FusedArgs f) {
     fused_layer<8, 2, true>(f, f.nlayers - 1, hb, row0, lane, wave);
 }
 
-/* The weights of one layer as bf16 MFMA fragments: for 16-column tile t,
- * 32-wide k step s and lane L, the 8 values W[k][n] with
+/* A matrix M[k][n] (k < kdim, n < ndim) as bf16 MFMA fragments: for
+ * 16-column tile t, 32-wide k step s and lane L, the 8 values M[k][n] with
  * n = 16 t + (L & 15), k = 32 s + 8 (L >> 4) + e (e = 0..7) are stored at
- * ((t * ksteps + s) * 64 + L) * 8 + e, zero for k >= in or n >= out. A
- * wave's fragment of one tile and step is then 1 KB of whole cache lines. */
-__global__ void fragment_pad_kernel(const float *w, int in, int out, int ksteps, int ntiles,
-                                    uint16_t *dst) {
+ * ((t * ksteps + s) * 64 + L) * 8 + e, zero for k >= kdim or n >= ndim. A
+ * wave's fragment of one tile and step is then 1 KB of whole cache lines.
+ * M is the layer's weights W[in][out] (trans = 0: k = in, n = out) or
+ * their transpose (trans = 1: k = out, n = in); ldw is the row length of W
+ * as stored (out). */
+__host__ __device__ inline uint64_t fragment_index(int k, int n, int ksteps) {
+    return ((uint64_t)((n >> 4) * ksteps + (k >> 5)) * 64 + (n & 15) + 16 * ((k & 31) >> 3)) * 8 +
+           (k & 7);
+}
+
+__global__ void fragment_pad_kernel(const float *w, int kdim, int ndim, int ldw, int trans,
+                                    int ksteps, int ntiles, uint16_t *dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntiles * ksteps * 512) return;
     const int e = i & 7, L = (i >> 3) & 63, ts = i >> 9;
     const int t = ts / ksteps, st = ts % ksteps;
     const int n = t * 16 + (L & 15), k = st * 32 + (L >> 4) * 8 + e;
-    dst[i] = (k < in && n < out) ? f2bf(w[(size_t)k * out + n]) : (uint16_t)0;
+    const size_t src = trans ? (size_t)n * ldw + k : (size_t)k * ldw + n;
+    dst[i] = (k < kdim && n < ndim) ? f2bf(w[src]) : (uint16_t)0;
 }
 
 
@@ -613,6 +627,595 @@ __global__ void refresh_kernel(const float *w, uint64_t n, const uint64_t *bf_in
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Fused training step: two launches                                    */
+/* ------------------------------------------------------------------ */
+/* For the networks of the fused inference kernel with at most TMAXL
+ * layers. train_rows_kernel takes TBM minibatch rows per block through the
+ * forward pass, the loss and the data gradients with every activation in
+ * LDS, and writes what the weight gradients read: each layer's input H_l
+ * and output gradient dZ_l, bf16, transposed (one row per unit, the batch
+ * contiguous). The products are those of the layer-by-layer path in the
+ * same k order (weights as fragment-major copies, W_l for the forward pass
+ * and W_l^T for the data gradients), so activations, loss gradient and data
+ * gradients are the same bits. train_weights_kernel forms each 32x32 tile
+ * of every dW_l over the whole batch in one block: its four waves take a
+ * quarter of the batch each, with the loads of the next WG k steps in
+ * flight while the MFMAs of the current WG issue, and their sums are
+ * added in wave order in LDS (deterministic; only the order of additions
+ * differs from the split sums of the layer-by-layer path). Blocks of the
+ * first row tile also form the bias gradient (a row of ones times the dZ_l
+ * fragments). The tile's gradient then goes to Adam in place (or out, for
+ * xo_mlp_gradient), which refreshes both fragment-major copies of the
+ * updated weights. The loss: each row block writes its partial sum, the
+ * first block of train_weights_kernel adds them in a fixed order (no
+ * atomics: a step's loss is deterministic). */
+constexpr int TMT = 2, TBM = 16 * TMT, TMAXL = 6;
+/* waves per block of train_rows_kernel: wave w owns 16 / TNW of the 16
+ * column tiles of a hidden-width product, 8 / TNW of the output layer's */
+constexpr int TNW = 4, TNT = 64 * TNW;
+constexpr int TXS = 40, TYS = 136; /* LDS row strides of the input and output tiles */
+enum TrainEpi : int { TE_RELU = 0, TE_LOSS = 1, TE_MASK = 2 };
+
+struct TrainArgs {
+    const float *X;          /* [n][nin] rows                                  */
+    const float *T;          /* [n][nout] targets                              */
+    uint32_t n;
+    int sampled;             /* the step's minibatch rows, else rows 0..B-1    */
+    uint64_t seed;
+    const uint32_t *step;
+    float *loss_part;        /* per row block: its squared errors times scale  */
+    float scale;             /* 1 / (B nout)                                   */
+    int B, nin, nout, nlayers, ldb;
+    const uint16_t *wf[TMAXL]; /* fragments of W_l                             */
+    const uint16_t *wb[TMAXL]; /* fragments of W_l^T (l >= 1)                  */
+    const float *bias[TMAXL];
+    uint16_t *ht[TMAXL];       /* H_l^T, the input of layer l, and            */
+    uint16_t *dzt[TMAXL];      /* dZ_l^T, the gradient at its output: fragments
+                                  over the batch (store_transposed)            */
+};
+
+/* All of a wave's weight fragments of one product (KS k steps x NJ
+ * 16-column tiles), loaded one product ahead of their use so their L2
+ * latency is hidden behind the current product. */
+constexpr int TKS = 8, TNJ = 16 / TNW;
+typedef bf16x8 TrainFrags[TKS][TNJ];
+
+/* Column tile j of a wave: hidden-width products (16 tiles) take
+ * consecutive tiles, the output layer (8 tiles) interleaves them. */
+template <bool OUT, int NJ> __device__ __forceinline__ int train_tile(int wave, int j) {
+    return OUT ? wave + TNW * j : wave * NJ + j;
+}
+
+template <int KS, int NJ, bool OUT>
+__device__ __forceinline__ void train_wload(const uint16_t *W, int lane, int wave,
+                                            TrainFrags &f) {
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int tile = train_tile<OUT, NJ>(wave, j);
+            f[s][j] = *reinterpret_cast<const bf16x8 *>(W + (((size_t)tile * KS + s) * 64 + lane) * 8);
+        }
+}
+
+/* Targets of the loss product, per lane: rows 16 i + l % 16 of the block,
+ * the four columns of each of its two column tiles (zero past nout and for
+ * rows past B). Loaded before the next product's weights are requested, so
+ * waiting for them does not wait for those. */
+constexpr int TNJO = 8 / TNW; /* output-layer column tiles per wave */
+typedef float4 TrainTargets[TMT][TNJO];
+
+__device__ __forceinline__ void train_targets(const TrainArgs &a, int row0, uint32_t stp,
+                                              int lane, int wave, TrainTargets &tv) {
+    const int fr = lane & 15, nq = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < TMT; ++i) {
+        const int grow = row0 + i * 16 + fr;
+        const uint32_t trow = grow < a.B ? (a.sampled ? sample_row(a.seed, stp, (uint32_t)grow, a.n)
+                                                      : (uint32_t)grow)
+                                         : 0u;
+        const float *t = a.T + (size_t)trow * a.nout;
+#pragma unroll
+        for (int j = 0; j < TNJO; ++j) {
+            const int n0 = train_tile<true, TNJO>(wave, j) * 16 + nq;
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = (grow < a.B && n0 + r < a.nout) ? t[n0 + r] : 0.f;
+            tv[i][j] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    }
+}
+
+/* One product of a TBM-row block: src (LDS, row stride ss) times the
+ * fragments f, written as bf16 to dst (row stride ds): bias (LDS) + ReLU,
+ * or the loss gradient 2 (y - t) scale against the targets tv (sq collects
+ * the squared errors), or the product masked by mask > 0. Operand order
+ * and result layout as fused_layer: lane l holds four consecutive columns
+ * of row l % 16. */
+template <int KS, int NJ, int EPI, bool OUT>
+__device__ __forceinline__ void train_mma(const TrainArgs &a, const TrainFrags &f,
+                                          const uint16_t *src, int ss, uint16_t *dst, int ds,
+                                          const float *bias, const uint16_t *mask,
+                                          const TrainTargets &tv, int row0, int lane, int wave,
+                                          float &sq) {
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    floatx4 acc[TMT][NJ];
+#pragma unroll
+    for (int i = 0; i < TMT; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 x[2][TMT];
+#pragma unroll
+    for (int i = 0; i < TMT; ++i)
+        x[0][i] = *reinterpret_cast<const bf16x8 *>(&src[(i * 16 + fr) * ss + fk]);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        if (s + 1 < KS) {
+#pragma unroll
+            for (int i = 0; i < TMT; ++i)
+                x[(s + 1) & 1][i] = *reinterpret_cast<const bf16x8 *>(
+                    &src[(i * 16 + fr) * ss + (s + 1) * 32 + fk]);
+        }
+#pragma unroll
+        for (int i = 0; i < TMT; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f[s][j], x[s & 1][i],
+                                                                   acc[i][j], 0, 0, 0);
+    }
+    const int nq = (lane >> 4) * 4;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int n0 = train_tile<OUT, NJ>(wave, j) * 16 + nq;
+#pragma unroll
+        for (int i = 0; i < TMT; ++i) {
+            const int row = i * 16 + fr;
+            float z[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+            if (EPI == TE_RELU) {
+                const float4 bv = *reinterpret_cast<const float4 *>(bias + n0);
+                const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float v = z[r] + bb[r];
+                    z[r] = v > 0.f ? v : 0.f;
+                }
+            } else if (EPI == TE_LOSS) {
+                const int grow = row0 + row;
+                if (grow < a.B) {
+                    const int jo = j < TNJO ? j : 0;
+                    const float t[4] = {tv[i][jo].x, tv[i][jo].y, tv[i][jo].z, tv[i][jo].w};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        if (n0 + r < a.nout) {
+                            const float d = (z[r] + bias[n0 + r]) - t[r];
+                            sq += d * d;
+                            z[r] = 2.f * d * a.scale;
+                        } else {
+                            z[r] = 0.f;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) z[r] = 0.f;
+                }
+            } else {
+                const uint2 mk = *reinterpret_cast<const uint2 *>(&mask[row * FSTRIDE + n0]);
+                const uint16_t mv[4] = {(uint16_t)mk.x, (uint16_t)(mk.x >> 16), (uint16_t)mk.y,
+                                        (uint16_t)(mk.y >> 16)};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) z[r] = bf2f(mv[r]) > 0.f ? z[r] : 0.f;
+            }
+            uint2 pk;
+            pk.x = bf16_pair(z[0], z[1]);
+            pk.y = bf16_pair(z[2], z[3]);
+            *reinterpret_cast<uint2 *>(&dst[row * ds + n0]) = pk;
+        }
+    }
+}
+
+/* Columns [0, ncols) of a TBM-row LDS tile, transposed (unit c, batch
+ * rows k) into dstT as MFMA fragments over the batch: unit c and batch rows
+ * [8 q, 8 q + 8) are the 16 bytes at fragment_index(8 q, c, nks) (nks: k
+ * steps of 32 batch rows per 16-unit tile), so the weight-gradient kernel
+ * reads each 16 x 32 fragment as 1 KB of whole cache lines. A thread of the
+ * block's TNT threads takes four units of eight rows: eight 8-byte LDS reads
+ * (a wave reads whole rows), a 4 x 8 transpose in registers and four
+ * consecutive 16-byte slots. ncols is a multiple of 4 and the tile's
+ * columns up to the next multiple of 4 are readable. */
+__device__ inline void store_transposed(const uint16_t *tile, int stride, int ncols,
+                                        uint16_t *dstT, int nks, int row0) {
+    constexpr int G = TBM / 8; /* 8-row groups per block */
+    const int nq = (ncols + 3) >> 2;
+    for (int it = threadIdx.x; it < nq * G; it += TNT) {
+        const int q = it % nq, g8 = (it / nq) * 8, c = 4 * q;
+        uint2 r[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            r[e] = *reinterpret_cast<const uint2 *>(&tile[(g8 + e) * stride + c]);
+        /* unit u of row e: 16 bits of r[e] (u = 0, 1 in .x, 2, 3 in .y) */
+        uint4 out[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            uint32_t h[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t wv = u < 2 ? r[e].x : r[e].y;
+                h[e] = (u & 1) ? (wv >> 16) : (wv & 0xFFFFu);
+            }
+            out[u] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
+                                h[6] | (h[7] << 16));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c + u < ncols)
+                *reinterpret_cast<uint4 *>(dstT + fragment_index(row0 + g8, c + u, nks)) = out[u];
+    }
+}
+
+/* The LDS tiles of a row block: the input rows, each hidden layer's
+ * output, the loss gradient, two data-gradient tiles, and every layer's
+ * bias (FHID floats each). */
+struct TrainTiles {
+    uint16_t *x, *h[TMAXL - 1], *y, *d[2];
+    float *bias;
+};
+
+/* Product I of a network of L layers: the forward products of layers
+ * 0..L-1, then the data gradients through layers L-1..1. */
+template <int L, int I> struct TrainOp {
+    static constexpr bool fwd = I < L;
+    static constexpr int layer = fwd ? I : 2 * L - 1 - I;
+    static constexpr int KS = fwd ? (layer == 0 ? 1 : 8) : (layer == L - 1 ? 4 : 8);
+    static constexpr bool OUT = fwd && layer == L - 1;
+    static constexpr int NJ = OUT ? TNJO : TNJ;
+    static constexpr int EPI = fwd ? (layer == L - 1 ? TE_LOSS : TE_RELU) : TE_MASK;
+};
+
+template <int L, int I>
+__device__ __forceinline__ void train_ops(const TrainArgs &a, TrainFrags (&f)[2],
+                                          const TrainTiles &t, const TrainTargets &tv, int row0,
+                                          int lane, int wave, float *wsq) {
+    if constexpr (I < 2 * L - 1) {
+        using O = TrainOp<L, I>;
+        if constexpr (I + 1 < 2 * L - 1) {
+            using Nx = TrainOp<L, I + 1>;
+            train_wload<Nx::KS, Nx::NJ, Nx::OUT>(Nx::fwd ? a.wf[Nx::layer] : a.wb[Nx::layer], lane,
+                                                 wave, f[(I + 1) & 1]);
+        }
+        const int l = O::layer;
+        const uint16_t *src;
+        uint16_t *dst, *outT;
+        const uint16_t *mask = nullptr;
+        int ss, ds = FSTRIDE, ncols = FHID;
+        if constexpr (O::fwd) {
+            if constexpr (O::layer == 0) {
+                src = t.x;
+                ss = TXS;
+            } else {
+                src = t.h[O::layer - 1];
+                ss = FSTRIDE;
+            }
+            if constexpr (O::layer + 1 < L) {
+                dst = t.h[l];
+                outT = a.ht[l + 1];
+            } else {
+                dst = t.y;
+                ds = TYS;
+                outT = a.dzt[L - 1];
+                ncols = a.nout;
+            }
+        } else {
+            constexpr int b = I - L;
+            if constexpr (b == 0) {
+                src = t.y;
+                ss = TYS;
+            } else {
+                src = t.d[(b - 1) & 1];
+                ss = FSTRIDE;
+            }
+            dst = t.d[b & 1];
+            mask = t.h[O::layer - 1];
+            outT = a.dzt[O::layer - 1];
+        }
+        float sq = 0.f;
+        train_mma<O::KS, O::NJ, O::EPI, O::OUT>(a, f[I & 1], src, ss, dst, ds, t.bias + l * FHID, mask,
+                                        tv, row0, lane, wave, sq);
+        if constexpr (O::EPI == TE_LOSS) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+            if (lane == 0) wsq[wave] = sq;
+        }
+        __syncthreads();
+        if constexpr (O::EPI == TE_LOSS) {
+            /* the block's squared errors, waves added in order */
+            if (threadIdx.x == 0) {
+                float b = 0.f;
+                for (int w = 0; w < TNW; ++w) b += wsq[w];
+                a.loss_part[blockIdx.x] = b * a.scale;
+            }
+        }
+        store_transposed(dst, ds, ncols, outT, a.ldb / 32, row0);
+        train_ops<L, I + 1>(a, f, t, tv, row0, lane, wave, wsq);
+    }
+}
+
+template <int L>
+__global__ void __launch_bounds__(TNT) train_rows_kernel(const TrainArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t sx[TBM * TXS];
+    __shared__ __attribute__((aligned(16))) uint16_t sh[L - 1][TBM * FSTRIDE];
+    __shared__ __attribute__((aligned(16))) uint16_t sy[TBM * TYS];
+    __shared__ __attribute__((aligned(16))) uint16_t sd[2][TBM * FSTRIDE];
+    __shared__ __attribute__((aligned(16))) float sb[L * FHID];
+    __shared__ float wsq[TNW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row0 = blockIdx.x * TBM;
+    TrainFrags f[2];
+    train_wload<TrainOp<L, 0>::KS, TrainOp<L, 0>::NJ, TrainOp<L, 0>::OUT>(a.wf[0], lane, wave,
+                                                                          f[0]);
+    const uint32_t stp = a.sampled ? *a.step : 0u;
+    for (int e = tid; e < TBM * 32; e += TNT) {
+        const int m = e >> 5, c = e & 31, grow = row0 + m;
+        float v = 0.f;
+        if (grow < a.B && c < a.nin) {
+            const uint32_t j =
+                a.sampled ? sample_row(a.seed, stp, (uint32_t)grow, a.n) : (uint32_t)grow;
+            v = a.X[(size_t)j * a.nin + c];
+        }
+        sx[m * TXS + c] = f2bf(v);
+    }
+    /* the loss product's targets, requested now: their latency hides
+     * behind the forward products */
+    TrainTargets tv;
+    train_targets(a, row0, stp, lane, wave, tv);
+    for (int e = tid; e < L * FHID; e += TNT) {
+        const int l = e / FHID, c = e % FHID;
+        sb[e] = (l + 1 < L || c < a.nout) ? a.bias[l][c] : 0.f;
+    }
+    __syncthreads();
+    store_transposed(sx, TXS, a.nin, a.ht[0], a.ldb / 32, row0);
+    TrainTiles t;
+    t.x = sx;
+    for (int l = 0; l < L - 1; ++l) t.h[l] = sh[l];
+    for (int l = L - 1; l < TMAXL - 1; ++l) t.h[l] = nullptr;
+    t.y = sy;
+    t.d[0] = sd[0];
+    t.d[1] = sd[1];
+    t.bias = sb;
+    train_ops<L, 0>(a, f, t, tv, row0, lane, wave, wsq);
+}
+
+typedef void (*TrainRowsFn)(TrainArgs);
+/* train_rows_kernel<L> for L = 2..TMAXL */
+const TrainRowsFn kTrainRows[TMAXL + 1] = {nullptr,
+                                           nullptr,
+                                           train_rows_kernel<2>,
+                                           train_rows_kernel<3>,
+                                           train_rows_kernel<4>,
+                                           train_rows_kernel<5>,
+                                           train_rows_kernel<6>};
+
+struct WgLayer {
+    const uint16_t *A;       /* H_l^T, fragments over the batch  */
+    const uint16_t *BT;      /* dZ_l^T, fragments over the batch */
+    uint16_t *wf, *wb;       /* fragment copies of W_l and W_l^T (wb null for l = 0) */
+    uint64_t woff, boff;
+    int in, out, tiles_n, block0, ksf, ksb;
+};
+
+struct WgArgs {
+    WgLayer ly[TMAXL];
+    int nlayers, nks, ldb, adam;
+    float *grad;             /* adam == 0: the gradient, packed like w */
+    float *w, *m, *v;
+    uint32_t *step, *done;
+    float lr, b1, b2, eps;
+    const uint16_t *zrow;    /* 16 bytes of zeros, read for tiles past a layer's width */
+    const float *loss_part;  /* the row blocks' partial losses ...           */
+    int nparts;
+    float *loss;             /* ... added in order into loss[0], or loss[step % ring] */
+    uint32_t ring;
+};
+
+/* 32x32 tiles (about 170 blocks for the surrogate's layers); each wave
+ * keeps two groups of WG k steps in flight */
+constexpr int WTI = 2, WTJ = 2, WG = 8;
+constexpr int WTM = 16 * WTI, WTN = 16 * WTJ, WEPT = WTM * WTN / 256;
+
+/* The k loop of a weight-gradient tile: steps [ks0, ks1) of 32 batch rows,
+ * two groups of WG steps in flight. With BIAS, a row of ones times the
+ * dZ_l fragments also accumulates the bias gradient (every row of accb
+ * holds the column sums). */
+template <bool BIAS>
+__device__ __forceinline__ void wg_loop(const uint16_t *const (&ap)[WTI], const int (&ainc)[WTI],
+                                        const uint16_t *const (&bp)[WTJ], const int (&binc)[WTJ],
+                                        int ks0, int ks1,
+                                        floatx4 (&acc)[WTI][WTJ], floatx4 (&accb)[WTJ]) {
+    bf16x8 fa[2][WG][WTI], fb[2][WG][WTJ];
+    bf16x8 ones;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
+    /* steps past ks1 read step ks1 - 1 again (in bounds) and contribute
+     * nothing: their dZ fragments are replaced by zeros */
+    auto load = [&](int buf, int s0) {
+#pragma unroll
+        for (int u = 0; u < WG; ++u) {
+            const int s = s0 + u;
+            const int sc = s < ks1 ? s : ks1 - 1;
+#pragma unroll
+            for (int i = 0; i < WTI; ++i)
+                fa[buf][u][i] = *reinterpret_cast<const bf16x8 *>(ap[i] + (size_t)sc * ainc[i]);
+#pragma unroll
+            for (int j = 0; j < WTJ; ++j) {
+                const bf16x8 v = *reinterpret_cast<const bf16x8 *>(bp[j] + (size_t)sc * binc[j]);
+                fb[buf][u][j] = s < ks1 ? v : bf16x8{};
+            }
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < WG; ++u) {
+#pragma unroll
+            for (int i = 0; i < WTI; ++i)
+#pragma unroll
+                for (int j = 0; j < WTJ; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        fa[buf][u][i], fb[buf][u][j], acc[i][j], 0, 0, 0);
+            if (BIAS) {
+#pragma unroll
+                for (int j = 0; j < WTJ; ++j)
+                    accb[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ones, fb[buf][u][j],
+                                                                     accb[j], 0, 0, 0);
+            }
+        }
+    };
+    load(0, ks0);
+    for (int s0 = ks0; s0 < ks1; s0 += 2 * WG) {
+        load(1, s0 + WG);
+        compute(0);
+        load(0, s0 + 2 * WG);
+        compute(1);
+    }
+}
+
+/* One WTM x WTN tile of dW_l (rows: inputs, columns: outputs) per block. */
+__global__ void __launch_bounds__(256) train_weights_kernel(const WgArgs g) {
+    __shared__ __attribute__((aligned(16))) float red[4][16 * WTI * 16 * WTJ];
+    __shared__ float bred[4][16 * WTJ];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int li = 0;
+    while (li + 1 < g.nlayers && (int)blockIdx.x >= g.ly[li + 1].block0) ++li;
+    const WgLayer &Ly = g.ly[li];
+    const int t = (int)blockIdx.x - Ly.block0;
+    const int m0 = (t / Ly.tiles_n) * WTM, n0 = (t % Ly.tiles_n) * WTN;
+    const bool bias_tile = m0 == 0;
+    const uint32_t stp = g.adam ? *g.step : 0u;
+    const int fr = lane & 15;
+    if (blockIdx.x == 0 && wave == 0) {
+        /* the step's loss: lane l adds parts l, l + 64, ..., then a fixed
+         * butterfly over the lanes */
+        float v = 0.f;
+        for (int i = lane; i < g.nparts; i += 64) v += g.loss_part[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) g.loss[g.ring ? stp % g.ring : 0u] = v;
+    }
+    /* fragment (16 units x 32 batch rows) of unit tile T at k step s:
+     * 1 KB at ((T nks + s) 64 + lane) 8 */
+    const int nksb = g.ldb / 32;
+    const uint16_t *ap[WTI], *bp[WTJ];
+    int ainc[WTI], binc[WTJ];
+#pragma unroll
+    for (int i = 0; i < WTI; ++i) {
+        const int T = (m0 >> 4) + i;
+        const bool ok = 16 * T < Ly.in;
+        ap[i] = ok ? Ly.A + ((size_t)T * nksb * 64 + lane) * 8 : g.zrow;
+        ainc[i] = ok ? 512 : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < WTJ; ++j) {
+        const int T = (n0 >> 4) + j;
+        const bool ok = 16 * T < Ly.out;
+        bp[j] = ok ? Ly.BT + ((size_t)T * nksb * 64 + lane) * 8 : g.zrow;
+        binc[j] = ok ? 512 : 0;
+    }
+    /* the wave's k steps (uniform, so the loads below are unconditional:
+     * steps past the range read zeros) */
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int ks0 = wv * g.nks / 4, ks1 = (wv + 1) * g.nks / 4;
+    floatx4 acc[WTI][WTJ], accb[WTJ];
+#pragma unroll
+    for (int i = 0; i < WTI; ++i)
+#pragma unroll
+        for (int j = 0; j < WTJ; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < WTJ; ++j) accb[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (bias_tile)
+        wg_loop<true>(ap, ainc, bp, binc, ks0, ks1, acc, accb);
+    else
+        wg_loop<false>(ap, ainc, bp, binc, ks0, ks1, acc, accb);
+    float bs[WTJ];
+#pragma unroll
+    for (int j = 0; j < WTJ; ++j) bs[j] = accb[j][0];
+    /* lane l holds rows 16 i + 4 (l / 16) + q, column 16 j + l % 16 */
+#pragma unroll
+    for (int i = 0; i < WTI; ++i)
+#pragma unroll
+        for (int j = 0; j < WTJ; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                red[wave][(16 * i + (lane >> 4) * 4 + q) * WTN + 16 * j + fr] = acc[i][j][q];
+    if (bias_tile && lane < 16) {
+#pragma unroll
+        for (int j = 0; j < WTJ; ++j) bred[wave][16 * j + lane] = bs[j];
+    }
+    __syncthreads();
+    const float tt = (float)(stp + 1u);
+    const float c1 = 1.f - powf(g.b1, tt), c2 = 1.f - powf(g.b2, tt);
+    auto adam = [&](float gi, float mo, float vo, float wo, float &mi, float &vi) {
+        mi = g.b1 * mo + (1.f - g.b1) * gi;
+        vi = g.b2 * vo + (1.f - g.b2) * gi * gi;
+        return wo - g.lr * (mi / c1) / (sqrtf(vi / c2) + g.eps);
+    };
+    /* thread: tile row tid / (WTN / WEPT), WEPT columns; every load is
+     * issued before the first update */
+    const int r = tid / (WTN / WEPT), cb = (tid % (WTN / WEPT)) * WEPT, in_r = m0 + r;
+    if (in_r < Ly.in) {
+        const uint64_t base = Ly.woff + (uint64_t)in_r * Ly.out + n0 + cb;
+        float gi[WEPT], mo[WEPT], vo[WEPT], wo[WEPT];
+#pragma unroll
+        for (int cc = 0; cc < WEPT; ++cc) {
+            const int e = r * WTN + cb + cc;
+            gi[cc] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+            mo[cc] = vo[cc] = wo[cc] = 0.f;
+            if (g.adam && n0 + cb + cc < Ly.out) {
+                mo[cc] = g.m[base + cc];
+                vo[cc] = g.v[base + cc];
+                wo[cc] = g.w[base + cc];
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < WEPT; ++cc) {
+            const int out_c = n0 + cb + cc;
+            if (out_c >= Ly.out) continue;
+            if (!g.adam) {
+                g.grad[base + cc] = gi[cc];
+                continue;
+            }
+            float mi, vi;
+            const float wi = adam(gi[cc], mo[cc], vo[cc], wo[cc], mi, vi);
+            g.m[base + cc] = mi;
+            g.v[base + cc] = vi;
+            g.w[base + cc] = wi;
+            const uint16_t h = f2bf(wi);
+            Ly.wf[fragment_index(in_r, out_c, Ly.ksf)] = h;
+            if (Ly.wb) Ly.wb[fragment_index(out_c, in_r, Ly.ksb)] = h;
+        }
+    }
+    if (bias_tile && tid < WTN && n0 + tid < Ly.out) {
+        const float gi = ((bred[0][tid] + bred[1][tid]) + bred[2][tid]) + bred[3][tid];
+        const uint64_t idx = Ly.boff + n0 + tid;
+        if (!g.adam) {
+            g.grad[idx] = gi;
+        } else {
+            float mi, vi;
+            const float wi = adam(gi, g.m[idx], g.v[idx], g.w[idx], mi, vi);
+            g.m[idx] = mi;
+            g.v[idx] = vi;
+            g.w[idx] = wi;
+        }
+    }
+    if (g.adam) {
+        /* every block read *step above; the last one to finish advances it
+         * for the next launch (kernel boundary) */
+        __syncthreads();
+        if (tid == 0 && atomicAdd(g.done, 1u) == gridDim.x - 1u) {
+            *g.step += 1u;
+            *g.done = 0u;
+        }
+    }
+}
+
 } // namespace
 
 struct xo_mlp {
@@ -642,17 +1245,23 @@ struct xo_mlp {
     float *y = nullptr;               /* [B][out] */
     float *part = nullptr;            /* [max_splits][nparams] partial sums */
     std::map<uint32_t, NtProblem *> dw_tables; /* weight-gradient launches per batch */
-    uint16_t *wt = nullptr;           /* fragment-major bf16 weights (inference) */
+    uint16_t *wt = nullptr;           /* fragment-major bf16 W_l (inference, training) */
     std::vector<uint64_t> wt_off;
-    bool fusable = false, wt_dirty = true;
+    uint16_t *wtt = nullptr;          /* fragment-major bf16 W_l^T, l >= 1 (training) */
+    std::vector<uint64_t> wtt_off;
+    uint16_t *zrow = nullptr;         /* zeros read for rows past a layer's width */
+    float *loss_part = nullptr;       /* fused step: one partial loss per row block */
+    bool fusable = false, train_fused = false, wt_dirty = true;
     float *loss = nullptr;            /* per training step */
     uint32_t loss_cap = 0;
     uint32_t *step = nullptr;
     uint32_t *done = nullptr;         /* Adam blocks finished this step */
     float *data_x = nullptr, *data_t = nullptr;
     uint32_t ndata = 0;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;       /* one training step */
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_k = nullptr;     /* kGraphSteps consecutive steps */
+    hipGraphExec_t exec_k = nullptr;
     uint32_t graph_batch = 0;
     uint64_t graph_seed = 0;
     float graph_lr = 0.f;
@@ -828,9 +1437,16 @@ int refresh_bf16(xo_mlp *N) {
     return 0;
 }
 
+int train_fused_step(xo_mlp *N, uint32_t B, const float *X, const LossSpec &loss, bool adam,
+                     float lr);
+
 /* One training step on a device-sampled minibatch (capturable: every
  * argument is fixed, the step counter lives on the device). */
 int train_step(xo_mlp *N, uint32_t B, uint64_t seed, float lr) {
+    if (N->train_fused) {
+        const LossSpec loss{N->data_t, N->ndata, 1, seed, N->loss, N->loss_cap};
+        return train_fused_step(N, B, N->data_x, loss, true, lr);
+    }
     hipLaunchKernelGGL(rows_kernel, dim3((B + 255) / 256), dim3(256), 0, N->stream, N->data_x,
                        N->ndata, (uint32_t)N->dims[0], B, 1, seed, N->step, N->xr, N->ld[0],
                        N->xt, N->ldb);
@@ -845,15 +1461,98 @@ int train_step(xo_mlp *N, uint32_t B, uint64_t seed, float lr) {
     return 0;
 }
 
+/* The fragment-major copies from the fp32 master weights: W_l for the
+ * fused kernels' forward products and, for fused training, W_l^T for the
+ * data gradients. */
 int refresh_transposed(xo_mlp *N) {
-    for (int l = 0; l < N->nlayers; ++l) {
-        const int ksteps = l == 0 ? 1 : FHID / 32, ntiles = l + 1 == N->nlayers ? 8 : FHID / 16;
+    const int L = N->nlayers;
+    for (int l = 0; l < L; ++l) {
+        const int in = N->dims[l], out = N->dims[l + 1];
+        const int ksteps = l == 0 ? 1 : FHID / 32, ntiles = l + 1 == L ? 8 : FHID / 16;
         hipLaunchKernelGGL(fragment_pad_kernel, dim3(ntiles * ksteps * 2), dim3(256), 0,
-                           N->stream, N->w + N->woff[l], N->dims[l], N->dims[l + 1], ksteps,
-                           ntiles, N->wt + N->wt_off[l]);
+                           N->stream, N->w + N->woff[l], in, out, out, 0, ksteps, ntiles,
+                           N->wt + N->wt_off[l]);
         MLP_CHECK(hipGetLastError());
+        if (N->train_fused && l >= 1) {
+            const int kb = l + 1 == L ? 4 : FHID / 32, nb = FHID / 16;
+            hipLaunchKernelGGL(fragment_pad_kernel, dim3(nb * kb * 2), dim3(256), 0, N->stream,
+                               N->w + N->woff[l], out, in, out, 1, kb, nb,
+                               N->wtt + N->wtt_off[l]);
+            MLP_CHECK(hipGetLastError());
+        }
     }
     N->wt_dirty = false;
+    return 0;
+}
+
+/* One step of the fused training kernels over B rows of X (the training
+ * set's minibatch when loss.sampled, else its first B rows): Adam with
+ * learning rate lr (adam), or the gradient into N->grad. Capturable. */
+int train_fused_step(xo_mlp *N, uint32_t B, const float *X, const LossSpec &loss, bool adam,
+                     float lr) {
+    const int L = N->nlayers;
+    TrainArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.X = X;
+    a.T = loss.T;
+    a.n = loss.nt;
+    a.sampled = loss.sampled;
+    a.seed = loss.seed;
+    a.step = N->step;
+    a.loss_part = N->loss_part;
+    a.scale = 1.0f / (float)(B * (uint32_t)N->dims[L]);
+    a.B = (int)B;
+    a.nin = N->dims[0];
+    a.nout = N->dims[L];
+    a.nlayers = L;
+    a.ldb = N->ldb;
+    WgArgs g;
+    std::memset(&g, 0, sizeof(g));
+    int blocks = 0;
+    for (int l = 0; l < L; ++l) {
+        a.wf[l] = N->wt + N->wt_off[l];
+        a.wb[l] = l ? N->wtt + N->wtt_off[l] : nullptr;
+        a.bias[l] = N->w + N->boff[l];
+        a.ht[l] = l == 0 ? N->xt : N->ht[l - 1];
+        a.dzt[l] = l + 1 == L ? N->dyt : N->dzt[l];
+        WgLayer &y = g.ly[l];
+        y.A = a.ht[l];
+        y.BT = a.dzt[l];
+        y.wf = N->wt + N->wt_off[l];
+        y.wb = l ? N->wtt + N->wtt_off[l] : nullptr;
+        y.woff = N->woff[l];
+        y.boff = N->boff[l];
+        y.in = N->dims[l];
+        y.out = N->dims[l + 1];
+        y.tiles_n = (y.out + WTN - 1) / WTN;
+        y.block0 = blocks;
+        y.ksf = l == 0 ? 1 : FHID / 32;
+        y.ksb = l + 1 == L ? 4 : FHID / 32;
+        blocks += ((y.in + WTM - 1) / WTM) * y.tiles_n;
+    }
+    g.nlayers = L;
+    g.nks = (int)((B + 31) / 32);
+    g.ldb = N->ldb;
+    g.adam = adam ? 1 : 0;
+    g.grad = N->grad;
+    g.w = N->w;
+    g.m = N->m;
+    g.v = N->v;
+    g.step = N->step;
+    g.done = N->done;
+    g.lr = lr;
+    g.b1 = 0.9f;
+    g.b2 = 0.999f;
+    g.eps = 1e-8f;
+    g.zrow = N->zrow;
+    g.loss_part = N->loss_part;
+    g.nparts = (int)((B + TBM - 1) / TBM);
+    g.loss = loss.loss;
+    g.ring = loss.ring;
+    hipLaunchKernelGGL(kTrainRows[L], dim3((B + TBM - 1) / TBM), dim3(TNT), 0, N->stream, a);
+    MLP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(train_weights_kernel, dim3(blocks), dim3(256), 0, N->stream, g);
+    MLP_CHECK(hipGetLastError());
     return 0;
 }
 
@@ -887,6 +1586,38 @@ int upload_rows(xo_mlp *N, const float *x, uint32_t B) {
     return 0;
 }
 
+/* Training steps per multi-step graph: one graph launch per kGraphSteps
+ * steps instead of one per step (each launch boundary costs about 8 us of
+ * idle device time). */
+constexpr uint32_t kGraphSteps = 16;
+
+void drop_graphs(xo_mlp *N) {
+    if (N->exec) (void)hipGraphExecDestroy(N->exec);
+    if (N->graph) (void)hipGraphDestroy(N->graph);
+    if (N->exec_k) (void)hipGraphExecDestroy(N->exec_k);
+    if (N->graph_k) (void)hipGraphDestroy(N->graph_k);
+    N->exec = N->exec_k = nullptr;
+    N->graph = N->graph_k = nullptr;
+}
+
+/* n consecutive training steps captured as one graph. */
+int capture_steps(xo_mlp *N, uint32_t n, uint32_t batch, uint64_t seed, float lr,
+                  hipGraph_t *graph, hipGraphExec_t *exec) {
+    MLP_CHECK(hipStreamBeginCapture(N->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (uint32_t i = 0; i < n && rc == 0; ++i) rc = train_step(N, batch, seed, lr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(N->stream, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) xo::set_error("xo_mlp_train: capture failed: %s", hipGetErrorString(e));
+        return -1;
+    }
+    *graph = g;
+    MLP_CHECK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -895,12 +1626,11 @@ void xo_mlp_destroy(xo_mlp *N) {
     if (!N) return;
     (void)hipSetDevice(N->ctx->device);
     if (N->stream) (void)hipStreamSynchronize(N->stream);
-    if (N->exec) (void)hipGraphExecDestroy(N->exec);
-    if (N->graph) (void)hipGraphDestroy(N->graph);
+    drop_graphs(N);
     void *bufs[] = {N->w,   N->m,    N->v,    N->grad,   N->wbf,    N->wtb,  N->bf_index,
                     N->bt_index, N->xf, N->tb, N->xr,    N->xt,     N->dy,   N->dyt,
                     N->y,   N->part, N->loss, N->step,   N->done, N->data_x, N->data_t,
-                    N->wt};
+                    N->wt,  N->wtt, N->zrow, N->loss_part};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (auto *list : {&N->hr, &N->ht, &N->dz, &N->dzt})
@@ -929,7 +1659,8 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
     N->dims.assign(dims, dims + nlayers + 1);
     for (uint32_t l = 0; l <= nlayers; ++l) N->ld.push_back(pad8((int)dims[l]));
     N->max_batch = max_batch;
-    N->ldb = pad8((int)max_batch);
+    /* transposed activations: whole 32-row blocks of the fused training step */
+    N->ldb = ((int)max_batch + 31) & ~31;
     N->max_splits = max_batch / 512u < 8u ? (max_batch / 512u > 0u ? max_batch / 512u : 1u) : 8u;
     for (uint32_t l = 0; l < nlayers; ++l) {
         N->woff.push_back(N->nparams);
@@ -968,13 +1699,17 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
         {(void **)&N->xf, 4 * B * dims[0]},
         {(void **)&N->tb, 4 * B * dims[nlayers]},
         {(void **)&N->xr, 2 * B * N->ld[0]},
-        {(void **)&N->xt, 2 * (uint64_t)dims[0] * LB},
+        /* transposed inputs and output gradients: whole 16-unit tiles (the
+         * fused step stores them as fragments) */
+        {(void **)&N->xt, 2 * (uint64_t)((dims[0] + 15) & ~15u) * LB},
         {(void **)&N->dy, 2 * B * N->ld[L]},
-        {(void **)&N->dyt, 2 * (uint64_t)dims[nlayers] * LB},
+        {(void **)&N->dyt, 2 * (uint64_t)((dims[nlayers] + 15) & ~15u) * LB},
         {(void **)&N->y, 4 * B * dims[nlayers]},
         {(void **)&N->part, 4ull * N->max_splits * P},
         {(void **)&N->step, 4},
         {(void **)&N->done, 4},
+        {(void **)&N->zrow, 256},
+        {(void **)&N->loss_part, 4ull * ((max_batch + TBM - 1) / TBM)},
     };
     for (auto &a : allocs) {
         if ((e = hipMalloc(a.p, a.bytes)) != hipSuccess) return bad("allocation", e);
@@ -1005,6 +1740,17 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
             total += kpad * npad;
         }
         if ((e = hipMalloc(&N->wt, 2 * total)) != hipSuccess) return bad("allocation", e);
+        /* fused training: at most TMAXL layers (the activations of a row
+         * block stay in LDS) */
+        N->train_fused = nlayers <= (uint32_t)TMAXL;
+        if (N->train_fused) {
+            uint64_t tb = 0;
+            for (uint32_t l = 0; l < nlayers; ++l) {
+                N->wtt_off.push_back(tb);
+                if (l >= 1) tb += (uint64_t)(l + 1 == nlayers ? 128 : FHID) * FHID;
+            }
+            if ((e = hipMalloc(&N->wtt, 2 * tb)) != hipSuccess) return bad("allocation", e);
+        }
     }
     /* where each matrix entry's bf16 copies are: W_l [in][ld out] and
      * W_l^T [out][ld in]; biases have none */
@@ -1108,17 +1854,25 @@ int xo_mlp_gradient(xo_mlp *N, const float *x, const float *t, uint32_t n, float
     }
     std::lock_guard<std::mutex> lock(N->mu);
     MLP_CHECK(hipSetDevice(N->ctx->device));
-    const uint32_t nout = N->dims[N->nlayers];
-    if (upload_rows(N, x, n)) return -1;
+    const uint32_t nin = N->dims[0], nout = N->dims[N->nlayers];
+    if (N->train_fused) {
+        MLP_CHECK(hipMemcpyAsync(N->xf, x, 4ull * n * nin, hipMemcpyHostToDevice, N->stream));
+        if (N->wt_dirty && refresh_transposed(N)) return -1;
+    } else if (upload_rows(N, x, n)) {
+        return -1;
+    }
     MLP_CHECK(hipMemcpyAsync(N->tb, t, 4ull * n * nout, hipMemcpyHostToDevice, N->stream));
     float *dloss = nullptr;
     MLP_CHECK(hipMalloc(&dloss, 4));
     int rc = 0;
     const LossSpec spec{N->tb, n, 0, 0ull, dloss, 0u};
-    if (hipMemsetAsync(dloss, 0, 4, N->stream) != hipSuccess || forward(N, n, &spec) ||
-        backward(N, n))
+    if (hipMemsetAsync(dloss, 0, 4, N->stream) != hipSuccess)
         rc = -1;
-    if (rc == 0) {
+    else if (N->train_fused)
+        rc = train_fused_step(N, n, N->xf, spec, false, 0.f);
+    else if (forward(N, n, &spec) || backward(N, n))
+        rc = -1;
+    if (rc == 0 && !N->train_fused) {
         hipLaunchKernelGGL(sum_partials_kernel, dim3(256), dim3(256), 0, N->stream, N->part,
                            batch_splits(N, n), N->nparams, N->grad);
         if (hipGetLastError() != hipSuccess) rc = -1;
@@ -1154,10 +1908,7 @@ int xo_mlp_set_data(xo_mlp *N, const float *x, const float *t, uint32_t n) {
     MLP_CHECK(hipMemcpy(N->data_t, t, 4ull * n * nout, hipMemcpyHostToDevice));
     N->ndata = n;
     /* a captured step names the data buffers: capture again */
-    if (N->exec) (void)hipGraphExecDestroy(N->exec);
-    if (N->graph) (void)hipGraphDestroy(N->graph);
-    N->exec = nullptr;
-    N->graph = nullptr;
+    drop_graphs(N);
     return 0;
 }
 
@@ -1171,47 +1922,39 @@ int xo_mlp_train(xo_mlp *N, uint32_t steps, uint32_t batch, float lr, uint64_t s
     }
     if (steps == 0) return 0;
     std::lock_guard<std::mutex> lock(N->mu);
-    N->wt_dirty = true;
     MLP_CHECK(hipSetDevice(N->ctx->device));
+    /* the fused step keeps the fragment-major copies current; the
+     * layer-by-layer step leaves them to be rebuilt */
+    if (N->train_fused) {
+        if (N->wt_dirty && refresh_transposed(N)) return -1;
+    } else {
+        N->wt_dirty = true;
+    }
     if (steps > N->loss_cap) {
         if (N->loss) (void)hipFree(N->loss);
         N->loss = nullptr;
         N->loss_cap = 0;
         MLP_CHECK(hipMalloc(&N->loss, 4ull * steps));
         N->loss_cap = steps;
-        if (N->exec) (void)hipGraphExecDestroy(N->exec);
-        if (N->graph) (void)hipGraphDestroy(N->graph);
-        N->exec = nullptr;
-        N->graph = nullptr;
+        drop_graphs(N);
     }
-    if (N->exec && (N->graph_batch != batch || N->graph_seed != seed || N->graph_lr != lr)) {
-        (void)hipGraphExecDestroy(N->exec);
-        (void)hipGraphDestroy(N->graph);
-        N->exec = nullptr;
-        N->graph = nullptr;
-    }
-    {
+    if (N->exec && (N->graph_batch != batch || N->graph_seed != seed || N->graph_lr != lr))
+        drop_graphs(N);
+    if (!N->train_fused) {
         /* the weight-gradient launch table is uploaded outside any capture */
         const NtProblem *table = nullptr;
         int blocks = 0;
         if (dw_table(N, batch, &table, &blocks)) return -1;
     }
     if (use_graph && !N->exec) {
-        MLP_CHECK(hipStreamBeginCapture(N->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = train_step(N, batch, seed, lr);
-        hipGraph_t graph = nullptr;
-        const hipError_t e = hipStreamEndCapture(N->stream, &graph);
-        if (rc || e != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            if (!rc) xo::set_error("xo_mlp_train: capture failed: %s", hipGetErrorString(e));
-            return -1;
-        }
-        N->graph = graph;
-        MLP_CHECK(hipGraphInstantiate(&N->exec, N->graph, nullptr, nullptr, 0));
+        if (capture_steps(N, 1, batch, seed, lr, &N->graph, &N->exec)) return -1;
         N->graph_batch = batch;
         N->graph_seed = seed;
         N->graph_lr = lr;
     }
+    if (use_graph && steps >= kGraphSteps && !N->exec_k &&
+        capture_steps(N, kGraphSteps, batch, seed, lr, &N->graph_k, &N->exec_k))
+        return -1;
     uint32_t step0 = 0;
     MLP_CHECK(hipMemcpyAsync(&step0, N->step, 4, hipMemcpyDeviceToHost, N->stream));
     /* losses start at zero: a fill kernel, outside the graph */
@@ -1223,11 +1966,16 @@ int xo_mlp_train(xo_mlp *N, uint32_t steps, uint32_t batch, float lr, uint64_t s
     MLP_CHECK(hipEventCreate(&e1));
     int rc = 0;
     if (hipEventRecord(e0, N->stream) != hipSuccess) rc = -1;
-    for (uint32_t s = 0; s < steps && rc == 0; ++s) {
-        if (use_graph) {
+    for (uint32_t s = 0; s < steps && rc == 0;) {
+        if (use_graph && steps - s >= kGraphSteps) {
+            if (hipGraphLaunch(N->exec_k, N->stream) != hipSuccess) rc = -1;
+            s += kGraphSteps;
+        } else if (use_graph) {
             if (hipGraphLaunch(N->exec, N->stream) != hipSuccess) rc = -1;
-        } else if (train_step(N, batch, seed, lr)) {
-            rc = -1;
+            ++s;
+        } else {
+            if (train_step(N, batch, seed, lr)) rc = -1;
+            ++s;
         }
     }
     if (rc == 0 && hipEventRecord(e1, N->stream) != hipSuccess) rc = -1;

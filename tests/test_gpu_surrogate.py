@@ -52,15 +52,18 @@ def test_bf16_product_layout(ctx, ta, tb, M_, N_, K):
     assert np.abs(C - ref).max() < 1e-3 * np.abs(ref).max()
 
 
-def test_gradient_matches_cpu_restatement(ctx):
-    dims = S.DIMS
-    net = S.Network(dims, ctx, max_batch=512)
+@pytest.mark.parametrize('dims,n', [(S.DIMS, 384), (S.DIMS, 1000),
+                                    ((5, 64, 64, 10), 384)])
+def test_gradient_matches_cpu_restatement(ctx, dims, n):
+    """S.DIMS runs the fused training kernels (1000 rows: a ragged last
+    row block), (5, 64, 64, 10) the layer-by-layer products."""
+    net = S.Network(dims, ctx, max_batch=1024)
     rng = np.random.default_rng(5)
     p = M.init_params(dims, seed=6)
-    p[-100:] = rng.normal(0, 0.1, 100)
+    p[-dims[-1]:] = rng.normal(0, 0.1, dims[-1])
     net.set_params(p)
-    x = rng.uniform(-1, 1, (384, 5)).astype(np.float32)
-    t = rng.normal(0, 1, (384, 100)).astype(np.float32)
+    x = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
+    t = rng.normal(0, 1, (n, dims[-1])).astype(np.float32)
     loss, g = net.gradient(x, t)
     lref, gref = M.loss_and_gradient(p, x, t, dims, emulate_bf16=True)
     assert abs(loss - lref) / lref < 1e-4
@@ -83,10 +86,11 @@ def test_gradient_matches_cpu_restatement(ctx):
 
 @pytest.mark.parametrize('rows', [1, 63, 64, 1000, 4096])
 def test_fused_inference_equals_layer_by_layer(ctx, rows):
-    """forward() runs the fused kernel (activations kept in LDS); the
-    gradient path runs one product per layer. Same products in the same
-    order: the network outputs agree to the bit, which the gradient's loss
-    against the fused outputs shows as exactly zero."""
+    """forward() runs the fused inference kernel, gradient() the fused
+    training kernels (row blocks of 32 instead of 64, every activation kept
+    for the backward pass). Same products in the same order: the network
+    outputs agree to the bit, which the gradient's loss against the
+    inference outputs shows as exactly zero."""
     dims = S.DIMS
     net = S.Network(dims, ctx, max_batch=4096)
     rng = np.random.default_rng(rows)
@@ -102,24 +106,84 @@ def test_fused_inference_equals_layer_by_layer(ctx, rows):
     net.close()
 
 
-def test_captured_steps_equal_eager_steps(ctx):
-    dims = (5, 64, 64, 10)
+@pytest.mark.parametrize('dims', [(5, 64, 64, 10), S.DIMS])
+def test_captured_steps_equal_eager_steps(ctx, dims):
     rng = np.random.default_rng(7)
     x = rng.uniform(-1, 1, (1000, 5)).astype(np.float32)
-    t = np.sin(x @ rng.normal(0, 1, (5, 10))).astype(np.float32)
+    t = np.sin(x @ rng.normal(0, 1, (5, dims[-1]))).astype(np.float32)
     p0 = M.init_params(dims, seed=8)
     out = []
     for use_graph in (False, True, True):
         net = S.Network(dims, ctx, max_batch=256)
         net.set_params(p0)
         net.set_data(x, t)
-        l1, _ = net.train(7, 128, 1e-3, seed=3, use_graph=use_graph)
+        # 21 steps: one 16-step graph, then single-step graphs
+        l1, _ = net.train(21, 128, 1e-3, seed=3, use_graph=use_graph)
         l2, _ = net.train(5, 128, 1e-3, seed=3, use_graph=use_graph)
         out.append((net.get_params(), np.concatenate([l1, l2])))
         net.close()
     for params, losses in out[1:]:
         np.testing.assert_array_equal(params, out[0][0])
         np.testing.assert_allclose(losses, out[0][1], rtol=1e-5)
+
+
+def _sample_rows(seed, step, batch, n):
+    """The minibatch rows of a training step: row i of step s is
+    mix64(seed + golden * ((s << 32) + i + 1)) mod n (xo_mlp_train)."""
+    m = (1 << 64) - 1
+    rows = []
+    for i in range(batch):
+        z = (seed + 0x9E3779B97F4A7C15 * ((step << 32) + i + 1)) & m
+        z ^= z >> 30
+        z = (z * 0xBF58476D1CE4E5B9) & m
+        z ^= z >> 27
+        z = (z * 0x94D049BB133111EB) & m
+        z ^= z >> 31
+        rows.append(z % n)
+    return np.array(rows)
+
+
+def test_fused_adam_step_and_weight_copies(ctx):
+    """One fused step from fresh moments moves each parameter by
+    -lr g / (|g| + eps) with g the gradient of the step's minibatch (the
+    CPU restatement's, on the rows the step draws); after more steps the
+    outputs (fragment copies of W_l) and the gradient (copies of W_l and
+    W_l^T), both rewritten by the step's Adam epilogue, equal the CPU
+    restatement's on the fp32 weights read back."""
+    dims = S.DIMS
+    rng = np.random.default_rng(21)
+    x = rng.uniform(-1, 1, (3000, 5)).astype(np.float32)
+    t = np.tanh(x @ rng.normal(0, 1, (5, 100))).astype(np.float32)
+    p0 = M.init_params(dims, seed=22)
+    net = S.Network(dims, ctx, max_batch=1024)
+    net.set_params(p0)
+    net.set_data(x, t)
+    lr, seed, B = 1e-3, 5, 1000
+    net.train(1, B, lr, seed=seed)
+    p1 = net.get_params()
+    rows = _sample_rows(seed, 0, B, len(x))
+    _, g = M.loss_and_gradient(p0, x[rows], t[rows], dims, emulate_bf16=True)
+    g = g.astype(np.float32)
+    step_ref = -lr * g / (np.abs(g) + 1e-8)
+    d = (p1.astype(np.float64) - p0) - step_ref
+    # the bf16 products can flip the sign of a gradient near zero
+    assert np.mean(np.abs(d) > 0.05 * lr) < 0.03
+    assert np.all(np.abs(p1 - p0) <= lr * (1 + 1e-3))
+    net.train(4, B, lr, seed=seed + 1)
+    p = net.get_params()
+    xq = rng.uniform(-1, 1, (700, 5)).astype(np.float32)
+    tq = rng.normal(0, 1, (700, 100)).astype(np.float32)
+    y, _ = net.forward(xq)
+    yref, _ = M.forward(p, xq, dims, emulate_bf16=True)
+    assert np.linalg.norm(y - yref) / np.linalg.norm(yref) < 1e-3
+    loss, gq = net.gradient(xq, tq)
+    lref, gqref = M.loss_and_gradient(p, xq, tq, dims, emulate_bf16=True)
+    assert abs(loss - lref) / lref < 1e-4
+    for w_idx, b_idx in M.unpack(np.arange(p.size), dims):
+        for idx in (w_idx.ravel(), b_idx.ravel()):
+            rel = np.linalg.norm(gq[idx] - gqref[idx]) / np.linalg.norm(gqref[idx])
+            assert rel < 1e-2, rel
+    net.close()
 
 
 def test_training_fits_a_smooth_function(ctx):
