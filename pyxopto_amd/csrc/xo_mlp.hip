@@ -639,8 +639,8 @@ __global__ void refresh_kernel(const float *w, uint64_t n, const uint64_t *bf_in
  * same k order (weights as fragment-major copies, W_l for the forward pass
  * and W_l^T for the data gradients), so activations, loss gradient and data
  * gradients are the same bits. train_weights_kernel forms each 32x32 tile
- * of every dW_l over the whole batch in one block: its four waves take a
- * quarter of the batch each, with the loads of the next WG k steps in
+ * of every dW_l over the whole batch in one block: its WNW waves take a
+ * 1/WNW share of the batch each, with the loads of the next WG k steps in
  * flight while the MFMAs of the current WG issue, and their sums are
  * added in wave order in LDS (deterministic; only the order of additions
  * differs from the split sums of the layer-by-layer path). Blocks of the
@@ -854,10 +854,12 @@ __device__ inline void store_transposed(const uint16_t *tile, int stride, int nc
 }
 
 /* The LDS tiles of a row block: the input rows, each hidden layer's
- * output, the loss gradient, two data-gradient tiles, and every layer's
- * bias (FHID floats each). */
+ * output, the loss gradient and every layer's bias (FHID floats each). The
+ * data gradient at a hidden layer's output overwrites that layer's
+ * activations, which are its mask: each lane reads the mask elements it
+ * then writes, and nothing else reads the tile during the product. */
 struct TrainTiles {
-    uint16_t *x, *h[TMAXL - 1], *y, *d[2];
+    uint16_t *x, *h[TMAXL - 1], *y;
     float *bias;
 };
 
@@ -906,15 +908,14 @@ __device__ __forceinline__ void train_ops(const TrainArgs &a, TrainFrags (&f)[2]
                 ncols = a.nout;
             }
         } else {
-            constexpr int b = I - L;
-            if constexpr (b == 0) {
+            if constexpr (O::layer == L - 1) {
                 src = t.y;
                 ss = TYS;
             } else {
-                src = t.d[(b - 1) & 1];
+                src = t.h[O::layer];
                 ss = FSTRIDE;
             }
-            dst = t.d[b & 1];
+            dst = t.h[O::layer - 1];
             mask = t.h[O::layer - 1];
             outT = a.dzt[O::layer - 1];
         }
@@ -945,7 +946,6 @@ __global__ void __launch_bounds__(TNT) train_rows_kernel(const TrainArgs a) {
     __shared__ __attribute__((aligned(16))) uint16_t sx[TBM * TXS];
     __shared__ __attribute__((aligned(16))) uint16_t sh[L - 1][TBM * FSTRIDE];
     __shared__ __attribute__((aligned(16))) uint16_t sy[TBM * TYS];
-    __shared__ __attribute__((aligned(16))) uint16_t sd[2][TBM * FSTRIDE];
     __shared__ __attribute__((aligned(16))) float sb[L * FHID];
     __shared__ float wsq[TNW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -979,8 +979,6 @@ __global__ void __launch_bounds__(TNT) train_rows_kernel(const TrainArgs a) {
     for (int l = 0; l < L - 1; ++l) t.h[l] = sh[l];
     for (int l = L - 1; l < TMAXL - 1; ++l) t.h[l] = nullptr;
     t.y = sy;
-    t.d[0] = sd[0];
-    t.d[1] = sd[1];
     t.bias = sb;
     train_ops<L, 0>(a, f, t, tv, row0, lane, wave, wsq);
 }
@@ -1019,8 +1017,11 @@ struct WgArgs {
 
 /* 32x32 tiles (about 170 blocks for the surrogate's layers); each wave
  * keeps two groups of WG k steps in flight */
-constexpr int WTI = 2, WTJ = 2, WG = 8;
-constexpr int WTM = 16 * WTI, WTN = 16 * WTJ, WEPT = WTM * WTN / 256;
+constexpr int WTI = 2, WTJ = 2, WG = 6;
+/* waves per block, each a 1/WNW share of the batch: two per SIMD, so
+ * twice the fragment loads in flight per CU */
+constexpr int WNW = 8, WNT = 64 * WNW;
+constexpr int WTM = 16 * WTI, WTN = 16 * WTJ, WEPT = WTM * WTN / WNT;
 
 /* The k loop of a weight-gradient tile: steps [ks0, ks1) of 32 batch rows,
  * two groups of WG steps in flight. With BIAS, a row of ones times the
@@ -1069,19 +1070,25 @@ __device__ __forceinline__ void wg_loop(const uint16_t *const (&ap)[WTI], const 
             }
         }
     };
+    /* the scheduling barriers keep each group's loads ahead of the MFMAs
+     * of the group before it */
     load(0, ks0);
     for (int s0 = ks0; s0 < ks1; s0 += 2 * WG) {
         load(1, s0 + WG);
+        __builtin_amdgcn_sched_barrier(0);
         compute(0);
+        __builtin_amdgcn_sched_barrier(0);
         load(0, s0 + 2 * WG);
+        __builtin_amdgcn_sched_barrier(0);
         compute(1);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 /* One WTM x WTN tile of dW_l (rows: inputs, columns: outputs) per block. */
-__global__ void __launch_bounds__(256) train_weights_kernel(const WgArgs g) {
-    __shared__ __attribute__((aligned(16))) float red[4][16 * WTI * 16 * WTJ];
-    __shared__ float bred[4][16 * WTJ];
+__global__ void __launch_bounds__(WNT) train_weights_kernel(const WgArgs g) {
+    __shared__ __attribute__((aligned(16))) float red[WNW][16 * WTI * 16 * WTJ];
+    __shared__ float bred[WNW][16 * WTJ];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int li = 0;
     while (li + 1 < g.nlayers && (int)blockIdx.x >= g.ly[li + 1].block0) ++li;
@@ -1122,7 +1129,7 @@ __global__ void __launch_bounds__(256) train_weights_kernel(const WgArgs g) {
     /* the wave's k steps (uniform, so the loads below are unconditional:
      * steps past the range read zeros) */
     const int wv = __builtin_amdgcn_readfirstlane(wave);
-    const int ks0 = wv * g.nks / 4, ks1 = (wv + 1) * g.nks / 4;
+    const int ks0 = wv * g.nks / WNW, ks1 = (wv + 1) * g.nks / WNW;
     floatx4 acc[WTI][WTJ], accb[WTJ];
 #pragma unroll
     for (int i = 0; i < WTI; ++i)
@@ -1166,7 +1173,9 @@ __global__ void __launch_bounds__(256) train_weights_kernel(const WgArgs g) {
 #pragma unroll
         for (int cc = 0; cc < WEPT; ++cc) {
             const int e = r * WTN + cb + cc;
-            gi[cc] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+            gi[cc] = red[0][e];
+#pragma unroll
+            for (int w = 1; w < WNW; ++w) gi[cc] += red[w][e];
             mo[cc] = vo[cc] = wo[cc] = 0.f;
             if (g.adam && n0 + cb + cc < Ly.out) {
                 mo[cc] = g.m[base + cc];
@@ -1193,7 +1202,9 @@ __global__ void __launch_bounds__(256) train_weights_kernel(const WgArgs g) {
         }
     }
     if (bias_tile && tid < WTN && n0 + tid < Ly.out) {
-        const float gi = ((bred[0][tid] + bred[1][tid]) + bred[2][tid]) + bred[3][tid];
+        float gi = bred[0][tid];
+#pragma unroll
+        for (int w = 1; w < WNW; ++w) gi += bred[w][tid];
         const uint64_t idx = Ly.boff + n0 + tid;
         if (!g.adam) {
             g.grad[idx] = gi;
@@ -1551,7 +1562,7 @@ int train_fused_step(xo_mlp *N, uint32_t B, const float *X, const LossSpec &loss
     g.ring = loss.ring;
     hipLaunchKernelGGL(kTrainRows[L], dim3((B + TBM - 1) / TBM), dim3(TNT), 0, N->stream, a);
     MLP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(train_weights_kernel, dim3(blocks), dim3(256), 0, N->stream, g);
+    hipLaunchKernelGGL(train_weights_kernel, dim3(blocks), dim3(WNT), 0, N->stream, g);
     MLP_CHECK(hipGetLastError());
     return 0;
 }
