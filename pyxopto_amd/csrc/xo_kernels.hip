@@ -134,6 +134,21 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
     uint32_t pend = kEvNone; /* event the lane's last step ended on      */
     float pstep = 0.f;       /* drawn step of a pending boundary event   */
     uint32_t bwait = 0;      /* iterations boundary events have waited   */
+    /* the per-step fields of the packet's current layer (top, bottom,
+     * 1/mut, mua/mut) and the phase-function parameters its kind uses, in
+     * registers: read from LDS again only when the layer may change (a new
+     * packet, a boundary event), not in every step (+5 % on the 1 mm slab) */
+    float4 lt = make_float4(0.f, 0.f, 0.f, 0.f);
+    PfL lpf{0.f, 0.f, 0.f, 0.f, 0.f, 0u, 0u};
+    auto layer_state = [&](const LayerL &Q) {
+        lt = make_float4(Q.top, Q.bottom, Q.inv_mut, Q.mua_inv_mut);
+        if (PF == XO_PF_HG) lpf.g = Q.g;
+        if (PF == XO_PF_MHG) {
+            lpf.g = Q.g;
+            lpf.beta = Q.beta;
+        }
+        if (PF == XO_PF_LUT) lpf = PfL{0.f, 0.f, Q.la, Q.lb, Q.lc, Q.loff, Q.lsize_m1};
+    };
 
     for (;;) {
         /* ---- refill lanes whose packet ended ---- */
@@ -202,6 +217,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                                      src.reflectance);
             }
             layer = 1;
+            layer_state(sl[1]);
             alive = true;
             pkt_steps = 0;
             st[XO_STAT_PACKETS]++;
@@ -299,6 +315,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                     st[XO_STAT_BOUNDARY]++;
                     const int32_t before = layer;
                     mcml_boundary<M>(sl, layer, next, dx, dy, dz, x, a);
+                    layer_state(sl[layer]);
                     if (TRACE)
                         ts.ev |= XO_EVENT_BOUNDARY_HIT |
                                  (layer != before ? XO_EVENT_REFRACTION : XO_EVENT_REFLECTION);
@@ -327,10 +344,10 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             if (METHOD == XO_METHOD_MBL)
                 step = M::div(-M::log_unit(rng_next(x, a)), L.mus); /* mcml.template.c:529-531 */
             else
-                step = -M::log_unit(rng_next(x, a)) * L.inv_mut;
+                step = -M::log_unit(rng_next(x, a)) * lt.z;
             step = fminf(step, XO_FLT_MAX);
             const float zend = __builtin_fmaf(step, dz, pz);
-            if (zend < L.top || zend >= L.bottom) {
+            if (zend < lt.x || zend >= lt.y) {
                 pend = kEvBoundary;
                 pstep = step;
             } else {
@@ -346,7 +363,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
             /* (C) */
             const LayerL &L = sl[layer];
             bool done = false;
-            if (METHOD == XO_METHOD_AR && rng_next(x, a) < L.mua_inv_mut) {
+            if (METHOD == XO_METHOD_AR && rng_next(x, a) < lt.w) {
                 /* albedo rejection: the whole weight is absorbed here,
                  * mcml.template.c:696-708 */
                 const float deposit = w;
@@ -359,7 +376,7 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 done = true;
             } else {
                 if (METHOD == XO_METHOD_AW) {
-                    const float deposit = w * L.mua_inv_mut;
+                    const float deposit = w * lt.w;
                     w -= deposit;
                     if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                     if (use_fluence &&
@@ -367,9 +384,8 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                                            L.mua))
                         st[XO_STAT_DEPOSITS]++;
                 }
-                PfL pf{L.g, L.beta, L.la, L.lb, L.lc, L.loff, L.lsize_m1};
                 float xi_fi;
-                float ct = pf_sample<PF, M>(pf, lut, x, a, xi_fi);
+                float ct = pf_sample<PF, M>(lpf, lut, x, a, xi_fi);
                 scatter_direction<M>(dx, dy, dz, ct, xi_fi);
                 if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
                 if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
