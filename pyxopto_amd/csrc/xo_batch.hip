@@ -51,6 +51,11 @@ constexpr uint32_t kEvNone = 0, kEvInteraction = 1, kEvBoundary = 2;
 /* Boundary events are handled once P.defer_min lanes of the wave hold one,
  * or after kDeferWait iterations (bounded wait, every event is handled). */
 constexpr uint32_t kDeferWait = 8;
+/* Transport rounds (B then C) per loop iteration: 1.66e9 -> 1.92e9
+ * packets/s on the 4096-instance batch going from 1 to 6 (2: 1.83e9, 3:
+ * 1.89e9, 8: 1.93e9; ending the rounds early once a quarter of the lanes
+ * wait: 1.86e9). */
+constexpr int kStepsPerIteration = 6;
 
 /* End of an event (mcml.template.c:756-782): the simulation radius and the
  * step limit; a finished packet frees its lane and adds its steps to the
@@ -317,52 +322,59 @@ batch_kernel(const BatchParams P) {
                 }
             }
         }
-        if (alive && pend == kEvNone) {
-            /* (B) */
-            float step;
-            if (METHOD == XO_METHOD_MBL)
-                step = M::div(-M::log_unit(rng_next(x, a)),
-                              layers[layer].mus); /* mcml.template.c:529-531 */
-            else
-                step = -M::log_unit(rng_next(x, a)) * lt.z;
-            step = fminf(step, XO_FLT_MAX);
-            const float zend = __builtin_fmaf(step, dz, pz);
-            if (zend < lt.x || zend >= lt.y) {
-                pend = kEvBoundary;
-                pstep = step;
-            } else {
-                px = __builtin_fmaf(dx, step, px);
-                py = __builtin_fmaf(dy, step, py);
-                pz = zend;
-                if (METHOD == XO_METHOD_MBL) mbl_absorb(layers[layer], step);
-                pend = kEvInteraction;
-            }
-        }
-        if (alive && pend == kEvInteraction) {
-            /* (C) absorption and scattering, mcml.template.c:696-753 */
-            const LayerL &L = layers[layer];
-            bool done = false;
-            if (METHOD == XO_METHOD_AR && rng_next(x, a) < lt.w) {
-                /* albedo rejection: the whole weight is absorbed here */
-                if (FLU) fluence_at(px, py, pz, w, L.mua);
-                w = 0.0f;
-                done = true;
-            } else {
-                if (METHOD == XO_METHOD_AW) {
-                    float deposit = w * lt.w;
-                    w -= deposit;
-                    if (FLU) fluence_at(px, py, pz, deposit, L.mua);
+        /* (B) and (C) kStepsPerIteration times per iteration: the loop-top
+         * work (refill and boundary ballots, the wave-uniform decisions)
+         * is shared by that many steps; a lane whose packet ends or whose
+         * step reaches a boundary sits out the remaining rounds */
+#pragma unroll
+        for (int rep = 0; rep < kStepsPerIteration; ++rep) {
+            if (alive && pend == kEvNone) {
+                /* (B) */
+                float step;
+                if (METHOD == XO_METHOD_MBL)
+                    step = M::div(-M::log_unit(rng_next(x, a)),
+                                  layers[layer].mus); /* mcml.template.c:529-531 */
+                else
+                    step = -M::log_unit(rng_next(x, a)) * lt.z;
+                step = fminf(step, XO_FLT_MAX);
+                const float zend = __builtin_fmaf(step, dz, pz);
+                if (zend < lt.x || zend >= lt.y) {
+                    pend = kEvBoundary;
+                    pstep = step;
+                } else {
+                    px = __builtin_fmaf(dx, step, px);
+                    py = __builtin_fmaf(dy, step, py);
+                    pz = zend;
+                    if (METHOD == XO_METHOD_MBL) mbl_absorb(layers[layer], step);
+                    pend = kEvInteraction;
                 }
-                float xi_fi;
-                float ct = pf_sample_rt<M>(pf_kind, lpf, P.lut, x, a, xi_fi);
-                scatter_direction<M>(dx, dy, dz, ct, xi_fi);
-                if (METHOD == XO_METHOD_AW)
-                    done = roulette(w, x, a, P.w_min, P.chance, P.use_lottery);
             }
-            done = mbl_roulette(done);
-            pend = kEvNone;
-            end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, n_trunc,
-                      n_steps);
+            if (alive && pend == kEvInteraction) {
+                /* (C) absorption and scattering, mcml.template.c:696-753 */
+                const LayerL &L = layers[layer];
+                bool done = false;
+                if (METHOD == XO_METHOD_AR && rng_next(x, a) < lt.w) {
+                    /* albedo rejection: the whole weight is absorbed here */
+                    if (FLU) fluence_at(px, py, pz, w, L.mua);
+                    w = 0.0f;
+                    done = true;
+                } else {
+                    if (METHOD == XO_METHOD_AW) {
+                        float deposit = w * lt.w;
+                        w -= deposit;
+                        if (FLU) fluence_at(px, py, pz, deposit, L.mua);
+                    }
+                    float xi_fi;
+                    float ct = pf_sample_rt<M>(pf_kind, lpf, P.lut, x, a, xi_fi);
+                    scatter_direction<M>(dx, dy, dz, ct, xi_fi);
+                    if (METHOD == XO_METHOD_AW)
+                        done = roulette(w, x, a, P.w_min, P.chance, P.use_lottery);
+                }
+                done = mbl_roulette(done);
+                pend = kEvNone;
+                end_event(done, alive, px, py, pz, spx, spy, spz, rmax2, pkt_steps, n_trunc,
+                          n_steps);
+            }
         }
     }
 

@@ -1017,7 +1017,7 @@ struct WgArgs {
 
 /* 32x32 tiles (about 170 blocks for the surrogate's layers); each wave
  * keeps two groups of WG k steps in flight */
-constexpr int WTI = 2, WTJ = 2, WG = 6;
+constexpr int WTI = 2, WTJ = 2, WG = 5;
 /* waves per block, each a 1/WNW share of the batch: two per SIMD, so
  * twice the fragment loads in flight per CU */
 constexpr int WNW = 8, WNT = 64 * WNW;
@@ -1098,6 +1098,29 @@ __global__ void __launch_bounds__(WNT) train_weights_kernel(const WgArgs g) {
     const bool bias_tile = m0 == 0;
     const uint32_t stp = g.adam ? *g.step : 0u;
     const int fr = lane & 15;
+    /* this thread's elements of the tile (row tid / (WTN / WEPT), WEPT
+     * columns) and, for the first row of tiles, one bias: their optimizer
+     * state and weights are requested now, so the epilogue does not wait
+     * for them */
+    const int r = tid / (WTN / WEPT), cb = (tid % (WTN / WEPT)) * WEPT, in_r = m0 + r;
+    const uint64_t base = Ly.woff + (uint64_t)in_r * Ly.out + n0 + cb;
+    const uint64_t bidx = Ly.boff + n0 + tid;
+    const bool has_bias = bias_tile && tid < WTN && n0 + tid < Ly.out;
+    float mo[WEPT], vo[WEPT], wo[WEPT], mb = 0.f, vb = 0.f, wb = 0.f;
+#pragma unroll
+    for (int cc = 0; cc < WEPT; ++cc) {
+        mo[cc] = vo[cc] = wo[cc] = 0.f;
+        if (g.adam && in_r < Ly.in && n0 + cb + cc < Ly.out) {
+            mo[cc] = g.m[base + cc];
+            vo[cc] = g.v[base + cc];
+            wo[cc] = g.w[base + cc];
+        }
+    }
+    if (g.adam && has_bias) {
+        mb = g.m[bidx];
+        vb = g.v[bidx];
+        wb = g.w[bidx];
+    }
     if (blockIdx.x == 0 && wave == 0) {
         /* the step's loss: lane l adds parts l, l + 64, ..., then a fixed
          * butterfly over the lanes */
@@ -1164,24 +1187,14 @@ __global__ void __launch_bounds__(WNT) train_weights_kernel(const WgArgs g) {
         vi = g.b2 * vo + (1.f - g.b2) * gi * gi;
         return wo - g.lr * (mi / c1) / (sqrtf(vi / c2) + g.eps);
     };
-    /* thread: tile row tid / (WTN / WEPT), WEPT columns; every load is
-     * issued before the first update */
-    const int r = tid / (WTN / WEPT), cb = (tid % (WTN / WEPT)) * WEPT, in_r = m0 + r;
     if (in_r < Ly.in) {
-        const uint64_t base = Ly.woff + (uint64_t)in_r * Ly.out + n0 + cb;
-        float gi[WEPT], mo[WEPT], vo[WEPT], wo[WEPT];
+        float gi[WEPT];
 #pragma unroll
         for (int cc = 0; cc < WEPT; ++cc) {
             const int e = r * WTN + cb + cc;
             gi[cc] = red[0][e];
 #pragma unroll
             for (int w = 1; w < WNW; ++w) gi[cc] += red[w][e];
-            mo[cc] = vo[cc] = wo[cc] = 0.f;
-            if (g.adam && n0 + cb + cc < Ly.out) {
-                mo[cc] = g.m[base + cc];
-                vo[cc] = g.v[base + cc];
-                wo[cc] = g.w[base + cc];
-            }
         }
 #pragma unroll
         for (int cc = 0; cc < WEPT; ++cc) {
@@ -1201,19 +1214,18 @@ __global__ void __launch_bounds__(WNT) train_weights_kernel(const WgArgs g) {
             if (Ly.wb) Ly.wb[fragment_index(out_c, in_r, Ly.ksb)] = h;
         }
     }
-    if (bias_tile && tid < WTN && n0 + tid < Ly.out) {
+    if (has_bias) {
         float gi = bred[0][tid];
 #pragma unroll
         for (int w = 1; w < WNW; ++w) gi += bred[w][tid];
-        const uint64_t idx = Ly.boff + n0 + tid;
         if (!g.adam) {
-            g.grad[idx] = gi;
+            g.grad[bidx] = gi;
         } else {
             float mi, vi;
-            const float wi = adam(gi, g.m[idx], g.v[idx], g.w[idx], mi, vi);
-            g.m[idx] = mi;
-            g.v[idx] = vi;
-            g.w[idx] = wi;
+            const float wi = adam(gi, mb, vb, wb, mi, vi);
+            g.m[bidx] = mi;
+            g.v[bidx] = vi;
+            g.w[bidx] = wi;
         }
     }
     if (g.adam) {
