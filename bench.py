@@ -42,7 +42,7 @@ SERVING_QPS = 250.0                     # requests/s of the serving run
 # profiles/mcvox_kernel_skin_vessel_200.md,
 # profiles/mcml_kernel_slab_1mm_and_surrogate.md). Offline figures: measured
 # for the kernels as they are in this tree, hardware arithmetic.
-TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 2.979e11,
+TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 2.585e11,
                             'skin_vessel_200': 7.52e11,
                             'slab_1mm': 1.09e7}
 

@@ -43,6 +43,8 @@ XO_DEV void stats_flush(BlockStats *ss, const uint32_t *mine,
  * kDeferWait iterations). */
 constexpr uint32_t kEvNone = 0, kEvInteraction = 1, kEvBoundary = 2;
 constexpr uint32_t kDeferMin = 12, kDeferWait = 8;
+/* Transport rounds (B then C) per loop iteration of the layered kernel. */
+constexpr int kRoundsPerIteration = 6;
 
 template <int PF, int METHOD, bool TRACE, class M>
 __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
@@ -336,62 +338,68 @@ __global__ void __launch_bounds__(XO_BLOCK) mcml_kernel(const McmlParams P) {
                 }
             }
         }
-        if (alive && pend == kEvNone) {
-            /* (B) */
-            const LayerL &L = sl[layer];
-            st[XO_STAT_STEPS]++;
-            float step;
-            if (METHOD == XO_METHOD_MBL)
-                step = M::div(-M::log_unit(rng_next(x, a)), L.mus); /* mcml.template.c:529-531 */
-            else
-                step = -M::log_unit(rng_next(x, a)) * lt.z;
-            step = fminf(step, XO_FLT_MAX);
-            const float zend = __builtin_fmaf(step, dz, pz);
-            if (zend < lt.x || zend >= lt.y) {
-                pend = kEvBoundary;
-                pstep = step;
-            } else {
-                px = __builtin_fmaf(dx, step, px);
-                py = __builtin_fmaf(dy, step, py);
-                pz = zend;
-                if (TRACE && T.plon) ts.pl += L.n * step;
-                if (METHOD == XO_METHOD_MBL) mbl_absorb(L, step);
-                pend = kEvInteraction;
+        /* (B) and (C) kRoundsPerIteration times per iteration, as in the
+         * batch kernel: the loop-top work is shared by that many steps
+         * (1 mm slab 0.464 -> 0.403 ms per 1e6 packets) */
+#pragma unroll
+        for (int rep = 0; rep < kRoundsPerIteration; ++rep) {
+            if (alive && pend == kEvNone) {
+                /* (B) */
+                const LayerL &L = sl[layer];
+                st[XO_STAT_STEPS]++;
+                float step;
+                if (METHOD == XO_METHOD_MBL)
+                    step = M::div(-M::log_unit(rng_next(x, a)), L.mus); /* mcml.template.c:529-531 */
+                else
+                    step = -M::log_unit(rng_next(x, a)) * lt.z;
+                step = fminf(step, XO_FLT_MAX);
+                const float zend = __builtin_fmaf(step, dz, pz);
+                if (zend < lt.x || zend >= lt.y) {
+                    pend = kEvBoundary;
+                    pstep = step;
+                } else {
+                    px = __builtin_fmaf(dx, step, px);
+                    py = __builtin_fmaf(dy, step, py);
+                    pz = zend;
+                    if (TRACE && T.plon) ts.pl += L.n * step;
+                    if (METHOD == XO_METHOD_MBL) mbl_absorb(L, step);
+                    pend = kEvInteraction;
+                }
             }
-        }
-        if (alive && pend == kEvInteraction) {
-            /* (C) */
-            const LayerL &L = sl[layer];
-            bool done = false;
-            if (METHOD == XO_METHOD_AR && rng_next(x, a) < lt.w) {
-                /* albedo rejection: the whole weight is absorbed here,
-                 * mcml.template.c:696-708 */
-                const float deposit = w;
-                w -= deposit;
-                if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-                if (use_fluence &&
-                    fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit,
-                                       L.mua))
-                    st[XO_STAT_DEPOSITS]++;
-                done = true;
-            } else {
-                if (METHOD == XO_METHOD_AW) {
-                    const float deposit = w * lt.w;
+            if (alive && pend == kEvInteraction) {
+                /* (C) */
+                const LayerL &L = sl[layer];
+                bool done = false;
+                if (METHOD == XO_METHOD_AR && rng_next(x, a) < lt.w) {
+                    /* albedo rejection: the whole weight is absorbed here,
+                     * mcml.template.c:696-708 */
+                    const float deposit = w;
                     w -= deposit;
                     if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
                     if (use_fluence &&
                         fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit,
                                            L.mua))
                         st[XO_STAT_DEPOSITS]++;
-                }
-                float xi_fi;
-                float ct = pf_sample<PF, M>(lpf, lut, x, a, xi_fi);
-                scatter_direction<M>(dx, dy, dz, ct, xi_fi);
-                if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
-                if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
                     done = true;
+                } else {
+                    if (METHOD == XO_METHOD_AW) {
+                        const float deposit = w * lt.w;
+                        w -= deposit;
+                        if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
+                        if (use_fluence &&
+                            fluence_deposit<M>(lds_fresh(sev.fluence), P.accum, px, py, pz, deposit,
+                                               L.mua))
+                            st[XO_STAT_DEPOSITS]++;
+                    }
+                    float xi_fi;
+                    float ct = pf_sample<PF, M>(lpf, lut, x, a, xi_fi);
+                    scatter_direction<M>(dx, dy, dz, ct, xi_fi);
+                    if (TRACE) ts.ev |= XO_EVENT_SCATTERING;
+                    if (METHOD == XO_METHOD_AW && roulette(w, x, a, w_min, chance, use_lottery))
+                        done = true;
+                }
+                end_step(done);
             }
-            end_step(done);
         }
     }
 
