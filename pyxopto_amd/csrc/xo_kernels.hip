@@ -476,6 +476,8 @@ XO_DEV bool box_intersect(const VoxGeom &g, const float pos[3], const float dir[
 /* Parked interactions of the voxel kernel are handled once this many lanes
  * of a wave hold one, or after kVoxDeferWait iterations. */
 constexpr uint32_t kVoxDeferMin = 24, kVoxDeferWait = 8;
+/* Steps per loop iteration of the voxel kernel (at most). */
+constexpr int kVoxRoundsPerIteration = 4;
 
 template <int PF, int METHOD, bool TRACE, class M>
 __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
@@ -788,120 +790,128 @@ __global__ void __launch_bounds__(XO_BLOCK) mcvox_kernel(const McvoxParams P) {
         }
         if (!alive || pend_int) continue;
 
-        /* ---- one step of while(!done), mcvox.template.c:748-924 ---- */
-        const MaterialL &Mt = sm[mi];
-        st[XO_STAT_STEPS]++;
-        float step;
-        if (METHOD == XO_METHOD_MBL)
-            step = M::div(-M::log_unit(rng_next(x, a)), Mt.mus); /* mcvox.template.c:763-766 */
-        else
-            step = -M::log_unit(rng_next(x, a)) * Mt.inv_mut;
-        step = fminf(step, XO_FLT_MAX);
-        /* mcsim_intersect, mcvox.template.c:173-196 */
-        float dist[3];
-        for (int k = 0; k < 3; ++k) {
-            float side = (float)((int32_t)(d[k] >= 0.0f) + vi[k]);
-            dist[k] = g.tl[k] + side * g.size[k] - p[k];
-            dist[k] = (d[k] != 0.0f) ? M::div(dist[k], d[k]) : XO_INF_F;
-        }
-        float dd = fminf(dist[0], fminf(dist[1], dist[2]));
-        float d_ok = fminf(dd, step);
-        p[0] = __builtin_fmaf(d_ok, d[0], p[0]);
-        p[1] = __builtin_fmaf(d_ok, d[1], p[1]);
-        p[2] = __builtin_fmaf(d_ok, d[2], p[2]);
-        /* optical path length, mcvox.template.c:792-798 */
-        if (TRACE && T.plon) ts.pl += Mt.n * d_ok;
-
-        bool done = false;
-        if (METHOD == XO_METHOD_MBL) {
-            /* absorption along the step inside this voxel,
-             * mcvox.template.c:800-818 */
-            const float mua = Mt.mua;
-            const float fraction = 1.0f - M::exp(-(mua * d_ok));
-            const float deposit = fraction * w;
-            w -= deposit;
-            if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
-            if (use_fluence) {
-                const float back =
-                    (mua != 0.0f)
-                        ? d_ok - M::div(-M::log_unit(1.0f - rng_next(x, a) * fraction), mua)
-                        : 0.0f;
-                if (deposit_fluence(p[0] - back * d[0], p[1] - back * d[1],
-                                    p[2] - back * d[2], deposit, mua))
-                    st[XO_STAT_DEPOSITS]++;
+        /* up to kVoxRoundsPerIteration steps per iteration: a lane goes on
+         * stepping across voxel faces until its step ends inside a voxel
+         * (the parked interaction) or its packet ends; the loop-top work
+         * is shared by those steps (+4 % on the 200^3 run) */
+#pragma unroll
+        for (int rep = 0; rep < kVoxRoundsPerIteration; ++rep) {
+            if (!alive || pend_int) break;
+            /* ---- one step of while(!done), mcvox.template.c:748-924 ---- */
+            const MaterialL &Mt = sm[mi];
+            st[XO_STAT_STEPS]++;
+            float step;
+            if (METHOD == XO_METHOD_MBL)
+                step = M::div(-M::log_unit(rng_next(x, a)), Mt.mus); /* mcvox.template.c:763-766 */
+            else
+                step = -M::log_unit(rng_next(x, a)) * Mt.inv_mut;
+            step = fminf(step, XO_FLT_MAX);
+            /* mcsim_intersect, mcvox.template.c:173-196 */
+            float dist[3];
+            for (int k = 0; k < 3; ++k) {
+                float side = (float)((int32_t)(d[k] >= 0.0f) + vi[k]);
+                dist[k] = g.tl[k] + side * g.size[k] - p[k];
+                dist[k] = (d[k] != 0.0f) ? M::div(dist[k], d[k]) : XO_INF_F;
             }
-        }
-        if (dd < step) {
-            st[XO_STAT_BOUNDARY]++;
-            /* mcsim_boundary, mcvox.template.c:275-388 */
-            int32_t nrm[3];
-            nrm[0] = (dist[0] <= dist[1]) && (dist[0] <= dist[2]);
-            nrm[1] = nrm[0] ? 0 : ((dist[1] <= dist[0]) && (dist[1] <= dist[2]));
-            nrm[2] = !(nrm[0] + nrm[1]);
-            for (int k = 0; k < 3; ++k)
-                if (d[k] < 0.0f) nrm[k] = -nrm[k];
-            int32_t nv0 = vi[0] + nrm[0], nv1 = vi[1] + nrm[1], nv2 = vi[2] + nrm[2];
-            bool escaping = !vox_valid(g, nv0, nv1, nv2);
-            uint32_t next_mi = !escaping ? vox_material(vox, g, nv0, nv1, nv2) : 0u;
-            bool crossed = false;
-            if (mi == next_mi && !escaping) {
-                crossed = true;
-            } else {
-                float n1 = sm[mi].n, n2 = sm[next_mi].n;
-                if (n1 == n2) {
+            float dd = fminf(dist[0], fminf(dist[1], dist[2]));
+            float d_ok = fminf(dd, step);
+            p[0] = __builtin_fmaf(d_ok, d[0], p[0]);
+            p[1] = __builtin_fmaf(d_ok, d[1], p[1]);
+            p[2] = __builtin_fmaf(d_ok, d[2], p[2]);
+            /* optical path length, mcvox.template.c:792-798 */
+            if (TRACE && T.plon) ts.pl += Mt.n * d_ok;
+
+            bool done = false;
+            if (METHOD == XO_METHOD_MBL) {
+                /* absorption along the step inside this voxel,
+                 * mcvox.template.c:800-818 */
+                const float mua = Mt.mua;
+                const float fraction = 1.0f - M::exp(-(mua * d_ok));
+                const float deposit = fraction * w;
+                w -= deposit;
+                if (TRACE) ts.ev |= XO_EVENT_ABSORPTION;
+                if (use_fluence) {
+                    const float back =
+                        (mua != 0.0f)
+                            ? d_ok - M::div(-M::log_unit(1.0f - rng_next(x, a) * fraction), mua)
+                            : 0.0f;
+                    if (deposit_fluence(p[0] - back * d[0], p[1] - back * d[1],
+                                        p[2] - back * d[2], deposit, mua))
+                        st[XO_STAT_DEPOSITS]++;
+                }
+            }
+            if (dd < step) {
+                st[XO_STAT_BOUNDARY]++;
+                /* mcsim_boundary, mcvox.template.c:275-388 */
+                int32_t nrm[3];
+                nrm[0] = (dist[0] <= dist[1]) && (dist[0] <= dist[2]);
+                nrm[1] = nrm[0] ? 0 : ((dist[1] <= dist[0]) && (dist[1] <= dist[2]));
+                nrm[2] = !(nrm[0] + nrm[1]);
+                for (int k = 0; k < 3; ++k)
+                    if (d[k] < 0.0f) nrm[k] = -nrm[k];
+                int32_t nv0 = vi[0] + nrm[0], nv1 = vi[1] + nrm[1], nv2 = vi[2] + nrm[2];
+                bool escaping = !vox_valid(g, nv0, nv1, nv2);
+                uint32_t next_mi = !escaping ? vox_material(vox, g, nv0, nv1, nv2) : 0u;
+                bool crossed = false;
+                if (mi == next_mi && !escaping) {
                     crossed = true;
                 } else {
-                    float cc = cos_critical<M>(n1, n2);
-                    float cos1 = (float)nrm[0] * d[0] + (float)nrm[1] * d[1] +
-                                 (float)nrm[2] * d[2];
-                    float fn0 = (float)nrm[0], fn1 = (float)nrm[1], fn2 = (float)nrm[2];
-                    bool refracted = false;
-                    if (cos1 > cc) {
-                        float R = reflectance<M>(n1, n2, cos1, cc);
-                        if (R < rng_next(x, a)) {
-                            /* refract, mcbase.template.c:1393-1410 */
-                            float c1 = d[0] * fn0 + d[1] * fn1 + d[2] * fn2;
-                            float n1_d_n2 = M::div(n1, n2);
-                            float s2 = n1_d_n2 * n1_d_n2 * (1.0f - c1 * c1);
-                            float k = fsign(c1) * (n1_d_n2 * fabsf(c1) - M::sqrt(1.0f - s2));
-                            d[0] = n1_d_n2 * d[0] - k * fn0;
-                            d[1] = n1_d_n2 * d[1] - k * fn1;
-                            d[2] = n1_d_n2 * d[2] - k * fn2;
-                            refracted = true;
-                            crossed = true;
+                    float n1 = sm[mi].n, n2 = sm[next_mi].n;
+                    if (n1 == n2) {
+                        crossed = true;
+                    } else {
+                        float cc = cos_critical<M>(n1, n2);
+                        float cos1 = (float)nrm[0] * d[0] + (float)nrm[1] * d[1] +
+                                     (float)nrm[2] * d[2];
+                        float fn0 = (float)nrm[0], fn1 = (float)nrm[1], fn2 = (float)nrm[2];
+                        bool refracted = false;
+                        if (cos1 > cc) {
+                            float R = reflectance<M>(n1, n2, cos1, cc);
+                            if (R < rng_next(x, a)) {
+                                /* refract, mcbase.template.c:1393-1410 */
+                                float c1 = d[0] * fn0 + d[1] * fn1 + d[2] * fn2;
+                                float n1_d_n2 = M::div(n1, n2);
+                                float s2 = n1_d_n2 * n1_d_n2 * (1.0f - c1 * c1);
+                                float k = fsign(c1) * (n1_d_n2 * fabsf(c1) - M::sqrt(1.0f - s2));
+                                d[0] = n1_d_n2 * d[0] - k * fn0;
+                                d[1] = n1_d_n2 * d[1] - k * fn1;
+                                d[2] = n1_d_n2 * d[2] - k * fn2;
+                                refracted = true;
+                                crossed = true;
+                            }
+                        }
+                        if (!refracted) {
+                            /* reflect, mcbase.template.c:1342-1350 */
+                            float pn2 = 2.0f * (d[0] * fn0 + d[1] * fn1 + d[2] * fn2);
+                            d[0] = d[0] - fn0 * pn2;
+                            d[1] = d[1] - fn1 * pn2;
+                            d[2] = d[2] - fn2 * pn2;
                         }
                     }
-                    if (!refracted) {
-                        /* reflect, mcbase.template.c:1342-1350 */
-                        float pn2 = 2.0f * (d[0] * fn0 + d[1] * fn1 + d[2] * fn2);
-                        d[0] = d[0] - fn0 * pn2;
-                        d[1] = d[1] - fn1 * pn2;
-                        d[2] = d[2] - fn2 * pn2;
+                }
+                if (TRACE)
+                    ts.ev |= XO_EVENT_BOUNDARY_HIT |
+                             (crossed ? XO_EVENT_REFRACTION : XO_EVENT_REFLECTION);
+                if (crossed) {
+                    vi[0] = nv0;
+                    vi[1] = nv1;
+                    vi[2] = nv2;
+                    mi = next_mi;
+                    st[XO_STAT_CROSSINGS]++;
+                    if (escaping) {
+                        if (vi[2] < 0)
+                            detector_deposit<M>(lds_fresh(sev.top), bins, p[0], p[1], d[0], d[1], d[2],
+                                                w);
+                        else if (vi[2] >= g.shape[2])
+                            detector_deposit<M>(lds_fresh(sev.bottom), bins, p[0], p[1], d[0], d[1],
+                                                d[2], w);
+                        done = true;
                     }
                 }
+                end_iteration(done);
+            } else {
+                pend_int = true;
             }
-            if (TRACE)
-                ts.ev |= XO_EVENT_BOUNDARY_HIT |
-                         (crossed ? XO_EVENT_REFRACTION : XO_EVENT_REFLECTION);
-            if (crossed) {
-                vi[0] = nv0;
-                vi[1] = nv1;
-                vi[2] = nv2;
-                mi = next_mi;
-                st[XO_STAT_CROSSINGS]++;
-                if (escaping) {
-                    if (vi[2] < 0)
-                        detector_deposit<M>(lds_fresh(sev.top), bins, p[0], p[1], d[0], d[1], d[2],
-                                            w);
-                    else if (vi[2] >= g.shape[2])
-                        detector_deposit<M>(lds_fresh(sev.bottom), bins, p[0], p[1], d[0], d[1],
-                                            d[2], w);
-                    done = true;
-                }
-            }
-            end_iteration(done);
-        } else {
-            pend_int = true;
         }
     }
 
