@@ -447,8 +447,9 @@ int xo_ctx_set_rng(xo_ctx *ctx, uint32_t n, const uint64_t *x, const uint32_t *a
 /* Run `npackets` packets through a layered medium: the McKernel enqueue of
  * Mc.run (xopto/mcml/mc.py:922-953; kernel mcml.template.c:346-824).
  * lut/nlut: the fp lookup-table array (Lut phase functions), may be NULL.
- * lanes: 0 = the persistent grid, or one lane per packet (at least 65,536)
- * for a run of fewer packets, capped by max_lanes; else the lane count
+ * lanes: 0 = the persistent grid, or one lane per 5 packets (at least
+ * 65,536) for a run of fewer than 5 packets per lane of that grid, capped
+ * by max_lanes; else the lane count
  * (maxthreads of Mc.run). accum/naccum: host output, zeroed and filled with
  * the integer accumulators (detector bins at their offsets). The RNG state
  * of the used lanes persists in the context across runs. */
