@@ -44,7 +44,7 @@ SERVING_QPS = 250.0                     # requests/s of the serving run
 # for the kernels as they are in this tree, hardware arithmetic.
 TRAFFIC_BYTES_PER_LAUNCH = {'layered_batch_4096': 2.585e11,
                             'skin_vessel_200': 7.52e11,
-                            'slab_1mm': 1.09e7}
+                            'slab_1mm': 5.3e6}
 
 METRIC = ('photon packets/s (1 GPU) batched vs plain; p50/p99 solve latency '
           'at fixed QPS')
