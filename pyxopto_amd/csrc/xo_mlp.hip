@@ -1077,6 +1077,8 @@ __device__ __forceinline__ void wg_loop(const uint16_t *const (&ap)[WTI], const 
 __global__ void __launch_bounds__(WNT) train_weights_kernel(const WgArgs g) {
     __shared__ __attribute__((aligned(16))) float red[WNW][16 * WTI * 16 * WTJ];
     __shared__ float bred[WNW][16 * WTJ];
+    /* the tile's updated weights as bf16, for the fragment stores */
+    __shared__ __attribute__((aligned(16))) uint16_t sw[WTM][WTN + 8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int li = 0;
     while (li + 1 < g.nlayers && (int)blockIdx.x >= g.ly[li + 1].block0) ++li;
@@ -1175,6 +1177,9 @@ __global__ void __launch_bounds__(WNT) train_weights_kernel(const WgArgs g) {
         vi = g.b2 * vo + (1.f - g.b2) * gi * gi;
         return wo - g.lr * (mi / c1) / (sqrtf(vi / c2) + g.eps);
     };
+    uint16_t hw[WEPT];
+#pragma unroll
+    for (int cc = 0; cc < WEPT; ++cc) hw[cc] = 0u; /* padding: stays zero */
     if (in_r < Ly.in) {
         float gi[WEPT];
 #pragma unroll
@@ -1197,10 +1202,12 @@ __global__ void __launch_bounds__(WNT) train_weights_kernel(const WgArgs g) {
             g.m[base + cc] = mi;
             g.v[base + cc] = vi;
             g.w[base + cc] = wi;
-            const uint16_t h = f2bf(wi);
-            Ly.wf[fragment_index(in_r, out_c, Ly.ksf)] = h;
-            if (Ly.wb) Ly.wb[fragment_index(out_c, in_r, Ly.ksb)] = h;
+            hw[cc] = f2bf(wi);
         }
+    }
+    if (g.adam) {
+#pragma unroll
+        for (int cc = 0; cc < WEPT; ++cc) sw[r][cb + cc] = hw[cc];
     }
     if (has_bias) {
         float gi = bred[0][tid];
@@ -1217,9 +1224,28 @@ __global__ void __launch_bounds__(WNT) train_weights_kernel(const WgArgs g) {
         }
     }
     if (g.adam) {
+        __syncthreads();
+        /* both fragment copies of the tile, one whole 16-byte slot per
+         * thread: W_l (k = input, the tile's rows; n = output) and W_l^T
+         * (k = output, n = input); the tile covers one k step and two
+         * 16-column tiles of each */
+        const int slot = tid & 63, jt = (tid >> 6) & 1;
+        const int nl = 16 * jt + (slot & 15), kl = 8 * (slot >> 4);
+        if (tid < 128) {
+            uint32_t h[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) h[e] = sw[kl + e][nl];
+            const uint4 pk = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16),
+                                        h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+            const size_t frag = (size_t)((n0 >> 4) + jt) * Ly.ksf + (m0 >> 5);
+            *reinterpret_cast<uint4 *>(Ly.wf + (frag * 64 + slot) * 8) = pk;
+        } else if (tid < 256 && Ly.wb) {
+            const uint4 pk = *reinterpret_cast<const uint4 *>(&sw[nl][kl]);
+            const size_t frag = (size_t)((m0 >> 4) + jt) * Ly.ksb + (n0 >> 5);
+            *reinterpret_cast<uint4 *>(Ly.wb + (frag * 64 + slot) * 8) = pk;
+        }
         /* every block read *step above; the last one to finish advances it
          * for the next launch (kernel boundary) */
-        __syncthreads();
         if (tid == 0 && atomicAdd(g.done, 1u) == gridDim.x - 1u) {
             *g.step += 1u;
             *g.done = 0u;
