@@ -12,7 +12,7 @@
  *
  * Training step (captured once as a hipGraph). Networks of input width
  * <= 32, hidden widths 256, output width <= 128 and at most six layers:
- * two fused launches, the forward and backward passes of 32-row blocks
+ * two fused launches, the forward and backward passes of 16-row blocks
  * with the activations resident in LDS, then every layer's weight and bias
  * gradients over the whole batch with Adam applied in place. Other shapes:
  * minibatch rows, one product per layer forward (the loss fused into the
@@ -650,7 +650,7 @@ __global__ void refresh_kernel(const float *w, uint64_t n, const uint64_t *bf_in
  * updated weights. The loss: each row block writes its partial sum, the
  * first block of train_weights_kernel adds them in a fixed order (no
  * atomics: a step's loss is deterministic). */
-constexpr int TMT = 2, TBM = 16 * TMT, TMAXL = 6;
+constexpr int TMT = 1, TBM = 16 * TMT, TMAXL = 6;
 /* waves per block of train_rows_kernel: wave w owns 16 / TNW of the 16
  * column tiles of a hidden-width product, 8 / TNW of the output layer's */
 constexpr int TNW = 4, TNT = 64 * TNW;
@@ -1583,10 +1583,12 @@ int train_fused_step(xo_mlp *N, uint32_t B, const float *X, const LossSpec &loss
     g.eps = 1e-8f;
     g.zrow = N->zrow;
     g.loss_part = N->loss_part;
-    g.nparts = (int)((B + TBM - 1) / TBM);
+    g.nparts = 32 * g.nks / TBM; /* the row kernel's blocks */
     g.loss = loss.loss;
     g.ring = loss.ring;
-    hipLaunchKernelGGL(kTrainRows[L], dim3((B + TBM - 1) / TBM), dim3(TNT), 0, N->stream, a);
+    /* row blocks through the last whole k step of 32 rows: the weight
+     * kernel reads every row of it (rows past B carry zero gradients) */
+    hipLaunchKernelGGL(kTrainRows[L], dim3(32 * g.nks / TBM), dim3(TNT), 0, N->stream, a);
     MLP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(train_weights_kernel, dim3(blocks), dim3(WNT), 0, N->stream, g);
     MLP_CHECK(hipGetLastError());
@@ -1696,7 +1698,8 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
     N->dims.assign(dims, dims + nlayers + 1);
     for (uint32_t l = 0; l <= nlayers; ++l) N->ld.push_back(pad8((int)dims[l]));
     N->max_batch = max_batch;
-    /* transposed activations: whole 32-row blocks of the fused training step */
+    /* transposed activations: whole k steps of 32 rows (the fused training
+     * step stores them as MFMA fragments over the batch) */
     N->ldb = ((int)max_batch + 31) & ~31;
     N->max_splits = max_batch / 512u < 8u ? (max_batch / 512u > 0u ? max_batch / 512u : 1u) : 8u;
     for (uint32_t l = 0; l < nlayers; ++l) {
@@ -1746,7 +1749,7 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
         {(void **)&N->step, 4},
         {(void **)&N->done, 4},
         {(void **)&N->zrow, 256},
-        {(void **)&N->loss_part, 4ull * ((max_batch + TBM - 1) / TBM)},
+        {(void **)&N->loss_part, 4ull * (((max_batch + 31) / 32) * 32 / TBM)},
     };
     for (auto &a : allocs) {
         if ((e = hipMalloc(a.p, a.bytes)) != hipSuccess) return bad("allocation", e);

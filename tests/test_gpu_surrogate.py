@@ -84,6 +84,28 @@ def test_gradient_matches_cpu_restatement(ctx, dims, n):
     net.close()
 
 
+def test_ragged_batch_after_a_full_one(ctx):
+    """A batch of 1000 rows after one of 1024 on the same network: the
+    rows past the batch in the last 32-row k step must carry zero
+    gradients, not what the larger batch left there."""
+    dims = S.DIMS
+    net = S.Network(dims, ctx, max_batch=1024)
+    rng = np.random.default_rng(31)
+    p = M.init_params(dims, seed=32)
+    net.set_params(p)
+    x = rng.uniform(-1, 1, (1024, 5)).astype(np.float32)
+    t = rng.normal(0, 1, (1024, 100)).astype(np.float32)
+    net.gradient(x, t)
+    loss, g = net.gradient(x[:1000], t[:1000])
+    lref, gref = M.loss_and_gradient(p, x[:1000], t[:1000], dims, emulate_bf16=True)
+    assert abs(loss - lref) / lref < 1e-4
+    for w_idx, b_idx in M.unpack(np.arange(p.size), dims):
+        for idx in (w_idx.ravel(), b_idx.ravel()):
+            rel = np.linalg.norm(g[idx] - gref[idx]) / np.linalg.norm(gref[idx])
+            assert rel < 1e-2, rel
+    net.close()
+
+
 @pytest.mark.parametrize('rows', [1, 63, 64, 1000, 4096])
 def test_fused_inference_equals_layer_by_layer(ctx, rows):
     """forward() runs the fused inference kernel, gradient() the fused
