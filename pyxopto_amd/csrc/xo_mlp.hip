@@ -1005,10 +1005,11 @@ struct WgArgs {
 
 /* 32x32 tiles (about 170 blocks for the surrogate's layers); each wave
  * keeps two groups of WG k steps in flight */
-constexpr int WTI = 2, WTJ = 2, WG = 5;
-/* waves per block, each a 1/WNW share of the batch: two per SIMD, so
- * twice the fragment loads in flight per CU */
-constexpr int WNW = 8, WNT = 64 * WNW;
+constexpr int WTI = 2, WTJ = 2, WG = 2;
+/* waves per block, each a 1/WNW share of the batch: four per SIMD, so
+ * more fragment loads in flight per CU (8 waves with 5-step groups: 17.8
+ * us per launch, 16 waves with 2-step groups: 15.4 us) */
+constexpr int WNW = 16, WNT = 64 * WNW;
 constexpr int WTM = 16 * WTI, WTN = 16 * WTJ, WEPT = WTM * WTN / WNT;
 
 /* The k loop of a weight-gradient tile: steps [ks0, ks1) of 32 batch rows,
