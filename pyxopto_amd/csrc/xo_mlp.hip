@@ -653,7 +653,7 @@ __global__ void refresh_kernel(const float *w, uint64_t n, const uint64_t *bf_in
 constexpr int TMT = 1, TBM = 16 * TMT, TMAXL = 6;
 /* waves per block of train_rows_kernel: wave w owns 16 / TNW of the 16
  * column tiles of a hidden-width product, 8 / TNW of the output layer's */
-constexpr int TNW = 4, TNT = 64 * TNW;
+constexpr int TNW = 8, TNT = 64 * TNW;
 constexpr int TXS = 40, TYS = 136; /* LDS row strides of the input and output tiles */
 enum TrainEpi : int { TE_RELU = 0, TE_LOSS = 1, TE_MASK = 2 };
 
