@@ -724,8 +724,9 @@ int xo_mlp_gradient(xo_mlp *mlp, const float *x, const float *t, uint32_t n,
 /* Keep a training set on the device. */
 int xo_mlp_set_data(xo_mlp *mlp, const float *x, const float *t, uint32_t n);
 /* `steps` Adam steps (beta 0.9, 0.999, eps 1e-8) on minibatches of `batch`
- * rows drawn with replacement by a hash of (seed, step, row); one captured
- * graph per step when use_graph. losses[s]: the minibatch loss of step s. */
+ * rows drawn with replacement by a hash of (seed, step, row); when
+ * use_graph, captured graphs of 64 steps and one of the remaining
+ * steps % 64. losses[s]: the minibatch loss of step s. */
 int xo_mlp_train(xo_mlp *mlp, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
                  int use_graph, float *losses, double *kernel_ms);
 /* C = op(A) op(B) with A and B rounded to bf16, fp32 accumulation: the

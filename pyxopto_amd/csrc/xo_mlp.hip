@@ -1296,11 +1296,12 @@ struct xo_mlp {
     uint32_t *done = nullptr;         /* Adam blocks finished this step */
     float *data_x = nullptr, *data_t = nullptr;
     uint32_t ndata = 0;
-    hipGraph_t graph = nullptr;       /* one training step */
+    hipGraph_t graph = nullptr;       /* graph_rem (< kGraphSteps) consecutive steps */
     hipGraphExec_t exec = nullptr;
+    uint32_t graph_rem = 0;
     hipGraph_t graph_k = nullptr;     /* kGraphSteps consecutive steps */
     hipGraphExec_t exec_k = nullptr;
-    uint32_t graph_batch = 0;
+    uint32_t graph_batch = 0;         /* the step arguments both graphs hold */
     uint64_t graph_seed = 0;
     float graph_lr = 0.f;
     std::mutex mu;
@@ -1626,18 +1627,26 @@ int upload_rows(xo_mlp *N, const float *x, uint32_t B) {
     return 0;
 }
 
-/* Training steps per multi-step graph: one graph launch per kGraphSteps
- * steps instead of one per step (each launch boundary costs about 8 us of
- * idle device time). */
-constexpr uint32_t kGraphSteps = 16;
+/* Training steps per multi-step graph: a run of s steps is s / kGraphSteps
+ * launches of one graph and at most one launch of a graph of the remaining
+ * s % kGraphSteps steps, instead of one launch per step (each launch
+ * boundary costs several microseconds of idle device time). */
+constexpr uint32_t kGraphSteps = 64;
 
-void drop_graphs(xo_mlp *N) {
+void drop_rem_graph(xo_mlp *N) {
     if (N->exec) (void)hipGraphExecDestroy(N->exec);
     if (N->graph) (void)hipGraphDestroy(N->graph);
+    N->exec = nullptr;
+    N->graph = nullptr;
+    N->graph_rem = 0;
+}
+
+void drop_graphs(xo_mlp *N) {
+    drop_rem_graph(N);
     if (N->exec_k) (void)hipGraphExecDestroy(N->exec_k);
     if (N->graph_k) (void)hipGraphDestroy(N->graph_k);
-    N->exec = N->exec_k = nullptr;
-    N->graph = N->graph_k = nullptr;
+    N->exec_k = nullptr;
+    N->graph_k = nullptr;
 }
 
 /* n consecutive training steps captured as one graph. */
@@ -1979,7 +1988,8 @@ int xo_mlp_train(xo_mlp *N, uint32_t steps, uint32_t batch, float lr, uint64_t s
         N->loss_cap = steps;
         drop_graphs(N);
     }
-    if (N->exec && (N->graph_batch != batch || N->graph_seed != seed || N->graph_lr != lr))
+    if ((N->exec || N->exec_k) &&
+        (N->graph_batch != batch || N->graph_seed != seed || N->graph_lr != lr))
         drop_graphs(N);
     if (!N->train_fused) {
         /* the weight-gradient launch table is uploaded outside any capture */
@@ -1987,15 +1997,18 @@ int xo_mlp_train(xo_mlp *N, uint32_t steps, uint32_t batch, float lr, uint64_t s
         int blocks = 0;
         if (dw_table(N, batch, &table, &blocks)) return -1;
     }
-    if (use_graph && !N->exec) {
-        if (capture_steps(N, 1, batch, seed, lr, &N->graph, &N->exec)) return -1;
-        N->graph_batch = batch;
-        N->graph_seed = seed;
-        N->graph_lr = lr;
+    const uint32_t rem = steps % kGraphSteps;
+    if (use_graph && rem && N->graph_rem != rem) {
+        drop_rem_graph(N);
+        if (capture_steps(N, rem, batch, seed, lr, &N->graph, &N->exec)) return -1;
+        N->graph_rem = rem;
     }
     if (use_graph && steps >= kGraphSteps && !N->exec_k &&
         capture_steps(N, kGraphSteps, batch, seed, lr, &N->graph_k, &N->exec_k))
         return -1;
+    N->graph_batch = batch;
+    N->graph_seed = seed;
+    N->graph_lr = lr;
     uint32_t step0 = 0;
     MLP_CHECK(hipMemcpyAsync(&step0, N->step, 4, hipMemcpyDeviceToHost, N->stream));
     /* losses start at zero: a fill kernel, outside the graph */
@@ -2013,7 +2026,7 @@ int xo_mlp_train(xo_mlp *N, uint32_t steps, uint32_t batch, float lr, uint64_t s
             s += kGraphSteps;
         } else if (use_graph) {
             if (hipGraphLaunch(N->exec, N->stream) != hipSuccess) rc = -1;
-            ++s;
+            s += rem;
         } else {
             if (train_step(N, batch, seed, lr)) rc = -1;
             ++s;

@@ -139,10 +139,12 @@ def test_captured_steps_equal_eager_steps(ctx, dims):
         net = S.Network(dims, ctx, max_batch=256)
         net.set_params(p0)
         net.set_data(x, t)
-        # 21 steps: one 16-step graph, then single-step graphs
-        l1, _ = net.train(21, 128, 1e-3, seed=3, use_graph=use_graph)
+        # 70 steps: one 64-step graph and a 6-step graph; then 5 steps (a
+        # 5-step graph captured in place of the 6-step one) and 70 again
+        l1, _ = net.train(70, 128, 1e-3, seed=3, use_graph=use_graph)
         l2, _ = net.train(5, 128, 1e-3, seed=3, use_graph=use_graph)
-        out.append((net.get_params(), np.concatenate([l1, l2])))
+        l3, _ = net.train(70, 128, 1e-3, seed=3, use_graph=use_graph)
+        out.append((net.get_params(), np.concatenate([l1, l2, l3])))
         net.close()
     for params, losses in out[1:]:
         np.testing.assert_array_equal(params, out[0][0])
