@@ -530,7 +530,7 @@ def main():
     ap.add_argument('--no-surrogate', action='store_true',
                     help='skip the surrogate network throughput')
     ap.add_argument('--qps', type=float, default=SERVING_QPS)
-    ap.add_argument('--qps-sweep', default='400,550',
+    ap.add_argument('--qps-sweep', default='400,550,800,1000',
                     help='further request rates of the serving run '
                          '(comma-separated, empty for none)')
     ap.add_argument('--tick-packets', type=int, default=None,
