@@ -1,20 +1,21 @@
 """Host-side scattering phase functions and the Monte Carlo lookup-table
 builder (xopto/pf/pfbase.py:102-567, xopto/pf/hg.py, xopto/pf/mhg.py).
 
-``mclut`` follows the published construction of the reference: the CDF of
-the phase function is integrated with Simpson's rule on a fine grid, the
-model F(x) = a/(x + b) + c with a = 2c(1 - c), b = 2c - 1 (so that F(-1) = 0
-and F(1) = 1) is fitted to it by least squares on both admissible branches
-(c > 1 and c < 0; bounded Brent searches instead of the reference's
-L-BFGS-B, same objective), and the table holds the inverse CDF evaluated at the model
-F of N uniformly spaced cosines. The sampler then draws
+``mclut`` follows the construction of the reference (pfbase.py:518-567):
+the CDF of the phase function is integrated with Simpson's rule on a fine
+grid (pfbase.py:439-452), the model F(x) = a/(x + b) + c with
+a = 2c(1 - c), b = 2c - 1 (so that F(-1) = 0 and F(1) = 1) is fitted to it
+by least squares on both admissible branches with the same optimizer calls
+(L-BFGS-B from c = 2 on c > 1 and from c = -1 on c < 0, the better of the
+two kept), and the table holds the inverse CDF evaluated at the model F of
+N uniformly spaced cosines. The sampler then draws
 index = (a/(xi - c) - b + 1)(N - 1)/2 and interpolates linearly.
 """
 from typing import Tuple
 
 import numpy as np
 from scipy.interpolate import interp1d
-from scipy.optimize import minimize_scalar
+from scipy.optimize import minimize
 
 
 def lut_function(costheta, params):
@@ -45,7 +46,8 @@ class PfBase:
         cum[1:] = dx / 3.0 * (p[:-2:2] + 4.0 * p[1:-1:2] + p[2::2])
         cum = np.cumsum(cum)
         out = np.interp(costheta, x[::2], cum)
-        return out / cum[-1]
+        out *= 1.0 / cum[-1]
+        return out
 
     def mclut(self, n: int = 1000, ncd: int = 10000) \
             -> Tuple[list, np.ndarray]:
@@ -59,21 +61,14 @@ class PfBase:
         cumpfi = self.cdf(costhetai)
 
         def cost(c):
-            diff = lut_function(costhetai, c) - cumpfi
-            return float(np.sqrt(np.sum(diff * diff)))
+            return np.linalg.norm(lut_function(costhetai, c) - cumpfi)
 
-        # The model has one free parameter on two branches, c > 1 and c < 0;
-        # each is searched with bounded Brent on a logarithmic scale of
-        # |c - 1| and |c| (the least-squares optimum of the reference's
-        # L-BFGS-B fit, found with fewer evaluations).
-        r1 = minimize_scalar(lambda t: cost(1.0 + np.exp(t)),
-                             bounds=(-30.0, 15.0), method='bounded',
-                             options={'xatol': 1e-10})
-        r2 = minimize_scalar(lambda t: cost(-np.exp(t)),
-                             bounds=(-30.0, 15.0), method='bounded',
-                             options={'xatol': 1e-10})
-        c = 1.0 + np.exp(r1.x) if r1.fun < r2.fun else -np.exp(r2.x)
-        res_x = np.array([c])
+        # one free parameter on two branches, c > 1 and c < 0: a bounded
+        # L-BFGS-B search on each, the lower residual kept
+        eps = np.finfo(np.float64).eps
+        r1 = minimize(cost, [2.0], bounds=((1.0 + eps, np.inf),))
+        r2 = minimize(cost, [-1.0], bounds=((-np.inf, -eps),))
+        res_x = r1.x if r1.fun < r2.fun else r2.x
         randlut = np.clip(lut_function(costheta, res_x), 0.0, 1.0)
         lut = interp1d(cumpfi, costhetai, bounds_error=False, fill_value=0.0,
                        kind='linear')(randlut)
