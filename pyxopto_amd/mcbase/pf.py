@@ -61,8 +61,12 @@ class PfBase:
         costhetai = np.linspace(-1.0, 1.0, ncd)
         cumpfi = self.cdf(costhetai)
 
+        # the residual norm without BLAS: the tables are built by pools of
+        # worker processes, where a threaded BLAS dot in each oversubscribes
+        # the host cores
         def cost(c):
-            return np.linalg.norm(lut_function(costhetai, c) - cumpfi)
+            diff = lut_function(costhetai, c) - cumpfi
+            return float(np.sqrt(np.sum(diff * diff)))
 
         # one free parameter on two branches, c > 1 and c < 0: bounded
         # Brent on a logarithmic scale of |c - 1| and |c|, the lower
