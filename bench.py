@@ -255,7 +255,7 @@ def serving(insts, rates, duration: float, seed: int = 20261203,
 
 
 def surrogate_rates(ctx, rows=(65536, 1048576), batch: int = 4096,
-                    steps: int = 256):
+                    steps: int = 1024):
     """MFMA throughput of the surrogate network (5-256-256-256-100, bf16
     products, fp32 accumulation): fused inference over each count of
     synthetic input rows in `rows` (the figures without a suffix are the
