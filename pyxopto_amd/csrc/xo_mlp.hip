@@ -665,7 +665,8 @@ struct TrainArgs {
     uint32_t n;
     int sampled;             /* the step's minibatch rows, else rows 0..B-1    */
     uint64_t seed;
-    const uint32_t *step;
+    const uint32_t *step;    /* the step counter, read when sampled ...       */
+    uint32_t *step_copy;     /* ... and copied here for the weight kernel     */
     float *loss_part;        /* per row block: its squared errors times scale  */
     float scale;             /* 1 / (B nout)                                   */
     int B, nin, nout, nlayers, ldb;
@@ -949,6 +950,7 @@ __global__ void __launch_bounds__(TNT) train_rows_kernel(const TrainArgs a) {
     train_wload<TrainOp<L, 1>::KS, TrainOp<L, 1>::NJ, TrainOp<L, 1>::OUT>(a.wf[1], lane, wave,
                                                                           f[1]);
     const uint32_t stp = a.sampled ? *a.step : 0u;
+    if (a.sampled && blockIdx.x == 0 && tid == 0) *a.step_copy = stp;
     for (int e = tid; e < TBM * 32; e += TNT) {
         const int m = e >> 5, c = e & 31, grow = row0 + m;
         float v = 0.f;
@@ -1001,7 +1003,11 @@ struct WgArgs {
     int nlayers, nks, ldb, adam;
     float *grad;             /* adam == 0: the gradient, packed like w */
     float *w, *m, *v;
-    uint32_t *step, *done;
+    /* adam: the step is read from the row kernel's copy and its successor
+     * written to the counter itself, which no block of this kernel reads:
+     * no counter of finished blocks, and no atomics */
+    const uint32_t *step;
+    uint32_t *step_next;
     float lr, b1, b2, eps;
     const uint16_t *zrow;    /* 16 bytes of zeros, read for tiles past a layer's width */
     const float *loss_part;  /* the row blocks' partial losses ...           */
@@ -1252,12 +1258,7 @@ __global__ void __launch_bounds__(WNT) train_weights_kernel(const WgArgs g) {
             const size_t frag = (size_t)((m0 >> 4) + jt) * Ly.ksb + (n0 >> 5);
             *reinterpret_cast<uint4 *>(Ly.wb + (frag * 64 + slot) * 8) = pk;
         }
-        /* every block read *step above; the last one to finish advances it
-         * for the next launch (kernel boundary) */
-        if (tid == 0 && atomicAdd(g.done, 1u) == gridDim.x - 1u) {
-            *g.step += 1u;
-            *g.done = 0u;
-        }
+        if (blockIdx.x == 0 && tid == 0) *g.step_next = stp + 1u;
     }
 }
 
@@ -1299,7 +1300,7 @@ struct xo_mlp {
     bool fusable = false, train_fused = false, wt_dirty = true;
     float *loss = nullptr;            /* per training step */
     uint32_t loss_cap = 0;
-    uint32_t *step = nullptr;
+    uint32_t *step = nullptr;         /* [0] the step counter, [1] the fused step's copy */
     uint32_t *done = nullptr;         /* Adam blocks finished this step */
     float *data_x = nullptr, *data_t = nullptr;
     uint32_t ndata = 0;
@@ -1545,6 +1546,7 @@ int train_fused_step(xo_mlp *N, uint32_t B, const float *X, const LossSpec &loss
     a.sampled = loss.sampled;
     a.seed = loss.seed;
     a.step = N->step;
+    a.step_copy = N->step + 1;
     a.loss_part = N->loss_part;
     a.scale = 1.0f / (float)(B * (uint32_t)N->dims[L]);
     a.B = (int)B;
@@ -1584,8 +1586,8 @@ int train_fused_step(xo_mlp *N, uint32_t B, const float *X, const LossSpec &loss
     g.w = N->w;
     g.m = N->m;
     g.v = N->v;
-    g.step = N->step;
-    g.done = N->done;
+    g.step = N->step + 1;
+    g.step_next = N->step;
     g.lr = lr;
     g.b1 = 0.9f;
     g.b2 = 0.999f;
@@ -1763,7 +1765,8 @@ xo_mlp *xo_mlp_create(xo_ctx *ctx, const uint32_t *dims, uint32_t nlayers, uint3
         {(void **)&N->dyt, 2 * (uint64_t)((dims[nlayers] + 15) & ~15u) * LB},
         {(void **)&N->y, 4 * B * dims[nlayers]},
         {(void **)&N->part, 4ull * N->max_splits * P},
-        {(void **)&N->step, 4},
+        /* the step counter and the fused training step's copy of it */
+        {(void **)&N->step, 8},
         {(void **)&N->done, 4},
         {(void **)&N->zrow, 256},
         {(void **)&N->loss_part, 4ull * (((max_batch + 31) / 32) * 32 / TBM)},
@@ -1840,7 +1843,7 @@ int xo_mlp_set_params(xo_mlp *N, const float *params, uint64_t n) {
     MLP_CHECK(hipMemcpyAsync(N->w, params, 4 * n, hipMemcpyHostToDevice, N->stream));
     MLP_CHECK(hipMemsetAsync(N->m, 0, 4 * n, N->stream));
     MLP_CHECK(hipMemsetAsync(N->v, 0, 4 * n, N->stream));
-    MLP_CHECK(hipMemsetAsync(N->step, 0, 4, N->stream));
+    MLP_CHECK(hipMemsetAsync(N->step, 0, 8, N->stream));
     if (refresh_bf16(N)) return -1;
     N->wt_dirty = true;
     MLP_CHECK(hipStreamSynchronize(N->stream));
