@@ -210,6 +210,42 @@ def test_fused_adam_step_and_weight_copies(ctx):
     net.close()
 
 
+def test_step_counter_across_calls_gradients_and_resets(ctx):
+    """The device step counter (it picks each step's minibatch rows and
+    Adam's bias correction) advances once per step across train calls, is
+    left alone by a gradient evaluation between them, and starts again at
+    zero after set_params: 3 + 4 steps with a gradient in between, and 7
+    steps after a reset, give the losses and weights of 7 steps in one
+    call, bit for bit."""
+    dims = S.DIMS
+    rng = np.random.default_rng(31)
+    x = rng.uniform(-1, 1, (2000, 5)).astype(np.float32)
+    t = np.tanh(x @ rng.normal(0, 1, (5, 100))).astype(np.float32)
+    xq = rng.uniform(-1, 1, (300, 5)).astype(np.float32)
+    tq = rng.normal(0, 1, (300, 100)).astype(np.float32)
+    p0 = M.init_params(dims, seed=32)
+    lr, seed, B = 1e-3, 9, 512
+    ref = S.Network(dims, ctx, max_batch=512)
+    ref.set_params(p0)
+    ref.set_data(x, t)
+    l_ref, _ = ref.train(7, B, lr, seed=seed)
+    p_ref = ref.get_params()
+    ref.close()
+    net = S.Network(dims, ctx, max_batch=512)
+    net.set_params(p0)
+    net.set_data(x, t)
+    l1, _ = net.train(3, B, lr, seed=seed)
+    net.gradient(xq, tq)
+    l2, _ = net.train(4, B, lr, seed=seed)
+    np.testing.assert_array_equal(np.concatenate([l1, l2]), l_ref)
+    np.testing.assert_array_equal(net.get_params(), p_ref)
+    net.set_params(p0)
+    l3, _ = net.train(7, B, lr, seed=seed)
+    np.testing.assert_array_equal(l3, l_ref)
+    np.testing.assert_array_equal(net.get_params(), p_ref)
+    net.close()
+
+
 def test_training_fits_a_smooth_function(ctx):
     dims = S.DIMS
     rng = np.random.default_rng(9)
