@@ -53,10 +53,14 @@ def test_bf16_product_layout(ctx, ta, tb, M_, N_, K):
 
 
 @pytest.mark.parametrize('dims,n', [(S.DIMS, 384), (S.DIMS, 1000),
+                                    ((5, 256, 100), 384),
+                                    ((5, 256, 256, 256, 256, 256, 128), 500),
                                     ((5, 64, 64, 10), 384)])
 def test_gradient_matches_cpu_restatement(ctx, dims, n):
-    """S.DIMS runs the fused training kernels (1000 rows: a ragged last
-    row block), (5, 64, 64, 10) the layer-by-layer products."""
+    """S.DIMS, (5, 256, 100) and the six-layer network run the fused
+    training kernels (the fewest and the most layers they take; 1000 and
+    500 rows: a ragged last row block), (5, 64, 64, 10) the layer-by-layer
+    products."""
     net = S.Network(dims, ctx, max_batch=1024)
     rng = np.random.default_rng(5)
     p = M.init_params(dims, seed=6)
@@ -128,7 +132,8 @@ def test_fused_inference_equals_layer_by_layer(ctx, rows):
     net.close()
 
 
-@pytest.mark.parametrize('dims', [(5, 64, 64, 10), S.DIMS])
+@pytest.mark.parametrize('dims', [(5, 64, 64, 10), S.DIMS, (5, 256, 100),
+                                  (5, 256, 256, 256, 256, 256, 128)])
 def test_captured_steps_equal_eager_steps(ctx, dims):
     rng = np.random.default_rng(7)
     x = rng.uniform(-1, 1, (1000, 5)).astype(np.float32)
